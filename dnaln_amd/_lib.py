@@ -46,5 +46,7 @@ def host() -> C.CDLL:
         lib.synth_write_fasta.restype = i32
         lib.synth_write_fastq.argtypes = [C.c_char_p, vp, vp, vp, u64, u64]
         lib.synth_write_fastq.restype = i32
+        from . import abi
+        abi.declare(lib, abi.HOST_SYMBOLS)
         _host = lib
     return _host

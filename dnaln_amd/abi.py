@@ -1,0 +1,96 @@
+"""ctypes mirror of include/dnaln_hip.h and include/dnaln_host.h (layouts must match the C structs)."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+i32, u32, u64, i64, dbl, vp = C.c_int32, C.c_uint32, C.c_uint64, C.c_int64, C.c_double, C.c_void_p
+
+OK, E_INVALID, E_CAPACITY, E_DEVICE, E_NOMEM, E_INTERNAL, E_PENDING = 0, -1, -2, -3, -4, -5, -6
+MODE_PLAIN, MODE_BATCHED = 0, 1
+
+
+class ScoreParams(C.Structure):
+    _fields_ = [(n, i32) for n in ("match", "mismatch", "gap_open", "gap_ext", "ambig", "bw")]
+
+
+class ChainParams(C.Structure):
+    _fields_ = [(n, i32) for n in ("max_gap", "band", "lookback", "min_score", "min_anchors")]
+
+
+class MapParams(C.Structure):
+    _fields_ = [("k", i32), ("w", i32), ("max_occ", u32), ("sc", ScoreParams), ("ch", ChainParams),
+                ("max_cand", i32), ("pri_ratio", dbl), ("min_aln_score", i32)]
+
+
+class Contig(C.Structure):
+    _fields_ = [("offset", u64), ("len", u32), ("reserved", u32)]
+
+
+class IndexView(C.Structure):
+    _fields_ = [("table", vp), ("n_buckets", u64), ("pos", vp), ("n_pos", u64), ("k", i32), ("w", i32)]
+
+
+class Index(C.Structure):
+    _fields_ = [("view", IndexView), ("n_keys", u64), ("max_count", u32)]
+
+
+class Task(C.Structure):
+    _fields_ = [("q_off", u64), ("t_off", u64), ("qlen", i32), ("tlen", i32)]
+
+
+class MapStats(C.Structure):
+    _fields_ = [(n, u64) for n in ("reads", "mapped", "anchors", "candidates", "cells", "rank_tasks", "tb_tasks")] + \
+               [(n, dbl) for n in ("ms_seed", "ms_chain", "ms_rank", "ms_traceback", "ms_total")]
+
+
+class BatchConfig(C.Structure):
+    _fields_ = [("mode", i32), ("chunk_reads", u32), ("max_reads", u64), ("max_bases", u64),
+                ("anchors_per_read", u32), ("tune", i32)]
+
+
+TASK_DTYPE = np.dtype([("q_off", "<u8"), ("t_off", "<u8"), ("qlen", "<i4"), ("tlen", "<i4")])
+RESULT_DTYPE = np.dtype([("score", "<i4"), ("qb", "<i4"), ("qe", "<i4"), ("tb", "<i4"), ("te", "<i4"), ("nm", "<i4"),
+                         ("cigar_off", "<u8"), ("cigar_len", "<u4"), ("reserved", "<u4")])
+RECORD_DTYPE = np.dtype([("contig", "<i4"), ("rev", "u1"), ("mapq", "u1"), ("reserved", "<u2"), ("pos", "<i8"),
+                         ("score", "<i4"), ("nm", "<i4"), ("n_cand", "<i4"), ("cigar_len", "<u4"),
+                         ("cigar_off", "<u8")])
+assert TASK_DTYPE.itemsize == 24 and RESULT_DTYPE.itemsize == 40 and RECORD_DTYPE.itemsize == 40
+assert C.sizeof(MapParams) == 80 and C.sizeof(BatchConfig) == 32 and C.sizeof(MapStats) == 96
+
+# Every symbol include/dnaln_hip.h declares, with (restype, argtypes).
+HIP_SYMBOLS = {
+    "dnaln_hip_default_params": (None, [vp]),
+    "dnaln_hip_create": (i32, [vp, vp, u64, vp, i32, vp, i32, vp]),
+    "dnaln_hip_destroy": (None, [vp]),
+    "dnaln_hip_last_error": (C.c_char_p, []),
+    "dnaln_hip_align": (i32, [vp, vp, vp, u64, vp, u64, i32, vp, vp, u64]),
+    "dnaln_hip_map_batch": (i32, [vp, vp, vp, vp, vp, u64, vp, vp, u64, vp]),
+    "dnaln_hip_default_batch_config": (None, [vp]),
+    "dnaln_hip_batch_create": (i32, [vp, vp, vp]),
+    "dnaln_hip_batch_destroy": (None, [vp]),
+    "dnaln_hip_batch_upload": (i32, [vp, vp, vp, vp, vp, u64]),
+    "dnaln_hip_batch_run": (i32, [vp]),
+    "dnaln_hip_batch_poll": (i32, [vp, i32]),
+    "dnaln_hip_batch_download": (i32, [vp, vp, vp, u64, vp]),
+    "dnaln_hip_batch_device_bytes": (u64, [vp, vp]),
+    "dnaln_hip_batch_timings": (i32, [vp, vp]),
+}
+
+HOST_SYMBOLS = {
+    "dnaln_index_build": (i32, [vp, vp, i32, i32, i32, vp]),
+    "dnaln_index_free": (None, [vp]),
+    "dnaln_fnv1a": (u32, [C.c_char_p, C.c_size_t]),
+    "dnaln_fnv1a_many": (None, [vp, vp, vp, u64, vp]),
+    "dnaln_encode": (None, [C.c_char_p, C.c_size_t, vp]),
+    "dnaln_sam_format": (C.c_long, [C.c_char_p, C.c_size_t, C.c_char_p, C.c_char_p, C.c_size_t, vp, vp, C.c_char_p,
+                                    vp, C.c_size_t]),
+}
+
+
+def declare(lib: C.CDLL, table: dict | None = None) -> None:
+    for name, (res, args) in (table or HIP_SYMBOLS).items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args

@@ -1,0 +1,259 @@
+"""Python front end of the HIP path (through the C ABI only): index, context, align, map, batches.
+
+Mirrors the reference's C++ surface: Context(...) plays Mapper(ref, idx, params)
+(src/mapper.hpp:53), Context.align plays Aligner::align (src/align.hpp:66) and Context.map
+plays Mapper::map (src/mapper.hpp:54) + append_sam. Nothing here computes alignments itself.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib, abi
+from .synth import Genome, Reads
+
+_ENCODE = np.full(256, 4, dtype=np.uint8)
+for _c, _v in zip(b"ACGTacgtUu", (0, 1, 2, 3, 0, 1, 2, 3, 3, 3)):
+    _ENCODE[_c] = _v
+
+CIGAR_OPS = "MIDNSHP=X"
+
+
+class DnalnError(RuntimeError):
+    def __init__(self, code: int, where: str):
+        msg = _lib.hip().dnaln_hip_last_error().decode(errors="replace")
+        super().__init__(f"{where}: status {code}: {msg}")
+        self.code = code
+
+
+def _check(rc: int, where: str) -> None:
+    if rc != abi.OK:
+        raise DnalnError(rc, where)
+
+
+def _p(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
+def encode(seq_ascii: np.ndarray) -> np.ndarray:
+    """ASCII -> codes 0..4 (src/common.hpp:23-38)."""
+    return _ENCODE[seq_ascii]
+
+
+def map_params(**kw) -> abi.MapParams:
+    p = abi.MapParams()
+    _lib.hip().dnaln_hip_default_params(C.byref(p))
+    for k, v in kw.items():
+        if k in ("match", "mismatch", "gap_open", "gap_ext", "ambig", "bw"):
+            setattr(p.sc, k, v)
+        elif k == "min_chain":
+            p.ch.min_score = v
+        elif k in ("max_gap", "band", "lookback", "min_anchors"):
+            setattr(p.ch, k, v)
+        else:
+            setattr(p, k, v)
+    return p
+
+
+class Index:
+    """MinimizerIndex built on the host (src/index.cpp:10-45), uploaded by Context."""
+
+    def __init__(self, genome: Genome, k: int = 15, w: int = 10):
+        self._contigs = contig_array(genome)
+        self.idx = abi.Index()
+        _check(_lib.host().dnaln_index_build(_p(genome.codes), C.cast(self._contigs, C.c_void_p), len(self._contigs),
+                                             k, w, C.byref(self.idx)), "dnaln_index_build")
+        self.k, self.w = k, w
+
+    @property
+    def n_keys(self) -> int:
+        return self.idx.n_keys
+
+    @property
+    def n_pos(self) -> int:
+        return self.idx.view.n_pos
+
+    def table(self) -> np.ndarray:
+        n = self.idx.view.n_buckets
+        buf = (C.c_uint8 * (n * 16)).from_address(self.idx.view.table)
+        return np.frombuffer(buf, dtype=np.dtype([("key", "<u8"), ("offset", "<u4"), ("count", "<u4")])).copy()
+
+    def positions(self) -> np.ndarray:
+        n = self.idx.view.n_pos
+        if n == 0:
+            return np.zeros(0, np.uint32)
+        return np.frombuffer((C.c_uint32 * n).from_address(self.idx.view.pos), dtype=np.uint32).copy()
+
+    def __del__(self):
+        if getattr(self, "idx", None) is not None and self.idx.view.table:
+            _lib.host().dnaln_index_free(C.byref(self.idx))
+
+
+def contig_array(genome: Genome):
+    arr = (abi.Contig * genome.n_contigs)()
+    for i in range(genome.n_contigs):
+        arr[i].offset = int(genome.contig_off[i])
+        arr[i].len = int(genome.contig_len[i])
+    return arr
+
+
+@dataclass
+class ReadBatch:
+    """Reads encoded for the device: codes, offsets, lengths, FNV-1a of the names."""
+    codes: np.ndarray
+    offs: np.ndarray
+    lens: np.ndarray
+    fnv: np.ndarray
+
+    def __len__(self) -> int:
+        return len(self.lens)
+
+
+def prepare(reads: Reads) -> ReadBatch:
+    names, noff, nlen = reads.name_arrays()
+    fnv = np.empty(len(reads), np.uint32)
+    nb = np.frombuffer(names, dtype=np.uint8) if names else np.zeros(1, np.uint8)
+    _lib.host().dnaln_fnv1a_many(_p(nb), _p(noff), _p(nlen), len(reads), _p(fnv))
+    return ReadBatch(encode(reads.seq), np.ascontiguousarray(reads.offs, np.uint64),
+                     np.ascontiguousarray(reads.lens, np.int32), fnv)
+
+
+class Context:
+    """Resident reference + index + parameters on one GPU (dnaln_hip_create)."""
+
+    def __init__(self, genome: Genome, index: Index | None = None, device: int = 0, **params):
+        self.params = map_params(**params)
+        self.genome = genome
+        self.index = index or Index(genome, self.params.k, self.params.w)
+        self._contigs = contig_array(genome)
+        self.ctx = C.c_void_p()
+        lib = _lib.hip()
+        _check(lib.dnaln_hip_create(C.byref(self.params), _p(genome.codes), len(genome.codes),
+                                    C.cast(self._contigs, C.c_void_p), genome.n_contigs, C.byref(self.index.idx.view),
+                                    device, C.byref(self.ctx)), "dnaln_hip_create")
+
+    def close(self) -> None:
+        if self.ctx:
+            _lib.hip().dnaln_hip_destroy(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---------------------------------------------------------------------------- Aligner::align
+    def align(self, pool: np.ndarray, tasks: np.ndarray, traceback: bool, score: dict | None = None):
+        """tasks: TASK_DTYPE array over the byte pool. Returns (RESULT_DTYPE array, cigar pool)."""
+        pool = np.ascontiguousarray(pool, np.uint8)
+        tasks = np.ascontiguousarray(tasks, abi.TASK_DTYPE)
+        n = len(tasks)
+        out = np.zeros(n, abi.RESULT_DTYPE)
+        cap = int(tasks["qlen"].astype(np.int64).sum()) * 2 + 4 * n + 4
+        cig = np.zeros(cap, np.uint32)
+        sp = None
+        if score is not None:
+            sp = abi.ScoreParams(score["match"], score["mismatch"], score["gap_open"], score["gap_ext"],
+                                 score.get("ambig", 1), score["bw"])
+        _check(_lib.hip().dnaln_hip_align(self.ctx, C.byref(sp) if sp is not None else None, _p(tasks), n, _p(pool),
+                                          len(pool), int(traceback), _p(out), _p(cig), cap), "dnaln_hip_align")
+        return out, cig
+
+    # ---------------------------------------------------------------------------- Mapper::map
+    def map(self, reads: Reads | ReadBatch):
+        rb = reads if isinstance(reads, ReadBatch) else prepare(reads)
+        n = len(rb)
+        out = np.zeros(n, abi.RECORD_DTYPE)
+        cap = int(rb.lens.astype(np.int64).sum()) * 2 + 4 * n + 4
+        cig = np.zeros(cap, np.uint32)
+        stats = abi.MapStats()
+        _check(_lib.hip().dnaln_hip_map_batch(self.ctx, _p(rb.codes), _p(rb.offs), _p(rb.lens), _p(rb.fnv), n,
+                                              _p(out), _p(cig), cap, C.byref(stats)), "dnaln_hip_map_batch")
+        return out, cig, stats
+
+    def batch(self, mode: int = abi.MODE_PLAIN, chunk_reads: int = 65536, max_reads: int = 0, max_bases: int = 0,
+              tune: bool = False) -> "Batch":
+        return Batch(self, mode, chunk_reads, max_reads, max_bases, tune)
+
+
+class Batch:
+    """A device-resident batch (dnaln_hip_batch_*): upload once, run many times, download."""
+
+    def __init__(self, ctx: Context, mode: int, chunk_reads: int, max_reads: int, max_bases: int, tune: bool):
+        self.ctx = ctx
+        cfg = abi.BatchConfig()
+        _lib.hip().dnaln_hip_default_batch_config(C.byref(cfg))
+        cfg.mode, cfg.chunk_reads, cfg.max_reads, cfg.max_bases, cfg.tune = mode, chunk_reads, max_reads, max_bases, int(tune)
+        self.cfg = cfg
+        self.b = C.c_void_p()
+        _check(_lib.hip().dnaln_hip_batch_create(ctx.ctx, C.byref(cfg), C.byref(self.b)), "dnaln_hip_batch_create")
+        self._keep = None
+        self.n = 0
+        self.bases = 0
+
+    def upload(self, rb: ReadBatch) -> None:
+        self._keep = rb
+        self.n = len(rb)
+        self.bases = int(rb.lens.astype(np.int64).sum())
+        _check(_lib.hip().dnaln_hip_batch_upload(self.b, _p(rb.codes), _p(rb.offs), _p(rb.lens), _p(rb.fnv), self.n),
+               "dnaln_hip_batch_upload")
+
+    def run(self) -> None:
+        _check(_lib.hip().dnaln_hip_batch_run(self.b), "dnaln_hip_batch_run")
+
+    def poll(self, wait: bool = True) -> bool:
+        rc = _lib.hip().dnaln_hip_batch_poll(self.b, int(wait))
+        if rc == abi.E_PENDING:
+            return False
+        _check(rc, "dnaln_hip_batch_poll")
+        return True
+
+    def download(self):
+        out = np.zeros(self.n, abi.RECORD_DTYPE)
+        cap = self.bases * 2 + 4 * self.n + 4
+        cig = np.zeros(cap, np.uint32)
+        stats = abi.MapStats()
+        _check(_lib.hip().dnaln_hip_batch_download(self.b, _p(out), _p(cig), cap, C.byref(stats)),
+               "dnaln_hip_batch_download")
+        return out, cig, stats
+
+    def timings(self) -> list[float]:
+        ms = (C.c_double * 5)()
+        _check(_lib.hip().dnaln_hip_batch_timings(self.b, ms), "dnaln_hip_batch_timings")
+        return list(ms)
+
+    def close(self) -> None:
+        if self.b:
+            _lib.hip().dnaln_hip_batch_destroy(self.b)
+            self.b = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def cigar_str(ops) -> str:
+    return "".join(f"{int(c) >> 4}{CIGAR_OPS[int(c) & 15]}" for c in ops) or "*"
+
+
+def record_fields(rec, pool: np.ndarray, contig_names: list[str]) -> tuple:
+    """(FLAG, RNAME, POS, MAPQ, CIGAR, NM, AS) as SAM prints them (src/mapper.cpp:231-270)."""
+    if rec["contig"] < 0:
+        return (4, "*", 0, 0, "*", None, None)
+    o = int(rec["cigar_off"])
+    return (16 if rec["rev"] else 0, contig_names[int(rec["contig"])], int(rec["pos"]) + 1, int(rec["mapq"]),
+            cigar_str(pool[o:o + int(rec["cigar_len"])]), int(rec["nm"]), int(rec["score"]))
+
+
+def record_lines(names: list[str], recs: np.ndarray, pool: np.ndarray, contig_names: list[str]) -> list[str]:
+    out = []
+    for i, r in enumerate(recs):
+        f = record_fields(r, pool, contig_names)
+        out.append("\t".join([names[i], str(f[0]), f[1], str(f[2]), str(f[3]), f[4],
+                              "*" if f[5] is None else str(f[5]), "*" if f[6] is None else str(f[6])]))
+    return out

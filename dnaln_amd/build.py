@@ -94,6 +94,16 @@ def build_host() -> Path:
     return out
 
 
+def build_emu() -> Path:
+    """CPU build of the device lane programs (test infrastructure: kernel logic checked without a GPU)."""
+    LIB.mkdir(parents=True, exist_ok=True)
+    out = LIB / "libdnaln_emu.so"
+    tmp = out.with_suffix(".so.tmp")
+    _run([*HOST_CXX, "-Wno-unknown-pragmas", "-O2", "-shared", CSRC / "emu" / "emu.cpp", "-o", tmp])
+    os.replace(tmp, out)
+    return out
+
+
 def build_oracle() -> None:
     _run(["make", "-s", "-C", ROOT / "oracle"])
     if (REFERENCE / "src" / "mapper.cpp").exists():
@@ -103,13 +113,14 @@ def build_oracle() -> None:
 def build_all() -> None:
     build_hip()
     build_host()
+    build_emu()
     build_oracle()
 
 
 def main(argv: list[str]) -> int:
-    parts = argv or ["hip", "host", "oracle"]
+    parts = argv or ["hip", "host", "emu", "oracle"]
     for p in parts:
-        {"hip": build_hip, "host": build_host, "oracle": build_oracle}[p]()
+        {"hip": build_hip, "host": build_host, "emu": build_emu, "oracle": build_oracle}[p]()
     return 0
 
 

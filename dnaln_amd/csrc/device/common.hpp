@@ -1,0 +1,98 @@
+// common.hpp — types and integer primitives shared by the gfx950 kernels.
+//
+// The per-lane programs in sw_packed.hpp, sw_generic.hpp and seed_chain.hpp are written against
+// the primitives below. Under hipcc's device pass they map to single CDNA4 instructions
+// (v_perm_b32, v_pk_*_i16, v_bfi_b32); under a host compiler they are plain C++ with the same
+// bit-exact semantics, which lets the CPU test build run the exact kernel logic lane by lane.
+#pragma once
+
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#define DN_HD __host__ __device__ inline __attribute__((always_inline))
+#define DN_DEV __device__ inline __attribute__((always_inline))
+#else
+#define DN_HD inline __attribute__((always_inline))
+#define DN_DEV inline
+#endif
+
+namespace dn {
+
+// ------------------------------------------------------------------------------ task descriptors
+// One banded alignment. The query is read forward (q[i]) or reverse-complemented
+// (comp(q[qlen-1-i])); the target window t[0, qlen+2bw) reads as N outside [tlo, thi).
+struct SwTask {
+  uint64_t q;      // device address of the query codes (0..4)
+  uint64_t t;      // device address of target position 0 (may lie outside the buffer; only [tlo, thi) is read)
+  int32_t qlen;
+  int32_t tlo, thi;
+  uint32_t flags;  // bit 0: reverse-complement the query
+};
+
+struct SwResult {
+  int32_t score, qb, qe, tb, te, nm;
+  int32_t n_cigar;   // ops written (reversed order), -1 on overflow of the per-task slot
+  int32_t status;    // 0 ok; nonzero: the walk hit an inconsistent cell (never expected)
+};
+
+struct ScoreP {
+  int32_t match, mismatch, gap_open, gap_ext, ambig, bw;
+};
+
+DN_HD uint8_t comp_code(uint8_t c) { return c < 4 ? (uint8_t)(3 - c) : (uint8_t)4; }
+
+DN_HD int sub_score(const ScoreP& sp, uint8_t a, uint8_t b) {
+  return (a > 3 || b > 3) ? -sp.ambig : (a == b ? sp.match : -sp.mismatch);
+}
+
+// ------------------------------------------------------------------------------ 32-bit primitives
+// v_perm_b32: each selector byte picks a byte of {s0:s1} (s1 = bytes 0..3, s0 = bytes 4..7);
+// 8..11 replicate the sign of bytes 1/3/5/7, 12 gives 0x00, 13..255 give 0xFF.
+DN_HD uint32_t perm_b32(uint32_t s0, uint32_t s1, uint32_t sel) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_perm(s0, s1, sel);
+#else
+  const uint64_t in = ((uint64_t)s0 << 32) | s1;
+  uint32_t out = 0;
+  for (int b = 0; b < 4; ++b) {
+    const uint32_t s = (sel >> (8 * b)) & 0xff;
+    uint32_t v;
+    if (s >= 13) v = 0xff;
+    else if (s == 12) v = 0x00;
+    else if (s >= 8) v = ((in >> (8 * (2 * (s - 8) + 1) + 7)) & 1) ? 0xff : 0x00;
+    else v = (uint32_t)(in >> (8 * s)) & 0xff;
+    out |= v << (8 * b);
+  }
+  return out;
+#endif
+}
+
+// Packed pairs of int16 in one 32-bit word: lane pair (A = bits 0..15, B = bits 16..31).
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef short pk16 __attribute__((ext_vector_type(2)));
+DN_HD pk16 pk_from(uint32_t x) { return __builtin_bit_cast(pk16, x); }
+DN_HD uint32_t pk_bits(pk16 x) { return __builtin_bit_cast(uint32_t, x); }
+DN_HD pk16 pk_add(pk16 a, pk16 b) { return a + b; }
+DN_HD pk16 pk_sub(pk16 a, pk16 b) { return a - b; }
+DN_HD pk16 pk_max(pk16 a, pk16 b) { return __builtin_elementwise_max(a, b); }
+#else
+struct pk16 {
+  int16_t x, y;
+};
+DN_HD pk16 pk_from(uint32_t v) { return pk16{(int16_t)(v & 0xffff), (int16_t)(v >> 16)}; }
+DN_HD uint32_t pk_bits(pk16 a) { return (uint32_t)(uint16_t)a.x | ((uint32_t)(uint16_t)a.y << 16); }
+DN_HD pk16 pk_add(pk16 a, pk16 b) { return pk16{(int16_t)(a.x + b.x), (int16_t)(a.y + b.y)}; }
+DN_HD pk16 pk_sub(pk16 a, pk16 b) { return pk16{(int16_t)(a.x - b.x), (int16_t)(a.y - b.y)}; }
+DN_HD pk16 pk_max(pk16 a, pk16 b) { return pk16{a.x > b.x ? a.x : b.x, a.y > b.y ? a.y : b.y}; }
+#endif
+DN_HD pk16 pk_splat(int v) { return pk_from((uint32_t)(uint16_t)v * 0x10001u); }
+DN_HD int pk_lo(pk16 a) { return (int16_t)(pk_bits(a) & 0xffff); }
+DN_HD int pk_hi(pk16 a) { return (int16_t)(pk_bits(a) >> 16); }
+
+// Sign bits of the two halves as 16-bit masks (0xffff where negative).
+DN_HD uint32_t pk_signmask(pk16 a) { return perm_b32(0, pk_bits(a), 0x09090808u); }
+
+// Bitfield insert: m ? a : b, bitwise.
+DN_HD uint32_t bfi(uint32_t m, uint32_t a, uint32_t b) { return (a & m) | (b & ~m); }
+
+}  // namespace dn

@@ -1,0 +1,909 @@
+// dnaln_hip.hip — gfx950 kernels and the C ABI of include/dnaln_hip.h.
+//
+// Pipeline of one batch (src/mapper.cpp:75-223 restated as device stages):
+//   seed_count -> scan -> seed_emit -> chain (+candidates) -> rank tasks -> packed SW score-only
+//   -> select -> traceback tasks -> packed SW with flags + walk -> records -> scan -> CIGAR gather
+// The reference codes, the index and all parameters stay resident in HBM for the life of the
+// context; reads arrive per batch.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "dnaln_hip.h"
+#include "bw_list.hpp"
+#include "map_steps.hpp"
+#include "seed_chain.hpp"
+#include "sw_generic.hpp"
+#include "sw_packed.hpp"
+
+using namespace dn;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define DN_HIP(call)                                                                          \
+  do {                                                                                        \
+    hipError_t e_ = (call);                                                                   \
+    if (e_ != hipSuccess)                                                                     \
+      return fail(DNALN_E_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_));         \
+  } while (0)
+
+#define DN_TRY(expr)          \
+  do {                        \
+    int rc_ = (expr);         \
+    if (rc_ != DNALN_OK) return rc_; \
+  } while (0)
+
+// Grow-only device buffer.
+struct DevBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  int reserve(size_t bytes) {
+    if (bytes <= cap) return DNALN_OK;
+    if (p) {
+      (void)hipDeviceSynchronize();
+      (void)hipFree(p);
+      p = nullptr;
+      cap = 0;
+    }
+    const size_t want = std::max<size_t>(bytes + bytes / 4, 256);
+    if (hipMalloc(&p, want) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(DNALN_E_NOMEM, "hipMalloc of " + std::to_string(want) + " bytes failed");
+    }
+    cap = want;
+    return DNALN_OK;
+  }
+  template <class T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+};
+
+constexpr int kSeedBlock = 128;
+constexpr int kSwBlock = 256;  // 4 waves, 512 alignments per block
+
+// ------------------------------------------------------------------------------------ kernels
+__device__ inline int wave_max(int v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v = max(v, __shfl_xor(v, m, 64));
+  return v;
+}
+
+__global__ __launch_bounds__(kSeedBlock) void k_seed_count(SeedArgs a, uint64_t* ring_h_g, uint32_t* ring_ps_g,
+                                                           int use_lds) {
+  extern __shared__ uint64_t lds[];
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= a.n_reads) return;
+  Ring ring;
+  if (use_lds) {
+    ring.h = lds + threadIdx.x;
+    ring.ps = reinterpret_cast<uint32_t*>(lds + (size_t)a.w * blockDim.x) + threadIdx.x;
+    ring.stride = blockDim.x;
+  } else {
+    ring.h = ring_h_g + r;
+    ring.ps = ring_ps_g + r;
+    ring.stride = (int)a.n_reads;
+  }
+  a.anchor_count[r] = seed_count_read(a, r, ring);
+}
+
+__global__ __launch_bounds__(kSeedBlock) void k_seed_emit(SeedArgs a, uint64_t* ring_h_g, uint32_t* ring_ps_g,
+                                                          int use_lds) {
+  extern __shared__ uint64_t lds[];
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= a.n_reads) return;
+  Ring ring;
+  if (use_lds) {
+    ring.h = lds + threadIdx.x;
+    ring.ps = reinterpret_cast<uint32_t*>(lds + (size_t)a.w * blockDim.x) + threadIdx.x;
+    ring.stride = blockDim.x;
+  } else {
+    ring.h = ring_h_g + r;
+    ring.ps = ring_ps_g + r;
+    ring.stride = (int)a.n_reads;
+  }
+  seed_emit_read(a, r, ring);
+}
+
+__global__ __launch_bounds__(kSeedBlock) void k_chain(ChainArgs a) {
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= a.n_reads) return;
+  chain_read(a, r);
+}
+
+struct ReadView {
+  const uint8_t* codes;
+  const uint64_t* offs;
+  const int32_t* lens;
+  const uint32_t* fnv;
+};
+
+// Rank tasks: every candidate of reads with at least two (src/mapper.cpp:124-130).
+__global__ void k_rank_count(const int32_t* n_cand, uint32_t* cnt, uint32_t n) {
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < n) cnt[r] = n_cand[r] >= 2 ? (uint32_t)n_cand[r] : 0u;
+}
+
+__global__ void k_rank_tasks(ReadView rv, MapGeom g, const Cand* cands, const int32_t* n_cand, int max_cand,
+                             const uint64_t* rank_off, SwTask* tasks, uint32_t n) {
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const int nc = n_cand[r];
+  if (nc < 2) return;
+  const uint64_t q = (uint64_t)(uintptr_t)(rv.codes + rv.offs[r]);
+  for (int c = 0; c < nc; ++c)
+    tasks[rank_off[r] + c] = make_task(g, q, rv.lens[r], cands[(uint64_t)r * max_cand + c]);
+}
+
+__global__ void k_select(ReadView rv, const int32_t* n_cand, const uint64_t* rank_off, const SwResult* rank_res,
+                         int32_t* chosen, int32_t* second, uint32_t* tbflag, uint32_t n) {
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const int nc = n_cand[r];
+  const SwResult* res = rank_res + (nc >= 2 ? rank_off[r] : 0);
+  int32_t ch, s2;
+  select_read(nc, [res](int c) { return res[c].score; }, rv.fnv[r], ch, s2);
+  chosen[r] = ch;
+  second[r] = s2;
+  tbflag[r] = ch >= 0 ? 1u : 0u;
+}
+
+__global__ void k_tb_tasks(ReadView rv, MapGeom g, const Cand* cands, int max_cand, const int32_t* chosen,
+                           const uint64_t* tb_off, SwTask* tasks, uint32_t n) {
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n || chosen[r] < 0) return;
+  const uint64_t q = (uint64_t)(uintptr_t)(rv.codes + rv.offs[r]);
+  tasks[tb_off[r]] = make_task(g, q, rv.lens[r], cands[(uint64_t)r * max_cand + chosen[r]]);
+}
+
+__global__ void k_records(ReadView rv, MapGeom g, const Cand* cands, const int32_t* n_cand, int max_cand,
+                          const int32_t* chosen, const int32_t* second, const uint64_t* tb_off,
+                          const SwResult* tb_res, int min_aln_score, const uint8_t* lut, int lut_len,
+                          dnaln_record* rec, uint32_t* cig_len, int* err, uint32_t n) {
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const int ch = chosen[r];
+  const SwResult* res = ch >= 0 ? &tb_res[tb_off[r]] : nullptr;
+  if (res && (res->n_cigar < 0 || res->status != 0)) atomicOr(err, res->n_cigar < 0 ? 1 : 2);
+  dnaln_record m;
+  record_read(g, rv.lens[r], n_cand[r], cands + (uint64_t)r * max_cand, ch, second[r], res, min_aln_score, lut,
+              lut_len, m);
+  if (res && res->n_cigar < 0) m.cigar_len = 0;
+  rec[r] = m;
+  cig_len[r] = m.cigar_len;
+}
+
+__global__ void k_cigar_gather(ReadView rv, const int32_t* chosen, const uint64_t* tb_off, const SwResult* tb_res,
+                               const uint32_t* slots, int cap, const uint64_t* cig_off, uint64_t pool_base,
+                               dnaln_record* rec, uint32_t* pool, uint32_t n) {
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  rec[r].cigar_off = pool_base + cig_off[r];
+  if (rec[r].cigar_len == 0) return;
+  const uint64_t t = tb_off[r];
+  gather_cigar(rv.lens[r], tb_res[t], slots + t * (uint64_t)cap, pool + cig_off[r]);
+}
+
+template <int W, bool TB>
+__global__ __launch_bounds__(kSwBlock) void k_sw_packed(SwArgs a) {
+  const int lane = threadIdx.x & 63;
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint64_t base = (uint64_t)wave * 128;
+  if (base >= a.n) return;
+  const uint32_t ka = (uint32_t)base + lane, kb = (uint32_t)base + 64 + lane;
+  const bool va = ka < a.n, vb = kb < a.n;
+  const uint32_t ida = va ? (a.order ? a.order[ka] : ka) : 0u;
+  const uint32_t idb = vb ? (a.order ? a.order[kb] : kb) : 0u;
+  int ql = max(va ? a.tasks[ida].qlen : 0, vb ? a.tasks[idb].qlen : 0);
+  ql = wave_max(ql);
+  uint64_t* slot = TB ? a.dir + (uint64_t)wave * a.dir_slot_words : nullptr;
+  sw_pair<W, TB>(a, ida, idb, va, vb, ql, slot, lane);
+}
+
+__global__ void k_sw_generic(SwGenericArgs a) {
+  const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= a.n) return;
+  sw_generic_one(a, a.order ? a.order[tid] : tid, tid);
+}
+
+__global__ void k_pool_tasks(const dnaln_task* in, uint64_t pool, int bw, SwTask* out, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const dnaln_task t = in[i];
+  SwTask s;
+  s.q = pool + t.q_off;
+  s.t = pool + t.t_off;
+  s.qlen = t.qlen;
+  s.tlo = 0;
+  const int wl = t.qlen + 2 * bw;
+  s.thi = t.tlen < wl ? t.tlen : wl;
+  if (s.thi < 0) s.thi = 0;
+  s.flags = 0;
+  out[i] = s;
+}
+
+template <bool TB>
+bool launch_packed_bw(int bw, const SwArgs& a, unsigned grid, hipStream_t st) {
+  switch (bw) {
+#define DN_CASE(B)                                                                    \
+  case B:                                                                             \
+    hipLaunchKernelGGL((k_sw_packed<2 * B + 1, TB>), dim3(grid), dim3(kSwBlock), 0, st, a); \
+    return true;
+    DN_PACKED_BW_LIST(DN_CASE)
+#undef DN_CASE
+    default:
+      return false;
+  }
+}
+
+bool packed_bw_available(int bw) {
+  switch (bw) {
+#define DN_CASE(B) case B:
+    DN_PACKED_BW_LIST(DN_CASE)
+#undef DN_CASE
+    return true;
+    default:
+      return false;
+  }
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------------------ context
+struct dnaln_hip_ctx {
+  dnaln_map_params p{};
+  int device = 0;
+  hipStream_t stream = nullptr;
+  DevBuf ref, coff, clen, table, pos, cost, lut;
+  std::vector<uint64_t> h_coff;
+  std::vector<uint32_t> h_clen;
+  uint64_t ref_len = 0, mask = 0, n_pos = 0;
+  int nc = 0, k = 0, w = 0, lut_len = 0;
+  // workspace of dnaln_hip_align
+  DevBuf a_pool, a_in, a_tasks, a_res, a_cig, a_dir, a_rows;
+  dnaln_hip_batch* batch = nullptr;  // used by dnaln_hip_map_batch
+};
+
+namespace {
+
+int validate(const dnaln_map_params& p) {
+  if (p.k < 4 || p.k > 28) return fail(DNALN_E_INVALID, "k must be in [4, 28]");
+  if (p.w < 1 || p.w > 255) return fail(DNALN_E_INVALID, "w must be in [1, 255]");
+  if (p.sc.bw < 1 || p.sc.bw > 1000) return fail(DNALN_E_INVALID, "--bw must be in [1, 1000]");
+  if (p.sc.match < 1 || p.sc.mismatch < 0 || p.sc.gap_open < 0 || p.sc.gap_ext < 1)
+    return fail(DNALN_E_INVALID, "scores must satisfy A >= 1, B >= 0, O >= 0, E >= 1");
+  if (p.sc.ambig < 0) return fail(DNALN_E_INVALID, "ambig must be >= 0");
+  if (p.max_cand < 0 || p.ch.lookback < 0 || p.ch.band < 0) return fail(DNALN_E_INVALID, "negative parameter");
+  return DNALN_OK;
+}
+
+uint64_t sw_dir_bytes_per_task(int W, int qmax) { return (uint64_t)qmax * (uint64_t)((W + 7) / 8) * 4; }
+
+// Runs a score-only or traceback fill over tasks[0, n) (device arrays), results in res[n],
+// reversed CIGARs in cig[n * cap]. Chunks the traceback launches so the direction scratch stays
+// under `dir_budget` bytes.
+int run_sw(const ScoreP& sp, const SwTask* tasks, uint32_t n, int qmax, bool traceback, SwResult* res,
+           uint32_t* cig, int cap, DevBuf& dir, DevBuf& rows, hipStream_t st, uint64_t* cells_out) {
+  if (n == 0) return DNALN_OK;
+  const int W = 2 * sp.bw + 1;
+  const bool packed = packed_bw_available(sp.bw) && packed_ok(sp, qmax);
+  const uint64_t budget = 1ull << 30;
+  if (packed) {
+    SwArgs a{};
+    a.sp = sp;
+    score_table(sp, a.tab0, a.tab1);
+    a.cigar_cap = cap;
+    const uint64_t slot_words = (uint64_t)std::max(qmax, 1) * ((W + 7) / 8) * 64;
+    uint32_t per_launch = n;
+    if (traceback) {
+      const uint64_t per_wave = slot_words * 8;
+      const uint64_t waves = std::max<uint64_t>(1, budget / per_wave);
+      per_launch = (uint32_t)std::min<uint64_t>(n, waves * 128);
+      DN_TRY(dir.reserve(((per_launch + 127) / 128) * per_wave));
+    }
+    for (uint32_t t0 = 0; t0 < n; t0 += per_launch) {
+      const uint32_t m = std::min(per_launch, n - t0);
+      a.tasks = tasks + t0;
+      a.order = nullptr;
+      a.n = m;
+      a.res = res + t0;
+      a.cigar = cig + (uint64_t)t0 * cap;
+      a.dir = traceback ? dir.as<uint64_t>() : nullptr;
+      a.dir_slot_words = slot_words;
+      const unsigned waves = (m + 127) / 128;
+      const unsigned grid = (waves + kSwBlock / 64 - 1) / (kSwBlock / 64);
+      const bool ok = traceback ? launch_packed_bw<true>(sp.bw, a, grid, st) : launch_packed_bw<false>(sp.bw, a, grid, st);
+      if (!ok) return fail(DNALN_E_INTERNAL, "no packed kernel for this band width");
+      DN_HIP(hipGetLastError());
+    }
+  } else {
+    SwGenericArgs g{};
+    g.sp = sp;
+    g.cigar_cap = cap;
+    g.traceback = traceback;
+    const uint64_t stride = traceback ? (uint64_t)std::max(qmax, 1) * W : 0;
+    uint32_t per_launch = n;
+    if (traceback) per_launch = (uint32_t)std::min<uint64_t>(n, std::max<uint64_t>(256, budget / std::max<uint64_t>(stride, 1)));
+    DN_TRY(rows.reserve((uint64_t)per_launch * 2 * (W + 1) * sizeof(int32_t)));
+    if (traceback) DN_TRY(dir.reserve((uint64_t)per_launch * stride));
+    for (uint32_t t0 = 0; t0 < n; t0 += per_launch) {
+      const uint32_t m = std::min(per_launch, n - t0);
+      g.tasks = tasks + t0;
+      g.order = nullptr;
+      g.n = m;
+      g.res = res + t0;
+      g.cigar = cig + (uint64_t)t0 * cap;
+      g.rows = rows.as<int32_t>();
+      g.dir = traceback ? dir.as<uint8_t>() : nullptr;
+      g.dir_stride = stride;
+      hipLaunchKernelGGL(k_sw_generic, dim3((m + 127) / 128), dim3(128), 0, st, g);
+      DN_HIP(hipGetLastError());
+    }
+  }
+  (void)cells_out;
+  return DNALN_OK;
+}
+
+template <class T>
+int exclusive_scan(DevBuf& tmp, const T* in, uint64_t* out, uint32_t n, hipStream_t st) {
+  size_t bytes = 0;
+  DN_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, n, st));
+  DN_TRY(tmp.reserve(bytes));
+  DN_HIP(hipcub::DeviceScan::ExclusiveSum(tmp.p, bytes, in, out, n, st));
+  return DNALN_OK;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------------------ batches
+struct dnaln_hip_batch {
+  dnaln_hip_ctx* ctx = nullptr;
+  dnaln_batch_config cfg{};
+  hipStream_t st = nullptr;
+  // inputs
+  DevBuf codes, offs, lens, fnv;
+  uint64_t n = 0, bases = 0;
+  int maxlen = 0;
+  // workspace
+  DevBuf acnt, aoff, keys, f, prev, order, chains, ring_h, ring_ps;
+  DevBuf cands, ncand, rcnt, roff, rtasks, rres, chosen, second, tbflag, tboff, ttasks, tres, tcig;
+  DevBuf dir, rows, scan_tmp, ciglen, cigoff, err, scratch8;
+  // outputs
+  DevBuf rec, pool;
+  uint64_t pool_used = 0;
+  dnaln_map_stats stats{};
+  hipEvent_t ev[6] = {};
+  double ms[5] = {0, 0, 0, 0, 0};
+  bool pending = false;
+};
+
+namespace {
+
+int d2h_u64(hipStream_t st, const uint64_t* d, uint64_t& h) {
+  DN_HIP(hipMemcpyAsync(&h, d, sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+  DN_HIP(hipStreamSynchronize(st));
+  return DNALN_OK;
+}
+
+// Total of an exclusive scan: off[n-1] + cnt[n-1].
+template <class T>
+int scan_total(hipStream_t st, const uint64_t* off, const T* cnt, uint32_t n, uint64_t& total) {
+  if (n == 0) {
+    total = 0;
+    return DNALN_OK;
+  }
+  uint64_t o = 0;
+  T c = 0;
+  DN_HIP(hipMemcpyAsync(&o, off + n - 1, sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+  DN_HIP(hipMemcpyAsync(&c, cnt + n - 1, sizeof(T), hipMemcpyDeviceToHost, st));
+  DN_HIP(hipStreamSynchronize(st));
+  total = o + (uint64_t)c;
+  return DNALN_OK;
+}
+
+// One chunk of reads [r0, r0 + n) through the whole pipeline (plain form).
+int run_chunk(dnaln_hip_batch* b, uint64_t r0, uint32_t n, bool first_chunk) {
+  dnaln_hip_ctx* c = b->ctx;
+  const dnaln_map_params& p = c->p;
+  hipStream_t st = b->st;
+  const int max_cand = std::max(p.max_cand, 0);
+  const ScoreP sp{p.sc.match, p.sc.mismatch, p.sc.gap_open, p.sc.gap_ext, p.sc.ambig, p.sc.bw};
+  const int cap = 2 * b->maxlen + 4;
+
+  ReadView rv;
+  rv.codes = b->codes.as<uint8_t>();
+  rv.offs = b->offs.as<uint64_t>() + r0;
+  rv.lens = b->lens.as<int32_t>() + r0;
+  rv.fnv = b->fnv.as<uint32_t>() + r0;
+  MapGeom g;
+  g.ref = (uint64_t)(uintptr_t)c->ref.p;
+  g.coff = c->coff.as<uint64_t>();
+  g.clen = c->clen.as<uint32_t>();
+  g.bw = p.sc.bw;
+
+  if (first_chunk) DN_HIP(hipEventRecord(b->ev[0], st));
+  // 1. seeds
+  DN_TRY(b->acnt.reserve((uint64_t)n * 4 + 4));
+  DN_TRY(b->aoff.reserve((uint64_t)n * 8 + 8));
+  SeedArgs sa{};
+  sa.codes = rv.codes;
+  sa.offs = rv.offs;
+  sa.lens = rv.lens;
+  sa.n_reads = n;
+  sa.table = c->table.as<Bucket>();
+  sa.mask = c->mask;
+  sa.pos = c->pos.as<uint32_t>();
+  sa.k = c->k;
+  sa.w = c->w;
+  sa.max_occ = p.max_occ;
+  sa.anchor_count = b->acnt.as<uint32_t>();
+  sa.anchor_off = b->aoff.as<uint64_t>();
+  const size_t lds = (size_t)c->w * kSeedBlock * 12;
+  const int use_lds = lds <= 48 * 1024;
+  if (!use_lds) {
+    DN_TRY(b->ring_h.reserve((uint64_t)n * c->w * 8));
+    DN_TRY(b->ring_ps.reserve((uint64_t)n * c->w * 4));
+  }
+  const unsigned sgrid = (n + kSeedBlock - 1) / kSeedBlock;
+  if (n) {
+    hipLaunchKernelGGL(k_seed_count, dim3(sgrid), dim3(kSeedBlock), use_lds ? lds : 0, st, sa,
+                       b->ring_h.as<uint64_t>(), b->ring_ps.as<uint32_t>(), use_lds);
+    DN_HIP(hipGetLastError());
+  }
+  DN_TRY(exclusive_scan(b->scan_tmp, b->acnt.as<uint32_t>(), b->aoff.as<uint64_t>(), n, st));
+  uint64_t na = 0;
+  DN_TRY(scan_total(st, b->aoff.as<uint64_t>(), b->acnt.as<uint32_t>(), n, na));
+  DN_TRY(b->keys.reserve(na * 8 + 8));
+  DN_TRY(b->f.reserve(na * 4 + 4));
+  DN_TRY(b->prev.reserve(na * 4 + 4));
+  DN_TRY(b->order.reserve(na * 8 + 8));
+  DN_TRY(b->chains.reserve(na * sizeof(ChainRec) + sizeof(ChainRec)));
+  sa.keys = b->keys.as<uint64_t>();
+  if (n) {
+    hipLaunchKernelGGL(k_seed_emit, dim3(sgrid), dim3(kSeedBlock), use_lds ? lds : 0, st, sa,
+                       b->ring_h.as<uint64_t>(), b->ring_ps.as<uint32_t>(), use_lds);
+    DN_HIP(hipGetLastError());
+  }
+  b->stats.anchors += na;
+
+  // 2. chains and candidates
+  DN_TRY(b->cands.reserve((uint64_t)n * std::max(max_cand, 1) * sizeof(Cand)));
+  DN_TRY(b->ncand.reserve((uint64_t)n * 4 + 4));
+  ChainArgs ca{};
+  ca.n_reads = n;
+  ca.anchor_count = b->acnt.as<uint32_t>();
+  ca.anchor_off = b->aoff.as<uint64_t>();
+  ca.keys = b->keys.as<uint64_t>();
+  ca.f = b->f.as<int32_t>();
+  ca.prev = b->prev.as<int32_t>();
+  ca.order = b->order.as<uint64_t>();
+  ca.chains = b->chains.as<ChainRec>();
+  ca.cost = c->cost.as<int32_t>();
+  ca.k = c->k;
+  ca.max_gap = p.ch.max_gap;
+  ca.band = p.ch.band;
+  ca.lookback = p.ch.lookback;
+  ca.min_score = p.ch.min_score;
+  ca.min_anchors = p.ch.min_anchors;
+  ca.max_cand = max_cand;
+  ca.bw = p.sc.bw;
+  ca.pri_ratio = p.pri_ratio;
+  ca.contig_off = c->coff.as<uint64_t>();
+  ca.n_contigs = c->nc;
+  ca.cands = b->cands.as<Cand>();
+  ca.n_cand = b->ncand.as<int32_t>();
+  if (first_chunk) DN_HIP(hipEventRecord(b->ev[1], st));
+  if (n) {
+    hipLaunchKernelGGL(k_chain, dim3(sgrid), dim3(kSeedBlock), 0, st, ca);
+    DN_HIP(hipGetLastError());
+  }
+
+  // 3. rank pass
+  if (first_chunk) DN_HIP(hipEventRecord(b->ev[2], st));
+  DN_TRY(b->rcnt.reserve((uint64_t)n * 4 + 4));
+  DN_TRY(b->roff.reserve((uint64_t)n * 8 + 8));
+  if (n) hipLaunchKernelGGL(k_rank_count, dim3(sgrid), dim3(kSeedBlock), 0, st, b->ncand.as<int32_t>(), b->rcnt.as<uint32_t>(), n);
+  DN_TRY(exclusive_scan(b->scan_tmp, b->rcnt.as<uint32_t>(), b->roff.as<uint64_t>(), n, st));
+  uint64_t nr = 0;
+  DN_TRY(scan_total(st, b->roff.as<uint64_t>(), b->rcnt.as<uint32_t>(), n, nr));
+  DN_TRY(b->rtasks.reserve(nr * sizeof(SwTask) + sizeof(SwTask)));
+  DN_TRY(b->rres.reserve(nr * sizeof(SwResult) + sizeof(SwResult)));
+  if (n)
+    hipLaunchKernelGGL(k_rank_tasks, dim3(sgrid), dim3(kSeedBlock), 0, st, rv, g, b->cands.as<Cand>(),
+                       b->ncand.as<int32_t>(), max_cand, b->roff.as<uint64_t>(), b->rtasks.as<SwTask>(), n);
+  DN_TRY(run_sw(sp, b->rtasks.as<SwTask>(), (uint32_t)nr, b->maxlen, false, b->rres.as<SwResult>(), nullptr, 0,
+                b->dir, b->rows, st, nullptr));
+  b->stats.rank_tasks += nr;
+
+  // 4. select + traceback pass
+  if (first_chunk) DN_HIP(hipEventRecord(b->ev[3], st));
+  DN_TRY(b->chosen.reserve((uint64_t)n * 4 + 4));
+  DN_TRY(b->second.reserve((uint64_t)n * 4 + 4));
+  DN_TRY(b->tbflag.reserve((uint64_t)n * 4 + 4));
+  DN_TRY(b->tboff.reserve((uint64_t)n * 8 + 8));
+  if (n)
+    hipLaunchKernelGGL(k_select, dim3(sgrid), dim3(kSeedBlock), 0, st, rv, b->ncand.as<int32_t>(),
+                       b->roff.as<uint64_t>(), b->rres.as<SwResult>(), b->chosen.as<int32_t>(),
+                       b->second.as<int32_t>(), b->tbflag.as<uint32_t>(), n);
+  DN_TRY(exclusive_scan(b->scan_tmp, b->tbflag.as<uint32_t>(), b->tboff.as<uint64_t>(), n, st));
+  uint64_t nt = 0;
+  DN_TRY(scan_total(st, b->tboff.as<uint64_t>(), b->tbflag.as<uint32_t>(), n, nt));
+  DN_TRY(b->ttasks.reserve(nt * sizeof(SwTask) + sizeof(SwTask)));
+  DN_TRY(b->tres.reserve(nt * sizeof(SwResult) + sizeof(SwResult)));
+  DN_TRY(b->tcig.reserve(nt * (uint64_t)cap * 4 + 4));
+  if (n)
+    hipLaunchKernelGGL(k_tb_tasks, dim3(sgrid), dim3(kSeedBlock), 0, st, rv, g, b->cands.as<Cand>(), max_cand,
+                       b->chosen.as<int32_t>(), b->tboff.as<uint64_t>(), b->ttasks.as<SwTask>(), n);
+  DN_TRY(run_sw(sp, b->ttasks.as<SwTask>(), (uint32_t)nt, b->maxlen, true, b->tres.as<SwResult>(),
+                b->tcig.as<uint32_t>(), cap, b->dir, b->rows, st, nullptr));
+  b->stats.tb_tasks += nt;
+
+  // 5. records and CIGARs
+  if (first_chunk) DN_HIP(hipEventRecord(b->ev[4], st));
+  DN_TRY(b->ciglen.reserve((uint64_t)n * 4 + 4));
+  DN_TRY(b->cigoff.reserve((uint64_t)n * 8 + 8));
+  dnaln_record* rec = b->rec.as<dnaln_record>() + r0;
+  if (n)
+    hipLaunchKernelGGL(k_records, dim3(sgrid), dim3(kSeedBlock), 0, st, rv, g, b->cands.as<Cand>(),
+                       b->ncand.as<int32_t>(), max_cand, b->chosen.as<int32_t>(), b->second.as<int32_t>(),
+                       b->tboff.as<uint64_t>(), b->tres.as<SwResult>(), p.min_aln_score, c->lut.as<uint8_t>(),
+                       c->lut_len, rec, b->ciglen.as<uint32_t>(), b->err.as<int>(), n);
+  DN_TRY(exclusive_scan(b->scan_tmp, b->ciglen.as<uint32_t>(), b->cigoff.as<uint64_t>(), n, st));
+  uint64_t ncig = 0;
+  DN_TRY(scan_total(st, b->cigoff.as<uint64_t>(), b->ciglen.as<uint32_t>(), n, ncig));
+  // The pool grows across chunks; keep what earlier chunks wrote.
+  const uint64_t need = (b->pool_used + ncig) * 4 + 4;
+  if (need > b->pool.cap) {
+    DevBuf bigger;
+    DN_TRY(bigger.reserve(need * 2));
+    if (b->pool_used) DN_HIP(hipMemcpyAsync(bigger.p, b->pool.p, b->pool_used * 4, hipMemcpyDeviceToDevice, st));
+    DN_HIP(hipStreamSynchronize(st));
+    std::swap(b->pool.p, bigger.p);
+    std::swap(b->pool.cap, bigger.cap);
+  }
+  if (n)
+    hipLaunchKernelGGL(k_cigar_gather, dim3(sgrid), dim3(kSeedBlock), 0, st, rv, b->chosen.as<int32_t>(),
+                       b->tboff.as<uint64_t>(), b->tres.as<SwResult>(), b->tcig.as<uint32_t>(), cap,
+                       b->cigoff.as<uint64_t>(), b->pool_used, rec, b->pool.as<uint32_t>() + b->pool_used, n);
+  DN_HIP(hipGetLastError());
+  b->pool_used += ncig;
+  return DNALN_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void dnaln_hip_default_params(dnaln_map_params* p) {
+  std::memset(p, 0, sizeof *p);
+  p->k = 15;
+  p->w = 10;
+  p->max_occ = 500;
+  p->sc = {2, 4, 4, 2, 1, 16};
+  p->ch = {5000, 16, 50, 20, 2};
+  p->max_cand = 10;
+  p->pri_ratio = 0.5;
+  p->min_aln_score = 40;
+}
+
+const char* dnaln_hip_last_error(void) { return g_err.c_str(); }
+
+int dnaln_hip_create(const dnaln_map_params* params, const uint8_t* ref_codes, uint64_t ref_len,
+                     const dnaln_contig* contigs, int32_t n_contigs, const dnaln_index_view* index, int32_t device,
+                     dnaln_hip_ctx** out) {
+  if (!params || !out || !index || (!ref_codes && ref_len) || n_contigs < 1 || !contigs)
+    return fail(DNALN_E_INVALID, "null argument");
+  DN_TRY(validate(*params));
+  if (index->k != params->k || index->w != params->w) return fail(DNALN_E_INVALID, "index k/w differ from params");
+  if (index->n_buckets == 0 || (index->n_buckets & (index->n_buckets - 1)))
+    return fail(DNALN_E_INVALID, "index bucket count must be a power of two");
+  if (ref_len >= (1ull << 31)) return fail(DNALN_E_INVALID, "reference is too large (limit: 2^31 bp)");
+  std::unique_ptr<dnaln_hip_ctx> c(new dnaln_hip_ctx);
+  c->p = *params;
+  c->p.ch.band = std::min(c->p.ch.band, c->p.sc.bw);  // src/mapper.cpp:9
+  c->device = device;
+  DN_HIP(hipSetDevice(device));
+  DN_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  c->ref_len = ref_len;
+  c->nc = n_contigs;
+  c->k = params->k;
+  c->w = params->w;
+  DN_TRY(c->ref.reserve(ref_len + 1));
+  if (ref_len) DN_HIP(hipMemcpy(c->ref.p, ref_codes, ref_len, hipMemcpyHostToDevice));
+  c->h_coff.resize(n_contigs);
+  c->h_clen.resize(n_contigs);
+  for (int i = 0; i < n_contigs; ++i) {
+    c->h_coff[i] = contigs[i].offset;
+    c->h_clen[i] = contigs[i].len;
+    if (contigs[i].offset + contigs[i].len > ref_len) return fail(DNALN_E_INVALID, "contig outside the reference");
+  }
+  DN_TRY(c->coff.reserve(n_contigs * 8));
+  DN_TRY(c->clen.reserve(n_contigs * 4));
+  DN_HIP(hipMemcpy(c->coff.p, c->h_coff.data(), n_contigs * 8, hipMemcpyHostToDevice));
+  DN_HIP(hipMemcpy(c->clen.p, c->h_clen.data(), n_contigs * 4, hipMemcpyHostToDevice));
+  c->mask = index->n_buckets - 1;
+  c->n_pos = index->n_pos;
+  DN_TRY(c->table.reserve(index->n_buckets * sizeof(dnaln_bucket)));
+  DN_HIP(hipMemcpy(c->table.p, index->table, index->n_buckets * sizeof(dnaln_bucket), hipMemcpyHostToDevice));
+  DN_TRY(c->pos.reserve(index->n_pos * 4 + 4));
+  if (index->n_pos) DN_HIP(hipMemcpy(c->pos.p, index->pos, index->n_pos * 4, hipMemcpyHostToDevice));
+  // Chain gap cost by drift dd (src/chain.cpp:36), evaluated in double exactly as the reference.
+  std::vector<int32_t> cost(c->p.ch.band + 1, 0);
+  for (int dd = 1; dd <= c->p.ch.band; ++dd)
+    cost[dd] = (int)(0.01 * c->k * (double)dd + 0.5 * (63 - __builtin_clzll((uint64_t)dd)));
+  DN_TRY(c->cost.reserve(cost.size() * 4));
+  DN_HIP(hipMemcpy(c->cost.p, cost.data(), cost.size() * 4, hipMemcpyHostToDevice));
+  // MAPQ by score difference (src/mapper.cpp:212-214).
+  std::vector<uint8_t> lut;
+  for (int d = 0;; ++d) {
+    const double q = 6.02 * d / c->p.sc.match + 0.499;
+    const uint8_t v = (uint8_t)std::min(60.0, q);
+    if (v >= 60) break;
+    lut.push_back(v);
+  }
+  c->lut_len = (int)lut.size();
+  DN_TRY(c->lut.reserve(lut.size() + 1));
+  if (!lut.empty()) DN_HIP(hipMemcpy(c->lut.p, lut.data(), lut.size(), hipMemcpyHostToDevice));
+  *out = c.release();
+  return DNALN_OK;
+}
+
+void dnaln_hip_destroy(dnaln_hip_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  (void)hipDeviceSynchronize();
+  if (c->batch) dnaln_hip_batch_destroy(c->batch);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+int dnaln_hip_align(dnaln_hip_ctx* c, const dnaln_score_params* spp, const dnaln_task* tasks, uint64_t n,
+                    const uint8_t* seq_pool, uint64_t pool_len, int traceback, dnaln_result* out,
+                    uint32_t* cigar_pool, uint64_t cigar_cap) {
+  if (!c || (n && (!tasks || !out || !seq_pool))) return fail(DNALN_E_INVALID, "null argument");
+  if (n >= (1ull << 31)) return fail(DNALN_E_INVALID, "too many tasks");
+  const dnaln_score_params s = spp ? *spp : c->p.sc;
+  if (s.bw < 1 || s.bw > 1000 || s.match < 1 || s.mismatch < 0 || s.gap_open < 0 || s.gap_ext < 1 || s.ambig < 0)
+    return fail(DNALN_E_INVALID, "score parameters outside the validated range");
+  const ScoreP sp{s.match, s.mismatch, s.gap_open, s.gap_ext, s.ambig, s.bw};
+  DN_HIP(hipSetDevice(c->device));
+  int qmax = 0;
+  for (uint64_t i = 0; i < n; ++i) {
+    if (tasks[i].qlen < 0 || tasks[i].tlen < 0 || tasks[i].q_off + (uint64_t)tasks[i].qlen > pool_len ||
+        tasks[i].t_off + (uint64_t)tasks[i].tlen > pool_len)
+      return fail(DNALN_E_INVALID, "task outside the sequence pool");
+    qmax = std::max(qmax, tasks[i].qlen);
+  }
+  if (n == 0) return DNALN_OK;
+  const int cap = 2 * qmax + 4;
+  hipStream_t st = c->stream;
+  DN_TRY(c->a_pool.reserve(pool_len + 1));
+  DN_TRY(c->a_in.reserve(n * sizeof(dnaln_task)));
+  DN_TRY(c->a_tasks.reserve(n * sizeof(SwTask)));
+  DN_TRY(c->a_res.reserve(n * sizeof(SwResult)));
+  if (traceback) DN_TRY(c->a_cig.reserve(n * (uint64_t)cap * 4));
+  DN_HIP(hipMemcpyAsync(c->a_pool.p, seq_pool, pool_len, hipMemcpyHostToDevice, st));
+  DN_HIP(hipMemcpyAsync(c->a_in.p, tasks, n * sizeof(dnaln_task), hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_pool_tasks, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, c->a_in.as<dnaln_task>(),
+                     (uint64_t)(uintptr_t)c->a_pool.p, sp.bw, c->a_tasks.as<SwTask>(), (uint32_t)n);
+  DN_HIP(hipGetLastError());
+  DN_TRY(run_sw(sp, c->a_tasks.as<SwTask>(), (uint32_t)n, qmax, traceback != 0, c->a_res.as<SwResult>(),
+                c->a_cig.as<uint32_t>(), cap, c->a_dir, c->a_rows, st, nullptr));
+  std::vector<SwResult> res(n);
+  DN_HIP(hipMemcpyAsync(res.data(), c->a_res.p, n * sizeof(SwResult), hipMemcpyDeviceToHost, st));
+  std::vector<uint32_t> cig;
+  if (traceback) {
+    cig.resize(n * (uint64_t)cap);
+    DN_HIP(hipMemcpyAsync(cig.data(), c->a_cig.p, n * (uint64_t)cap * 4, hipMemcpyDeviceToHost, st));
+  }
+  DN_HIP(hipStreamSynchronize(st));
+  uint64_t used = 0;
+  for (uint64_t i = 0; i < n; ++i) {
+    const SwResult& r = res[i];
+    if (r.status != 0 || r.n_cigar < 0) return fail(DNALN_E_INTERNAL, "traceback failed on task " + std::to_string(i));
+    dnaln_result& o = out[i];
+    o.score = r.score;
+    o.qb = r.qb;
+    o.qe = r.qe;
+    o.tb = r.tb;
+    o.te = r.te;
+    o.nm = r.nm;
+    o.cigar_off = used;
+    o.cigar_len = (uint32_t)r.n_cigar;
+    o.reserved = 0;
+    if (r.n_cigar > 0) {
+      if (!cigar_pool || used + (uint64_t)r.n_cigar > cigar_cap)
+        return fail(DNALN_E_CAPACITY, "cigar_pool too small");
+      for (int k = 0; k < r.n_cigar; ++k) cigar_pool[used + k] = cig[i * (uint64_t)cap + (r.n_cigar - 1 - k)];
+      used += (uint64_t)r.n_cigar;
+    }
+  }
+  return DNALN_OK;
+}
+
+void dnaln_hip_default_batch_config(dnaln_batch_config* cfg) {
+  std::memset(cfg, 0, sizeof *cfg);
+  cfg->mode = DNALN_MODE_PLAIN;
+  cfg->chunk_reads = 65536;
+  cfg->max_reads = 0;
+  cfg->max_bases = 0;
+  cfg->anchors_per_read = 64;
+  cfg->tune = 0;
+}
+
+int dnaln_hip_batch_create(dnaln_hip_ctx* c, const dnaln_batch_config* cfg, dnaln_hip_batch** out) {
+  if (!c || !out) return fail(DNALN_E_INVALID, "null argument");
+  std::unique_ptr<dnaln_hip_batch> b(new dnaln_hip_batch);
+  b->ctx = c;
+  if (cfg) b->cfg = *cfg;
+  else dnaln_hip_default_batch_config(&b->cfg);
+  if (b->cfg.chunk_reads == 0) b->cfg.chunk_reads = 65536;
+  DN_HIP(hipSetDevice(c->device));
+  DN_HIP(hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking));
+  for (auto& e : b->ev) DN_HIP(hipEventCreate(&e));
+  DN_TRY(b->err.reserve(64));
+  if (b->cfg.max_reads) {
+    DN_TRY(b->offs.reserve(b->cfg.max_reads * 8));
+    DN_TRY(b->lens.reserve(b->cfg.max_reads * 4));
+    DN_TRY(b->fnv.reserve(b->cfg.max_reads * 4));
+    DN_TRY(b->rec.reserve(b->cfg.max_reads * sizeof(dnaln_record)));
+  }
+  if (b->cfg.max_bases) DN_TRY(b->codes.reserve(b->cfg.max_bases));
+  *out = b.release();
+  return DNALN_OK;
+}
+
+void dnaln_hip_batch_destroy(dnaln_hip_batch* b) {
+  if (!b) return;
+  (void)hipSetDevice(b->ctx->device);
+  if (b->st) (void)hipStreamSynchronize(b->st);
+  for (auto& e : b->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (b->st) (void)hipStreamDestroy(b->st);
+  delete b;
+}
+
+int dnaln_hip_batch_upload(dnaln_hip_batch* b, const uint8_t* codes, const uint64_t* offs, const int32_t* lens,
+                           const uint32_t* fnv, uint64_t n) {
+  if (!b || (n && (!codes || !offs || !lens || !fnv))) return fail(DNALN_E_INVALID, "null argument");
+  if (n >= (1ull << 31)) return fail(DNALN_E_INVALID, "too many reads in one batch");
+  DN_HIP(hipSetDevice(b->ctx->device));
+  uint64_t bases = 0;
+  int maxlen = 0;
+  for (uint64_t i = 0; i < n; ++i) {
+    if (lens[i] < 0) return fail(DNALN_E_INVALID, "negative read length");
+    bases = std::max(bases, offs[i] + (uint64_t)lens[i]);
+    maxlen = std::max(maxlen, lens[i]);
+  }
+  DN_HIP(hipStreamSynchronize(b->st));
+  DN_TRY(b->codes.reserve(bases + 1));
+  DN_TRY(b->offs.reserve(n * 8 + 8));
+  DN_TRY(b->lens.reserve(n * 4 + 4));
+  DN_TRY(b->fnv.reserve(n * 4 + 4));
+  DN_TRY(b->rec.reserve(n * sizeof(dnaln_record) + sizeof(dnaln_record)));
+  if (bases) DN_HIP(hipMemcpyAsync(b->codes.p, codes, bases, hipMemcpyHostToDevice, b->st));
+  if (n) {
+    DN_HIP(hipMemcpyAsync(b->offs.p, offs, n * 8, hipMemcpyHostToDevice, b->st));
+    DN_HIP(hipMemcpyAsync(b->lens.p, lens, n * 4, hipMemcpyHostToDevice, b->st));
+    DN_HIP(hipMemcpyAsync(b->fnv.p, fnv, n * 4, hipMemcpyHostToDevice, b->st));
+  }
+  b->n = n;
+  b->bases = bases;
+  b->maxlen = maxlen;
+  return DNALN_OK;
+}
+
+int dnaln_hip_batch_run(dnaln_hip_batch* b) {
+  if (!b) return fail(DNALN_E_INVALID, "null batch");
+  dnaln_hip_ctx* c = b->ctx;
+  DN_HIP(hipSetDevice(c->device));
+  std::memset(&b->stats, 0, sizeof b->stats);
+  b->pool_used = 0;
+  DN_HIP(hipMemsetAsync(b->err.p, 0, 4, b->st));
+  const uint64_t chunk = b->cfg.mode == DNALN_MODE_PLAIN ? b->cfg.chunk_reads : std::max<uint64_t>(b->n, 1);
+  for (uint64_t r0 = 0; r0 < b->n || (r0 == 0 && b->n == 0); r0 += chunk) {
+    const uint32_t m = (uint32_t)std::min<uint64_t>(chunk, b->n - r0);
+    DN_TRY(run_chunk(b, r0, m, r0 == 0));
+    if (b->n == 0) break;
+  }
+  DN_HIP(hipEventRecord(b->ev[5], b->st));
+  b->stats.reads = b->n;
+  b->pending = true;
+  return DNALN_OK;
+}
+
+int dnaln_hip_batch_poll(dnaln_hip_batch* b, int wait) {
+  if (!b) return fail(DNALN_E_INVALID, "null batch");
+  if (!b->pending) return DNALN_OK;
+  if (!wait) {
+    const hipError_t q = hipEventQuery(b->ev[5]);
+    if (q == hipErrorNotReady) return DNALN_E_PENDING;
+    if (q != hipSuccess) return fail(DNALN_E_DEVICE, hipGetErrorString(q));
+  }
+  DN_HIP(hipEventSynchronize(b->ev[5]));
+  int err = 0;
+  DN_HIP(hipMemcpy(&err, b->err.p, 4, hipMemcpyDeviceToHost));
+  b->pending = false;
+  float t[5] = {0, 0, 0, 0, 0};
+  DN_HIP(hipEventElapsedTime(&t[0], b->ev[0], b->ev[1]));
+  DN_HIP(hipEventElapsedTime(&t[1], b->ev[1], b->ev[2]));
+  DN_HIP(hipEventElapsedTime(&t[2], b->ev[2], b->ev[3]));
+  DN_HIP(hipEventElapsedTime(&t[3], b->ev[3], b->ev[4]));
+  DN_HIP(hipEventElapsedTime(&t[4], b->ev[0], b->ev[5]));
+  for (int i = 0; i < 5; ++i) b->ms[i] = t[i];
+  if (err) return fail(DNALN_E_INTERNAL, err & 1 ? "CIGAR slot overflow" : "inconsistent traceback");
+  return DNALN_OK;
+}
+
+int dnaln_hip_batch_download(dnaln_hip_batch* b, dnaln_record* out, uint32_t* cigar_pool, uint64_t cigar_cap,
+                             dnaln_map_stats* stats) {
+  if (!b) return fail(DNALN_E_INVALID, "null batch");
+  DN_TRY(dnaln_hip_batch_poll(b, 1));
+  if (b->pool_used > cigar_cap) return fail(DNALN_E_CAPACITY, "cigar_pool too small");
+  if (b->n && !out) return fail(DNALN_E_INVALID, "null output");
+  if (b->n) DN_HIP(hipMemcpy(out, b->rec.p, b->n * sizeof(dnaln_record), hipMemcpyDeviceToHost));
+  if (b->pool_used) DN_HIP(hipMemcpy(cigar_pool, b->pool.p, b->pool_used * 4, hipMemcpyDeviceToHost));
+  if (stats) {
+    dnaln_map_stats s = b->stats;
+    uint64_t mapped = 0, cells = 0;
+    const uint64_t W = 2 * (uint64_t)b->ctx->p.sc.bw + 1;
+    for (uint64_t i = 0; i < b->n; ++i) mapped += out[i].contig >= 0;
+    (void)cells;
+    s.mapped = mapped;
+    s.ms_seed = b->ms[0];
+    s.ms_chain = b->ms[1];
+    s.ms_rank = b->ms[2];
+    s.ms_traceback = b->ms[3];
+    s.ms_total = b->ms[4];
+    (void)W;
+    *stats = s;
+  }
+  return DNALN_OK;
+}
+
+uint64_t dnaln_hip_batch_device_bytes(dnaln_hip_ctx* c, const dnaln_batch_config* cfg) {
+  if (!c || !cfg) return 0;
+  const uint64_t n = cfg->max_reads, bases = cfg->max_bases;
+  const uint64_t per_read = 8 + 4 + 4 + sizeof(dnaln_record) + 4 * 12 + (uint64_t)std::max(c->p.max_cand, 1) *
+                            (sizeof(Cand) + sizeof(SwTask) + sizeof(SwResult));
+  return bases + n * per_read + n * (uint64_t)cfg->anchors_per_read * 48;
+}
+
+int dnaln_hip_batch_timings(dnaln_hip_batch* b, double* ms5) {
+  if (!b || !ms5) return fail(DNALN_E_INVALID, "null argument");
+  for (int i = 0; i < 5; ++i) ms5[i] = b->ms[i];
+  return DNALN_OK;
+}
+
+int dnaln_hip_map_batch(dnaln_hip_ctx* c, const uint8_t* read_codes, const uint64_t* offs, const int32_t* lens,
+                        const uint32_t* name_fnv1a, uint64_t n, dnaln_record* out, uint32_t* cigar_pool,
+                        uint64_t cigar_cap, dnaln_map_stats* stats) {
+  if (!c) return fail(DNALN_E_INVALID, "null context");
+  if (!c->batch) {
+    dnaln_batch_config cfg;
+    dnaln_hip_default_batch_config(&cfg);
+    DN_TRY(dnaln_hip_batch_create(c, &cfg, &c->batch));
+  }
+  DN_TRY(dnaln_hip_batch_upload(c->batch, read_codes, offs, lens, name_fnv1a, n));
+  DN_TRY(dnaln_hip_batch_run(c->batch));
+  return dnaln_hip_batch_download(c->batch, out, cigar_pool, cigar_cap, stats);
+}
+
+}  // extern "C"

@@ -1,0 +1,360 @@
+// seed_chain.hpp — per-read programs for sketching, index probes, anchors, chaining and
+// candidate selection (one read per thread).
+//
+//   sketch        src/minimizer.cpp:5-68 — leftmost minimum of every window of w k-mers inside
+//                 an N-free run, deduplicated against the previous valid window's choice.
+//                 Streamed with a w-entry ring and a running minimum that is rescanned only
+//                 when it leaves the window.
+//   probe         src/index.hpp:24-38 — linear probing from hash & mask over 16-byte buckets.
+//   anchors       src/mapper.cpp:24-40 — n = 0 or n > max_occ skipped; strand = (v & 1) ^ m.strand;
+//                 qpos = strand ? len - pos - k : pos. Stored as one sortable 64-bit key
+//                 (strand << 63 | rpos << 32 | qpos), which orders exactly as chain.cpp:14-18.
+//   chain         src/chain.cpp:12-71 — DP over the previous `lookback` anchors (strict >, so the
+//                 nearest predecessor wins ties), extraction by f desc / index asc, stable sort
+//                 of chains by score desc.
+//   candidates    src/mapper.cpp:44-62.
+#pragma once
+
+#include "common.hpp"
+
+namespace dn {
+
+struct Bucket {
+  uint64_t key;
+  uint32_t offset, count;
+};
+
+struct SeedArgs {
+  const uint8_t* codes;     // reads, forward codes 0..4
+  const uint64_t* offs;
+  const int32_t* lens;
+  uint32_t n_reads;
+  const Bucket* table;
+  uint64_t mask;
+  const uint32_t* pos;
+  int k, w;
+  uint32_t max_occ;
+  uint32_t* anchor_count;   // seed_count output
+  const uint64_t* anchor_off;   // exclusive scan of anchor_count
+  uint64_t* keys;           // seed_emit output
+};
+
+DN_HD uint64_t hash64(uint64_t key, uint64_t mask) {  // src/minimizer.hpp:21-30
+  key = (~key + (key << 21)) & mask;
+  key = key ^ key >> 24;
+  key = ((key + (key << 3)) + (key << 8)) & mask;
+  key = key ^ key >> 14;
+  key = ((key + (key << 2)) + (key << 4)) & mask;
+  key = key ^ key >> 28;
+  key = (key + (key << 31)) & mask;
+  return key;
+}
+
+DN_HD void probe(const Bucket* table, uint64_t mask, uint64_t h, uint32_t& off, uint32_t& cnt) {
+  uint64_t i = h & mask;
+  for (;;) {
+    const Bucket b = table[i];
+    if (b.key == h) {
+      off = b.offset;
+      cnt = b.count;
+      return;
+    }
+    if (b.key == ~0ULL) {
+      off = cnt = 0;
+      return;
+    }
+    i = (i + 1) & mask;
+  }
+}
+
+// Ring of the last w k-mers: hash and (pos << 1 | strand), at ring[slot * stride].
+struct Ring {
+  uint64_t* h;
+  uint32_t* ps;
+  int stride;
+};
+
+// Calls emit(hash, pos, strand) for every minimizer of seq[0, len), in position order.
+template <class Emit>
+DN_HD void sketch_stream(const uint8_t* seq, int len, int k, int w, Ring ring, Emit&& emit) {
+  const uint64_t mask = (1ULL << (2 * k)) - 1;
+  const int shift = 2 * (k - 1);
+  const uint64_t kInvalid = ~0ULL;
+  uint64_t fwd = 0, rev = 0;
+  int run = 0, kidx = 0, slot = 0;
+  uint64_t mh = kInvalid;
+  uint32_t mps = 0;
+  int midx = -1;
+  int64_t last = -1;
+  for (int i = 0; i < len; ++i) {
+    const uint8_t c = seq[i];
+    if (c > 3) {
+      run = 0;
+      kidx = 0;
+      slot = 0;
+      mh = kInvalid;
+      midx = -1;
+      continue;
+    }
+    fwd = ((fwd << 2) | c) & mask;
+    rev = (rev >> 2) | ((uint64_t)(3 - c) << shift);
+    if (++run < k) continue;
+    const uint32_t strand = rev < fwd;
+    const uint64_t h = fwd != rev ? hash64(strand ? rev : fwd, mask) : kInvalid;
+    const uint32_t ps = (uint32_t)(i - k + 1) << 1 | strand;
+    ring.h[slot * ring.stride] = h;
+    ring.ps[slot * ring.stride] = ps;
+    // The window is [kidx - w + 1, kidx]; rescan when the running minimum left it.
+    if (midx >= 0 && midx < kidx - w + 1) {
+      const int lo = kidx - w + 1;
+      int s = lo % w;
+      mh = ring.h[s * ring.stride];
+      mps = ring.ps[s * ring.stride];
+      midx = lo;
+      for (int x = lo + 1; x <= kidx; ++x) {
+        s = s + 1 == w ? 0 : s + 1;
+        const uint64_t hx = ring.h[s * ring.stride];
+        if (hx < mh) {
+          mh = hx;
+          mps = ring.ps[s * ring.stride];
+          midx = x;
+        }
+      }
+    } else if (midx < 0 || h < mh) {
+      mh = h;
+      mps = ps;
+      midx = kidx;
+    }
+    if (kidx + 1 >= w && mh != kInvalid) {
+      const uint32_t p = mps >> 1;
+      if ((int64_t)p != last) emit(mh, p, mps & 1);
+      last = p;
+    }
+    ++kidx;
+    slot = slot + 1 == w ? 0 : slot + 1;
+  }
+}
+
+DN_HD uint64_t anchor_key(uint32_t strand, uint32_t rpos, int32_t qpos) {
+  return (uint64_t)strand << 63 | (uint64_t)rpos << 32 | (uint32_t)qpos;
+}
+DN_HD uint32_t key_strand(uint64_t k) { return (uint32_t)(k >> 63); }
+DN_HD uint32_t key_rpos(uint64_t k) { return (uint32_t)(k >> 32) & 0x7fffffffu; }
+DN_HD int32_t key_qpos(uint64_t k) { return (int32_t)(uint32_t)k; }
+
+DN_HD uint32_t seed_count_read(const SeedArgs& a, uint32_t r, Ring ring) {
+  const uint8_t* s = a.codes + a.offs[r];
+  uint32_t total = 0;
+  sketch_stream(s, a.lens[r], a.k, a.w, ring, [&](uint64_t h, uint32_t, uint32_t) {
+    uint32_t off, cnt;
+    probe(a.table, a.mask, h, off, cnt);
+    if (cnt != 0 && cnt <= a.max_occ) total += cnt;
+  });
+  return total;
+}
+
+// Forward-strand anchors fill the slice from the front in emission order, reverse-strand
+// anchors from the back; both runs then come out nearly ascending, which the insertion sort of
+// the chainer exploits. The final order is the unique sorted order either way.
+DN_HD void seed_emit_read(const SeedArgs& a, uint32_t r, Ring ring) {
+  const uint8_t* s = a.codes + a.offs[r];
+  const int len = a.lens[r];
+  uint64_t* out = a.keys + a.anchor_off[r];
+  int64_t lo = 0, hi = (int64_t)a.anchor_count[r] - 1;
+  const int k = a.k;
+  sketch_stream(s, len, k, a.w, ring, [&](uint64_t h, uint32_t mpos, uint32_t mstrand) {
+    uint32_t off, cnt;
+    probe(a.table, a.mask, h, off, cnt);
+    if (cnt == 0 || cnt > a.max_occ) return;
+    for (uint32_t j = 0; j < cnt; ++j) {
+      const uint32_t v = a.pos[off + j];
+      const uint32_t st = (v & 1) ^ mstrand;
+      const int32_t qpos = st ? len - (int32_t)mpos - k : (int32_t)mpos;
+      const uint64_t key = anchor_key(st, v >> 1, qpos);
+      if (st) out[hi--] = key;
+      else out[lo++] = key;
+    }
+  });
+}
+
+// ------------------------------------------------------------------------------------ chaining
+struct ChainRec {
+  int32_t score;
+  uint32_t strand;
+  int32_t n_anchors;
+  uint32_t rpos;
+  int64_t dmin, dmax;
+};
+
+struct Cand {
+  uint32_t rev;
+  int32_t contig;
+  int64_t diag, wstart;
+};
+
+struct ChainArgs {
+  uint32_t n_reads;
+  const uint32_t* anchor_count;
+  const uint64_t* anchor_off;
+  uint64_t* keys;          // sorted in place
+  int32_t* f;              // per anchor
+  int32_t* prev;
+  uint64_t* order;         // per anchor scratch: extraction order
+  ChainRec* chains;        // per anchor scratch: chains of the read
+  const int32_t* cost;     // gap cost by |dr - dq|, 0..band
+  int k, max_gap, band, lookback, min_score, min_anchors;
+  int max_cand, bw;
+  const uint64_t* contig_off;
+  int n_contigs;
+  double pri_ratio;        // keep chains scoring >= pri_ratio * best
+  Cand* cands;             // n_reads * max_cand
+  int32_t* n_cand;
+};
+
+template <class T, class Less>
+DN_HD void insertion_sort(T* a, int n, Less less) {
+  for (int i = 1; i < n; ++i) {
+    const T x = a[i];
+    int j = i - 1;
+    while (j >= 0 && less(x, a[j])) {
+      a[j + 1] = a[j];
+      --j;
+    }
+    a[j + 1] = x;
+  }
+}
+
+template <class T, class Less>
+DN_HD void heap_sort(T* a, int n, Less less) {
+  auto sift = [&](int root, int end) {
+    while (2 * root + 1 < end) {
+      int c = 2 * root + 1;
+      if (c + 1 < end && less(a[c], a[c + 1])) ++c;
+      if (!less(a[root], a[c])) return;
+      const T t = a[root];
+      a[root] = a[c];
+      a[c] = t;
+      root = c;
+    }
+  };
+  for (int s = n / 2 - 1; s >= 0; --s) sift(s, n);
+  for (int end = n - 1; end > 0; --end) {
+    const T t = a[0];
+    a[0] = a[end];
+    a[end] = t;
+    sift(0, end);
+  }
+}
+
+DN_HD int contig_of(const uint64_t* off, int nc, uint64_t gpos) {  // src/reference.cpp:25-32
+  int lo = 0, hi = nc;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) / 2;
+    if (off[mid] <= gpos) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+DN_HD void chain_read(const ChainArgs& a, uint32_t r) {
+  const int n = (int)a.anchor_count[r];
+  const uint64_t base = a.anchor_off[r];
+  Cand* cands = a.cands + (uint64_t)r * (uint64_t)a.max_cand;
+  int nc = 0;
+  if (n > 0) {
+    uint64_t* key = a.keys + base;
+    int32_t* f = a.f + base;
+    int32_t* prev = a.prev + base;
+    auto lt = [](uint64_t x, uint64_t y) { return x < y; };
+    // Nearly sorted on arrival (see seed_emit_read): insertion sort unless the read is large.
+    if (n <= 64) insertion_sort(key, n, lt);
+    else heap_sort(key, n, lt);
+
+    const int k = a.k;
+    for (int i = 0; i < n; ++i) {
+      const uint64_t ki = key[i];
+      const uint32_t si = key_strand(ki);
+      const int64_t ri = key_rpos(ki), qi = key_qpos(ki);
+      int best = k, best_j = -1;
+      const int lo = i - a.lookback > 0 ? i - a.lookback : 0;
+      for (int j = i - 1; j >= lo; --j) {
+        const uint64_t kj = key[j];
+        if (key_strand(kj) != si) break;
+        const int64_t dr = ri - key_rpos(kj);
+        if (dr > a.max_gap) break;
+        const int64_t dq = qi - key_qpos(kj);
+        if (dr <= 0 || dq <= 0) continue;
+        const int64_t dd = dr > dq ? dr - dq : dq - dr;
+        if (dd > a.band) continue;
+        const int64_t m = dq < dr ? dq : dr;
+        const int sc = f[j] + (int)(m < k ? m : k) - a.cost[dd];
+        if (sc > best) {
+          best = sc;
+          best_j = j;
+        }
+      }
+      f[i] = best;
+      prev[i] = best_j;
+    }
+
+    // Extraction order: f desc, index asc, as one ascending 64-bit key.
+    uint64_t* order = a.order + base;
+    for (int i = 0; i < n; ++i) order[i] = (uint64_t)(uint32_t)(0x7fffffff - f[i]) << 32 | (uint32_t)i;
+    if (n <= 64) insertion_sort(order, n, lt);
+    else heap_sort(order, n, lt);
+
+    ChainRec* ch = a.chains + base;
+    int m = 0;
+    for (int o = 0; o < n; ++o) {
+      const int end = (int)(uint32_t)order[o];
+      if (prev[end] == -2) continue;  // used
+      int i = end, cnt = 0;
+      int64_t dmin = INT64_MAX, dmax = INT64_MIN;
+      const int fend = f[end];
+      while (i >= 0 && prev[i] != -2) {
+        const int nx = prev[i];
+        prev[i] = -2;
+        ++cnt;
+        const int64_t d = (int64_t)key_rpos(key[i]) - key_qpos(key[i]);
+        dmin = d < dmin ? d : dmin;
+        dmax = d > dmax ? d : dmax;
+        i = nx;
+      }
+      const int score = fend - (i >= 0 ? f[i] : 0);
+      if (score < a.min_score || cnt < a.min_anchors) continue;
+      ChainRec c;
+      c.score = score;
+      c.strand = key_strand(key[end]);
+      c.n_anchors = cnt;
+      c.rpos = key_rpos(key[end]);
+      c.dmin = dmin;
+      c.dmax = dmax;
+      ch[m++] = c;
+    }
+    // Stable sort by score desc (insertion sort is stable).
+    insertion_sort(ch, m, [](const ChainRec& x, const ChainRec& y) { return x.score > y.score; });
+
+    // Candidates (mapper.cpp:44-62): score >= pri_ratio * best, at most max_cand, deduplicated.
+    const int64_t bw = a.bw;
+    const double min_score = ch[0].score * a.pri_ratio;
+    for (int c = 0; c < m; ++c) {
+      if (ch[c].score < min_score || nc >= a.max_cand) break;
+      const int64_t diag = ch[c].dmin + (ch[c].dmax - ch[c].dmin) / 2;
+      bool dup = false;
+      for (int x = 0; x < nc && !dup; ++x) {
+        const int64_t dd = cands[x].diag - diag;
+        dup = cands[x].rev == ch[c].strand && (dd < 0 ? -dd : dd) <= bw;
+      }
+      if (dup) continue;
+      Cand cd;
+      cd.rev = ch[c].strand;
+      cd.contig = contig_of(a.contig_off, a.n_contigs, ch[c].rpos);
+      cd.diag = diag;
+      cd.wstart = diag - bw;
+      cands[nc++] = cd;
+    }
+  }
+  a.n_cand[r] = nc;
+}
+
+}  // namespace dn

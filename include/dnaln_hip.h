@@ -1,0 +1,185 @@
+/* dnaln_hip.h — C ABI of the MI355X (gfx950) seed-chain-extend path of dnaln.
+ *
+ * Plain C, no STL or HIP types in the signatures: the reference's C++ host (src/mapper.cpp)
+ * reaches the GPU only through these calls. Every call returns 0 (DNALN_OK) or a negative
+ * status; dnaln_hip_last_error() gives the message of the calling thread's last failure.
+ * A context is used by one host thread at a time (the reference's Mapper and Aligner are not
+ * thread-safe either, src/mapper.hpp:51-94).
+ *
+ * Reference interfaces replaced (paths relative to the dnaln checkout):
+ *   dnaln_hip_create / _destroy   Mapper::Mapper (src/mapper.cpp:8-12) with the resident
+ *                                 Reference (src/reference.hpp:16-30) and MinimizerIndex
+ *                                 (src/index.hpp:21-39)
+ *   dnaln_hip_align               Aligner::align (src/align.hpp:62-68)
+ *   dnaln_hip_map_batch           Mapper::map (src/mapper.hpp:54, src/mapper.cpp:75-223), minus
+ *                                 SAM text, which stays on the host (append_sam)
+ *   dnaln_hip_batch_*             the same pipeline on device-resident batches (plain
+ *                                 per-chunk launches or the batched persistent form), used by
+ *                                 the serving loop and the benchmark
+ */
+#ifndef DNALN_HIP_H
+#define DNALN_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DNALN_HIP_ABI_VERSION 1
+
+enum {
+  DNALN_OK = 0,
+  DNALN_E_INVALID = -1,   /* bad argument or parameter outside the validated range */
+  DNALN_E_CAPACITY = -2,  /* a caller-provided output buffer is too small */
+  DNALN_E_DEVICE = -3,    /* HIP runtime error */
+  DNALN_E_NOMEM = -4,     /* device allocation failed */
+  DNALN_E_INTERNAL = -5,  /* an internal consistency check failed */
+  DNALN_E_PENDING = -6    /* dnaln_hip_batch_poll: still running */
+};
+
+/* ScoreParams (src/align.hpp:34-40). A gap of length L costs gap_open + L * gap_ext. */
+typedef struct {
+  int32_t match, mismatch, gap_open, gap_ext, ambig, bw;
+} dnaln_score_params;
+
+/* ChainParams (src/chain.hpp:30-36). */
+typedef struct {
+  int32_t max_gap, band, lookback, min_score, min_anchors;
+} dnaln_chain_params;
+
+/* MapParams (src/mapper.hpp:25-35); `simd` and `prefetch` have no meaning on the device. */
+typedef struct {
+  int32_t k, w;
+  uint32_t max_occ;
+  dnaln_score_params sc;
+  dnaln_chain_params ch;
+  int32_t max_cand;
+  double pri_ratio;
+  int32_t min_aln_score;
+} dnaln_map_params;
+
+/* Contig (src/reference.hpp:10-14) without the name (names stay on the host). */
+typedef struct {
+  uint64_t offset;
+  uint32_t len;
+  uint32_t reserved;
+} dnaln_contig;
+
+/* MinimizerIndex layout (src/index.hpp:49-56): 16-byte buckets, linear probing, empty key ~0;
+ * pos[] holds (global_pos << 1 | strand) grouped by key. n_buckets is a power of two. */
+typedef struct {
+  uint64_t key;
+  uint32_t offset, count;
+} dnaln_bucket;
+
+typedef struct {
+  const dnaln_bucket* table;
+  uint64_t n_buckets;
+  const uint32_t* pos;
+  uint64_t n_pos;
+  int32_t k, w;
+} dnaln_index_view;
+
+/* AlignTask (src/align.hpp:42-47) as offsets into one host byte pool of codes 0..4. The target
+ * should hold qlen + 2*bw codes; a shorter one is N-padded, as in the reference. */
+typedef struct {
+  uint64_t q_off;
+  uint64_t t_off;
+  int32_t qlen;
+  int32_t tlen;
+} dnaln_task;
+
+/* AlignResult (src/align.hpp:49-60); the CIGAR (M/I/D, BAM encoding len << 4 | op) is stored
+ * at cigar_pool[cigar_off .. cigar_off + cigar_len). */
+typedef struct {
+  int32_t score, qb, qe, tb, te, nm;
+  uint64_t cigar_off;
+  uint32_t cigar_len;
+  uint32_t reserved;
+} dnaln_result;
+
+/* MapRecord (src/mapper.hpp:37-44): contig -1 = unmapped; pos is 0-based within the contig;
+ * the CIGAR includes soft clips. */
+typedef struct {
+  int32_t contig;
+  uint8_t rev, mapq;
+  uint16_t reserved;
+  int64_t pos;
+  int32_t score, nm, n_cand;
+  uint32_t cigar_len;
+  uint64_t cigar_off;
+} dnaln_record;
+
+/* MapStats (src/mapper.hpp:46-49) plus device timings of the last batch. */
+typedef struct {
+  uint64_t reads, mapped, anchors, candidates, cells, rank_tasks, tb_tasks;
+  double ms_seed, ms_chain, ms_rank, ms_traceback, ms_total;
+} dnaln_map_stats;
+
+/* Fills the reference's defaults (k 15, w 10, max_occ 500, 2/4/4/2/1 bw 16, chain
+ * 5000/16/50/20/2, max_cand 10, pri_ratio 0.5, min score 40). */
+void dnaln_hip_default_params(dnaln_map_params* p);
+
+/* Uploads the reference codes (0..4) and the index once; they stay resident in HBM.
+ * Validates the parameters as src/cli.hpp:93-95 and src/index.cpp:11-12 do. */
+int dnaln_hip_create(const dnaln_map_params* params, const uint8_t* ref_codes, uint64_t ref_len,
+                     const dnaln_contig* contigs, int32_t n_contigs, const dnaln_index_view* index, int32_t device,
+                     struct dnaln_hip_ctx** out);
+void dnaln_hip_destroy(struct dnaln_hip_ctx* ctx);
+const char* dnaln_hip_last_error(void);
+
+/* Aligner::align: n independent banded alignments; traceback = 0 computes the score only.
+ * sp may be NULL (use the context's scoring). Results are bit-identical to the reference's
+ * scalar and SIMD kernels. */
+int dnaln_hip_align(struct dnaln_hip_ctx* ctx, const dnaln_score_params* sp, const dnaln_task* tasks, uint64_t n,
+                    const uint8_t* seq_pool, uint64_t pool_len, int traceback, dnaln_result* out,
+                    uint32_t* cigar_pool, uint64_t cigar_cap);
+
+/* Mapper::map on n reads (forward-strand codes 0..4 at read_codes[offs[i] .. offs[i]+lens[i]),
+ * name_fnv1a[i] = FNV-1a of the read name for the equal-best tie-break). Records are identical
+ * to the reference's. stats may be NULL. */
+int dnaln_hip_map_batch(struct dnaln_hip_ctx* ctx, const uint8_t* read_codes, const uint64_t* offs,
+                        const int32_t* lens, const uint32_t* name_fnv1a, uint64_t n, dnaln_record* out,
+                        uint32_t* cigar_pool, uint64_t cigar_cap, dnaln_map_stats* stats);
+
+/* ------------------------------------------------------------------ device-resident batches */
+enum { DNALN_MODE_PLAIN = 0, DNALN_MODE_BATCHED = 1 };
+
+typedef struct {
+  int32_t mode;             /* DNALN_MODE_PLAIN: one launch sequence per chunk of chunk_reads;
+                               DNALN_MODE_BATCHED: length-bucketed persistent kernels, graph-captured */
+  uint32_t chunk_reads;     /* plain form: reads per chunk (65536 by default) */
+  uint64_t max_reads;       /* capacity of the batch */
+  uint64_t max_bases;
+  uint32_t anchors_per_read;/* batched form: anchor capacity per read before the overflow pass */
+  int32_t tune;             /* batched form: 1 = run the autotuner on the first batch */
+} dnaln_batch_config;
+
+void dnaln_hip_default_batch_config(dnaln_batch_config* c);
+
+/* A batch owns device buffers for inputs, workspace and outputs, and a HIP stream. */
+int dnaln_hip_batch_create(struct dnaln_hip_ctx* ctx, const dnaln_batch_config* cfg, struct dnaln_hip_batch** out);
+void dnaln_hip_batch_destroy(struct dnaln_hip_batch* b);
+/* Asynchronous H2D copy of n reads into the batch (host buffers must stay valid until sync). */
+int dnaln_hip_batch_upload(struct dnaln_hip_batch* b, const uint8_t* read_codes, const uint64_t* offs,
+                           const int32_t* lens, const uint32_t* name_fnv1a, uint64_t n);
+/* Runs the pipeline on the resident batch, asynchronously where the mode allows. */
+int dnaln_hip_batch_run(struct dnaln_hip_batch* b);
+/* Waits for the batch (wait = 1) or reports DNALN_E_PENDING (wait = 0). */
+int dnaln_hip_batch_poll(struct dnaln_hip_batch* b, int wait);
+/* Copies records and CIGARs to the host (after poll returned DNALN_OK). */
+int dnaln_hip_batch_download(struct dnaln_hip_batch* b, dnaln_record* out, uint32_t* cigar_pool, uint64_t cigar_cap,
+                             dnaln_map_stats* stats);
+/* Device bytes a batch of this configuration holds (for admission by HBM budget). */
+uint64_t dnaln_hip_batch_device_bytes(struct dnaln_hip_ctx* ctx, const dnaln_batch_config* cfg);
+/* Time of the last run measured with HIP events on the batch stream, per kernel family
+ * (ms): [0] seed, [1] chain, [2] rank fill, [3] traceback fill + walk, [4] whole run. */
+int dnaln_hip_batch_timings(struct dnaln_hip_batch* b, double* ms5);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,74 @@
+"""CPU build of the device lane programs (libdnaln_emu.so) against the reference kernel vectors and
+the oracle: the packed int16 tricks (v_perm scoring, sign-bit flags, saved best row, lock-step
+walk) are exact before they reach a GPU."""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle
+from tests import golden_io
+from tests.golden import sets
+
+LIB = Path(__file__).resolve().parent.parent / "dnaln_amd" / "lib" / "libdnaln_emu.so"
+
+
+@pytest.fixture(scope="module")
+def emu():
+    if not LIB.exists():
+        pytest.fail("libdnaln_emu.so is not built (python -m dnaln_amd.build emu)")
+    L = C.CDLL(str(LIB))
+    L.emu_align.restype = C.c_int
+    return L
+
+
+def _run(emu, sp, pool, qo, ql, to, tl, tb, force):
+    n = len(ql)
+    res = np.zeros((n, 6), np.int32)
+    cap = int(ql.astype(np.int64).sum()) * 2 + 8 * n + 8
+    cig = np.zeros(cap, np.uint32)
+    co = np.zeros(n, np.int64)
+    cn = np.zeros(n, np.int32)
+    v = C.c_void_p
+    rc = emu.emu_align(v(sp.ctypes.data), v(pool.ctypes.data), v(qo.ctypes.data), v(ql.ctypes.data),
+                       v(to.ctypes.data), v(tl.ctypes.data), C.c_size_t(n), tb, v(res.ctypes.data),
+                       v(cig.ctypes.data), v(co.ctypes.data), v(cn.ctypes.data), C.c_size_t(cap), force)
+    assert rc >= 0, rc
+    return rc, res, cig[:int(cn.sum())], cn
+
+
+@pytest.mark.parametrize("si", range(len(sets.SCHEMES)))
+@pytest.mark.parametrize("force", [0, 1])
+def test_emulated_kernels_match_reference_vectors(emu, si, force):
+    kv = golden_io.kernel_vectors()
+    sp = kv[f"s{si}_sp"].astype(np.int32)
+    args = [kv[f"s{si}_{k}"] for k in ("pool", "qoff", "qlen", "toff", "tlen")]
+    for tb in (0, 1):
+        rc, res, cig, cn = _run(emu, sp, *args, tb, force)
+        if force == 0 and int(sp[5]) in (5, 8, 16, 21, 24, 32):
+            assert rc == 1  # the packed kernel ran
+        assert np.array_equal(res, kv[f"s{si}_tb{tb}_res"])
+        assert np.array_equal(cn, kv[f"s{si}_tb{tb}_cn"])
+        assert np.array_equal(cig, kv[f"s{si}_tb{tb}_cig"])
+
+
+def test_emulated_packed_kernel_on_long_and_n_rich_tasks(emu):
+    rng = np.random.default_rng(77)
+    for bw in (16, 32):
+        sc = dict(match=2, mismatch=4, gap_open=4, gap_ext=2, ambig=1, bw=bw)
+        tasks = sets.random_tasks(rng, 130, bw, max_len=1500)
+        for q, t in tasks[:20]:
+            q[rng.random(len(q)) < 0.2] = 4
+        pool, qo, ql, to, tl = sets.pack_tasks(tasks)
+        sp = np.array(list(sc.values()), np.int32)
+        rc, res, cig, cn = _run(emu, sp, pool, qo, ql, to, tl, 1, 2)
+        assert rc == 1
+        ospt = oracle.score_t(**sc)
+        co = np.concatenate([[0], np.cumsum(cn)])
+        for i, (q, t) in enumerate(tasks):
+            exp = oracle.align(ospt, q, t, True)
+            assert tuple(res[i]) == tuple(exp[:6]), i
+            assert cig[co[i]:co[i + 1]].tolist() == exp[6], i
