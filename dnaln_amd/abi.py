@@ -75,7 +75,7 @@ HIP_SYMBOLS = {
     "dnaln_hip_batch_poll": (i32, [vp, i32]),
     "dnaln_hip_batch_download": (i32, [vp, vp, vp, u64, vp]),
     "dnaln_hip_batch_device_bytes": (u64, [vp, vp]),
-    "dnaln_hip_batch_timings": (i32, [vp, vp]),
+    "dnaln_hip_batch_timings": (i32, [vp, vp, vp]),
 }
 
 HOST_SYMBOLS = {

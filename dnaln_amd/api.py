@@ -220,10 +220,12 @@ class Batch:
                "dnaln_hip_batch_download")
         return out, cig, stats
 
-    def timings(self) -> list[float]:
+    def timings(self) -> tuple[list[float], list[int]]:
+        """(ms, launches) per family: seed, chain, rank fill, traceback fill + walk, whole run."""
         ms = (C.c_double * 5)()
-        _check(_lib.hip().dnaln_hip_batch_timings(self.b, ms), "dnaln_hip_batch_timings")
-        return list(ms)
+        ln = (C.c_uint64 * 5)()
+        _check(_lib.hip().dnaln_hip_batch_timings(self.b, ms, ln), "dnaln_hip_batch_timings")
+        return list(ms), list(ln)
 
     def close(self) -> None:
         if self.b:

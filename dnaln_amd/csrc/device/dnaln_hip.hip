@@ -300,7 +300,7 @@ uint64_t sw_dir_bytes_per_task(int W, int qmax) { return (uint64_t)qmax * (uint6
 // reversed CIGARs in cig[n * cap]. Chunks the traceback launches so the direction scratch stays
 // under `dir_budget` bytes.
 int run_sw(const ScoreP& sp, const SwTask* tasks, uint32_t n, int qmax, bool traceback, SwResult* res,
-           uint32_t* cig, int cap, DevBuf& dir, DevBuf& rows, hipStream_t st, uint64_t* cells_out) {
+           uint32_t* cig, int cap, DevBuf& dir, DevBuf& rows, hipStream_t st, uint64_t* launches) {
   if (n == 0) return DNALN_OK;
   const int W = 2 * sp.bw + 1;
   const bool packed = packed_bw_available(sp.bw) && packed_ok(sp, qmax);
@@ -332,6 +332,7 @@ int run_sw(const ScoreP& sp, const SwTask* tasks, uint32_t n, int qmax, bool tra
       const bool ok = traceback ? launch_packed_bw<true>(sp.bw, a, grid, st) : launch_packed_bw<false>(sp.bw, a, grid, st);
       if (!ok) return fail(DNALN_E_INTERNAL, "no packed kernel for this band width");
       DN_HIP(hipGetLastError());
+      if (launches) ++*launches;
     }
   } else {
     SwGenericArgs g{};
@@ -355,9 +356,9 @@ int run_sw(const ScoreP& sp, const SwTask* tasks, uint32_t n, int qmax, bool tra
       g.dir_stride = stride;
       hipLaunchKernelGGL(k_sw_generic, dim3((m + 127) / 128), dim3(128), 0, st, g);
       DN_HIP(hipGetLastError());
+      if (launches) ++*launches;
     }
   }
-  (void)cells_out;
   return DNALN_OK;
 }
 
@@ -389,9 +390,38 @@ struct dnaln_hip_batch {
   DevBuf rec, pool;
   uint64_t pool_used = 0;
   dnaln_map_stats stats{};
-  hipEvent_t ev[6] = {};
+  // Timing: event pairs recorded on the batch stream around each launch group, summed per
+  // family at poll time: 0 seed, 1 chain, 2 rank fill, 3 traceback fill + walk, 4 whole run.
+  struct Span {
+    hipEvent_t a = nullptr, b = nullptr;
+    int family = 0;
+  };
+  std::vector<Span> spans;
+  size_t n_spans = 0;
   double ms[5] = {0, 0, 0, 0, 0};
+  uint64_t launches[5] = {0, 0, 0, 0, 0};
+  hipEvent_t done = nullptr;
   bool pending = false;
+  int last_rc = DNALN_OK;  // status of the last run; poll and download report it
+
+  // Spans may nest (the whole-run span holds the stage spans), so each caller keeps its index.
+  int span_begin(int family, size_t& idx) {
+    if (n_spans == spans.size()) {
+      Span s;
+      DN_HIP(hipEventCreate(&s.a));
+      DN_HIP(hipEventCreate(&s.b));
+      spans.push_back(s);
+    }
+    idx = n_spans++;
+    spans[idx].family = family;
+    DN_HIP(hipEventRecord(spans[idx].a, st));
+    return DNALN_OK;
+  }
+  int span_end(size_t idx) {
+    if (idx >= n_spans) return fail(DNALN_E_INTERNAL, "timing span out of range");
+    DN_HIP(hipEventRecord(spans[idx].b, st));
+    return DNALN_OK;
+  }
 };
 
 namespace {
@@ -426,6 +456,7 @@ int run_chunk(dnaln_hip_batch* b, uint64_t r0, uint32_t n, bool first_chunk) {
   const int max_cand = std::max(p.max_cand, 0);
   const ScoreP sp{p.sc.match, p.sc.mismatch, p.sc.gap_open, p.sc.gap_ext, p.sc.ambig, p.sc.bw};
   const int cap = 2 * b->maxlen + 4;
+  size_t sp_i = 0;
 
   ReadView rv;
   rv.codes = b->codes.as<uint8_t>();
@@ -438,7 +469,6 @@ int run_chunk(dnaln_hip_batch* b, uint64_t r0, uint32_t n, bool first_chunk) {
   g.clen = c->clen.as<uint32_t>();
   g.bw = p.sc.bw;
 
-  if (first_chunk) DN_HIP(hipEventRecord(b->ev[0], st));
   // 1. seeds
   DN_TRY(b->acnt.reserve((uint64_t)n * 4 + 4));
   DN_TRY(b->aoff.reserve((uint64_t)n * 8 + 8));
@@ -462,12 +492,14 @@ int run_chunk(dnaln_hip_batch* b, uint64_t r0, uint32_t n, bool first_chunk) {
     DN_TRY(b->ring_ps.reserve((uint64_t)n * c->w * 4));
   }
   const unsigned sgrid = (n + kSeedBlock - 1) / kSeedBlock;
+  DN_TRY(b->span_begin(0, sp_i));
   if (n) {
     hipLaunchKernelGGL(k_seed_count, dim3(sgrid), dim3(kSeedBlock), use_lds ? lds : 0, st, sa,
                        b->ring_h.as<uint64_t>(), b->ring_ps.as<uint32_t>(), use_lds);
     DN_HIP(hipGetLastError());
   }
   DN_TRY(exclusive_scan(b->scan_tmp, b->acnt.as<uint32_t>(), b->aoff.as<uint64_t>(), n, st));
+  DN_TRY(b->span_end(sp_i));
   uint64_t na = 0;
   DN_TRY(scan_total(st, b->aoff.as<uint64_t>(), b->acnt.as<uint32_t>(), n, na));
   DN_TRY(b->keys.reserve(na * 8 + 8));
@@ -476,11 +508,13 @@ int run_chunk(dnaln_hip_batch* b, uint64_t r0, uint32_t n, bool first_chunk) {
   DN_TRY(b->order.reserve(na * 8 + 8));
   DN_TRY(b->chains.reserve(na * sizeof(ChainRec) + sizeof(ChainRec)));
   sa.keys = b->keys.as<uint64_t>();
+  DN_TRY(b->span_begin(0, sp_i));
   if (n) {
     hipLaunchKernelGGL(k_seed_emit, dim3(sgrid), dim3(kSeedBlock), use_lds ? lds : 0, st, sa,
                        b->ring_h.as<uint64_t>(), b->ring_ps.as<uint32_t>(), use_lds);
     DN_HIP(hipGetLastError());
   }
+  DN_TRY(b->span_end(sp_i));
   b->stats.anchors += na;
 
   // 2. chains and candidates
@@ -509,14 +543,14 @@ int run_chunk(dnaln_hip_batch* b, uint64_t r0, uint32_t n, bool first_chunk) {
   ca.n_contigs = c->nc;
   ca.cands = b->cands.as<Cand>();
   ca.n_cand = b->ncand.as<int32_t>();
-  if (first_chunk) DN_HIP(hipEventRecord(b->ev[1], st));
+  DN_TRY(b->span_begin(1, sp_i));
   if (n) {
     hipLaunchKernelGGL(k_chain, dim3(sgrid), dim3(kSeedBlock), 0, st, ca);
     DN_HIP(hipGetLastError());
   }
+  DN_TRY(b->span_end(sp_i));
 
   // 3. rank pass
-  if (first_chunk) DN_HIP(hipEventRecord(b->ev[2], st));
   DN_TRY(b->rcnt.reserve((uint64_t)n * 4 + 4));
   DN_TRY(b->roff.reserve((uint64_t)n * 8 + 8));
   if (n) hipLaunchKernelGGL(k_rank_count, dim3(sgrid), dim3(kSeedBlock), 0, st, b->ncand.as<int32_t>(), b->rcnt.as<uint32_t>(), n);
@@ -528,12 +562,13 @@ int run_chunk(dnaln_hip_batch* b, uint64_t r0, uint32_t n, bool first_chunk) {
   if (n)
     hipLaunchKernelGGL(k_rank_tasks, dim3(sgrid), dim3(kSeedBlock), 0, st, rv, g, b->cands.as<Cand>(),
                        b->ncand.as<int32_t>(), max_cand, b->roff.as<uint64_t>(), b->rtasks.as<SwTask>(), n);
+  DN_TRY(b->span_begin(2, sp_i));
   DN_TRY(run_sw(sp, b->rtasks.as<SwTask>(), (uint32_t)nr, b->maxlen, false, b->rres.as<SwResult>(), nullptr, 0,
-                b->dir, b->rows, st, nullptr));
+                b->dir, b->rows, st, &b->launches[2]));
+  DN_TRY(b->span_end(sp_i));
   b->stats.rank_tasks += nr;
 
   // 4. select + traceback pass
-  if (first_chunk) DN_HIP(hipEventRecord(b->ev[3], st));
   DN_TRY(b->chosen.reserve((uint64_t)n * 4 + 4));
   DN_TRY(b->second.reserve((uint64_t)n * 4 + 4));
   DN_TRY(b->tbflag.reserve((uint64_t)n * 4 + 4));
@@ -551,12 +586,13 @@ int run_chunk(dnaln_hip_batch* b, uint64_t r0, uint32_t n, bool first_chunk) {
   if (n)
     hipLaunchKernelGGL(k_tb_tasks, dim3(sgrid), dim3(kSeedBlock), 0, st, rv, g, b->cands.as<Cand>(), max_cand,
                        b->chosen.as<int32_t>(), b->tboff.as<uint64_t>(), b->ttasks.as<SwTask>(), n);
+  DN_TRY(b->span_begin(3, sp_i));
   DN_TRY(run_sw(sp, b->ttasks.as<SwTask>(), (uint32_t)nt, b->maxlen, true, b->tres.as<SwResult>(),
-                b->tcig.as<uint32_t>(), cap, b->dir, b->rows, st, nullptr));
+                b->tcig.as<uint32_t>(), cap, b->dir, b->rows, st, &b->launches[3]));
+  DN_TRY(b->span_end(sp_i));
   b->stats.tb_tasks += nt;
 
   // 5. records and CIGARs
-  if (first_chunk) DN_HIP(hipEventRecord(b->ev[4], st));
   DN_TRY(b->ciglen.reserve((uint64_t)n * 4 + 4));
   DN_TRY(b->cigoff.reserve((uint64_t)n * 8 + 8));
   dnaln_record* rec = b->rec.as<dnaln_record>() + r0;
@@ -757,7 +793,7 @@ int dnaln_hip_batch_create(dnaln_hip_ctx* c, const dnaln_batch_config* cfg, dnal
   if (b->cfg.chunk_reads == 0) b->cfg.chunk_reads = 65536;
   DN_HIP(hipSetDevice(c->device));
   DN_HIP(hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking));
-  for (auto& e : b->ev) DN_HIP(hipEventCreate(&e));
+  DN_HIP(hipEventCreate(&b->done));
   DN_TRY(b->err.reserve(64));
   if (b->cfg.max_reads) {
     DN_TRY(b->offs.reserve(b->cfg.max_reads * 8));
@@ -774,8 +810,11 @@ void dnaln_hip_batch_destroy(dnaln_hip_batch* b) {
   if (!b) return;
   (void)hipSetDevice(b->ctx->device);
   if (b->st) (void)hipStreamSynchronize(b->st);
-  for (auto& e : b->ev)
-    if (e) (void)hipEventDestroy(e);
+  for (auto& sp : b->spans) {
+    (void)hipEventDestroy(sp.a);
+    (void)hipEventDestroy(sp.b);
+  }
+  if (b->done) (void)hipEventDestroy(b->done);
   if (b->st) (void)hipStreamDestroy(b->st);
   delete b;
 }
@@ -785,6 +824,7 @@ int dnaln_hip_batch_upload(dnaln_hip_batch* b, const uint8_t* codes, const uint6
   if (!b || (n && (!codes || !offs || !lens || !fnv))) return fail(DNALN_E_INVALID, "null argument");
   if (n >= (1ull << 31)) return fail(DNALN_E_INVALID, "too many reads in one batch");
   DN_HIP(hipSetDevice(b->ctx->device));
+  b->last_rc = DNALN_OK;
   uint64_t bases = 0;
   int maxlen = 0;
   for (uint64_t i = 0; i < n; ++i) {
@@ -816,14 +856,26 @@ int dnaln_hip_batch_run(dnaln_hip_batch* b) {
   DN_HIP(hipSetDevice(c->device));
   std::memset(&b->stats, 0, sizeof b->stats);
   b->pool_used = 0;
+  b->n_spans = 0;
+  for (auto& l : b->launches) l = 0;
   DN_HIP(hipMemsetAsync(b->err.p, 0, 4, b->st));
+  size_t run_span = 0;
+  DN_TRY(b->span_begin(4, run_span));
   const uint64_t chunk = b->cfg.mode == DNALN_MODE_PLAIN ? b->cfg.chunk_reads : std::max<uint64_t>(b->n, 1);
+  b->last_rc = DNALN_OK;
   for (uint64_t r0 = 0; r0 < b->n || (r0 == 0 && b->n == 0); r0 += chunk) {
     const uint32_t m = (uint32_t)std::min<uint64_t>(chunk, b->n - r0);
-    DN_TRY(run_chunk(b, r0, m, r0 == 0));
+    const int rc = run_chunk(b, r0, m, r0 == 0);
+    if (rc != DNALN_OK) {
+      (void)hipStreamSynchronize(b->st);
+      b->last_rc = rc;
+      b->pending = false;
+      return rc;
+    }
     if (b->n == 0) break;
   }
-  DN_HIP(hipEventRecord(b->ev[5], b->st));
+  DN_TRY(b->span_end(run_span));
+  DN_HIP(hipEventRecord(b->done, b->st));
   b->stats.reads = b->n;
   b->pending = true;
   return DNALN_OK;
@@ -831,24 +883,27 @@ int dnaln_hip_batch_run(dnaln_hip_batch* b) {
 
 int dnaln_hip_batch_poll(dnaln_hip_batch* b, int wait) {
   if (!b) return fail(DNALN_E_INVALID, "null batch");
+  if (b->last_rc != DNALN_OK) return b->last_rc;
   if (!b->pending) return DNALN_OK;
   if (!wait) {
-    const hipError_t q = hipEventQuery(b->ev[5]);
+    const hipError_t q = hipEventQuery(b->done);
     if (q == hipErrorNotReady) return DNALN_E_PENDING;
     if (q != hipSuccess) return fail(DNALN_E_DEVICE, hipGetErrorString(q));
   }
-  DN_HIP(hipEventSynchronize(b->ev[5]));
+  DN_HIP(hipEventSynchronize(b->done));
   int err = 0;
   DN_HIP(hipMemcpy(&err, b->err.p, 4, hipMemcpyDeviceToHost));
   b->pending = false;
-  float t[5] = {0, 0, 0, 0, 0};
-  DN_HIP(hipEventElapsedTime(&t[0], b->ev[0], b->ev[1]));
-  DN_HIP(hipEventElapsedTime(&t[1], b->ev[1], b->ev[2]));
-  DN_HIP(hipEventElapsedTime(&t[2], b->ev[2], b->ev[3]));
-  DN_HIP(hipEventElapsedTime(&t[3], b->ev[3], b->ev[4]));
-  DN_HIP(hipEventElapsedTime(&t[4], b->ev[0], b->ev[5]));
-  for (int i = 0; i < 5; ++i) b->ms[i] = t[i];
-  if (err) return fail(DNALN_E_INTERNAL, err & 1 ? "CIGAR slot overflow" : "inconsistent traceback");
+  for (auto& m : b->ms) m = 0;
+  for (size_t i = 0; i < b->n_spans; ++i) {
+    float t = 0;
+    DN_HIP(hipEventElapsedTime(&t, b->spans[i].a, b->spans[i].b));
+    b->ms[b->spans[i].family] += t;
+  }
+  if (err) {
+    b->last_rc = DNALN_E_INTERNAL;
+    return fail(DNALN_E_INTERNAL, err & 1 ? "CIGAR slot overflow" : "inconsistent traceback");
+  }
   return DNALN_OK;
 }
 
@@ -886,9 +941,11 @@ uint64_t dnaln_hip_batch_device_bytes(dnaln_hip_ctx* c, const dnaln_batch_config
   return bases + n * per_read + n * (uint64_t)cfg->anchors_per_read * 48;
 }
 
-int dnaln_hip_batch_timings(dnaln_hip_batch* b, double* ms5) {
+int dnaln_hip_batch_timings(dnaln_hip_batch* b, double* ms5, uint64_t* launches5) {
   if (!b || !ms5) return fail(DNALN_E_INVALID, "null argument");
   for (int i = 0; i < 5; ++i) ms5[i] = b->ms[i];
+  if (launches5)
+    for (int i = 0; i < 5; ++i) launches5[i] = b->launches[i];
   return DNALN_OK;
 }
 

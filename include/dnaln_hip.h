@@ -175,8 +175,9 @@ int dnaln_hip_batch_download(struct dnaln_hip_batch* b, dnaln_record* out, uint3
 /* Device bytes a batch of this configuration holds (for admission by HBM budget). */
 uint64_t dnaln_hip_batch_device_bytes(struct dnaln_hip_ctx* ctx, const dnaln_batch_config* cfg);
 /* Time of the last run measured with HIP events on the batch stream, per kernel family
- * (ms): [0] seed, [1] chain, [2] rank fill, [3] traceback fill + walk, [4] whole run. */
-int dnaln_hip_batch_timings(struct dnaln_hip_batch* b, double* ms5);
+ * (ms): [0] seed, [1] chain, [2] rank fill, [3] traceback fill + walk, [4] whole run.
+ * launches5 (may be NULL) receives the number of fill kernel launches per family. */
+int dnaln_hip_batch_timings(struct dnaln_hip_batch* b, double* ms5, uint64_t* launches5);
 
 #ifdef __cplusplus
 }
