@@ -22,6 +22,7 @@
 #include "seed_chain.hpp"
 #include "sw_generic.hpp"
 #include "sw_packed.hpp"
+#include "fused_seed_chain.hpp"
 
 using namespace dn;
 
@@ -77,6 +78,10 @@ struct DevBuf {
 };
 
 constexpr int kSeedBlock = 128;
+constexpr int kFusedBlock = 64;  // one wave per block: LDS is the occupancy limit
+// Guard bytes before and after read and task pools (the packed kernel loads aligned dwords
+// that may reach up to 4 bytes past a sequence).
+constexpr uint64_t kReadGuard = 64;
 constexpr int kSwBlock = 256;  // 4 waves, 512 alignments per block
 
 // ------------------------------------------------------------------------------------ kernels
@@ -126,6 +131,37 @@ __global__ __launch_bounds__(kSeedBlock) void k_chain(ChainArgs a) {
   const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= a.n_reads) return;
   chain_read(a, r);
+}
+
+
+// One read per thread; the read's ring, anchors, f and prev sit in LDS, interleaved by thread.
+template <int CAP>
+__global__ __launch_bounds__(kFusedBlock) void k_seed_chain_fused(SeedArgs s, ChainArgs c, uint32_t* ovf,
+                                                                  uint32_t* novf) {
+  extern __shared__ uint64_t lds[];
+  const int t = threadIdx.x;
+  const uint32_t i = blockIdx.x * kFusedBlock + t;
+  if (i >= s.n_reads) return;
+  uint8_t* base = reinterpret_cast<uint8_t*>(lds);
+  const int w = s.w;
+  Ring ring;
+  ring.h = reinterpret_cast<uint64_t*>(base) + t;
+  ring.ps = reinterpret_cast<uint32_t*>(base + (size_t)w * kFusedBlock * 8) + t;
+  ring.stride = kFusedBlock;
+  uint8_t* p = base + (size_t)w * kFusedBlock * 12;
+  LdsView v;
+  v.keys = reinterpret_cast<uint64_t*>(p) + t;
+  v.f = reinterpret_cast<int16_t*>(p + (size_t)CAP * kFusedBlock * 8) + t;
+  v.prev = reinterpret_cast<int8_t*>(p + (size_t)CAP * kFusedBlock * 10) + t;
+  v.stride = kFusedBlock;
+  const uint32_t r = read_id(s.list, i);
+  const int nc = seed_chain_lds<8>(s, c, r, ring, v, CAP);
+  if (nc < 0) {
+    ovf[atomicAdd(novf, 1u)] = r;
+    c.n_cand[r] = 0;
+  } else {
+    c.n_cand[r] = nc;
+  }
 }
 
 struct ReadView {
@@ -217,6 +253,13 @@ __global__ __launch_bounds__(kSwBlock) void k_sw_packed(SwArgs a) {
   sw_pair<W, TB>(a, ida, idb, va, vb, ql, slot, lane);
 }
 
+template <int W>
+__global__ __launch_bounds__(256) void k_walk(SwArgs a) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= a.n) return;
+  walk_task<W>(a, k);
+}
+
 __global__ void k_sw_generic(SwGenericArgs a) {
   const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
   if (tid >= a.n) return;
@@ -253,6 +296,20 @@ bool launch_packed_bw(int bw, const SwArgs& a, unsigned grid, hipStream_t st) {
   }
 }
 
+bool launch_walk_bw(int bw, const SwArgs& a, hipStream_t st) {
+  const unsigned grid = (a.n + 255) / 256;
+  switch (bw) {
+#define DN_CASE(B)                                                                 \
+  case B:                                                                          \
+    hipLaunchKernelGGL((k_walk<2 * B + 1>), dim3(grid), dim3(256), 0, st, a);      \
+    return true;
+    DN_PACKED_BW_LIST(DN_CASE)
+#undef DN_CASE
+    default:
+      return false;
+  }
+}
+
 bool packed_bw_available(int bw) {
   switch (bw) {
 #define DN_CASE(B) case B:
@@ -272,6 +329,7 @@ struct dnaln_hip_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   DevBuf ref, coff, clen, table, pos, cost, lut;
+  uint8_t* ref_base = nullptr;  // reference codes, kSeqGuard bytes into `ref`
   std::vector<uint64_t> h_coff;
   std::vector<uint32_t> h_clen;
   uint64_t ref_len = 0, mask = 0, n_pos = 0;
@@ -333,6 +391,10 @@ int run_sw(const ScoreP& sp, const SwTask* tasks, uint32_t n, int qmax, bool tra
       if (!ok) return fail(DNALN_E_INTERNAL, "no packed kernel for this band width");
       DN_HIP(hipGetLastError());
       if (launches) ++*launches;
+      if (traceback) {
+        if (!launch_walk_bw(sp.bw, a, st)) return fail(DNALN_E_INTERNAL, "no walk kernel for this band width");
+        DN_HIP(hipGetLastError());
+      }
     }
   } else {
     SwGenericArgs g{};
@@ -385,7 +447,7 @@ struct dnaln_hip_batch {
   // workspace
   DevBuf acnt, aoff, keys, f, prev, order, chains, ring_h, ring_ps;
   DevBuf cands, ncand, rcnt, roff, rtasks, rres, chosen, second, tbflag, tboff, ttasks, tres, tcig;
-  DevBuf dir, rows, scan_tmp, ciglen, cigoff, err, scratch8;
+  DevBuf dir, rows, scan_tmp, ciglen, cigoff, err, scratch8, ovf;
   // outputs
   DevBuf rec, pool;
   uint64_t pool_used = 0;
@@ -448,6 +510,88 @@ int scan_total(hipStream_t st, const uint64_t* off, const T* cnt, uint32_t n, ui
   return DNALN_OK;
 }
 
+// Global-memory seed + chain path for reads list[0, nl) (list null: reads 0 .. nl-1): count,
+// scan, emit anchors, chain. No limit on anchors per read.
+int seed_chain_global(dnaln_hip_batch* b, SeedArgs sa, ChainArgs ca, const uint32_t* list, uint32_t nl,
+                      size_t& sp_i) {
+  dnaln_hip_ctx* c = b->ctx;
+  hipStream_t st = b->st;
+  DN_TRY(b->acnt.reserve((uint64_t)nl * 4 + 4));
+  DN_TRY(b->aoff.reserve((uint64_t)nl * 8 + 8));
+  sa.n_reads = nl;
+  sa.list = list;
+  sa.anchor_count = b->acnt.as<uint32_t>();
+  sa.anchor_off = b->aoff.as<uint64_t>();
+  const size_t lds = (size_t)c->w * kSeedBlock * 12;
+  const int use_lds = lds <= 48 * 1024;
+  if (!use_lds) {
+    DN_TRY(b->ring_h.reserve((uint64_t)nl * c->w * 8));
+    DN_TRY(b->ring_ps.reserve((uint64_t)nl * c->w * 4));
+  }
+  const unsigned sgrid = (nl + kSeedBlock - 1) / kSeedBlock;
+  DN_TRY(b->span_begin(0, sp_i));
+  if (nl) {
+    hipLaunchKernelGGL(k_seed_count, dim3(sgrid), dim3(kSeedBlock), use_lds ? lds : 0, st, sa,
+                       b->ring_h.as<uint64_t>(), b->ring_ps.as<uint32_t>(), use_lds);
+    DN_HIP(hipGetLastError());
+  }
+  DN_TRY(exclusive_scan(b->scan_tmp, b->acnt.as<uint32_t>(), b->aoff.as<uint64_t>(), nl, st));
+  DN_TRY(b->span_end(sp_i));
+  uint64_t na = 0;
+  DN_TRY(scan_total(st, b->aoff.as<uint64_t>(), b->acnt.as<uint32_t>(), nl, na));
+  DN_TRY(b->keys.reserve(na * 8 + 8));
+  DN_TRY(b->f.reserve(na * 4 + 4));
+  DN_TRY(b->prev.reserve(na * 4 + 4));
+  DN_TRY(b->order.reserve(na * 8 + 8));
+  DN_TRY(b->chains.reserve(na * sizeof(ChainRec) + sizeof(ChainRec)));
+  sa.keys = b->keys.as<uint64_t>();
+  DN_TRY(b->span_begin(0, sp_i));
+  if (nl) {
+    hipLaunchKernelGGL(k_seed_emit, dim3(sgrid), dim3(kSeedBlock), use_lds ? lds : 0, st, sa,
+                       b->ring_h.as<uint64_t>(), b->ring_ps.as<uint32_t>(), use_lds);
+    DN_HIP(hipGetLastError());
+  }
+  DN_TRY(b->span_end(sp_i));
+  b->stats.anchors += na;
+  ca.n_reads = nl;
+  ca.list = list;
+  ca.anchor_count = b->acnt.as<uint32_t>();
+  ca.anchor_off = b->aoff.as<uint64_t>();
+  ca.keys = b->keys.as<uint64_t>();
+  ca.f = b->f.as<int32_t>();
+  ca.prev = b->prev.as<int32_t>();
+  ca.order = b->order.as<uint64_t>();
+  ca.chains = b->chains.as<ChainRec>();
+  DN_TRY(b->span_begin(1, sp_i));
+  if (nl) {
+    hipLaunchKernelGGL(k_chain, dim3(sgrid), dim3(kSeedBlock), 0, st, ca);
+    DN_HIP(hipGetLastError());
+  }
+  DN_TRY(b->span_end(sp_i));
+  return DNALN_OK;
+}
+
+// One read from bases to candidates in LDS (fused_seed_chain.hpp). CAP anchors per read.
+size_t fused_lds_bytes(int cap, int w) { return (size_t)kFusedBlock * ((size_t)12 * w + 11 * (size_t)cap); }
+
+// LDS anchor capacity for this batch, 0 when the fused path does not apply.
+int fused_capacity(int w, int maxlen) {
+  const int cap = maxlen <= 200 ? 32 : 64;
+  return fused_lds_bytes(cap, w) <= 64 * 1024 ? cap : 0;
+}
+
+int launch_fused(int cap, const SeedArgs& sa, const ChainArgs& ca, uint32_t* ovf, uint32_t* novf, int w,
+                 hipStream_t st) {
+  const unsigned grid = (sa.n_reads + kFusedBlock - 1) / kFusedBlock;
+  const size_t lds = fused_lds_bytes(cap, w);
+  if (cap == 32)
+    hipLaunchKernelGGL((k_seed_chain_fused<32>), dim3(grid), dim3(kFusedBlock), lds, st, sa, ca, ovf, novf);
+  else
+    hipLaunchKernelGGL((k_seed_chain_fused<64>), dim3(grid), dim3(kFusedBlock), lds, st, sa, ca, ovf, novf);
+  DN_HIP(hipGetLastError());
+  return DNALN_OK;
+}
+
 // One chunk of reads [r0, r0 + n) through the whole pipeline (plain form).
 int run_chunk(dnaln_hip_batch* b, uint64_t r0, uint32_t n, bool first_chunk) {
   dnaln_hip_ctx* c = b->ctx;
@@ -459,76 +603,30 @@ int run_chunk(dnaln_hip_batch* b, uint64_t r0, uint32_t n, bool first_chunk) {
   size_t sp_i = 0;
 
   ReadView rv;
-  rv.codes = b->codes.as<uint8_t>();
+  rv.codes = b->codes.as<uint8_t>() + kReadGuard;
   rv.offs = b->offs.as<uint64_t>() + r0;
   rv.lens = b->lens.as<int32_t>() + r0;
   rv.fnv = b->fnv.as<uint32_t>() + r0;
   MapGeom g;
-  g.ref = (uint64_t)(uintptr_t)c->ref.p;
+  g.ref = (uint64_t)(uintptr_t)c->ref_base;
   g.coff = c->coff.as<uint64_t>();
   g.clen = c->clen.as<uint32_t>();
   g.bw = p.sc.bw;
 
-  // 1. seeds
-  DN_TRY(b->acnt.reserve((uint64_t)n * 4 + 4));
-  DN_TRY(b->aoff.reserve((uint64_t)n * 8 + 8));
+  // 1-2. seeds, chains and candidates
+  DN_TRY(b->cands.reserve((uint64_t)n * std::max(max_cand, 1) * sizeof(Cand)));
+  DN_TRY(b->ncand.reserve((uint64_t)n * 4 + 4));
   SeedArgs sa{};
-  sa.codes = rv.codes;
+  sa.codes = rv.codes;  // kReadGuard bytes into the buffer
   sa.offs = rv.offs;
   sa.lens = rv.lens;
-  sa.n_reads = n;
   sa.table = c->table.as<Bucket>();
   sa.mask = c->mask;
   sa.pos = c->pos.as<uint32_t>();
   sa.k = c->k;
   sa.w = c->w;
   sa.max_occ = p.max_occ;
-  sa.anchor_count = b->acnt.as<uint32_t>();
-  sa.anchor_off = b->aoff.as<uint64_t>();
-  const size_t lds = (size_t)c->w * kSeedBlock * 12;
-  const int use_lds = lds <= 48 * 1024;
-  if (!use_lds) {
-    DN_TRY(b->ring_h.reserve((uint64_t)n * c->w * 8));
-    DN_TRY(b->ring_ps.reserve((uint64_t)n * c->w * 4));
-  }
-  const unsigned sgrid = (n + kSeedBlock - 1) / kSeedBlock;
-  DN_TRY(b->span_begin(0, sp_i));
-  if (n) {
-    hipLaunchKernelGGL(k_seed_count, dim3(sgrid), dim3(kSeedBlock), use_lds ? lds : 0, st, sa,
-                       b->ring_h.as<uint64_t>(), b->ring_ps.as<uint32_t>(), use_lds);
-    DN_HIP(hipGetLastError());
-  }
-  DN_TRY(exclusive_scan(b->scan_tmp, b->acnt.as<uint32_t>(), b->aoff.as<uint64_t>(), n, st));
-  DN_TRY(b->span_end(sp_i));
-  uint64_t na = 0;
-  DN_TRY(scan_total(st, b->aoff.as<uint64_t>(), b->acnt.as<uint32_t>(), n, na));
-  DN_TRY(b->keys.reserve(na * 8 + 8));
-  DN_TRY(b->f.reserve(na * 4 + 4));
-  DN_TRY(b->prev.reserve(na * 4 + 4));
-  DN_TRY(b->order.reserve(na * 8 + 8));
-  DN_TRY(b->chains.reserve(na * sizeof(ChainRec) + sizeof(ChainRec)));
-  sa.keys = b->keys.as<uint64_t>();
-  DN_TRY(b->span_begin(0, sp_i));
-  if (n) {
-    hipLaunchKernelGGL(k_seed_emit, dim3(sgrid), dim3(kSeedBlock), use_lds ? lds : 0, st, sa,
-                       b->ring_h.as<uint64_t>(), b->ring_ps.as<uint32_t>(), use_lds);
-    DN_HIP(hipGetLastError());
-  }
-  DN_TRY(b->span_end(sp_i));
-  b->stats.anchors += na;
-
-  // 2. chains and candidates
-  DN_TRY(b->cands.reserve((uint64_t)n * std::max(max_cand, 1) * sizeof(Cand)));
-  DN_TRY(b->ncand.reserve((uint64_t)n * 4 + 4));
   ChainArgs ca{};
-  ca.n_reads = n;
-  ca.anchor_count = b->acnt.as<uint32_t>();
-  ca.anchor_off = b->aoff.as<uint64_t>();
-  ca.keys = b->keys.as<uint64_t>();
-  ca.f = b->f.as<int32_t>();
-  ca.prev = b->prev.as<int32_t>();
-  ca.order = b->order.as<uint64_t>();
-  ca.chains = b->chains.as<ChainRec>();
   ca.cost = c->cost.as<int32_t>();
   ca.k = c->k;
   ca.max_gap = p.ch.max_gap;
@@ -543,12 +641,26 @@ int run_chunk(dnaln_hip_batch* b, uint64_t r0, uint32_t n, bool first_chunk) {
   ca.n_contigs = c->nc;
   ca.cands = b->cands.as<Cand>();
   ca.n_cand = b->ncand.as<int32_t>();
-  DN_TRY(b->span_begin(1, sp_i));
-  if (n) {
-    hipLaunchKernelGGL(k_chain, dim3(sgrid), dim3(kSeedBlock), 0, st, ca);
-    DN_HIP(hipGetLastError());
+  const unsigned sgrid = (n + kSeedBlock - 1) / kSeedBlock;
+  const int fcap = b->cfg.mode == DNALN_MODE_BATCHED ? fused_capacity(c->w, b->maxlen) : 0;
+  if (fcap && n) {
+    // Batched form: every read from bases to candidates in LDS; reads that do not fit are listed
+    // and take the global-memory path below.
+    DN_TRY(b->ovf.reserve((uint64_t)n * 4 + 8));
+    DN_HIP(hipMemsetAsync(b->ovf.as<uint32_t>() + n, 0, 4, st));
+    sa.n_reads = n;
+    sa.list = nullptr;
+    DN_TRY(b->span_begin(0, sp_i));
+    DN_TRY(launch_fused(fcap, sa, ca, b->ovf.as<uint32_t>(), b->ovf.as<uint32_t>() + n, c->w, st));
+    DN_TRY(b->span_end(sp_i));
+    uint32_t novf = 0;
+    DN_HIP(hipMemcpyAsync(&novf, b->ovf.as<uint32_t>() + n, 4, hipMemcpyDeviceToHost, st));
+    DN_HIP(hipStreamSynchronize(st));
+    b->stats.global_path_reads += novf;
+    if (novf) DN_TRY(seed_chain_global(b, sa, ca, b->ovf.as<uint32_t>(), novf, sp_i));
+  } else {
+    DN_TRY(seed_chain_global(b, sa, ca, nullptr, n, sp_i));
   }
-  DN_TRY(b->span_end(sp_i));
 
   // 3. rank pass
   DN_TRY(b->rcnt.reserve((uint64_t)n * 4 + 4));
@@ -661,8 +773,12 @@ int dnaln_hip_create(const dnaln_map_params* params, const uint8_t* ref_codes, u
   c->nc = n_contigs;
   c->k = params->k;
   c->w = params->w;
-  DN_TRY(c->ref.reserve(ref_len + 1));
-  if (ref_len) DN_HIP(hipMemcpy(c->ref.p, ref_codes, ref_len, hipMemcpyHostToDevice));
+  // The packed kernel streams aligned dwords and may touch bytes just outside a window: the
+  // reference sits between two guard regions of kSeqGuard bytes (never used as sequence).
+  DN_TRY(c->ref.reserve(ref_len + 2 * kSeqGuard));
+  DN_HIP(hipMemset(c->ref.p, 4, ref_len + 2 * kSeqGuard));
+  c->ref_base = c->ref.as<uint8_t>() + kSeqGuard;
+  if (ref_len) DN_HIP(hipMemcpy(c->ref_base, ref_codes, ref_len, hipMemcpyHostToDevice));
   c->h_coff.resize(n_contigs);
   c->h_clen.resize(n_contigs);
   for (int i = 0; i < n_contigs; ++i) {
@@ -730,15 +846,16 @@ int dnaln_hip_align(dnaln_hip_ctx* c, const dnaln_score_params* spp, const dnaln
   if (n == 0) return DNALN_OK;
   const int cap = 2 * qmax + 4;
   hipStream_t st = c->stream;
-  DN_TRY(c->a_pool.reserve(pool_len + 1));
+  DN_TRY(c->a_pool.reserve(pool_len + 2 * kReadGuard));
   DN_TRY(c->a_in.reserve(n * sizeof(dnaln_task)));
   DN_TRY(c->a_tasks.reserve(n * sizeof(SwTask)));
   DN_TRY(c->a_res.reserve(n * sizeof(SwResult)));
   if (traceback) DN_TRY(c->a_cig.reserve(n * (uint64_t)cap * 4));
-  DN_HIP(hipMemcpyAsync(c->a_pool.p, seq_pool, pool_len, hipMemcpyHostToDevice, st));
+  DN_HIP(hipMemcpyAsync(c->a_pool.as<uint8_t>() + kReadGuard, seq_pool, pool_len, hipMemcpyHostToDevice, st));
   DN_HIP(hipMemcpyAsync(c->a_in.p, tasks, n * sizeof(dnaln_task), hipMemcpyHostToDevice, st));
   hipLaunchKernelGGL(k_pool_tasks, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, c->a_in.as<dnaln_task>(),
-                     (uint64_t)(uintptr_t)c->a_pool.p, sp.bw, c->a_tasks.as<SwTask>(), (uint32_t)n);
+                     (uint64_t)(uintptr_t)(c->a_pool.as<uint8_t>() + kReadGuard), sp.bw, c->a_tasks.as<SwTask>(),
+                     (uint32_t)n);
   DN_HIP(hipGetLastError());
   DN_TRY(run_sw(sp, c->a_tasks.as<SwTask>(), (uint32_t)n, qmax, traceback != 0, c->a_res.as<SwResult>(),
                 c->a_cig.as<uint32_t>(), cap, c->a_dir, c->a_rows, st, nullptr));
@@ -833,12 +950,12 @@ int dnaln_hip_batch_upload(dnaln_hip_batch* b, const uint8_t* codes, const uint6
     maxlen = std::max(maxlen, lens[i]);
   }
   DN_HIP(hipStreamSynchronize(b->st));
-  DN_TRY(b->codes.reserve(bases + 1));
+  DN_TRY(b->codes.reserve(bases + 2 * kReadGuard));
   DN_TRY(b->offs.reserve(n * 8 + 8));
   DN_TRY(b->lens.reserve(n * 4 + 4));
   DN_TRY(b->fnv.reserve(n * 4 + 4));
   DN_TRY(b->rec.reserve(n * sizeof(dnaln_record) + sizeof(dnaln_record)));
-  if (bases) DN_HIP(hipMemcpyAsync(b->codes.p, codes, bases, hipMemcpyHostToDevice, b->st));
+  if (bases) DN_HIP(hipMemcpyAsync(b->codes.as<uint8_t>() + kReadGuard, codes, bases, hipMemcpyHostToDevice, b->st));
   if (n) {
     DN_HIP(hipMemcpyAsync(b->offs.p, offs, n * 8, hipMemcpyHostToDevice, b->st));
     DN_HIP(hipMemcpyAsync(b->lens.p, lens, n * 4, hipMemcpyHostToDevice, b->st));
@@ -919,7 +1036,12 @@ int dnaln_hip_batch_download(dnaln_hip_batch* b, dnaln_record* out, uint32_t* ci
     dnaln_map_stats s = b->stats;
     uint64_t mapped = 0, cells = 0;
     const uint64_t W = 2 * (uint64_t)b->ctx->p.sc.bw + 1;
-    for (uint64_t i = 0; i < b->n; ++i) mapped += out[i].contig >= 0;
+    uint64_t cand = 0;
+    for (uint64_t i = 0; i < b->n; ++i) {
+      mapped += out[i].contig >= 0;
+      cand += (uint64_t)out[i].n_cand;
+    }
+    s.candidates = cand;
     (void)cells;
     s.mapped = mapped;
     s.ms_seed = b->ms[0];

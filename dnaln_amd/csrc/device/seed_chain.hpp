@@ -28,16 +28,19 @@ struct SeedArgs {
   const uint8_t* codes;     // reads, forward codes 0..4
   const uint64_t* offs;
   const int32_t* lens;
-  uint32_t n_reads;
+  uint32_t n_reads;         // entries to process (of `list` when set)
+  const uint32_t* list;     // read ids to process (null: 0 .. n_reads-1)
   const Bucket* table;
   uint64_t mask;
   const uint32_t* pos;
   int k, w;
   uint32_t max_occ;
-  uint32_t* anchor_count;   // seed_count output
+  uint32_t* anchor_count;   // seed_count output, by list position
   const uint64_t* anchor_off;   // exclusive scan of anchor_count
   uint64_t* keys;           // seed_emit output
 };
+
+DN_HD uint32_t read_id(const uint32_t* list, uint32_t i) { return list ? list[i] : i; }
 
 DN_HD uint64_t hash64(uint64_t key, uint64_t mask) {  // src/minimizer.hpp:21-30
   key = (~key + (key << 21)) & mask;
@@ -74,9 +77,47 @@ struct Ring {
   int stride;
 };
 
+// Reads a read's codes front to back in 16-byte aligned chunks, one chunk of look-ahead, so a
+// 150-bp read costs ~11 independent vector loads instead of 150 dependent byte loads. It touches
+// up to 15 bytes before and 31 bytes after the read: the buffer carries guard bytes.
+struct ChunkReader {
+  const uint32_t* base;  // 16-byte aligned chunk holding byte 0
+  uint32_t off;          // offset of byte 0 in that chunk
+  int cur;
+  uint32_t c0, c1, c2, c3, n0, n1, n2, n3;
+  DN_HD explicit ChunkReader(const uint8_t* p) {
+    const uintptr_t a = (uintptr_t)p;
+    base = (const uint32_t*)(a & ~(uintptr_t)15);
+    off = (uint32_t)(a & 15);
+    cur = 0;
+    c0 = base[0], c1 = base[1], c2 = base[2], c3 = base[3];
+    n0 = base[4], n1 = base[5], n2 = base[6], n3 = base[7];
+  }
+  // Sequential access only: i grows by one per call.
+  DN_HD uint8_t operator()(int i) {
+    const uint32_t x = off + (uint32_t)i;
+    const int c = (int)(x >> 4);
+    if (c != cur) {
+      c0 = n0, c1 = n1, c2 = n2, c3 = n3;
+      cur = c;
+      const uint32_t* q = base + 4 * (c + 1);
+      n0 = q[0], n1 = q[1], n2 = q[2], n3 = q[3];
+    }
+    const uint32_t sel = (x >> 2) & 3;
+    const uint32_t wv = sel == 0 ? c0 : (sel == 1 ? c1 : (sel == 2 ? c2 : c3));
+    return (uint8_t)(wv >> (8 * (x & 3)));
+  }
+};
+
+// Plain indexed reader (host-side genome sketching).
+struct PtrReader {
+  const uint8_t* p;
+  DN_HD uint8_t operator()(int i) const { return p[i]; }
+};
+
 // Calls emit(hash, pos, strand) for every minimizer of seq[0, len), in position order.
-template <class Emit>
-DN_HD void sketch_stream(const uint8_t* seq, int len, int k, int w, Ring ring, Emit&& emit) {
+template <class Reader, class Emit>
+DN_HD void sketch_stream(Reader&& seq, int len, int k, int w, Ring ring, Emit&& emit) {
   const uint64_t mask = (1ULL << (2 * k)) - 1;
   const int shift = 2 * (k - 1);
   const uint64_t kInvalid = ~0ULL;
@@ -87,7 +128,7 @@ DN_HD void sketch_stream(const uint8_t* seq, int len, int k, int w, Ring ring, E
   int midx = -1;
   int64_t last = -1;
   for (int i = 0; i < len; ++i) {
-    const uint8_t c = seq[i];
+    const uint8_t c = seq(i);
     if (c > 3) {
       run = 0;
       kidx = 0;
@@ -142,39 +183,111 @@ DN_HD uint32_t key_strand(uint64_t k) { return (uint32_t)(k >> 63); }
 DN_HD uint32_t key_rpos(uint64_t k) { return (uint32_t)(k >> 32) & 0x7fffffffu; }
 DN_HD int32_t key_qpos(uint64_t k) { return (int32_t)(uint32_t)k; }
 
-DN_HD uint32_t seed_count_read(const SeedArgs& a, uint32_t r, Ring ring) {
-  const uint8_t* s = a.codes + a.offs[r];
+// Continues linear probing for h from slot i (index.hpp:24-38).
+DN_HD void probe_from(const Bucket* table, uint64_t mask, uint64_t h, uint64_t i, uint32_t& off, uint32_t& cnt) {
+  for (;;) {
+    const Bucket b = table[i];
+    if (b.key == h) {
+      off = b.offset;
+      cnt = b.count;
+      return;
+    }
+    if (b.key == ~0ULL) {
+      off = cnt = 0;
+      return;
+    }
+    i = (i + 1) & mask;
+  }
+}
+
+// Probes minimizers as the sketch emits them, two deep: the home bucket of minimizer m is
+// loaded while minimizer m-1's bucket is resolved (and its first occurrence loaded) and m-2's
+// occurrences are consumed, so one read keeps several independent loads in flight. Calls
+// on(mpos, mstrand, off, cnt, first) in minimizer order for every minimizer with
+// 1 <= cnt <= max_occ (src/mapper.cpp:24-30); first = pos[off] when NeedPos.
+template <bool NeedPos, class OnHit>
+struct ProbePipe {
+  const SeedArgs& s;
+  OnHit& on;
+  bool v1 = false, v2 = false;
+  uint64_t h1 = 0;
+  uint32_t mp1 = 0, ms1 = 0;
+  Bucket b1{};
+  uint32_t mp2 = 0, ms2 = 0, off2 = 0, cnt2 = 0, first2 = 0;
+  DN_HD ProbePipe(const SeedArgs& a, OnHit& f) : s(a), on(f) {}
+  DN_HD void advance() {
+    if (v2) {
+      on(mp2, ms2, off2, cnt2, first2);
+      v2 = false;
+    }
+    if (v1) {
+      uint32_t off = 0, cnt = 0;
+      if (b1.key == h1) {
+        off = b1.offset;
+        cnt = b1.count;
+      } else if (b1.key != ~0ULL) {
+        probe_from(s.table, s.mask, h1, ((h1 & s.mask) + 1) & s.mask, off, cnt);
+      }
+      if (cnt != 0 && cnt <= s.max_occ) {
+        mp2 = mp1;
+        ms2 = ms1;
+        off2 = off;
+        cnt2 = cnt;
+        first2 = NeedPos ? s.pos[off] : 0u;
+        v2 = true;
+      }
+      v1 = false;
+    }
+  }
+  DN_HD void push(uint64_t h, uint32_t mpos, uint32_t mstrand) {
+    const Bucket bn = s.table[h & s.mask];
+    advance();
+    h1 = h;
+    mp1 = mpos;
+    ms1 = mstrand;
+    b1 = bn;
+    v1 = true;
+  }
+  DN_HD void drain() {
+    advance();
+    advance();
+  }
+};
+
+DN_HD uint32_t seed_count_read(const SeedArgs& a, uint32_t i, Ring ring) {
+  const uint32_t r = read_id(a.list, i);
   uint32_t total = 0;
-  sketch_stream(s, a.lens[r], a.k, a.w, ring, [&](uint64_t h, uint32_t, uint32_t) {
-    uint32_t off, cnt;
-    probe(a.table, a.mask, h, off, cnt);
-    if (cnt != 0 && cnt <= a.max_occ) total += cnt;
-  });
+  auto on = [&](uint32_t, uint32_t, uint32_t, uint32_t cnt, uint32_t) { total += cnt; };
+  ProbePipe<false, decltype(on)> pipe(a, on);
+  sketch_stream(ChunkReader(a.codes + a.offs[r]), a.lens[r], a.k, a.w, ring,
+                [&](uint64_t h, uint32_t p, uint32_t st) { pipe.push(h, p, st); });
+  pipe.drain();
   return total;
 }
 
 // Forward-strand anchors fill the slice from the front in emission order, reverse-strand
 // anchors from the back; both runs then come out nearly ascending, which the insertion sort of
 // the chainer exploits. The final order is the unique sorted order either way.
-DN_HD void seed_emit_read(const SeedArgs& a, uint32_t r, Ring ring) {
-  const uint8_t* s = a.codes + a.offs[r];
+DN_HD void seed_emit_read(const SeedArgs& a, uint32_t i, Ring ring) {
+  const uint32_t r = read_id(a.list, i);
   const int len = a.lens[r];
-  uint64_t* out = a.keys + a.anchor_off[r];
-  int64_t lo = 0, hi = (int64_t)a.anchor_count[r] - 1;
+  uint64_t* out = a.keys + a.anchor_off[i];
+  int64_t lo = 0, hi = (int64_t)a.anchor_count[i] - 1;
   const int k = a.k;
-  sketch_stream(s, len, k, a.w, ring, [&](uint64_t h, uint32_t mpos, uint32_t mstrand) {
-    uint32_t off, cnt;
-    probe(a.table, a.mask, h, off, cnt);
-    if (cnt == 0 || cnt > a.max_occ) return;
+  auto on = [&](uint32_t mpos, uint32_t mstrand, uint32_t off, uint32_t cnt, uint32_t first) {
     for (uint32_t j = 0; j < cnt; ++j) {
-      const uint32_t v = a.pos[off + j];
+      const uint32_t v = j == 0 ? first : a.pos[off + j];
       const uint32_t st = (v & 1) ^ mstrand;
       const int32_t qpos = st ? len - (int32_t)mpos - k : (int32_t)mpos;
       const uint64_t key = anchor_key(st, v >> 1, qpos);
       if (st) out[hi--] = key;
       else out[lo++] = key;
     }
-  });
+  };
+  ProbePipe<true, decltype(on)> pipe(a, on);
+  sketch_stream(ChunkReader(a.codes + a.offs[r]), len, k, a.w, ring,
+                [&](uint64_t h, uint32_t p, uint32_t st) { pipe.push(h, p, st); });
+  pipe.drain();
 }
 
 // ------------------------------------------------------------------------------------ chaining
@@ -193,7 +306,8 @@ struct Cand {
 };
 
 struct ChainArgs {
-  uint32_t n_reads;
+  uint32_t n_reads;        // entries to process (of `list` when set)
+  const uint32_t* list;    // read ids (null: identity); anchor arrays are by list position
   const uint32_t* anchor_count;
   const uint64_t* anchor_off;
   uint64_t* keys;          // sorted in place
@@ -256,9 +370,10 @@ DN_HD int contig_of(const uint64_t* off, int nc, uint64_t gpos) {  // src/refere
   return lo;
 }
 
-DN_HD void chain_read(const ChainArgs& a, uint32_t r) {
-  const int n = (int)a.anchor_count[r];
-  const uint64_t base = a.anchor_off[r];
+DN_HD void chain_read(const ChainArgs& a, uint32_t li) {
+  const uint32_t r = read_id(a.list, li);
+  const int n = (int)a.anchor_count[li];
+  const uint64_t base = a.anchor_off[li];
   Cand* cands = a.cands + (uint64_t)r * (uint64_t)a.max_cand;
   int nc = 0;
   if (n > 0) {

@@ -14,6 +14,9 @@
 // (the band loop is fully unrolled for a compile-time W). Substitution scores come from one
 // v_xor_b32 + one v_perm_b32 per cell and pair (see score_table()).
 //
+// Sequence bytes are streamed with one aligned dword load per stream per 4 rows (v_alignbyte),
+// so every device sequence buffer carries guard bytes on both sides (kSeqGuard).
+//
 // Traceback storage: 4 flag bits per cell and alignment. Flags are the sign bits of four packed
 // differences, gathered by v_perm_b32 into byte masks and accumulated 8 cells to a byte:
 //   byte 0/1 extF (A/B), 2/3 extE, 4/5 notE, 6/7 notDiag — bit (d & 7) of the 8-byte word of
@@ -26,10 +29,9 @@
 namespace dn {
 
 // Selector encodings (per 16-bit half): query [4|q, 8|q] or [0xF0, 0xF0] for N; target [t, t] or
-// [0x30, 0x30] for N. x = qsel ^ tsel indexes the table below: low byte 4|(q^t) picks the score,
-// high byte 8|(q^t) replicates its sign; any N gives a selector >= 13, i.e. 0xFFFF = -1 (ambig 1).
-DN_HD uint32_t qsel_half(uint8_t c) { return c < 4 ? (uint32_t)((8u | c) << 8 | (4u | c)) : 0xF0F0u; }
-DN_HD uint32_t tsel_half(uint8_t c) { return c < 4 ? (uint32_t)c * 0x0101u : 0x3030u; }
+// [0x30, 0x30] for N (qsel_from / tsel_from below). x = qsel ^ tsel indexes the table below: low
+// byte 4|(q^t) picks the score, high byte 8|(q^t) replicates its sign; any N gives a selector
+// >= 13, i.e. 0xFFFF = -1 (ambig 1).
 
 // v_perm sources: s0 bytes = {A, -B, -B, -B} (x = 0..3), s1 byte 1 = 0 (sign source of a match),
 // s1 byte 3 = -B (sign source of a mismatch).
@@ -72,13 +74,87 @@ DN_HD uint8_t t_at(const SwTask& t, int j) {
   return ((const uint8_t*)(uintptr_t)t.t)[j];
 }
 
+// Guard bytes kept readable on both sides of every sequence buffer the packed kernel streams:
+// it may load up to 4 bytes past a read, and a target window may start or end up to
+// qlen + 2*bw bytes outside the reference (those bytes are masked to N, never used).
+constexpr uint64_t kSeqGuard = 65536;
+
+// The 4 bytes at addr [j, j+4) as one little-endian word, from two aligned dword loads.
+DN_HD uint32_t load4(uint64_t base, int64_t j) {
+  const uint64_t a = base + (uint64_t)j;
+  const uint32_t* p = (const uint32_t*)(uintptr_t)(a & ~(uint64_t)3);
+  const uint32_t sh = (uint32_t)(a & 3);
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_alignbyte(p[1], p[0], sh);  // v_alignbyte_b32 shifts by S2[1:0] bytes
+#else
+  return (uint32_t)((((uint64_t)p[1] << 32) | p[0]) >> (8 * sh));
+#endif
+}
+
+// Query codes of rows r0 .. r0+3 as bytes 0..3 (reverse-complemented reads walk backwards;
+// the complement is applied later). Bytes of rows >= qlen are undefined and masked by the caller.
+DN_HD uint32_t qword4(const SwTask& t, int r0) {
+  if (t.flags & 1) {
+    int j = t.qlen - 4 - r0;
+    j = j < -4 ? -4 : j;
+    return perm_b32(0, load4(t.q, j), 0x00010203u);  // byte reverse
+  }
+  const int last = t.qlen > 0 ? t.qlen - 1 : 0;
+  return load4(t.q, r0 < last ? r0 : last);
+}
+
+// Target codes at window positions j0 .. j0+3; positions outside [tlo, thi) are masked by the
+// caller. The address is clamped so it stays next to the valid range.
+DN_HD uint32_t tword4(const SwTask& t, int j0) {
+  int j = j0 < t.tlo - 4 ? t.tlo - 4 : j0;
+  j = j > t.thi ? t.thi : j;
+  return load4(t.t, j);
+}
+
+DN_HD uint32_t byte_at(uint32_t w, int u) { return (w >> (8 * u)) & 0xffu; }
+
+// Selector halves with validity, complement for reverse-complemented queries, and N.
+DN_HD uint32_t qsel_from(uint32_t c, bool valid, bool rev) {
+  if (!valid || c > 3) return 0xF0F0u;
+  return rev ? 0x0B07u - c * 0x0101u : 0x0804u + c * 0x0101u;
+}
+DN_HD uint32_t tsel_from(uint32_t c, bool valid) { return (!valid || c > 3) ? 0x3030u : c * 0x0101u; }
+
 // ------------------------------------------------------------------------------------ walk
-// src/align.cpp:87-143, one cell per step so the two alignments of a lane walk in lock-step.
+// src/align.cpp:87-143, one cell per step.
 struct Walk {
   int i, d, v, nm, run, state;  // state 0: H (diagonal), 1: E (insertion), 2: F (deletion)
   int n, pend_len, status;
   uint32_t pend_op;
   bool done;
+};
+
+// The aligned dword last loaded from one sequence: the walk moves through the query and the
+// target one position at a time, so most bases come from a register.
+struct CachedDword {
+  uint64_t at = ~0ull;
+  uint32_t w = 0;
+  DN_HD uint8_t get(uint64_t addr) {
+    const uint64_t a = addr & ~(uint64_t)3;
+    if (a != at) {
+      w = *(const uint32_t*)(uintptr_t)a;
+      at = a;
+    }
+    return (uint8_t)(w >> (8 * (addr & 3)));
+  }
+};
+
+struct WalkSeq {
+  CachedDword q, t;
+  DN_HD uint8_t qb(const SwTask& tk, int i) {
+    if (i < 0 || i >= tk.qlen) return 4;
+    if (tk.flags & 1) return comp_code(q.get(tk.q + (uint64_t)(tk.qlen - 1 - i)));
+    return q.get(tk.q + (uint64_t)i);
+  }
+  DN_HD uint8_t tb(const SwTask& tk, int j) {
+    if (j < tk.tlo || j >= tk.thi) return 4;
+    return t.get(tk.t + (uint64_t)(int64_t)j);
+  }
 };
 
 DN_HD void walk_push(Walk& w, uint32_t op, int len, uint32_t* out, int cap) {
@@ -104,7 +180,7 @@ DN_HD void walk_flush(Walk& w, uint32_t* out, int cap) {
 
 template <int W>
 DN_HD void walk_step(Walk& w, int half, const SwTask& tk, const uint64_t* slot_lane, const ScoreP& sp,
-                     uint32_t* out, int cap) {
+                     uint32_t* out, int cap, WalkSeq& seq) {
   constexpr int G = (W + 7) / 8;
   const int oe = sp.gap_open + sp.gap_ext, ge = sp.gap_ext;
   if (w.state == 0 && (w.i < 0 || w.v <= 0)) {
@@ -119,11 +195,12 @@ DN_HD void walk_step(Walk& w, int half, const SwTask& tk, const uint64_t* slot_l
     w.done = true;
     return;
   }
+  // The flag word and, speculatively, the two bases of the cell (mostly from registers).
   const uint64_t word = slot_lane[(uint64_t)(w.i * G + (w.d >> 3)) * 64];
+  const uint8_t a = seq.qb(tk, w.i), c = seq.tb(tk, w.i + w.d);
   const int b = w.d & 7;
   if (w.state == 0) {
     if (!((word >> (48 + 8 * half + b)) & 1)) {  // diagonal
-      const uint8_t a = q_at(tk, w.i), c = t_at(tk, w.i + w.d);
       w.v -= sub_score(sp, a, c);
       w.nm += (a != c || a > 3);
       ++w.run;
@@ -195,7 +272,9 @@ DN_HD void walk_finish(const Walk& w, int i_end, int d_end, int score, int cap, 
 }
 
 // ------------------------------------------------------------------------------------ fill
-// One lane: tasks ta and tb (task ids, >= n_tasks for an empty half), rows 0..qmax-1.
+// One lane: tasks ida and idb (va / vb false for an empty half), rows 0..qmax-1 of the wave.
+// Score-only: writes the scores. With flags (TB): stores the flag words into the wave's slot
+// and writes the score and the best cell (row in qe, diagonal in te) for walk_task().
 template <int W, bool TB>
 DN_HD void sw_pair(const SwArgs& a, uint32_t ida, uint32_t idb, bool va, bool vb, int qmax, uint64_t* slot,
                    int lane) {
@@ -205,6 +284,7 @@ DN_HD void sw_pair(const SwArgs& a, uint32_t ida, uint32_t idb, bool va, bool vb
   else ta.qlen = ta.tlo = ta.thi = 0, ta.flags = 0, ta.q = ta.t = 0;
   if (vb) tb = a.tasks[idb];
   else tb.qlen = tb.tlo = tb.thi = 0, tb.flags = 0, tb.q = tb.t = 0;
+  const bool ra = ta.flags & 1, rb = tb.flags & 1;
 
   const pk16 zero = pk_splat(0);
   const pk16 oe = pk_splat(a.sp.gap_open + a.sp.gap_ext), ge = pk_splat(a.sp.gap_ext);
@@ -214,9 +294,23 @@ DN_HD void sw_pair(const SwArgs& a, uint32_t ida, uint32_t idb, bool va, bool vb
   uint32_t T[W];
 #pragma unroll
   for (int d = 0; d <= W; ++d) H[d] = E[d] = zero;
+  // Target positions 0..W-1, four at a time.
 #pragma unroll
-  for (int d = 0; d < W; ++d) T[d] = tsel_half(t_at(ta, d)) | tsel_half(t_at(tb, d)) << 16;
-  uint32_t qsel = qsel_half(q_at(ta, 0)) | qsel_half(q_at(tb, 0)) << 16;
+  for (int d0 = 0; d0 < W; d0 += 4) {
+    const uint32_t wa = va ? tword4(ta, d0) : 0u, wb = vb ? tword4(tb, d0) : 0u;
+#pragma unroll
+    for (int u = 0; u < 4 && d0 + u < W; ++u) {
+      const int j = d0 + u;
+      T[j] = tsel_from(byte_at(wa, u), j >= ta.tlo && j < ta.thi) |
+             tsel_from(byte_at(wb, u), j >= tb.tlo && j < tb.thi) << 16;
+    }
+  }
+  // Streams, per block of four rows r = 4b .. 4b+3: the query bytes of those rows, and the
+  // target bytes that enter the band after each of them (positions r + W); the next block's
+  // words are fetched at the start of the current one.
+  uint32_t cqa = va ? qword4(ta, 0) : 0u, cqb = vb ? qword4(tb, 0) : 0u;
+  uint32_t cta = va ? tword4(ta, W) : 0u, ctb = vb ? tword4(tb, W) : 0u;
+  uint32_t nqa = 0, nqb = 0, nta = 0, ntb = 0;
 
   pk16 best = zero;
   pk16 BH[TB ? W : 1];
@@ -228,9 +322,14 @@ DN_HD void sw_pair(const SwArgs& a, uint32_t ida, uint32_t idb, bool va, bool vb
   uint64_t* slot_lane = slot + lane;
 
   for (int i = 0; i < qmax; ++i) {
-    // Next row's bases, loaded a row ahead.
-    const uint8_t qa_n = q_at(ta, i + 1), qb_n = q_at(tb, i + 1);
-    const uint8_t ta_n = t_at(ta, i + W), tb_n = t_at(tb, i + W);
+    const int u = i & 3;
+    if (u == 0) {  // fetch the next block of four rows
+      nqa = va ? qword4(ta, i + 4) : 0u;
+      nqb = vb ? qword4(tb, i + 4) : 0u;
+      nta = va ? tword4(ta, i + 4 + W) : 0u;
+      ntb = vb ? tword4(tb, i + 4 + W) : 0u;
+    }
+    const uint32_t qsel = qsel_from(byte_at(cqa, u), i < ta.qlen, ra) | qsel_from(byte_at(cqb, u), i < tb.qlen, rb) << 16;
 
     pk16 hdiag = H[0], f = zero, hleft = zero, rowmax = zero;
     uint32_t acc12 = 0, acc34 = 0;
@@ -272,10 +371,18 @@ DN_HD void sw_pair(const SwArgs& a, uint32_t ida, uint32_t idb, bool va, bool vb
         best_i = bfi(m, (uint32_t)i * 0x10001u, best_i);
       }
     }
+    // Shift the band one position and bring in target position i + W.
 #pragma unroll
     for (int d = 0; d + 1 < W; ++d) T[d] = T[d + 1];
-    T[W - 1] = tsel_half(ta_n) | tsel_half(tb_n) << 16;
-    qsel = qsel_half(qa_n) | qsel_half(qb_n) << 16;
+    const int jn = i + W;
+    T[W - 1] = tsel_from(byte_at(cta, u), jn >= ta.tlo && jn < ta.thi) |
+               tsel_from(byte_at(ctb, u), jn >= tb.tlo && jn < tb.thi) << 16;
+    if (u == 3) {
+      cqa = nqa;
+      cqb = nqb;
+      cta = nta;
+      ctb = ntb;
+    }
   }
 
   const int sa = pk_lo(best), sb = pk_hi(best);
@@ -299,16 +406,42 @@ DN_HD void sw_pair(const SwArgs& a, uint32_t ida, uint32_t idb, bool va, bool vb
     if (pk_lo(BH[d]) == sa) da = d;
     if (pk_hi(BH[d]) == sb) db = d;
   }
-  const int ia = (int16_t)(best_i & 0xffff), ib = (int16_t)(best_i >> 16);
-  uint32_t* oa = a.cigar + (uint64_t)(va ? ida : 0) * (uint64_t)a.cigar_cap;
-  uint32_t* ob = a.cigar + (uint64_t)(vb ? idb : 0) * (uint64_t)a.cigar_cap;
-  Walk wa = walk_init(ia, da, va ? sa : 0), wb = walk_init(ib, db, vb ? sb : 0);
-  while (!(wa.done && wb.done)) {
-    if (!wa.done) walk_step<W>(wa, 0, ta, slot_lane, a.sp, oa, a.cigar_cap);
-    if (!wb.done) walk_step<W>(wb, 1, tb, slot_lane, a.sp, ob, a.cigar_cap);
+  if (va) {
+    SwResult& r = a.res[ida];
+    r.score = sa;
+    r.qe = (int16_t)(best_i & 0xffff);
+    r.te = da;
+    r.qb = r.tb = r.nm = r.n_cigar = r.status = 0;
   }
-  if (va) walk_finish(wa, ia, da, sa, a.cigar_cap, a.res[ida]);
-  if (vb) walk_finish(wb, ib, db, sb, a.cigar_cap, a.res[idb]);
+  if (vb) {
+    SwResult& r = a.res[idb];
+    r.score = sb;
+    r.qe = (int16_t)(best_i >> 16);
+    r.te = db;
+    r.qb = r.tb = r.nm = r.n_cigar = r.status = 0;
+  }
+}
+
+// Traceback of one alignment from the end cell sw_pair<W, true> left in its result. k is the
+// launch-relative position of the task (the fill's wave k / 128 holds it, in lane k % 64, half
+// (k / 64) % 2), so the 64 threads of a walk wave read the words of one fill wave.
+template <int W>
+DN_HD void walk_task(const SwArgs& a, uint32_t k) {
+  const uint32_t id = a.order ? a.order[k] : k;
+  SwResult& r = a.res[id];
+  const int score = r.score, i_end = r.qe, d_end = r.te;
+  if (score <= 0) {
+    r.qb = r.qe = r.tb = r.te = r.nm = r.n_cigar = r.status = 0;
+    return;
+  }
+  const SwTask tk = a.tasks[id];
+  const uint32_t wave = k >> 7, lane = k & 63, half = (k >> 6) & 1;
+  const uint64_t* slot_lane = a.dir + (uint64_t)wave * a.dir_slot_words + lane;
+  uint32_t* out = a.cigar + (uint64_t)id * (uint64_t)a.cigar_cap;
+  Walk w = walk_init(i_end, d_end, score);
+  WalkSeq seq;
+  while (!w.done) walk_step<W>(w, (int)half, tk, slot_lane, a.sp, out, a.cigar_cap, seq);
+  walk_finish(w, i_end, d_end, score, a.cigar_cap, r);
 }
 
 }  // namespace dn

@@ -7,10 +7,13 @@
 #include <stdint.h>
 #include <string.h>
 
+#include <cstring>
+
 #include <algorithm>
 #include <vector>
 
 #include "device/bw_list.hpp"
+#include "device/fused_seed_chain.hpp"
 #include "device/sw_generic.hpp"
 #include "device/sw_packed.hpp"
 
@@ -19,17 +22,27 @@ using namespace dn;
 namespace {
 
 template <int W, bool TB>
-void run_packed(const SwArgs& a) {
+void run_packed(const SwArgs& a0) {
   constexpr int G = (W + 7) / 8;
-  for (uint32_t base = 0; base < a.n; base += 128) {
+  int qall = 1;
+  for (uint32_t k = 0; k < a0.n; ++k) qall = std::max(qall, a0.tasks[k].qlen);
+  const uint64_t slot_words = (uint64_t)qall * G * 64;
+  const uint32_t waves = (a0.n + 127) / 128;
+  std::vector<uint64_t> dir(TB ? (size_t)waves * slot_words : 1, 0xdeadbeefdeadbeefULL);
+  SwArgs a = a0;
+  a.dir = dir.data();
+  a.dir_slot_words = slot_words;
+  for (uint32_t wv = 0; wv < waves; ++wv) {
+    const uint32_t base = wv * 128;
     int qmax = 0;
     for (uint32_t k = base; k < a.n && k < base + 128; ++k) qmax = std::max(qmax, a.tasks[k].qlen);
-    std::vector<uint64_t> slot((size_t)std::max(qmax, 1) * G * 64, 0xdeadbeefdeadbeefULL);
     for (int lane = 0; lane < 64; ++lane) {
       const uint32_t ia = base + lane, ib = base + 64 + lane;
-      sw_pair<W, TB>(a, ia, ib, ia < a.n, ib < a.n, qmax, slot.data(), lane);
+      sw_pair<W, TB>(a, ia, ib, ia < a.n, ib < a.n, qmax, TB ? a.dir + (uint64_t)wv * slot_words : nullptr, lane);
     }
   }
+  if (TB)
+    for (uint32_t k = 0; k < a.n; ++k) walk_task<W>(a, k);
 }
 
 template <bool TB>
@@ -52,10 +65,32 @@ extern "C" {
 
 // Same contract as the reference shim's ref_align (res6 = score qb qe tb te nm), CIGAR in
 // forward order. force: 0 auto, 1 generic int32 kernel, 2 packed (fails if unavailable).
+int emu_align_ex(const int* sp6, const uint8_t* pool, const int64_t* q_off, const int32_t* q_len,
+                 const int64_t* t_off, const int32_t* t_len, const uint8_t* rev, const int32_t* tlo, size_t n,
+                 int traceback, int32_t* res6, uint32_t* cig, int64_t* cig_off, int32_t* cig_n, size_t cap,
+                 int force);
+
 int emu_align(const int* sp6, const uint8_t* pool, const int64_t* q_off, const int32_t* q_len, const int64_t* t_off,
               const int32_t* t_len, size_t n, int traceback, int32_t* res6, uint32_t* cig, int64_t* cig_off,
               int32_t* cig_n, size_t cap, int force) {
+  return emu_align_ex(sp6, pool, q_off, q_len, t_off, t_len, nullptr, nullptr, n, traceback, res6, cig, cig_off,
+                      cig_n, cap, force);
+}
+
+// rev[i] != 0 aligns the reverse complement of query i (read backwards, as for reverse-strand
+// candidates); target positions below tlo[i] read as N (a window starting before its contig).
+int emu_align_ex(const int* sp6, const uint8_t* pool, const int64_t* q_off, const int32_t* q_len,
+                 const int64_t* t_off, const int32_t* t_len, const uint8_t* rev, const int32_t* tlo, size_t n,
+                 int traceback, int32_t* res6, uint32_t* cig, int64_t* cig_off, int32_t* cig_n, size_t cap,
+                 int force) {
   ScoreP sp{sp6[0], sp6[1], sp6[2], sp6[3], sp6[4], sp6[5]};
+  // The packed kernel streams aligned dwords around each sequence: give the pool guard bytes.
+  uint64_t pool_len = 0;
+  for (size_t i = 0; i < n; ++i)
+    pool_len = std::max<uint64_t>(pool_len, std::max<uint64_t>(q_off[i] + q_len[i], t_off[i] + t_len[i]));
+  std::vector<uint8_t> guarded(pool_len + 2 * kSeqGuard, 0xEE);
+  std::memcpy(guarded.data() + kSeqGuard, pool, pool_len);
+  pool = guarded.data() + kSeqGuard;
   std::vector<SwTask> tasks(n);
   int maxq = 0;
   for (size_t i = 0; i < n; ++i) {
@@ -63,9 +98,9 @@ int emu_align(const int* sp6, const uint8_t* pool, const int64_t* q_off, const i
     t.q = (uint64_t)(uintptr_t)(pool + q_off[i]);
     t.t = (uint64_t)(uintptr_t)(pool + t_off[i]);
     t.qlen = q_len[i];
-    t.tlo = 0;
     t.thi = std::min(t_len[i], q_len[i] + 2 * sp.bw);
-    t.flags = 0;
+    t.tlo = tlo ? std::min(tlo[i], t.thi) : 0;
+    t.flags = rev && rev[i] ? 1u : 0u;
     maxq = std::max(maxq, q_len[i]);
   }
   const int ccap = 2 * maxq + 4;
@@ -121,6 +156,109 @@ int emu_align(const int* sp6, const uint8_t* pool, const int64_t* q_off, const i
     used += (size_t)r.n_cigar;
   }
   return done ? 1 : 0;
+}
+
+// Runs both seed+chain programs on n reads (codes 0..4) and compares their candidates.
+// mp = k, w, max_occ, bw, max_cand, min_chain, lookback, max_gap, band, min_anchors.
+// Returns the number of reads whose candidates differ (-1 on bad input); stats[0] = reads the
+// LDS program sent to the global path (overflow), stats[1] = total anchors, stats[2] = max.
+long emu_seed_chain(const uint8_t* codes, const uint64_t* offs, const int32_t* lens, uint64_t n,
+                    const void* table, uint64_t n_buckets, const uint32_t* pos, const uint64_t* contig_off,
+                    int n_contigs, const int* mp, double pri_ratio, const int32_t* cost, int cap,
+                    uint64_t* stats) {
+  uint64_t total = 0;
+  for (uint64_t r = 0; r < n; ++r) total = std::max<uint64_t>(total, offs[r] + (uint64_t)lens[r]);
+  std::vector<uint8_t> guarded(total + 256, 0xEE);
+  uint8_t* gcodes = guarded.data() + (64 - ((uintptr_t)guarded.data() & 15));  // 16-aligned, >= 49 bytes before
+  std::memcpy(gcodes, codes, total);
+  SeedArgs sa{};
+  sa.codes = gcodes;
+  sa.offs = offs;
+  sa.lens = lens;
+  sa.table = static_cast<const Bucket*>(table);
+  sa.mask = n_buckets - 1;
+  sa.pos = pos;
+  sa.k = mp[0];
+  sa.w = mp[1];
+  sa.max_occ = (uint32_t)mp[2];
+  ChainArgs ca{};
+  ca.cost = cost;
+  ca.k = mp[0];
+  ca.bw = mp[3];
+  ca.max_cand = mp[4];
+  ca.min_score = mp[5];
+  ca.lookback = mp[6];
+  ca.max_gap = mp[7];
+  ca.band = mp[8];
+  ca.min_anchors = mp[9];
+  ca.pri_ratio = pri_ratio;
+  ca.contig_off = contig_off;
+  ca.n_contigs = n_contigs;
+  const int mc = std::max(mp[4], 1);
+  std::vector<Cand> c1((size_t)n * mc), c2((size_t)n * mc);
+  std::vector<int32_t> n1(n), n2(n);
+  std::vector<uint64_t> rh(sa.w);
+  std::vector<uint32_t> rps(sa.w);
+  Ring ring{rh.data(), rps.data(), 1};
+  std::vector<uint64_t> keys(cap), gkeys, order;
+  std::vector<int16_t> f16(cap);
+  std::vector<int8_t> p8(cap);
+  std::vector<int32_t> f32, prev;
+  std::vector<ChainRec> chains;
+  long differ = 0;
+  stats[0] = stats[1] = stats[2] = 0;
+  for (uint64_t r = 0; r < n; ++r) {
+    // global path for read r alone
+    uint32_t cnt = 0;
+    uint64_t off0 = 0;
+    SeedArgs s1 = sa;
+    const uint32_t rid = (uint32_t)r;
+    s1.list = &rid;
+    s1.n_reads = 1;
+    s1.anchor_count = &cnt;
+    cnt = seed_count_read(s1, 0, ring);
+    gkeys.assign(cnt + 1, 0);
+    f32.assign(cnt + 1, 0);
+    prev.assign(cnt + 1, 0);
+    order.assign(cnt + 1, 0);
+    chains.assign(cnt + 1, ChainRec{});
+    s1.anchor_off = &off0;
+    s1.keys = gkeys.data();
+    seed_emit_read(s1, 0, ring);
+    stats[1] += cnt;
+    stats[2] = std::max<uint64_t>(stats[2], cnt);
+    ChainArgs g = ca;
+    g.list = &rid;
+    g.n_reads = 1;
+    g.anchor_count = &cnt;
+    g.anchor_off = &off0;
+    g.keys = gkeys.data();
+    g.f = f32.data();
+    g.prev = prev.data();
+    g.order = order.data();
+    g.chains = chains.data();
+    g.cands = c1.data();
+    g.n_cand = n1.data();
+    chain_read(g, 0);
+    // LDS path
+    ChainArgs l = ca;
+    l.cands = c2.data();
+    LdsView v{keys.data(), f16.data(), p8.data(), 1};
+    const int nc = seed_chain_lds<8>(sa, l, (uint32_t)r, ring, v, cap);
+    if (nc < 0) {
+      ++stats[0];
+      continue;
+    }
+    n2[r] = nc;
+    bool same = n1[r] == nc;
+    for (int c = 0; same && c < nc; ++c) {
+      const Cand& x = c1[r * mc + c];
+      const Cand& y = c2[r * mc + c];
+      same = x.rev == y.rev && x.contig == y.contig && x.diag == y.diag && x.wstart == y.wstart;
+    }
+    differ += !same;
+  }
+  return differ;
 }
 
 }  // extern "C"

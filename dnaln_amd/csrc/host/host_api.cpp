@@ -57,7 +57,7 @@ int dnaln_index_build(const uint8_t* ref, const dnaln_contig* contigs, int32_t n
   dn::Ring ring{rh.data(), rps.data(), 1};
   for (int c = 0; c < n_contigs; ++c) {
     const uint32_t base = (uint32_t)contigs[c].offset;
-    dn::sketch_stream(ref + contigs[c].offset, (int)contigs[c].len, k, w, ring,
+    dn::sketch_stream(dn::PtrReader{ref + contigs[c].offset}, (int)contigs[c].len, k, w, ring,
                       [&](uint64_t h, uint32_t p, uint32_t s) { occ.push_back({h, (p + base) << 1 | s}); });
   }
   std::sort(occ.begin(), occ.end(),

@@ -112,10 +112,13 @@ typedef struct {
   uint64_t cigar_off;
 } dnaln_record;
 
-/* MapStats (src/mapper.hpp:46-49) plus device timings of the last batch. */
+/* MapStats (src/mapper.hpp:46-49) plus device timings of the last batch. anchors counts the
+ * reads that took the global-memory seed path; global_path_reads counts those of them that the
+ * batched form's LDS path handed over. */
 typedef struct {
   uint64_t reads, mapped, anchors, candidates, cells, rank_tasks, tb_tasks;
   double ms_seed, ms_chain, ms_rank, ms_traceback, ms_total;
+  uint64_t global_path_reads;
 } dnaln_map_stats;
 
 /* Fills the reference's defaults (k 15, w 10, max_occ 500, 2/4/4/2/1 bw 16, chain

@@ -72,3 +72,37 @@ def test_emulated_packed_kernel_on_long_and_n_rich_tasks(emu):
             exp = oracle.align(ospt, q, t, True)
             assert tuple(res[i]) == tuple(exp[:6]), i
             assert cig[co[i]:co[i + 1]].tolist() == exp[6], i
+
+
+def test_emulated_packed_kernel_reverse_queries_and_clipped_windows(emu):
+    """Reverse-complemented queries stream backwards and windows may start inside the target
+    (positions below tlo read as N): both against the oracle on the equivalent plain task."""
+    rng = np.random.default_rng(91)
+    for bw in (8, 16, 32):
+        sc = dict(match=2, mismatch=4, gap_open=4, gap_ext=2, ambig=1, bw=bw)
+        tasks = sets.random_tasks(rng, 260, bw, max_len=700)
+        pool, qo, ql, to, tl = sets.pack_tasks(tasks)
+        n = len(tasks)
+        rev = (rng.random(n) < 0.5).astype(np.uint8)
+        tlo = np.where(rng.random(n) < 0.3, rng.integers(0, 40, n), 0).astype(np.int32)
+        res = np.zeros((n, 6), np.int32)
+        cap = int(ql.astype(np.int64).sum()) * 2 + 8 * n + 8
+        cig = np.zeros(cap, np.uint32)
+        co = np.zeros(n, np.int64)
+        cn = np.zeros(n, np.int32)
+        v = C.c_void_p
+        sp = np.array(list(sc.values()), np.int32)
+        rc = emu.emu_align_ex(v(sp.ctypes.data), v(pool.ctypes.data), v(qo.ctypes.data), v(ql.ctypes.data),
+                              v(to.ctypes.data), v(tl.ctypes.data), v(rev.ctypes.data), v(tlo.ctypes.data),
+                              C.c_size_t(n), 1, v(res.ctypes.data), v(cig.ctypes.data), v(co.ctypes.data),
+                              v(cn.ctypes.data), C.c_size_t(cap), 2)
+        assert rc == 1
+        ospt = oracle.score_t(**sc)
+        offs = np.concatenate([[0], np.cumsum(cn)])
+        for i, (q, t) in enumerate(tasks):
+            qq = np.where(q[::-1] < 4, 3 - q[::-1], 4).astype(np.uint8) if rev[i] else q
+            tt = t.copy()
+            tt[:min(int(tlo[i]), len(tt))] = 4
+            exp = oracle.align(ospt, qq, tt, True)
+            assert tuple(res[i]) == tuple(exp[:6]), (bw, i)
+            assert cig[offs[i]:offs[i + 1]].tolist() == exp[6], (bw, i)

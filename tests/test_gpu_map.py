@@ -40,6 +40,23 @@ def test_map_matches_reference_records(name):
     assert stats.reads == len(r)
 
 
+@pytest.mark.parametrize("name", list(sets.SETS))
+def test_batched_form_matches_reference_records(name):
+    """The batched form (fused LDS seed+chain, overflow reads on the global path) gives the
+    reference's records too, and reports how many reads took the global path."""
+    spec = sets.SETS[name]
+    g = sets.genome(spec)
+    r = sets.reads(spec, g)
+    print(f"set {name} batched", flush=True)
+    b = _context(spec).batch(mode=abi.MODE_BATCHED)
+    b.upload(api.prepare(r))
+    b.run()
+    recs, pool, stats = b.download()
+    b.close()
+    _compare(name, recs, pool, g)
+    assert stats.global_path_reads <= len(r)
+
+
 @pytest.mark.parametrize("name", ["uniform4m_150bp_10k", "edge_repeats_bw32", "edge_twins_bw16"])
 @pytest.mark.parametrize("mode,chunk", [(abi.MODE_PLAIN, 1000), (abi.MODE_PLAIN, 777), (abi.MODE_BATCHED, 0)])
 def test_map_is_batch_invariant(name, mode, chunk):
