@@ -134,28 +134,30 @@ __global__ __launch_bounds__(kSeedBlock) void k_chain(ChainArgs a) {
 }
 
 
-// One read per thread; the read's ring, anchors, f and prev sit in LDS, interleaved by thread.
-template <int CAP>
-__global__ __launch_bounds__(kFusedBlock) void k_seed_chain_fused(SeedArgs s, ChainArgs c, uint32_t* ovf,
-                                                                  uint32_t* novf) {
+// One read per thread; the read's anchor keys and minimizer list sit in LDS, interleaved by
+// thread. Per thread: `cap` u64 key slots, then `mslots` u64 slots holding the minimizer list
+// during the probes and f (int16) and prev (int8) during the DP. The fallback sketch's ring
+// (w u64 hashes + w u32 positions) uses the key slots before any anchor is written.
+template <int WT>
+__global__ __launch_bounds__(kFusedBlock) void k_seed_chain_fused(SeedArgs s, ChainArgs c, int cap, int mslots,
+                                                                  int maxmin, uint32_t* ovf, uint32_t* novf) {
   extern __shared__ uint64_t lds[];
   const int t = threadIdx.x;
   const uint32_t i = blockIdx.x * kFusedBlock + t;
   if (i >= s.n_reads) return;
-  uint8_t* base = reinterpret_cast<uint8_t*>(lds);
-  const int w = s.w;
+  uint64_t* keys = lds;
+  uint64_t* mins = lds + (size_t)cap * kFusedBlock;
   Ring ring;
-  ring.h = reinterpret_cast<uint64_t*>(base) + t;
-  ring.ps = reinterpret_cast<uint32_t*>(base + (size_t)w * kFusedBlock * 8) + t;
+  ring.h = keys + t;
+  ring.ps = reinterpret_cast<uint32_t*>(keys + (size_t)s.w * kFusedBlock) + t;
   ring.stride = kFusedBlock;
-  uint8_t* p = base + (size_t)w * kFusedBlock * 12;
   LdsView v;
-  v.keys = reinterpret_cast<uint64_t*>(p) + t;
-  v.f = reinterpret_cast<int16_t*>(p + (size_t)CAP * kFusedBlock * 8) + t;
-  v.prev = reinterpret_cast<int8_t*>(p + (size_t)CAP * kFusedBlock * 10) + t;
+  v.keys = keys + t;
+  v.f = reinterpret_cast<int16_t*>(mins) + t;
+  v.prev = reinterpret_cast<int8_t*>(mins) + (size_t)2 * cap * kFusedBlock + t;
   v.stride = kFusedBlock;
   const uint32_t r = read_id(s.list, i);
-  const int nc = seed_chain_lds<8>(s, c, r, ring, v, CAP);
+  const int nc = seed_chain_lds<8, 10>(s, c, r, v, mins + t, maxmin, ring, cap);
   if (nc < 0) {
     ovf[atomicAdd(novf, 1u)] = r;
     c.n_cand[r] = 0;
@@ -571,23 +573,32 @@ int seed_chain_global(dnaln_hip_batch* b, SeedArgs sa, ChainArgs ca, const uint3
   return DNALN_OK;
 }
 
-// One read from bases to candidates in LDS (fused_seed_chain.hpp). CAP anchors per read.
-size_t fused_lds_bytes(int cap, int w) { return (size_t)kFusedBlock * ((size_t)12 * w + 11 * (size_t)cap); }
+// LDS shape of the fused seed + chain kernel: anchor slots, minimizer-list entries and the
+// u64 slots that hold the list (and later f and prev), per read.
+struct FusedShape {
+  int cap = 0, maxmin = 0, mslots = 0;
+  size_t lds = 0;
+};
 
-// LDS anchor capacity for this batch, 0 when the fused path does not apply.
-int fused_capacity(int w, int maxlen) {
-  const int cap = maxlen <= 200 ? 32 : 64;
-  return fused_lds_bytes(cap, w) <= 64 * 1024 ? cap : 0;
+// Sized for the longest read of the batch: 32 anchors up to 200 bp, else 64 (the chain
+// extraction's used-mask is 64 bits); the minimizer list holds 0.22 per k-mer plus slack (a
+// random read has 0.18 on average), at most 64. cap == 0: the fused path does not apply.
+FusedShape fused_shape(int k, int w, int maxlen) {
+  FusedShape f;
+  const int nk = std::max(maxlen - k + 1, 1);
+  f.cap = maxlen <= 200 ? 32 : 64;
+  f.maxmin = std::min(64, std::max(16, ((nk * 22 / 100 + 2) + 7) / 8 * 8));
+  f.mslots = std::max(f.maxmin, (3 * f.cap + 7) / 8);
+  f.lds = (size_t)kFusedBlock * 8 * (f.cap + f.mslots);
+  if (12 * w > 8 * f.cap || f.lds > 64 * 1024) f.cap = 0;
+  return f;
 }
 
-int launch_fused(int cap, const SeedArgs& sa, const ChainArgs& ca, uint32_t* ovf, uint32_t* novf, int w,
+int launch_fused(const FusedShape& fs, const SeedArgs& sa, const ChainArgs& ca, uint32_t* ovf, uint32_t* novf,
                  hipStream_t st) {
   const unsigned grid = (sa.n_reads + kFusedBlock - 1) / kFusedBlock;
-  const size_t lds = fused_lds_bytes(cap, w);
-  if (cap == 32)
-    hipLaunchKernelGGL((k_seed_chain_fused<32>), dim3(grid), dim3(kFusedBlock), lds, st, sa, ca, ovf, novf);
-  else
-    hipLaunchKernelGGL((k_seed_chain_fused<64>), dim3(grid), dim3(kFusedBlock), lds, st, sa, ca, ovf, novf);
+  hipLaunchKernelGGL((k_seed_chain_fused<10>), dim3(grid), dim3(kFusedBlock), fs.lds, st, sa, ca, fs.cap, fs.mslots,
+                     fs.maxmin, ovf, novf);
   DN_HIP(hipGetLastError());
   return DNALN_OK;
 }
@@ -642,8 +653,8 @@ int run_chunk(dnaln_hip_batch* b, uint64_t r0, uint32_t n, bool first_chunk) {
   ca.cands = b->cands.as<Cand>();
   ca.n_cand = b->ncand.as<int32_t>();
   const unsigned sgrid = (n + kSeedBlock - 1) / kSeedBlock;
-  const int fcap = b->cfg.mode == DNALN_MODE_BATCHED ? fused_capacity(c->w, b->maxlen) : 0;
-  if (fcap && n) {
+  const FusedShape fs = b->cfg.mode == DNALN_MODE_BATCHED ? fused_shape(c->k, c->w, b->maxlen) : FusedShape{};
+  if (fs.cap && n) {
     // Batched form: every read from bases to candidates in LDS; reads that do not fit are listed
     // and take the global-memory path below.
     DN_TRY(b->ovf.reserve((uint64_t)n * 4 + 8));
@@ -651,7 +662,7 @@ int run_chunk(dnaln_hip_batch* b, uint64_t r0, uint32_t n, bool first_chunk) {
     sa.n_reads = n;
     sa.list = nullptr;
     DN_TRY(b->span_begin(0, sp_i));
-    DN_TRY(launch_fused(fcap, sa, ca, b->ovf.as<uint32_t>(), b->ovf.as<uint32_t>() + n, c->w, st));
+    DN_TRY(launch_fused(fs, sa, ca, b->ovf.as<uint32_t>(), b->ovf.as<uint32_t>() + n, st));
     DN_TRY(b->span_end(sp_i));
     uint32_t novf = 0;
     DN_HIP(hipMemcpyAsync(&novf, b->ovf.as<uint32_t>() + n, 4, hipMemcpyDeviceToHost, st));

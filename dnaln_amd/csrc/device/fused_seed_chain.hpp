@@ -70,73 +70,152 @@ DN_HD ChainLite walk_chain(const LdsView& v, int end, uint64_t& used, int& cnt) 
   return c;
 }
 
-template <int MAXCH>
-DN_HD int seed_chain_lds(const SeedArgs& s, const ChainArgs& a, uint32_t r, Ring ring, LdsView v, int cap) {
-  // 1. anchors
-  const int len = s.lens[r], k = s.k;
-  int lo = 0, hi = cap - 1;
-  bool overflow = false;
-  auto on = [&](uint32_t mpos, uint32_t mstrand, uint32_t off, uint32_t cnt, uint32_t first) {
-    if (overflow) return;
-    if ((int)cnt > hi - lo + 1) {
-      overflow = true;
-      return;
-    }
-    for (uint32_t j = 0; j < cnt; ++j) {
-      const uint32_t val = j == 0 ? first : s.pos[off + j];
-      const uint32_t st = (val & 1) ^ mstrand;
-      const int32_t qpos = st ? len - (int32_t)mpos - k : (int32_t)mpos;
-      const uint64_t key = anchor_key(st, val >> 1, qpos);
-      if (st) v.keys[(hi--) * v.stride] = key;
-      else v.keys[(lo++) * v.stride] = key;
-    }
-  };
-  ProbePipe<true, decltype(on)> pipe(s, on);
-  sketch_stream(ChunkReader(s.codes + s.offs[r]), len, k, s.w, ring,
-                [&](uint64_t h, uint32_t p, uint32_t st) { pipe.push(h, p, st); });
-  pipe.drain();
-  if (overflow) return -1;
-  const int nrev = cap - 1 - hi, n = lo + nrev;
-  for (int x = 0; x < nrev; ++x) v.keys[(lo + x) * v.stride] = v.keys[(hi + 1 + x) * v.stride];
-  if (n == 0) return 0;
+// Minimizer list entry: hash << 17 | pos << 1 | strand (2k <= 46 bits, reads < 65536 bp).
+DN_HD uint64_t pack_min(uint64_t h, uint32_t pos, uint32_t strand) { return h << 17 | (uint64_t)pos << 1 | strand; }
 
-  // 2. sort (nearly sorted: insertion sort) and DP (src/chain.cpp:21-45)
-  for (int i = 1; i < n; ++i) {
-    const uint64_t x = v.keys[i * v.stride];
-    int j = i - 1;
-    while (j >= 0 && x < v.keys[j * v.stride]) {
-      v.keys[(j + 1) * v.stride] = v.keys[j * v.stride];
-      --j;
-    }
-    v.keys[(j + 1) * v.stride] = x;
+// hash64 (src/minimizer.hpp:21-30) in the k-mer's own word type. Every step either masks to 2k
+// bits or shifts a masked value right, so for 2k <= 32 the 32-bit form gives the same hash.
+template <class T>
+DN_HD T hash_kmer(T key, T mask) {
+  key = (~key + (key << 21)) & mask;
+  key = key ^ key >> 24;
+  key = ((key + (key << 3)) + (key << 8)) & mask;
+  key = key ^ key >> 14;
+  key = ((key + (key << 2)) + (key << 4)) & mask;
+  key = key ^ key >> 28;
+  key = (key + (key << 31)) & mask;
+  return key;
+}
+
+// The reference's streamed van Herk / Gil-Werman window minimum (src/minimizer.cpp:5-68) with
+// the window length fixed at compile time: block and suffix minima are registers and every lane
+// of a wave runs the same block step, so the sketch does not diverge. T is uint32_t for k <= 15
+// (the hash then stays below the all-ones sentinel) and uint64_t otherwise. Appends the
+// minimizers of an N-free read to `mins`; returns 1, or 0 when the read holds an N (the caller
+// then uses sketch_stream), or -1 when the list would exceed maxmin.
+template <int WT, class T>
+DN_HD int sketch_vh(ChunkReader rd, int len, int k, uint64_t* mins, int stride, int maxmin, int& nmin) {
+  const T mask = (T)((1ULL << (2 * k)) - 1);
+  const int shift = 2 * (k - 1);
+  const T kInvalid = ~(T)0;
+  T fwd = 0, rev = 0;
+  nmin = 0;
+  if (len < k) return 1;
+  for (int i = 0; i < k - 1; ++i) {
+    const uint8_t c = rd(i);
+    if (c > 3) return 0;
+    fwd = ((fwd << 2) | c) & mask;
+    rev = (rev >> 2) | ((T)(3 - c) << shift);
   }
+  T bh[WT], sh[WT];
+  uint32_t bp[WT], sp[WT];
+#pragma unroll
+  for (int j = 0; j < WT; ++j) bh[j] = sh[j] = kInvalid, bp[j] = sp[j] = 0;
+  T ph = kInvalid;
+  uint32_t pp = 0;
+  int64_t last = -1;
+  const int nk = len - k + 1;
+  for (int x0 = 0; x0 < nk; x0 += WT) {
+#pragma unroll
+    for (int j = 0; j < WT; ++j) {
+      const int x = x0 + j;
+      if (x < nk) {
+        const uint8_t c = rd(x + k - 1);
+        if (c > 3) return 0;
+        fwd = ((fwd << 2) | c) & mask;
+        rev = (rev >> 2) | ((T)(3 - c) << shift);
+        const uint32_t strand = rev < fwd;
+        const T h = fwd != rev ? hash_kmer<T>(strand ? rev : fwd, mask) : kInvalid;
+        const uint32_t ps = (uint32_t)x << 1 | strand;
+        bh[j] = h;
+        bp[j] = ps;
+        if (j == 0 || h < ph) {
+          ph = h;
+          pp = ps;
+        }
+        if (x + 1 >= WT) {
+          T mh = ph;
+          uint32_t mps = pp;
+          if (j + 1 < WT && !(ph < sh[j + 1])) {
+            mh = sh[j + 1];
+            mps = sp[j + 1];
+          }
+          if (mh != kInvalid) {
+            const uint32_t p = mps >> 1;
+            if ((int64_t)p != last) {
+              if (nmin >= maxmin) return -1;
+              mins[(nmin++) * stride] = pack_min(mh, p, mps & 1);
+            }
+            last = p;
+          }
+        }
+      }
+    }
+    // block complete: suffix minima, right to left (ties keep the left k-mer)
+    sh[WT - 1] = bh[WT - 1];
+    sp[WT - 1] = bp[WT - 1];
+#pragma unroll
+    for (int r = WT - 2; r >= 0; --r) {
+      const bool left = bh[r] <= sh[r + 1];
+      sh[r] = left ? bh[r] : sh[r + 1];
+      sp[r] = left ? bp[r] : sp[r + 1];
+    }
+  }
+  return 1;
+}
+
+// Chain DP over sorted anchors in LDS (src/chain.cpp:21-45); predecessors are visited nearest
+// first in groups of four whose loads are issued together.
+DN_HD void chain_dp_lds(const ChainArgs& a, const LdsView& v, int n, int k) {
   for (int i = 0; i < n; ++i) {
     const uint64_t ki = v.keys[i * v.stride];
     const uint32_t si = key_strand(ki);
     const int64_t ri = key_rpos(ki), qi = key_qpos(ki);
     int best = k, best_j = -1;
     const int jlo = i - a.lookback > 0 ? i - a.lookback : 0;
-    for (int j = i - 1; j >= jlo; --j) {
-      const uint64_t kj = v.keys[j * v.stride];
-      if (key_strand(kj) != si) break;
-      const int64_t dr = ri - key_rpos(kj);
-      if (dr > a.max_gap) break;
-      const int64_t dq = qi - key_qpos(kj);
-      if (dr <= 0 || dq <= 0) continue;
-      const int64_t dd = dr > dq ? dr - dq : dq - dr;
-      if (dd > a.band) continue;
-      const int64_t m = dq < dr ? dq : dr;
-      const int sc = v.f[j * v.stride] + (int)(m < k ? m : k) - a.cost[dd];
-      if (sc > best) {
-        best = sc;
-        best_j = j;
+    bool stop = false;
+    for (int j0 = i - 1; j0 >= jlo && !stop; j0 -= 4) {
+      uint64_t kj[4];
+      int fj[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int j = j0 - u;
+        kj[u] = j >= jlo ? v.keys[j * v.stride] : 0;
+        fj[u] = j >= jlo ? v.f[j * v.stride] : 0;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int j = j0 - u;
+        if (stop || j < jlo) continue;
+        if (key_strand(kj[u]) != si) {
+          stop = true;
+          continue;
+        }
+        const int64_t dr = ri - key_rpos(kj[u]);
+        if (dr > a.max_gap) {
+          stop = true;
+          continue;
+        }
+        const int64_t dq = qi - key_qpos(kj[u]);
+        if (dr <= 0 || dq <= 0) continue;
+        const int64_t dd = dr > dq ? dr - dq : dq - dr;
+        if (dd > a.band) continue;
+        const int64_t m = dq < dr ? dq : dr;
+        const int sc = fj[u] + (int)(m < k ? m : k) - a.cost[dd];
+        if (sc > best) {
+          best = sc;
+          best_j = j;
+        }
       }
     }
     v.f[i * v.stride] = (int16_t)best;
     v.prev[i * v.stride] = (int8_t)best_j;
   }
+}
 
-  // 3. extraction, pass 1: best qualifying chain score
+template <int MAXCH>
+DN_HD int chain_extract_lds(const ChainArgs& a, const LdsView& v, int n, uint32_t r) {
+  // pass 1: best qualifying chain score
   uint64_t used = 0;
   int best_score = -1;
   for (int end = next_end(v, n, used); end >= 0; end = next_end(v, n, used)) {
@@ -170,7 +249,7 @@ DN_HD int seed_chain_lds(const SeedArgs& s, const ChainArgs& a, uint32_t r, Ring
     count = count < MAXCH ? count + 1 : MAXCH;
   }
 
-  // 4. candidates (src/mapper.cpp:44-62)
+  // candidates (src/mapper.cpp:44-62)
   Cand* cands = a.cands + (uint64_t)r * (uint64_t)a.max_cand;
   const int64_t bw = a.bw;
   int nc = 0;
@@ -205,6 +284,96 @@ DN_HD int seed_chain_lds(const SeedArgs& s, const ChainArgs& a, uint32_t r, Ring
   // The list ran out before max_cand candidates while chains were dropped from it.
   if (truncated && !done && nc < a.max_cand) return -1;
   return nc;
+}
+
+// Phases per read (one thread): 1. minimizers into an LDS list; 2. probes, four minimizers at a
+// time, anchors into LDS; 3. sort, DP, extraction, candidates. `mins` holds maxmin entries and
+// is dead once the probes are done, so v.f and v.prev may alias it. `ring` (the fallback sketch
+// for reads with N or a window length other than WT) may alias the anchor keys, which are free
+// until phase 2. Returns the candidate count or -1 (the read does not fit; global path).
+template <int MAXCH, int WT>
+DN_HD int seed_chain_lds(const SeedArgs& s, const ChainArgs& a, uint32_t r, LdsView v, uint64_t* mins, int maxmin,
+                         Ring ring, int cap) {
+  const int len = s.lens[r], k = s.k;
+  if (len >= 65536 || 2 * k > 46) return -1;
+  const uint8_t* seq = s.codes + s.offs[r];
+  int nmin = 0;
+  int got = 0;
+  if (s.w == WT)
+    got = k <= 15 ? sketch_vh<WT, uint32_t>(ChunkReader(seq), len, k, mins, v.stride, maxmin, nmin)
+                  : sketch_vh<WT, uint64_t>(ChunkReader(seq), len, k, mins, v.stride, maxmin, nmin);
+  if (got < 0) return -1;
+  if (got == 0) {
+    nmin = 0;
+    bool full = false;
+    sketch_stream(ChunkReader(seq), len, k, s.w, ring, [&](uint64_t h, uint32_t p, uint32_t st) {
+      if (nmin >= maxmin) full = true;
+      else mins[(nmin++) * v.stride] = pack_min(h, p, st);
+    });
+    if (full) return -1;
+  }
+
+  // 2. probes, four minimizers in flight
+  int lo = 0, hi = cap - 1;
+  bool overflow = false;
+  for (int m0 = 0; m0 < nmin && !overflow; m0 += 4) {
+    uint64_t e[4];
+    Bucket b[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      e[u] = m0 + u < nmin ? mins[(m0 + u) * v.stride] : 0;
+      if (m0 + u < nmin) b[u] = s.table[(e[u] >> 17) & s.mask];
+    }
+    uint32_t off[4], cnt[4], first[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      off[u] = cnt[u] = first[u] = 0;
+      if (m0 + u >= nmin) continue;
+      const uint64_t h = e[u] >> 17;
+      if (b[u].key == h) {
+        off[u] = b[u].offset;
+        cnt[u] = b[u].count;
+      } else if (b[u].key != ~0ULL) {
+        probe_from(s.table, s.mask, h, ((h & s.mask) + 1) & s.mask, off[u], cnt[u]);
+      }
+      if (cnt[u] > s.max_occ) cnt[u] = 0;
+      if (cnt[u]) first[u] = s.pos[off[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (!cnt[u] || overflow) continue;
+      if ((int)cnt[u] > hi - lo + 1) {
+        overflow = true;
+        continue;
+      }
+      const uint32_t mpos = (uint32_t)(e[u] >> 1) & 0xffffu, mstrand = (uint32_t)(e[u] & 1);
+      for (uint32_t j = 0; j < cnt[u]; ++j) {
+        const uint32_t val = j == 0 ? first[u] : s.pos[off[u] + j];
+        const uint32_t st = (val & 1) ^ mstrand;
+        const int32_t qpos = st ? len - (int32_t)mpos - k : (int32_t)mpos;
+        const uint64_t key = anchor_key(st, val >> 1, qpos);
+        if (st) v.keys[(hi--) * v.stride] = key;
+        else v.keys[(lo++) * v.stride] = key;
+      }
+    }
+  }
+  if (overflow) return -1;
+  const int nrev = cap - 1 - hi, n = lo + nrev;
+  for (int x = 0; x < nrev; ++x) v.keys[(lo + x) * v.stride] = v.keys[(hi + 1 + x) * v.stride];
+  if (n == 0) return 0;
+
+  // 3. sort (nearly sorted: insertion sort), DP, extraction, candidates
+  for (int i = 1; i < n; ++i) {
+    const uint64_t x = v.keys[i * v.stride];
+    int j = i - 1;
+    while (j >= 0 && x < v.keys[j * v.stride]) {
+      v.keys[(j + 1) * v.stride] = v.keys[j * v.stride];
+      --j;
+    }
+    v.keys[(j + 1) * v.stride] = x;
+  }
+  chain_dp_lds(a, v, n, k);
+  return chain_extract_lds<MAXCH>(a, v, n, r);
 }
 
 }  // namespace dn

@@ -165,7 +165,7 @@ int emu_align_ex(const int* sp6, const uint8_t* pool, const int64_t* q_off, cons
 long emu_seed_chain(const uint8_t* codes, const uint64_t* offs, const int32_t* lens, uint64_t n,
                     const void* table, uint64_t n_buckets, const uint32_t* pos, const uint64_t* contig_off,
                     int n_contigs, const int* mp, double pri_ratio, const int32_t* cost, int cap,
-                    uint64_t* stats) {
+                    int maxmin, uint64_t* stats) {
   uint64_t total = 0;
   for (uint64_t r = 0; r < n; ++r) total = std::max<uint64_t>(total, offs[r] + (uint64_t)lens[r]);
   std::vector<uint8_t> guarded(total + 256, 0xEE);
@@ -200,7 +200,7 @@ long emu_seed_chain(const uint8_t* codes, const uint64_t* offs, const int32_t* l
   std::vector<uint64_t> rh(sa.w);
   std::vector<uint32_t> rps(sa.w);
   Ring ring{rh.data(), rps.data(), 1};
-  std::vector<uint64_t> keys(cap), gkeys, order;
+  std::vector<uint64_t> keys(cap), mins(maxmin), gkeys, order;
   std::vector<int16_t> f16(cap);
   std::vector<int8_t> p8(cap);
   std::vector<int32_t> f32, prev;
@@ -244,7 +244,7 @@ long emu_seed_chain(const uint8_t* codes, const uint64_t* offs, const int32_t* l
     ChainArgs l = ca;
     l.cands = c2.data();
     LdsView v{keys.data(), f16.data(), p8.data(), 1};
-    const int nc = seed_chain_lds<8>(sa, l, (uint32_t)r, ring, v, cap);
+    const int nc = seed_chain_lds<8, 10>(sa, l, (uint32_t)r, v, mins.data(), maxmin, ring, cap);
     if (nc < 0) {
       ++stats[0];
       continue;

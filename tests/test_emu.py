@@ -106,3 +106,38 @@ def test_emulated_packed_kernel_reverse_queries_and_clipped_windows(emu):
             exp = oracle.align(ospt, qq, tt, True)
             assert tuple(res[i]) == tuple(exp[:6]), (bw, i)
             assert cig[offs[i]:offs[i + 1]].tolist() == exp[6], (bw, i)
+
+
+def _emu_seed_chain(emu, spec, g, r, cap, maxmin):
+    from dnaln_amd import api
+    mp = sets.map_params(spec)
+    k, w, bw = mp["k"], mp["w"], mp["bw"]
+    idx = api.Index(g, k, w)
+    tab, pos = idx.table(), idx.positions()
+    rb = api.prepare(r)
+    band = min(16, bw)  # ChainParams::band, clamped to the SW band (src/mapper.cpp:9)
+    cost = np.array([0] + [int(0.01 * k * dd + 0.5 * (dd.bit_length() - 1)) for dd in range(1, band + 1)], np.int32)
+    m = np.array([k, w, mp["max_occ"], bw, mp["max_cand"], mp["min_chain"], 50, 5000, band, 2], np.int32)
+    st = np.zeros(3, np.uint64)
+    v = C.c_void_p
+    emu.emu_seed_chain.restype = C.c_long
+    d = emu.emu_seed_chain(v(rb.codes.ctypes.data), v(rb.offs.ctypes.data), v(rb.lens.ctypes.data),
+                           C.c_uint64(len(r)), v(tab.ctypes.data), C.c_uint64(len(tab)), v(pos.ctypes.data),
+                           v(g.contig_off.ctypes.data), g.n_contigs, v(m.ctypes.data), C.c_double(0.5),
+                           v(cost.ctypes.data), cap, maxmin, v(st.ctypes.data))
+    return d, int(st[0])
+
+
+@pytest.mark.parametrize("name", ["uniform4m_150bp_10k", "edge_twins_bw16", "edge_repeats_params",
+                                  "edge_twins_sr_bw5", "edge_uniform_linear_bw21"])
+@pytest.mark.parametrize("cap,maxmin", [(32, 32), (64, 56), (32, 24)])
+def test_emulated_fused_seed_chain_matches_global_path(emu, name, cap, maxmin):
+    """The fused one-read-per-thread program (register van Herk sketch, LDS minimizer list,
+    batched probes, LDS chain DP and extraction) gives every read the candidates of the
+    global-memory program, or declines the read (it then takes the global path)."""
+    spec = sets.SETS[name]
+    g = sets.genome(spec)
+    r = sets.reads(spec, g)
+    differ, declined = _emu_seed_chain(emu, spec, g, r, cap, maxmin)
+    assert differ == 0
+    assert declined < len(r)
