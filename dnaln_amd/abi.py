@@ -48,7 +48,7 @@ class MapStats(C.Structure):
 
 class BatchConfig(C.Structure):
     _fields_ = [("mode", i32), ("chunk_reads", u32), ("max_reads", u64), ("max_bases", u64),
-                ("anchors_per_read", u32), ("tune", i32)]
+                ("anchors_per_read", u32), ("tune", i32), ("len_split", i32 * 2), ("dir_budget", u64)]
 
 
 TASK_DTYPE = np.dtype([("q_off", "<u8"), ("t_off", "<u8"), ("qlen", "<i4"), ("tlen", "<i4")])
@@ -58,7 +58,7 @@ RECORD_DTYPE = np.dtype([("contig", "<i4"), ("rev", "u1"), ("mapq", "u1"), ("res
                          ("score", "<i4"), ("nm", "<i4"), ("n_cand", "<i4"), ("cigar_len", "<u4"),
                          ("cigar_off", "<u8")])
 assert TASK_DTYPE.itemsize == 24 and RESULT_DTYPE.itemsize == 40 and RECORD_DTYPE.itemsize == 40
-assert C.sizeof(MapParams) == 80 and C.sizeof(BatchConfig) == 32 and C.sizeof(MapStats) == 104
+assert C.sizeof(MapParams) == 80 and C.sizeof(BatchConfig) == 48 and C.sizeof(MapStats) == 104
 
 # Every symbol include/dnaln_hip.h declares, with (restype, argtypes).
 HIP_SYMBOLS = {

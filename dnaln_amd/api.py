@@ -175,18 +175,21 @@ class Context:
         return out, cig, stats
 
     def batch(self, mode: int = abi.MODE_PLAIN, chunk_reads: int = 65536, max_reads: int = 0, max_bases: int = 0,
-              tune: bool = False) -> "Batch":
-        return Batch(self, mode, chunk_reads, max_reads, max_bases, tune)
+              tune: bool = False, len_split: tuple[int, int] = (0, 0), dir_budget: int = 0) -> "Batch":
+        return Batch(self, mode, chunk_reads, max_reads, max_bases, tune, len_split, dir_budget)
 
 
 class Batch:
     """A device-resident batch (dnaln_hip_batch_*): upload once, run many times, download."""
 
-    def __init__(self, ctx: Context, mode: int, chunk_reads: int, max_reads: int, max_bases: int, tune: bool):
+    def __init__(self, ctx: Context, mode: int, chunk_reads: int, max_reads: int, max_bases: int, tune: bool,
+                 len_split: tuple[int, int] = (0, 0), dir_budget: int = 0):
         self.ctx = ctx
         cfg = abi.BatchConfig()
         _lib.hip().dnaln_hip_default_batch_config(C.byref(cfg))
         cfg.mode, cfg.chunk_reads, cfg.max_reads, cfg.max_bases, cfg.tune = mode, chunk_reads, max_reads, max_bases, int(tune)
+        cfg.len_split[0], cfg.len_split[1] = len_split
+        cfg.dir_budget = dir_budget
         self.cfg = cfg
         self.b = C.c_void_p()
         _check(_lib.hip().dnaln_hip_batch_create(ctx.ctx, C.byref(cfg), C.byref(self.b)), "dnaln_hip_batch_create")

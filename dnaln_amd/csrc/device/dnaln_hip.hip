@@ -77,6 +77,19 @@ struct DevBuf {
   }
 };
 
+// Grow-only device buffers of run_sw, and the host copy of the per-wave slot offsets.
+struct SwWork {
+  DevBuf dir, rows, keys, keys2, ids, ids2, woff, tmp;
+  std::vector<uint64_t> woff_h;
+  uint64_t dir_budget = 1ull << 30;  // bytes of direction words per launch
+};
+
+// Layouts shared with other languages' bindings.
+static_assert(sizeof(dnaln_map_params) == 80, "dnaln_map_params layout");
+static_assert(sizeof(dnaln_batch_config) == 48, "dnaln_batch_config layout");
+static_assert(sizeof(dnaln_map_stats) == 104, "dnaln_map_stats layout");
+static_assert(sizeof(dnaln_record) == 40 && sizeof(dnaln_result) == 40, "record/result layout");
+
 constexpr int kSeedBlock = 128;
 constexpr int kFusedBlock = 64;  // one wave per block: LDS is the occupancy limit
 // Guard bytes before and after read and task pools (the packed kernel loads aligned dwords
@@ -251,7 +264,7 @@ __global__ __launch_bounds__(kSwBlock) void k_sw_packed(SwArgs a) {
   const uint32_t idb = vb ? (a.order ? a.order[kb] : kb) : 0u;
   int ql = max(va ? a.tasks[ida].qlen : 0, vb ? a.tasks[idb].qlen : 0);
   ql = wave_max(ql);
-  uint64_t* slot = TB ? a.dir + (uint64_t)wave * a.dir_slot_words : nullptr;
+  uint64_t* slot = TB ? dir_slot(a, wave) : nullptr;
   sw_pair<W, TB>(a, ida, idb, va, vb, ql, slot, lane);
 }
 
@@ -337,7 +350,8 @@ struct dnaln_hip_ctx {
   uint64_t ref_len = 0, mask = 0, n_pos = 0;
   int nc = 0, k = 0, w = 0, lut_len = 0;
   // workspace of dnaln_hip_align
-  DevBuf a_pool, a_in, a_tasks, a_res, a_cig, a_dir, a_rows;
+  DevBuf a_pool, a_in, a_tasks, a_res, a_cig;
+  SwWork a_sw;
   dnaln_hip_batch* batch = nullptr;  // used by dnaln_hip_map_batch
 };
 
@@ -354,41 +368,128 @@ int validate(const dnaln_map_params& p) {
   return DNALN_OK;
 }
 
-uint64_t sw_dir_bytes_per_task(int W, int qmax) { return (uint64_t)qmax * (uint64_t)((W + 7) / 8) * 4; }
 
 // Runs a score-only or traceback fill over tasks[0, n) (device arrays), results in res[n],
 // reversed CIGARs in cig[n * cap]. Chunks the traceback launches so the direction scratch stays
 // under `dir_budget` bytes.
-int run_sw(const ScoreP& sp, const SwTask* tasks, uint32_t n, int qmax, bool traceback, SwResult* res,
-           uint32_t* cig, int cap, DevBuf& dir, DevBuf& rows, hipStream_t st, uint64_t* launches) {
+__global__ void k_iota(uint32_t* out, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (uint32_t)i;
+}
+
+__global__ void k_len_keys(const SwTask* tasks, uint32_t n, uint32_t* keys, uint32_t* ids) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  keys[i] = (uint32_t)tasks[i].qlen;
+  ids[i] = i;
+}
+
+// Slot words of every fill wave, from its longest query (the first of its 128 tasks, which are
+// sorted longest first); entry `waves` is 0 so that the exclusive scan ends with the total.
+__global__ void k_wave_words(const uint32_t* sorted_len, uint32_t waves, uint64_t row_words, uint64_t* words) {
+  const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w > waves) return;
+  words[w] = w < waves ? (uint64_t)max(sorted_len[(uint64_t)w * 128], 1u) * row_words : 0;
+}
+
+int scan_u64(DevBuf& tmp, const uint64_t* in, uint64_t* out, uint32_t n, hipStream_t st) {
+  size_t bytes = 0;
+  DN_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, n, st));
+  DN_TRY(tmp.reserve(bytes));
+  DN_HIP(hipcub::DeviceScan::ExclusiveSum(tmp.p, bytes, in, out, n, st));
+  return DNALN_OK;
+}
+
+// Fills (and with traceback walks) n tasks. Query lengths lie in [qmin, qmax]. When they differ,
+// the tasks are sorted longest first on the device, so each wave of 128 runs to nearly its own
+// tasks' length, and each wave's direction slot is sized by its longest query. Launches are cut
+// at wave boundaries so that one launch's direction words fit wk.dir_budget.
+int run_sw(const ScoreP& sp, const SwTask* tasks, uint32_t n, int qmin, int qmax, bool traceback, SwResult* res,
+           uint32_t* cig, int cap, SwWork& wk, hipStream_t st, uint64_t* launches) {
   if (n == 0) return DNALN_OK;
   const int W = 2 * sp.bw + 1;
   const bool packed = packed_bw_available(sp.bw) && packed_ok(sp, qmax);
-  const uint64_t budget = 1ull << 30;
+  const uint64_t budget = wk.dir_budget;
   if (packed) {
     SwArgs a{};
     a.sp = sp;
     score_table(sp, a.tab0, a.tab1);
     a.cigar_cap = cap;
-    const uint64_t slot_words = (uint64_t)std::max(qmax, 1) * ((W + 7) / 8) * 64;
-    uint32_t per_launch = n;
-    if (traceback) {
-      const uint64_t per_wave = slot_words * 8;
-      const uint64_t waves = std::max<uint64_t>(1, budget / per_wave);
-      per_launch = (uint32_t)std::min<uint64_t>(n, waves * 128);
-      DN_TRY(dir.reserve(((per_launch + 127) / 128) * per_wave));
+    const uint64_t row_words = (uint64_t)((W + 7) / 8) * 64;
+    const uint64_t slot_words = (uint64_t)std::max(qmax, 1) * row_words;
+    const uint32_t waves = (n + 127) / 128;
+    const uint32_t* order = nullptr;
+    const uint64_t* woff = nullptr;
+    if (qmax > qmin && n > 128) {
+      DN_TRY(wk.keys.reserve((uint64_t)n * 4));
+      DN_TRY(wk.keys2.reserve((uint64_t)n * 4));
+      DN_TRY(wk.ids.reserve((uint64_t)n * 4));
+      DN_TRY(wk.ids2.reserve((uint64_t)n * 4));
+      hipLaunchKernelGGL(k_len_keys, dim3((n + 255) / 256), dim3(256), 0, st, tasks, n, wk.keys.as<uint32_t>(),
+                         wk.ids.as<uint32_t>());
+      DN_HIP(hipGetLastError());
+      const int bits = 32 - __builtin_clz((unsigned)qmax);
+      size_t bytes = 0;
+      DN_HIP(hipcub::DeviceRadixSort::SortPairsDescending(nullptr, bytes, wk.keys.as<uint32_t>(), wk.keys2.as<uint32_t>(),
+                                                          wk.ids.as<uint32_t>(), wk.ids2.as<uint32_t>(), n, 0, bits, st));
+      DN_TRY(wk.tmp.reserve(bytes));
+      DN_HIP(hipcub::DeviceRadixSort::SortPairsDescending(wk.tmp.p, bytes, wk.keys.as<uint32_t>(), wk.keys2.as<uint32_t>(),
+                                                          wk.ids.as<uint32_t>(), wk.ids2.as<uint32_t>(), n, 0, bits, st));
+      order = wk.ids2.as<uint32_t>();
+      if (traceback) {
+        DN_TRY(wk.woff.reserve((uint64_t)(waves + 1) * 16));
+        uint64_t* words = wk.woff.as<uint64_t>();
+        uint64_t* offs = words + (waves + 1);
+        hipLaunchKernelGGL(k_wave_words, dim3((waves + 256) / 256), dim3(256), 0, st, wk.keys2.as<uint32_t>(), waves,
+                           row_words, words);
+        DN_HIP(hipGetLastError());
+        DN_TRY(scan_u64(wk.tmp, words, offs, waves + 1, st));
+        wk.woff_h.resize(waves + 1);
+        DN_HIP(hipMemcpyAsync(wk.woff_h.data(), offs, (waves + 1) * 8, hipMemcpyDeviceToHost, st));
+        DN_HIP(hipStreamSynchronize(st));
+        woff = offs;
+      }
     }
-    for (uint32_t t0 = 0; t0 < n; t0 += per_launch) {
-      const uint32_t m = std::min(per_launch, n - t0);
-      a.tasks = tasks + t0;
-      a.order = nullptr;
+    // Launch cuts [w0, w1) in waves.
+    const uint64_t budget_words = std::max<uint64_t>(budget / 8, 1);
+    std::vector<std::pair<uint32_t, uint32_t>> cuts;
+    uint64_t dir_words = 0;
+    if (!traceback) {
+      cuts.emplace_back(0, waves);
+    } else if (woff) {
+      for (uint32_t w0 = 0; w0 < waves;) {
+        uint32_t w1 = w0 + 1;
+        while (w1 < waves && wk.woff_h[w1 + 1] - wk.woff_h[w0] <= budget_words) ++w1;
+        cuts.emplace_back(w0, w1);
+        dir_words = std::max(dir_words, wk.woff_h[w1] - wk.woff_h[w0]);
+        w0 = w1;
+      }
+    } else {
+      const uint32_t per = (uint32_t)std::max<uint64_t>(1, budget_words / slot_words);
+      for (uint32_t w0 = 0; w0 < waves; w0 += per) cuts.emplace_back(w0, std::min(waves, w0 + per));
+      dir_words = (uint64_t)std::min(waves, per) * slot_words;
+    }
+    if (traceback) DN_TRY(wk.dir.reserve(dir_words * 8));
+    for (const auto& c : cuts) {
+      const uint32_t t0 = c.first * 128, m = std::min<uint64_t>(n, (uint64_t)c.second * 128) - t0;
+      if (order) {
+        a.tasks = tasks;
+        a.order = order + t0;
+        a.res = res;
+        a.cigar = cig;
+      } else {
+        a.tasks = tasks + t0;
+        a.order = nullptr;
+        a.res = res + t0;
+        a.cigar = cig + (uint64_t)t0 * cap;
+      }
       a.n = m;
-      a.res = res + t0;
-      a.cigar = cig + (uint64_t)t0 * cap;
-      a.dir = traceback ? dir.as<uint64_t>() : nullptr;
+      a.dir = traceback ? wk.dir.as<uint64_t>() : nullptr;
       a.dir_slot_words = slot_words;
-      const unsigned waves = (m + 127) / 128;
-      const unsigned grid = (waves + kSwBlock / 64 - 1) / (kSwBlock / 64);
+      a.woff = woff ? woff + c.first : nullptr;
+      a.woff_base = woff ? wk.woff_h[c.first] : 0;
+      const unsigned lw = (m + 127) / 128;
+      const unsigned grid = (lw + kSwBlock / 64 - 1) / (kSwBlock / 64);
       const bool ok = traceback ? launch_packed_bw<true>(sp.bw, a, grid, st) : launch_packed_bw<false>(sp.bw, a, grid, st);
       if (!ok) return fail(DNALN_E_INTERNAL, "no packed kernel for this band width");
       DN_HIP(hipGetLastError());
@@ -406,8 +507,8 @@ int run_sw(const ScoreP& sp, const SwTask* tasks, uint32_t n, int qmax, bool tra
     const uint64_t stride = traceback ? (uint64_t)std::max(qmax, 1) * W : 0;
     uint32_t per_launch = n;
     if (traceback) per_launch = (uint32_t)std::min<uint64_t>(n, std::max<uint64_t>(256, budget / std::max<uint64_t>(stride, 1)));
-    DN_TRY(rows.reserve((uint64_t)per_launch * 2 * (W + 1) * sizeof(int32_t)));
-    if (traceback) DN_TRY(dir.reserve((uint64_t)per_launch * stride));
+    DN_TRY(wk.rows.reserve((uint64_t)per_launch * 2 * (W + 1) * sizeof(int32_t)));
+    if (traceback) DN_TRY(wk.dir.reserve((uint64_t)per_launch * stride));
     for (uint32_t t0 = 0; t0 < n; t0 += per_launch) {
       const uint32_t m = std::min(per_launch, n - t0);
       g.tasks = tasks + t0;
@@ -415,8 +516,8 @@ int run_sw(const ScoreP& sp, const SwTask* tasks, uint32_t n, int qmax, bool tra
       g.n = m;
       g.res = res + t0;
       g.cigar = cig + (uint64_t)t0 * cap;
-      g.rows = rows.as<int32_t>();
-      g.dir = traceback ? dir.as<uint8_t>() : nullptr;
+      g.rows = wk.rows.as<int32_t>();
+      g.dir = traceback ? wk.dir.as<uint8_t>() : nullptr;
       g.dir_stride = stride;
       hipLaunchKernelGGL(k_sw_generic, dim3((m + 127) / 128), dim3(128), 0, st, g);
       DN_HIP(hipGetLastError());
@@ -445,11 +546,15 @@ struct dnaln_hip_batch {
   // inputs
   DevBuf codes, offs, lens, fnv;
   uint64_t n = 0, bases = 0;
-  int maxlen = 0;
+  int minlen = 0, maxlen = 0;
+  int split[2] = {0, 0};             // resolved len_split
+  uint64_t n_short = 0, n_mid = 0;   // reads with length <= split[0], <= split[1]
+  DevBuf lids, lkeys, lorder, ltmp;  // length-sorted read order
   // workspace
   DevBuf acnt, aoff, keys, f, prev, order, chains, ring_h, ring_ps;
   DevBuf cands, ncand, rcnt, roff, rtasks, rres, chosen, second, tbflag, tboff, ttasks, tres, tcig;
-  DevBuf dir, rows, scan_tmp, ciglen, cigoff, err, scratch8, ovf;
+  SwWork sw;
+  DevBuf scan_tmp, ciglen, cigoff, err, scratch8, ovf;
   // outputs
   DevBuf rec, pool;
   uint64_t pool_used = 0;
@@ -489,12 +594,6 @@ struct dnaln_hip_batch {
 };
 
 namespace {
-
-int d2h_u64(hipStream_t st, const uint64_t* d, uint64_t& h) {
-  DN_HIP(hipMemcpyAsync(&h, d, sizeof(uint64_t), hipMemcpyDeviceToHost, st));
-  DN_HIP(hipStreamSynchronize(st));
-  return DNALN_OK;
-}
 
 // Total of an exclusive scan: off[n-1] + cnt[n-1].
 template <class T>
@@ -580,18 +679,40 @@ struct FusedShape {
   size_t lds = 0;
 };
 
-// Sized for the longest read of the batch: 32 anchors up to 200 bp, else 64 (the chain
-// extraction's used-mask is 64 bits); the minimizer list holds 0.22 per k-mer plus slack (a
-// random read has 0.18 on average), at most 64. cap == 0: the fused path does not apply.
+// Minimizer-list entries for reads of up to maxlen bases: 0.22 per k-mer plus slack (a random
+// read has 0.18 on average), in steps of 8.
+int fused_list_need(int k, int maxlen) {
+  const int nk = std::max(maxlen - k + 1, 1);
+  return ((nk * 22 / 100 + 2) + 7) / 8 * 8;
+}
+
+// Sized for the longest read of a bucket: 32 anchors up to 200 bp, else 64 (the chain
+// extraction's used-mask is 64 bits); the list holds fused_list_need entries, at most 64.
+// cap == 0: the fused path does not apply.
 FusedShape fused_shape(int k, int w, int maxlen) {
   FusedShape f;
-  const int nk = std::max(maxlen - k + 1, 1);
   f.cap = maxlen <= 200 ? 32 : 64;
-  f.maxmin = std::min(64, std::max(16, ((nk * 22 / 100 + 2) + 7) / 8 * 8));
+  f.maxmin = std::min(64, std::max(16, fused_list_need(k, maxlen)));
   f.mslots = std::max(f.maxmin, (3 * f.cap + 7) / 8);
   f.lds = (size_t)kFusedBlock * 8 * (f.cap + f.mslots);
   if (12 * w > 8 * f.cap || f.lds > 64 * 1024) f.cap = 0;
   return f;
+}
+
+// Default length buckets: the longest reads whose expected minimizer list fits the 32-entry
+// shape (200 bp at most) and the 64-entry shape; 0 where the shape cannot be used.
+void default_splits(int k, int w, int out[2]) {
+  out[0] = out[1] = 0;
+  for (int L = 200; L >= k; --L)
+    if (fused_list_need(k, L) <= 32) {
+      out[0] = fused_shape(k, w, L).cap ? L : 0;
+      break;
+    }
+  for (int L = 4000; L >= k; --L)
+    if (fused_list_need(k, L) <= 64) {
+      out[1] = fused_shape(k, w, L).cap ? L : 0;
+      break;
+    }
 }
 
 int launch_fused(const FusedShape& fs, const SeedArgs& sa, const ChainArgs& ca, uint32_t* ovf, uint32_t* novf,
@@ -611,6 +732,8 @@ int run_chunk(dnaln_hip_batch* b, uint64_t r0, uint32_t n, bool first_chunk) {
   const int max_cand = std::max(p.max_cand, 0);
   const ScoreP sp{p.sc.match, p.sc.mismatch, p.sc.gap_open, p.sc.gap_ext, p.sc.ambig, p.sc.bw};
   const int cap = 2 * b->maxlen + 4;
+  // The batched form sorts SW tasks by length; the plain form runs them in read order.
+  const int qmin = b->cfg.mode == DNALN_MODE_BATCHED ? b->minlen : b->maxlen;
   size_t sp_i = 0;
 
   ReadView rv;
@@ -653,22 +776,61 @@ int run_chunk(dnaln_hip_batch* b, uint64_t r0, uint32_t n, bool first_chunk) {
   ca.cands = b->cands.as<Cand>();
   ca.n_cand = b->ncand.as<int32_t>();
   const unsigned sgrid = (n + kSeedBlock - 1) / kSeedBlock;
-  const FusedShape fs = b->cfg.mode == DNALN_MODE_BATCHED ? fused_shape(c->k, c->w, b->maxlen) : FusedShape{};
-  if (fs.cap && n) {
-    // Batched form: every read from bases to candidates in LDS; reads that do not fit are listed
-    // and take the global-memory path below.
-    DN_TRY(b->ovf.reserve((uint64_t)n * 4 + 8));
-    DN_HIP(hipMemsetAsync(b->ovf.as<uint32_t>() + n, 0, 4, st));
-    sa.n_reads = n;
-    sa.list = nullptr;
+  if (b->cfg.mode == DNALN_MODE_BATCHED && n) {
+    // Batched form (one chunk: the whole batch). Length buckets: reads of up to split[0] and
+    // split[1] bases run the fused LDS kernel in its two shapes, longer ones the global-memory
+    // path, and reads the fused kernel declines are listed for the global path too. With more
+    // than one bucket in use the reads are sorted by length on the device first.
+    if (r0 != 0 || n != b->n) return fail(DNALN_E_INTERNAL, "batched form runs the whole batch as one chunk");
+    const uint64_t na = b->n_short, nb = b->n_mid;
+    const int used = (na > 0) + (nb > na) + (n > nb);
+    const uint32_t* lorder = nullptr;
     DN_TRY(b->span_begin(0, sp_i));
-    DN_TRY(launch_fused(fs, sa, ca, b->ovf.as<uint32_t>(), b->ovf.as<uint32_t>() + n, st));
+    if (used > 1) {
+      DN_TRY(b->lids.reserve((uint64_t)n * 4));
+      DN_TRY(b->lkeys.reserve((uint64_t)n * 4));
+      DN_TRY(b->lorder.reserve((uint64_t)n * 4));
+      hipLaunchKernelGGL(k_iota, dim3((n + 255) / 256), dim3(256), 0, st, b->lids.as<uint32_t>(), n);
+      DN_HIP(hipGetLastError());
+      const int bits = 32 - __builtin_clz((unsigned)std::max(b->maxlen, 1));
+      const uint32_t* lens_u = reinterpret_cast<const uint32_t*>(b->lens.as<int32_t>());
+      size_t bytes = 0;
+      DN_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, lens_u, b->lkeys.as<uint32_t>(), b->lids.as<uint32_t>(),
+                                                b->lorder.as<uint32_t>(), (uint32_t)n, 0, bits, st));
+      DN_TRY(b->ltmp.reserve(bytes));
+      DN_HIP(hipcub::DeviceRadixSort::SortPairs(b->ltmp.p, bytes, lens_u, b->lkeys.as<uint32_t>(), b->lids.as<uint32_t>(),
+                                                b->lorder.as<uint32_t>(), (uint32_t)n, 0, bits, st));
+      lorder = b->lorder.as<uint32_t>();
+    }
+    DN_TRY(b->ovf.reserve((uint64_t)n * 4 + 8));
+    uint32_t* ovf = b->ovf.as<uint32_t>();
+    uint32_t* novf_d = ovf + n;
+    DN_HIP(hipMemsetAsync(novf_d, 0, 4, st));
+    const uint64_t lo[2] = {0, na}, hi[2] = {na, nb};
+    for (int x = 0; x < 2; ++x) {
+      if (hi[x] <= lo[x]) continue;
+      const FusedShape fs = fused_shape(c->k, c->w, std::min(b->split[x], b->maxlen));
+      sa.list = lorder ? lorder + lo[x] : nullptr;
+      sa.n_reads = (uint32_t)(hi[x] - lo[x]);
+      DN_TRY(launch_fused(fs, sa, ca, ovf, novf_d, st));
+    }
     DN_TRY(b->span_end(sp_i));
     uint32_t novf = 0;
-    DN_HIP(hipMemcpyAsync(&novf, b->ovf.as<uint32_t>() + n, 4, hipMemcpyDeviceToHost, st));
+    DN_HIP(hipMemcpyAsync(&novf, novf_d, 4, hipMemcpyDeviceToHost, st));
     DN_HIP(hipStreamSynchronize(st));
-    b->stats.global_path_reads += novf;
-    if (novf) DN_TRY(seed_chain_global(b, sa, ca, b->ovf.as<uint32_t>(), novf, sp_i));
+    const uint32_t* list = ovf;
+    uint32_t nl = novf;
+    if (n > nb) {
+      if (lorder) {
+        DN_HIP(hipMemcpyAsync(ovf + novf, lorder + nb, (n - nb) * 4, hipMemcpyDeviceToDevice, st));
+        nl += (uint32_t)(n - nb);
+      } else {  // every read is long: the global path in read order
+        list = nullptr;
+        nl = n;
+      }
+    }
+    b->stats.global_path_reads += nl;
+    if (nl) DN_TRY(seed_chain_global(b, sa, ca, list, nl, sp_i));
   } else {
     DN_TRY(seed_chain_global(b, sa, ca, nullptr, n, sp_i));
   }
@@ -686,8 +848,8 @@ int run_chunk(dnaln_hip_batch* b, uint64_t r0, uint32_t n, bool first_chunk) {
     hipLaunchKernelGGL(k_rank_tasks, dim3(sgrid), dim3(kSeedBlock), 0, st, rv, g, b->cands.as<Cand>(),
                        b->ncand.as<int32_t>(), max_cand, b->roff.as<uint64_t>(), b->rtasks.as<SwTask>(), n);
   DN_TRY(b->span_begin(2, sp_i));
-  DN_TRY(run_sw(sp, b->rtasks.as<SwTask>(), (uint32_t)nr, b->maxlen, false, b->rres.as<SwResult>(), nullptr, 0,
-                b->dir, b->rows, st, &b->launches[2]));
+  DN_TRY(run_sw(sp, b->rtasks.as<SwTask>(), (uint32_t)nr, qmin, b->maxlen, false, b->rres.as<SwResult>(),
+                nullptr, 0, b->sw, st, &b->launches[2]));
   DN_TRY(b->span_end(sp_i));
   b->stats.rank_tasks += nr;
 
@@ -710,8 +872,8 @@ int run_chunk(dnaln_hip_batch* b, uint64_t r0, uint32_t n, bool first_chunk) {
     hipLaunchKernelGGL(k_tb_tasks, dim3(sgrid), dim3(kSeedBlock), 0, st, rv, g, b->cands.as<Cand>(), max_cand,
                        b->chosen.as<int32_t>(), b->tboff.as<uint64_t>(), b->ttasks.as<SwTask>(), n);
   DN_TRY(b->span_begin(3, sp_i));
-  DN_TRY(run_sw(sp, b->ttasks.as<SwTask>(), (uint32_t)nt, b->maxlen, true, b->tres.as<SwResult>(),
-                b->tcig.as<uint32_t>(), cap, b->dir, b->rows, st, &b->launches[3]));
+  DN_TRY(run_sw(sp, b->ttasks.as<SwTask>(), (uint32_t)nt, qmin, b->maxlen, true, b->tres.as<SwResult>(),
+                b->tcig.as<uint32_t>(), cap, b->sw, st, &b->launches[3]));
   DN_TRY(b->span_end(sp_i));
   b->stats.tb_tasks += nt;
 
@@ -847,12 +1009,13 @@ int dnaln_hip_align(dnaln_hip_ctx* c, const dnaln_score_params* spp, const dnaln
     return fail(DNALN_E_INVALID, "score parameters outside the validated range");
   const ScoreP sp{s.match, s.mismatch, s.gap_open, s.gap_ext, s.ambig, s.bw};
   DN_HIP(hipSetDevice(c->device));
-  int qmax = 0;
+  int qmax = 0, qmin = n ? tasks[0].qlen : 0;
   for (uint64_t i = 0; i < n; ++i) {
     if (tasks[i].qlen < 0 || tasks[i].tlen < 0 || tasks[i].q_off + (uint64_t)tasks[i].qlen > pool_len ||
         tasks[i].t_off + (uint64_t)tasks[i].tlen > pool_len)
       return fail(DNALN_E_INVALID, "task outside the sequence pool");
     qmax = std::max(qmax, tasks[i].qlen);
+    qmin = std::min(qmin, tasks[i].qlen);
   }
   if (n == 0) return DNALN_OK;
   const int cap = 2 * qmax + 4;
@@ -868,8 +1031,8 @@ int dnaln_hip_align(dnaln_hip_ctx* c, const dnaln_score_params* spp, const dnaln
                      (uint64_t)(uintptr_t)(c->a_pool.as<uint8_t>() + kReadGuard), sp.bw, c->a_tasks.as<SwTask>(),
                      (uint32_t)n);
   DN_HIP(hipGetLastError());
-  DN_TRY(run_sw(sp, c->a_tasks.as<SwTask>(), (uint32_t)n, qmax, traceback != 0, c->a_res.as<SwResult>(),
-                c->a_cig.as<uint32_t>(), cap, c->a_dir, c->a_rows, st, nullptr));
+  DN_TRY(run_sw(sp, c->a_tasks.as<SwTask>(), (uint32_t)n, qmin, qmax, traceback != 0, c->a_res.as<SwResult>(),
+                c->a_cig.as<uint32_t>(), cap, c->a_sw, st, nullptr));
   std::vector<SwResult> res(n);
   DN_HIP(hipMemcpyAsync(res.data(), c->a_res.p, n * sizeof(SwResult), hipMemcpyDeviceToHost, st));
   std::vector<uint32_t> cig;
@@ -910,6 +1073,8 @@ void dnaln_hip_default_batch_config(dnaln_batch_config* cfg) {
   cfg->max_bases = 0;
   cfg->anchors_per_read = 64;
   cfg->tune = 0;
+  cfg->len_split[0] = cfg->len_split[1] = 0;
+  cfg->dir_budget = 0;
 }
 
 int dnaln_hip_batch_create(dnaln_hip_ctx* c, const dnaln_batch_config* cfg, dnaln_hip_batch** out) {
@@ -919,6 +1084,7 @@ int dnaln_hip_batch_create(dnaln_hip_ctx* c, const dnaln_batch_config* cfg, dnal
   if (cfg) b->cfg = *cfg;
   else dnaln_hip_default_batch_config(&b->cfg);
   if (b->cfg.chunk_reads == 0) b->cfg.chunk_reads = 65536;
+  if (b->cfg.dir_budget) b->sw.dir_budget = std::max<uint64_t>(b->cfg.dir_budget, 1 << 20);
   DN_HIP(hipSetDevice(c->device));
   DN_HIP(hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking));
   DN_HIP(hipEventCreate(&b->done));
@@ -954,10 +1120,11 @@ int dnaln_hip_batch_upload(dnaln_hip_batch* b, const uint8_t* codes, const uint6
   DN_HIP(hipSetDevice(b->ctx->device));
   b->last_rc = DNALN_OK;
   uint64_t bases = 0;
-  int maxlen = 0;
+  int minlen = n ? lens[0] : 0, maxlen = 0;
   for (uint64_t i = 0; i < n; ++i) {
     if (lens[i] < 0) return fail(DNALN_E_INVALID, "negative read length");
     bases = std::max(bases, offs[i] + (uint64_t)lens[i]);
+    minlen = std::min(minlen, lens[i]);
     maxlen = std::max(maxlen, lens[i]);
   }
   DN_HIP(hipStreamSynchronize(b->st));
@@ -974,7 +1141,20 @@ int dnaln_hip_batch_upload(dnaln_hip_batch* b, const uint8_t* codes, const uint6
   }
   b->n = n;
   b->bases = bases;
+  b->minlen = minlen;
   b->maxlen = maxlen;
+  int split[2];
+  default_splits(b->ctx->k, b->ctx->w, split);
+  for (int x = 0; x < 2; ++x) {
+    const int want = b->cfg.len_split[x];
+    b->split[x] = want < 0 ? 0 : (want > 0 && fused_shape(b->ctx->k, b->ctx->w, want).cap ? want : split[x]);
+  }
+  b->split[1] = std::max(b->split[1], b->split[0]);
+  b->n_short = b->n_mid = 0;
+  for (uint64_t i = 0; i < n; ++i) {
+    b->n_short += lens[i] <= b->split[0];
+    b->n_mid += lens[i] <= b->split[1];
+  }
   return DNALN_OK;
 }
 

@@ -57,11 +57,19 @@ struct SwArgs {
   SwResult* res;           // indexed by task id
   uint32_t* cigar;         // reversed CIGAR ops, slot of cigar_cap per task id
   int32_t cigar_cap;
-  uint64_t* dir;           // direction words: one slot of dir_slot_words per wave
-  uint64_t dir_slot_words;
+  uint64_t* dir;           // direction words, one slot per wave of the launch
+  uint64_t dir_slot_words; // words per slot when woff is null
+  const uint64_t* woff;    // per-wave slot offsets in words (absolute; null: wave * dir_slot_words)
+  uint64_t woff_base;      // woff of the launch's first wave, which sits at dir[0]
   ScoreP sp;
   uint32_t tab0, tab1;
 };
+
+// Direction slot of launch-relative wave `wave`: fixed-size slots, or (tasks sorted by length)
+// slots sized by each wave's longest query.
+DN_HD uint64_t* dir_slot(const SwArgs& a, uint32_t wave) {
+  return a.dir + (a.woff ? a.woff[wave] - a.woff_base : (uint64_t)wave * a.dir_slot_words);
+}
 
 DN_HD uint8_t q_at(const SwTask& t, int i) {
   if (i < 0 || i >= t.qlen) return 4;
@@ -436,7 +444,7 @@ DN_HD void walk_task(const SwArgs& a, uint32_t k) {
   }
   const SwTask tk = a.tasks[id];
   const uint32_t wave = k >> 7, lane = k & 63, half = (k >> 6) & 1;
-  const uint64_t* slot_lane = a.dir + (uint64_t)wave * a.dir_slot_words + lane;
+  const uint64_t* slot_lane = dir_slot(a, wave) + lane;
   uint32_t* out = a.cigar + (uint64_t)id * (uint64_t)a.cigar_cap;
   Walk w = walk_init(i_end, d_end, score);
   WalkSeq seq;

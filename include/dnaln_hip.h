@@ -158,6 +158,10 @@ typedef struct {
   uint64_t max_bases;
   uint32_t anchors_per_read;/* batched form: anchor capacity per read before the overflow pass */
   int32_t tune;             /* batched form: 1 = run the autotuner on the first batch */
+  int32_t len_split[2];     /* batched form: read-length limits of the two LDS shapes of the fused
+                               seed+chain kernel; longer reads take the global-memory path
+                               (0: the largest lengths the shapes hold on average; -1: bucket off) */
+  uint64_t dir_budget;      /* bytes of traceback direction words per SW launch (0: 1 GiB) */
 } dnaln_batch_config;
 
 void dnaln_hip_default_batch_config(dnaln_batch_config* c);

@@ -48,7 +48,7 @@ def test_emulated_kernels_match_reference_vectors(emu, si, force):
     args = [kv[f"s{si}_{k}"] for k in ("pool", "qoff", "qlen", "toff", "tlen")]
     for tb in (0, 1):
         rc, res, cig, cn = _run(emu, sp, *args, tb, force)
-        if force == 0 and int(sp[5]) in (5, 8, 16, 21, 24, 32):
+        if force == 0 and int(sp[5]) in (5, 8, 16, 21, 24, 32, 64):
             assert rc == 1  # the packed kernel ran
         assert np.array_equal(res, kv[f"s{si}_tb{tb}_res"])
         assert np.array_equal(cn, kv[f"s{si}_tb{tb}_cn"])
