@@ -133,8 +133,9 @@ def main() -> int:
     ctx = api.Context(g, bw=w.bw)
     W = 2 * w.bw + 1
 
-    plain = time_form(ctx, rb, abi.MODE_PLAIN, a.plain_steps, 1)
-    print(f"[bench] plain: {plain['reads_per_s']:.0f} reads/s", file=sys.stderr, flush=True)
+    plain = time_form(ctx, rb, abi.MODE_PLAIN, a.plain_steps, 1) if a.plain_steps > 0 else None
+    if plain:
+        print(f"[bench] plain: {plain['reads_per_s']:.0f} reads/s", file=sys.stderr, flush=True)
     batched = time_form(ctx, rb, abi.MODE_BATCHED, a.steps, a.warmup)
     print(f"[bench] batched: {batched['reads_per_s']:.0f} reads/s", file=sys.stderr, flush=True)
 
@@ -145,8 +146,9 @@ def main() -> int:
     tb_launches = max(batched["launches"][3], 1)
 
     parity = {"records": len(recs), "mapped": int((recs["contig"] >= 0).sum())}
-    same = all(np.array_equal(plain["recs"][k], recs[k]) for k in ("contig", "pos", "rev", "mapq", "score", "nm"))
-    parity["plain_equals_batched"] = bool(same)
+    if plain:
+        same = all(np.array_equal(plain["recs"][k], recs[k]) for k in ("contig", "pos", "rev", "mapq", "score", "nm"))
+        parity["plain_equals_batched"] = bool(same)
     if not a.no_parity:
         dg = json.loads((ROOT / "tests" / "golden" / "chunk_digests.json").read_text()).get(w.name) \
             if (ROOT / "tests" / "golden" / "chunk_digests.json").exists() else None
@@ -159,7 +161,7 @@ def main() -> int:
     # Roofline of the dominant kernel: the traceback fill (packed int16 VALU), time from HIP events.
     tb_ops = tb_cells * OPS_PER_CELL["traceback"]
     achieved = tb_ops / (tb_ms * 1e-3) if tb_ms > 0 else 0.0
-    roofline = {"bound": "valu-int16", "kernel": "k_sw_packed<33,traceback> (fill + walk)",
+    roofline = {"bound": "valu-int16", "kernel": f"k_sw_packed<{W},traceback> + k_walk<{W}>",
                 "achieved": achieved / 1e12, "peak": P16_PEAK / 1e12, "unit": "Tops/s (packed int16)",
                 "frac": achieved / P16_PEAK, "traffic": None,
                 "ops_per_launch": tb_ops / tb_launches, "ms_per_launch": tb_ms / tb_launches,
@@ -174,13 +176,17 @@ def main() -> int:
         "vs_baseline": None, "dtype": "int16", "data": "synthetic (seeded generator, reads resident in HBM)",
         "config": {"workload": w.name, "reads": len(reads), "read_len": f"{w.len_a}-{w.len_b}", "genome_bp": int(len(g.codes)),
                    "bw": w.bw, "form": "batched", "parallelism": "1 GPU"},
-        "plain": {"reads_per_s": plain["reads_per_s"], "ms_per_step": plain["ms_per_step"], "chunk_reads": 65536},
-        "batched_over_plain": batched["reads_per_s"] / plain["reads_per_s"],
+        "plain": {"reads_per_s": plain["reads_per_s"], "ms_per_step": plain["ms_per_step"], "chunk_reads": 65536}
+        if plain else None,
+        "batched_over_plain": batched["reads_per_s"] / plain["reads_per_s"] if plain else None,
         "gcups": (rank_cells + tb_cells) / ((rank_ms + tb_ms) * 1e-3) / 1e9 if rank_ms + tb_ms > 0 else None,
         "gcups_per_step_wall": (rank_cells + tb_cells) / (batched["ms_per_step"] * 1e-3) / 1e9,
         "cells": {"rank": rank_cells, "traceback": tb_cells},
         "stage_ms": dict(zip(["seed", "chain", "rank_fill", "traceback_fill_walk", "run"], fam)),
-        "plain_stage_ms": dict(zip(["seed", "chain", "rank_fill", "traceback_fill_walk", "run"], plain["family_ms"])),
+        "plain_stage_ms": dict(zip(["seed", "chain", "rank_fill", "traceback_fill_walk", "run"], plain["family_ms"]))
+        if plain else None,
+        "batched_stats": {k: int(getattr(batched["stats"], k)) for k in
+                          ("anchors", "candidates", "rank_tasks", "tb_tasks", "global_path_reads")},
         "parity": parity, "roofline": roofline,
     }
     if not a.no_cpu:

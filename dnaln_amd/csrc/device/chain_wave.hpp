@@ -1,0 +1,207 @@
+// chain_wave.hpp — chaining with one wave per read (the batched form's chain stage for reads
+// that leave the fused LDS kernel: long reads and reads with many anchors).
+//
+// Same results as chain_read (src/chain.cpp:12-71, src/mapper.cpp:44-62), organised for a wave:
+//   * the read's anchors are sorted with a wave-wide bitonic network whose comparators all put
+//     the smaller key first, so a virtual +inf pad beyond n never moves into range;
+//   * DP: lane l holds anchor i-1-l (its key and f) in registers and scores it as predecessor
+//     of anchor i. The reference's scan stops at the first j that changes strand or lies more
+//     than max_gap behind; both conditions are monotone in j on sorted anchors, so each lane
+//     tests its own j. A DPP max-reduction over (score << 6 | 63 - l) picks the best score and,
+//     on ties, the nearest j (smallest l), as the reference's strict > scan from j = i-1 down
+//     does. The window then shifts one lane (DPP wave_shr) and lane 0 takes anchor i;
+//   * extraction walks ends in (f desc, index asc) order, a second bitonic sort of packed
+//     keys; chains go to a list that is sorted (score desc, extraction order asc) the same way;
+//   * candidates: lane x keeps accepted candidate x's diagonal, so the dedupe test is a ballot.
+// Every lane writes the same value wherever all lanes later read (f, prev, used), so each lane
+// reads its own writes. Lookback <= 64 and max_cand <= 64 (checked by the host).
+#pragma once
+
+#include "seed_chain.hpp"
+
+namespace dn {
+
+// Makes a wave's earlier LDS and global writes visible to its later reads.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// Maximum over the 64 lanes (all lanes active), returned to every lane.
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+  v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, true));  // row_shr:1
+  v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, true));  // row_shr:2
+  v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, true));  // row_shr:4
+  v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, true));  // row_shr:8
+  v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false)); // row_bcast:15
+  v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false)); // row_bcast:31
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+}
+
+// Lane l receives lane l-1's value; lane 0 receives 0.
+__device__ __forceinline__ uint32_t wave_shr1(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x138, 0xf, 0xf, false);
+}
+
+// Ascending sort of a[0..n) by the wave (the 64 lanes take the comparators of each step).
+template <class T>
+__device__ __forceinline__ void wave_sort(T* a, int n, int lane) {
+  int N = 1;
+  while (N < n) N <<= 1;
+  for (int k = 2; k <= N; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = lane; i < N; i += 64) {
+        const int p = j == (k >> 1) ? (i ^ (k - 1)) : (i ^ j);
+        if (p > i && p < n) {
+          const T x = a[i], y = a[p];
+          if (y < x) {
+            a[i] = y;
+            a[p] = x;
+          }
+        }
+      }
+      wave_sync();
+    }
+  }
+}
+
+// Per-read working arrays (LDS, or global scratch for reads too large for LDS).
+struct WaveChainMem {
+  uint64_t* keys;     // n anchors
+  uint64_t* order;    // n sort keys
+  int32_t* f;         // n
+  int32_t* prev;      // n
+  int32_t* chains;    // chain_cap x {score, rpos | strand << 31, dmin, dmax}
+  uint8_t* used;      // n
+};
+
+__device__ __forceinline__ void chain_wave_read(const ChainArgs& a, uint32_t r, int n, const WaveChainMem& s,
+                                                const int32_t* cost, int lane) {
+  Cand* cands = a.cands + (uint64_t)r * (uint64_t)a.max_cand;
+  if (n == 0) {
+    if (lane == 0) a.n_cand[r] = 0;
+    return;
+  }
+  // 1. anchors in (strand, rpos, qpos) order
+  wave_sort(s.keys, n, lane);
+
+  // 2. DP (src/chain.cpp:21-45)
+  const int k = a.k;
+  const int64_t max_gap = a.max_gap, band = a.band;
+  constexpr int kBias = 1 << 20;
+  uint64_t kw = 0;  // key of anchor i-1-lane
+  int fw = 0;       // f of anchor i-1-lane
+  uint64_t knext = s.keys[0];
+  for (int i = 0; i < n; ++i) {
+    const uint64_t ki = knext;
+    if (i + 1 < n) knext = s.keys[i + 1];
+    const uint32_t si = key_strand(ki);
+    const int64_t ri = key_rpos(ki), qi = key_qpos(ki);
+    const int j = i - 1 - lane;
+    uint32_t cand = 0;
+    if (lane < a.lookback && j >= 0) {
+      const int64_t dr = ri - key_rpos(kw);
+      const int64_t dq = qi - key_qpos(kw);
+      const int64_t dd = dr > dq ? dr - dq : dq - dr;
+      const bool alive = key_strand(kw) == si && dr <= max_gap;
+      if (alive && dr > 0 && dq > 0 && dd <= band) {
+        const int64_t m = dq < dr ? dq : dr;
+        const int sc = fw + (int)(m < k ? m : k) - cost[dd];
+        cand = (uint32_t)(sc + kBias) << 6 | (uint32_t)(63 - lane);
+      }
+    }
+    const uint32_t best = wave_max_u32(cand);
+    int fi = k, pi = -1;
+    if (best) {
+      const int sc = (int)(best >> 6) - kBias;
+      if (sc > k) {
+        fi = sc;
+        pi = i - 1 - (63 - (int)(best & 63));
+      }
+    }
+    s.f[i] = fi;
+    s.prev[i] = pi;
+    const uint32_t klo = wave_shr1((uint32_t)kw), khi = wave_shr1((uint32_t)(kw >> 32));
+    const uint32_t fs = wave_shr1((uint32_t)fw);
+    kw = lane == 0 ? ki : ((uint64_t)khi << 32 | klo);
+    fw = lane == 0 ? fi : (int)fs;
+  }
+  wave_sync();
+
+  // 3. extraction order: f desc, index asc (src/chain.cpp:47-52)
+  for (int x = lane; x < n; x += 64) {
+    s.order[x] = (uint64_t)(uint32_t)(0x7fffffff - s.f[x]) << 32 | (uint32_t)x;
+    s.used[x] = 0;
+  }
+  wave_sync();
+  wave_sort(s.order, n, lane);
+
+  // 4. chains (src/chain.cpp:53-68), lane-uniform walks
+  int m = 0;
+  for (int o = 0; o < n; ++o) {
+    const int end = (int)(uint32_t)s.order[o];
+    if (s.used[end]) continue;
+    int i = end, cnt = 0;
+    int64_t dmin = INT64_MAX, dmax = INT64_MIN;
+    while (i >= 0 && !s.used[i]) {
+      s.used[i] = 1;
+      ++cnt;
+      const uint64_t kk = s.keys[i];
+      const int64_t d = (int64_t)key_rpos(kk) - key_qpos(kk);
+      dmin = d < dmin ? d : dmin;
+      dmax = d > dmax ? d : dmax;
+      i = s.prev[i];
+    }
+    const int score = s.f[end] - (i >= 0 ? s.f[i] : 0);
+    if (score < a.min_score || cnt < a.min_anchors) continue;
+    const uint64_t ke = s.keys[end];
+    int32_t* c = s.chains + 4 * m;
+    c[0] = score;
+    c[1] = (int32_t)(key_rpos(ke) | key_strand(ke) << 31);
+    c[2] = (int32_t)dmin;
+    c[3] = (int32_t)dmax;
+    ++m;
+  }
+  wave_sync();
+  if (m == 0) {
+    if (lane == 0) a.n_cand[r] = 0;
+    return;
+  }
+
+  // 5. chains by score desc, extraction order asc (the reference's stable sort)
+  for (int x = lane; x < m; x += 64)
+    s.order[x] = (uint64_t)(uint32_t)(0x7fffffff - s.chains[4 * x]) << 32 | (uint32_t)x;
+  wave_sync();
+  wave_sort(s.order, m, lane);
+
+  // 6. candidates (src/mapper.cpp:44-62)
+  const double threshold = s.chains[4 * (int)(uint32_t)s.order[0]] * a.pri_ratio;
+  const int64_t bw = a.bw;
+  int nc = 0;
+  int64_t my_diag = 0;   // lane x < nc: candidate x
+  uint32_t my_strand = 2;
+  for (int o = 0; o < m; ++o) {
+    const int32_t* c = s.chains + 4 * (int)(uint32_t)s.order[o];
+    if (c[0] < threshold || nc >= a.max_cand) break;
+    const uint32_t strand = (uint32_t)c[1] >> 31;
+    const int64_t diag = (int64_t)c[2] + ((int64_t)c[3] - c[2]) / 2;
+    const int64_t dd = my_diag - diag;
+    const bool dup_here = lane < nc && my_strand == strand && (dd < 0 ? -dd : dd) <= bw;
+    if (__builtin_amdgcn_ballot_w64(dup_here)) continue;
+    if (lane == nc) {
+      my_diag = diag;
+      my_strand = strand;
+      Cand cd;
+      cd.rev = strand;
+      cd.contig = contig_of(a.contig_off, a.n_contigs, (uint32_t)c[1] & 0x7fffffffu);
+      cd.diag = diag;
+      cd.wstart = diag - bw;
+      cands[nc] = cd;
+    }
+    ++nc;
+  }
+  if (lane == 0) a.n_cand[r] = nc;
+}
+
+}  // namespace dn

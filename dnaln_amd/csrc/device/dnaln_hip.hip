@@ -23,6 +23,7 @@
 #include "sw_generic.hpp"
 #include "sw_packed.hpp"
 #include "fused_seed_chain.hpp"
+#include "chain_wave.hpp"
 
 using namespace dn;
 
@@ -144,6 +145,49 @@ __global__ __launch_bounds__(kSeedBlock) void k_chain(ChainArgs a) {
   const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= a.n_reads) return;
   chain_read(a, r);
+}
+
+// One wave per read (chain_wave.hpp), four waves per block. LDS: the gap-cost table, then per
+// wave kWaveChainLds anchors' keys, sort keys, f, prev, chains and used flags. Reads with more
+// anchors, or min_anchors < 2 (more chains than the LDS list holds), work in the global
+// scratch arrays of the chain stage instead.
+constexpr int kWaveChainLds = 256;
+constexpr size_t kWaveChainBytes = (size_t)33 * kWaveChainLds;
+
+size_t wave_chain_lds_bytes(int band) { return (size_t)((band + 4) & ~3) * 4 + 4 * kWaveChainBytes; }
+
+__global__ __launch_bounds__(256) void k_chain_wave(ChainArgs a) {
+  extern __shared__ uint64_t lds[];
+  int32_t* cost = reinterpret_cast<int32_t*>(lds);
+  for (int x = threadIdx.x; x <= a.band; x += blockDim.x) cost[x] = a.cost[x];
+  __syncthreads();
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const uint32_t li = blockIdx.x * 4 + wv;
+  if (li >= a.n_reads) return;
+  const uint32_t r = read_id(a.list, li);
+  const int n = (int)a.anchor_count[li];
+  const uint64_t off = a.anchor_off[li];
+  WaveChainMem s;
+  if (n <= kWaveChainLds && a.min_anchors >= 2) {
+    uint8_t* base = reinterpret_cast<uint8_t*>(cost + ((a.band + 4) & ~3)) + wv * kWaveChainBytes;
+    s.keys = reinterpret_cast<uint64_t*>(base);
+    s.order = s.keys + kWaveChainLds;
+    s.f = reinterpret_cast<int32_t*>(s.order + kWaveChainLds);
+    s.prev = s.f + kWaveChainLds;
+    s.chains = s.prev + kWaveChainLds;
+    s.used = reinterpret_cast<uint8_t*>(s.chains + 2 * kWaveChainLds);
+    for (int x = lane; x < n; x += 64) s.keys[x] = a.keys[off + x];
+    wave_sync();
+    chain_wave_read(a, r, n, s, cost, lane);
+  } else {
+    s.keys = a.keys + off;
+    s.order = a.order + off;
+    s.f = a.f + off;
+    s.prev = a.prev + off;
+    s.chains = reinterpret_cast<int32_t*>(a.chains + off);
+    s.used = reinterpret_cast<uint8_t*>(a.chains + off) + (size_t)16 * n;
+    chain_wave_read(a, r, n, s, cost, lane);
+  }
 }
 
 
@@ -613,8 +657,11 @@ int scan_total(hipStream_t st, const uint64_t* off, const T* cnt, uint32_t n, ui
 
 // Global-memory seed + chain path for reads list[0, nl) (list null: reads 0 .. nl-1): count,
 // scan, emit anchors, chain. No limit on anchors per read.
+// Seeds and chains the listed reads (all reads when list is null) through global memory: count,
+// scan, emit, chain. wave_chain: one wave per read in the chain step (batched form), else one
+// thread per read.
 int seed_chain_global(dnaln_hip_batch* b, SeedArgs sa, ChainArgs ca, const uint32_t* list, uint32_t nl,
-                      size_t& sp_i) {
+                      size_t& sp_i, bool wave_chain = false) {
   dnaln_hip_ctx* c = b->ctx;
   hipStream_t st = b->st;
   DN_TRY(b->acnt.reserve((uint64_t)nl * 4 + 4));
@@ -665,7 +712,11 @@ int seed_chain_global(dnaln_hip_batch* b, SeedArgs sa, ChainArgs ca, const uint3
   ca.chains = b->chains.as<ChainRec>();
   DN_TRY(b->span_begin(1, sp_i));
   if (nl) {
-    hipLaunchKernelGGL(k_chain, dim3(sgrid), dim3(kSeedBlock), 0, st, ca);
+    if (wave_chain && ca.lookback <= 64 && ca.max_cand <= 64) {
+      hipLaunchKernelGGL(k_chain_wave, dim3((nl + 3) / 4), dim3(256), wave_chain_lds_bytes(ca.band), st, ca);
+    } else {
+      hipLaunchKernelGGL(k_chain, dim3(sgrid), dim3(kSeedBlock), 0, st, ca);
+    }
     DN_HIP(hipGetLastError());
   }
   DN_TRY(b->span_end(sp_i));
@@ -830,7 +881,7 @@ int run_chunk(dnaln_hip_batch* b, uint64_t r0, uint32_t n, bool first_chunk) {
       }
     }
     b->stats.global_path_reads += nl;
-    if (nl) DN_TRY(seed_chain_global(b, sa, ca, list, nl, sp_i));
+    if (nl) DN_TRY(seed_chain_global(b, sa, ca, list, nl, sp_i, true));
   } else {
     DN_TRY(seed_chain_global(b, sa, ca, nullptr, n, sp_i));
   }
