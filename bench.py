@@ -50,9 +50,10 @@ def chunk_digests(recs, pool, names: list[str], contig_names, chunk: int) -> lis
             for i in range(0, len(lines), chunk)]
 
 
-def time_form(ctx, rb, mode: int, steps: int, warmup: int) -> dict:
-    b = ctx.batch(mode=mode, chunk_reads=65536)
+def time_form(ctx, rb, mode: int, steps: int, warmup: int, tune: bool = False, **kw) -> dict:
+    b = ctx.batch(mode=mode, chunk_reads=65536, **kw)
     b.upload(rb)
+    report = b.tune() if tune else None
     b.run()
     b.poll()
     for _ in range(warmup):
@@ -71,7 +72,7 @@ def time_form(ctx, rb, mode: int, steps: int, warmup: int) -> dict:
     recs, pool, stats = b.download()
     b.close()
     return {"ms_per_step": 1e3 * dt / steps, "reads_per_s": len(rb) * steps / dt, "family_ms": (fam / steps).tolist(),
-            "launches": (launches / steps).tolist(), "recs": recs, "pool": pool, "stats": stats}
+            "launches": (launches / steps).tolist(), "recs": recs, "pool": pool, "stats": stats, "tune": report}
 
 
 def cpu_baseline(w: synth.Workload, g: synth.Genome, reads: synth.Reads, per_proc: int) -> dict | None:
@@ -118,6 +119,9 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-parity", action="store_true")
     ap.add_argument("--cpu-reads-per-proc", type=int, default=60000)
+    ap.add_argument("--len-split", default="0,0", help="batched form: read-length bucket limits (0 default, -1 off)")
+    ap.add_argument("--dir-budget", type=int, default=0, help="bytes of traceback directions per SW launch")
+    ap.add_argument("--no-tune", action="store_true", help="batched form: skip the autotuner (it runs before timing)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported (reads shard "
@@ -136,7 +140,9 @@ def main() -> int:
     plain = time_form(ctx, rb, abi.MODE_PLAIN, a.plain_steps, 1) if a.plain_steps > 0 else None
     if plain:
         print(f"[bench] plain: {plain['reads_per_s']:.0f} reads/s", file=sys.stderr, flush=True)
-    batched = time_form(ctx, rb, abi.MODE_BATCHED, a.steps, a.warmup)
+    split = tuple(int(x) for x in a.len_split.split(","))
+    batched = time_form(ctx, rb, abi.MODE_BATCHED, a.steps, a.warmup, tune=not a.no_tune and split == (0, 0)
+                        and not a.dir_budget, len_split=split, dir_budget=a.dir_budget)
     print(f"[bench] batched: {batched['reads_per_s']:.0f} reads/s", file=sys.stderr, flush=True)
 
     recs, pool = batched["recs"], batched["pool"]
@@ -185,6 +191,9 @@ def main() -> int:
         "stage_ms": dict(zip(["seed", "chain", "rank_fill", "traceback_fill_walk", "run"], fam)),
         "plain_stage_ms": dict(zip(["seed", "chain", "rank_fill", "traceback_fill_walk", "run"], plain["family_ms"]))
         if plain else None,
+        "autotune": {"len_split": batched["tune"]["len_split"], "dir_budget": batched["tune"]["dir_budget"],
+                     "tried": [(t["len_split"], t["dir_budget"] >> 20, round(t["ms"], 2)) for t in batched["tune"]["tried"]]}
+        if batched["tune"] else None,
         "batched_stats": {k: int(getattr(batched["stats"], k)) for k in
                           ("anchors", "candidates", "rank_tasks", "tb_tasks", "global_path_reads")},
         "parity": parity, "roofline": roofline,

@@ -51,6 +51,11 @@ class BatchConfig(C.Structure):
                 ("anchors_per_read", u32), ("tune", i32), ("len_split", i32 * 2), ("dir_budget", u64)]
 
 
+class TuneReport(C.Structure):
+    _fields_ = [("len_split", i32 * 2), ("dir_budget", u64), ("n_tried", i32), ("chosen", i32),
+                ("ms", dbl * 16), ("cand_split", (i32 * 2) * 16), ("cand_budget", u64 * 16)]
+
+
 TASK_DTYPE = np.dtype([("q_off", "<u8"), ("t_off", "<u8"), ("qlen", "<i4"), ("tlen", "<i4")])
 RESULT_DTYPE = np.dtype([("score", "<i4"), ("qb", "<i4"), ("qe", "<i4"), ("tb", "<i4"), ("te", "<i4"), ("nm", "<i4"),
                          ("cigar_off", "<u8"), ("cigar_len", "<u4"), ("reserved", "<u4")])
@@ -77,6 +82,7 @@ HIP_SYMBOLS = {
     "dnaln_hip_batch_download": (i32, [vp, vp, vp, u64, vp]),
     "dnaln_hip_batch_device_bytes": (u64, [vp, vp]),
     "dnaln_hip_batch_timings": (i32, [vp, vp, vp]),
+    "dnaln_hip_batch_tune": (i32, [vp, vp]),
 }
 
 HOST_SYMBOLS = {

@@ -230,6 +230,15 @@ class Batch:
         _check(_lib.hip().dnaln_hip_batch_timings(self.b, ms, ln), "dnaln_hip_batch_timings")
         return list(ms), list(ln)
 
+    def tune(self) -> dict:
+        """Runs the autotuner on the resident batch (dnaln_hip_batch_tune) and returns its report."""
+        r = abi.TuneReport()
+        _check(_lib.hip().dnaln_hip_batch_tune(self.b, C.byref(r)), "dnaln_hip_batch_tune")
+        tried = [{"len_split": (r.cand_split[i][0], r.cand_split[i][1]), "dir_budget": int(r.cand_budget[i]),
+                  "ms": r.ms[i]} for i in range(r.n_tried)]
+        return {"len_split": (r.len_split[0], r.len_split[1]), "dir_budget": int(r.dir_budget), "chosen": r.chosen,
+                "tried": tried}
+
     def close(self) -> None:
         if self.b:
             _lib.hip().dnaln_hip_batch_destroy(self.b)

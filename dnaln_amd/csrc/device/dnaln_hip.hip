@@ -78,6 +78,17 @@ struct DevBuf {
   }
 };
 
+// Traceback directions per SW launch: enough fill waves to occupy every SIMD several times at
+// W = 129 and 1000-bp reads (8.7 MB per wave), within an eighth of the free HBM.
+uint64_t default_dir_budget() {
+  size_t free_b = 0, total_b = 0;
+  if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
+    (void)hipGetLastError();
+    return 1ull << 30;
+  }
+  return std::min<uint64_t>(16ull << 30, std::max<uint64_t>(1ull << 30, free_b / 8));
+}
+
 // Grow-only device buffers of run_sw, and the host copy of the per-wave slot offsets.
 struct SwWork {
   DevBuf dir, rows, keys, keys2, ids, ids2, woff, tmp;
@@ -593,6 +604,8 @@ struct dnaln_hip_batch {
   int minlen = 0, maxlen = 0;
   int split[2] = {0, 0};             // resolved len_split
   uint64_t n_short = 0, n_mid = 0;   // reads with length <= split[0], <= split[1]
+  std::vector<int32_t> lens_h;       // host copy of the read lengths (bucket counts)
+  bool tuned = false;
   DevBuf lids, lkeys, lorder, ltmp;  // length-sorted read order
   // workspace
   DevBuf acnt, aoff, keys, f, prev, order, chains, ring_h, ring_ps;
@@ -1135,8 +1148,8 @@ int dnaln_hip_batch_create(dnaln_hip_ctx* c, const dnaln_batch_config* cfg, dnal
   if (cfg) b->cfg = *cfg;
   else dnaln_hip_default_batch_config(&b->cfg);
   if (b->cfg.chunk_reads == 0) b->cfg.chunk_reads = 65536;
-  if (b->cfg.dir_budget) b->sw.dir_budget = std::max<uint64_t>(b->cfg.dir_budget, 1 << 20);
   DN_HIP(hipSetDevice(c->device));
+  b->sw.dir_budget = b->cfg.dir_budget ? std::max<uint64_t>(b->cfg.dir_budget, 1 << 20) : default_dir_budget();
   DN_HIP(hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking));
   DN_HIP(hipEventCreate(&b->done));
   DN_TRY(b->err.reserve(64));
@@ -1162,6 +1175,22 @@ void dnaln_hip_batch_destroy(dnaln_hip_batch* b) {
   if (b->done) (void)hipEventDestroy(b->done);
   if (b->st) (void)hipStreamDestroy(b->st);
   delete b;
+}
+
+// Resolves requested bucket limits (0: default, -1: bucket unused) and counts the reads of
+// each bucket.
+static void set_splits(dnaln_hip_batch* b, int want0, int want1) {
+  int def[2];
+  default_splits(b->ctx->k, b->ctx->w, def);
+  const int want[2] = {want0, want1};
+  for (int x = 0; x < 2; ++x)
+    b->split[x] = want[x] < 0 ? 0 : (want[x] > 0 && fused_shape(b->ctx->k, b->ctx->w, want[x]).cap ? want[x] : def[x]);
+  b->split[1] = std::max(b->split[1], b->split[0]);
+  b->n_short = b->n_mid = 0;
+  for (const int32_t l : b->lens_h) {
+    b->n_short += l <= b->split[0];
+    b->n_mid += l <= b->split[1];
+  }
 }
 
 int dnaln_hip_batch_upload(dnaln_hip_batch* b, const uint8_t* codes, const uint64_t* offs, const int32_t* lens,
@@ -1194,23 +1223,14 @@ int dnaln_hip_batch_upload(dnaln_hip_batch* b, const uint8_t* codes, const uint6
   b->bases = bases;
   b->minlen = minlen;
   b->maxlen = maxlen;
-  int split[2];
-  default_splits(b->ctx->k, b->ctx->w, split);
-  for (int x = 0; x < 2; ++x) {
-    const int want = b->cfg.len_split[x];
-    b->split[x] = want < 0 ? 0 : (want > 0 && fused_shape(b->ctx->k, b->ctx->w, want).cap ? want : split[x]);
-  }
-  b->split[1] = std::max(b->split[1], b->split[0]);
-  b->n_short = b->n_mid = 0;
-  for (uint64_t i = 0; i < n; ++i) {
-    b->n_short += lens[i] <= b->split[0];
-    b->n_mid += lens[i] <= b->split[1];
-  }
+  b->lens_h.assign(lens, lens + n);
+  set_splits(b, b->cfg.len_split[0], b->cfg.len_split[1]);
   return DNALN_OK;
 }
 
 int dnaln_hip_batch_run(dnaln_hip_batch* b) {
   if (!b) return fail(DNALN_E_INVALID, "null batch");
+  if (b->cfg.tune && !b->tuned && b->cfg.mode == DNALN_MODE_BATCHED && b->n) DN_TRY(dnaln_hip_batch_tune(b, nullptr));
   dnaln_hip_ctx* c = b->ctx;
   DN_HIP(hipSetDevice(c->device));
   std::memset(&b->stats, 0, sizeof b->stats);
@@ -1237,6 +1257,58 @@ int dnaln_hip_batch_run(dnaln_hip_batch* b) {
   DN_HIP(hipEventRecord(b->done, b->st));
   b->stats.reads = b->n;
   b->pending = true;
+  return DNALN_OK;
+}
+
+int dnaln_hip_batch_tune(dnaln_hip_batch* b, dnaln_tune_report* rep) {
+  if (!b) return fail(DNALN_E_INVALID, "null batch");
+  if (b->cfg.mode != DNALN_MODE_BATCHED) return fail(DNALN_E_INVALID, "the autotuner applies to the batched form");
+  b->tuned = true;
+  const uint64_t big = default_dir_budget();
+  std::vector<uint64_t> budgets{big};
+  if (big != (1ull << 30)) budgets.push_back(1ull << 30);
+  std::vector<std::pair<std::pair<int, int>, uint64_t>> cand;
+  for (const auto& sp : {std::make_pair(0, 0), std::make_pair(0, -1), std::make_pair(-1, 0), std::make_pair(-1, -1)})
+    for (const uint64_t bud : budgets) cand.push_back({sp, bud});
+  dnaln_tune_report r{};
+  double best = 1e300;
+  int best_i = 0;
+  std::vector<std::pair<int, int>> seen;
+  for (size_t i = 0; i < cand.size() && r.n_tried < 16; ++i) {
+    set_splits(b, cand[i].first.first, cand[i].first.second);
+    b->sw.dir_budget = cand[i].second;
+    const std::pair<int, int> eff{b->n_short > 0 ? b->split[0] : 0, b->n_mid > b->n_short ? b->split[1] : 0};
+    bool dup = false;
+    for (size_t x = 0; x < seen.size(); ++x)
+      dup = dup || (seen[x] == eff && r.cand_budget[x] == cand[i].second);
+    if (dup) continue;
+    seen.push_back(eff);
+    double ms = 0;
+    for (int rep_i = 0; rep_i < 2; ++rep_i) {  // the first run grows the workspace
+      DN_TRY(dnaln_hip_batch_run(b));
+      DN_TRY(dnaln_hip_batch_poll(b, 1));
+      ms = b->ms[4];
+    }
+    const int t = r.n_tried++;
+    r.ms[t] = ms;
+    r.cand_split[t][0] = eff.first;
+    r.cand_split[t][1] = eff.second;
+    r.cand_budget[t] = cand[i].second;
+    if (ms < best) {
+      best = ms;
+      best_i = t;
+    }
+  }
+  r.chosen = best_i;
+  r.len_split[0] = r.cand_split[best_i][0];
+  r.len_split[1] = r.cand_split[best_i][1];
+  r.dir_budget = r.cand_budget[best_i];
+  b->cfg.len_split[0] = r.len_split[0] > 0 ? r.len_split[0] : -1;
+  b->cfg.len_split[1] = r.len_split[1] > 0 ? r.len_split[1] : -1;
+  b->cfg.dir_budget = r.dir_budget;
+  set_splits(b, b->cfg.len_split[0], b->cfg.len_split[1]);
+  b->sw.dir_budget = r.dir_budget;
+  if (rep) *rep = r;
   return DNALN_OK;
 }
 

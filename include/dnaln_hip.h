@@ -186,6 +186,23 @@ uint64_t dnaln_hip_batch_device_bytes(struct dnaln_hip_ctx* ctx, const dnaln_bat
  * launches5 (may be NULL) receives the number of fill kernel launches per family. */
 int dnaln_hip_batch_timings(struct dnaln_hip_batch* b, double* ms5, uint64_t* launches5);
 
+/* Autotuner of the batched form (no counterpart in the reference, whose CPU path has no such
+ * choices). Runs the resident batch under each candidate setting and keeps the fastest:
+ * read-length bucket limits (which reads chain one per lane in the fused LDS kernel, in which
+ * LDS shape, and which one per wave through global memory) and the traceback direction budget
+ * per SW launch. Every candidate gives the same records; only the time differs. */
+typedef struct {
+  int32_t len_split[2];     /* chosen bucket limits (0 = bucket unused) */
+  uint64_t dir_budget;      /* chosen direction budget, bytes */
+  int32_t n_tried;
+  int32_t chosen;           /* index into ms[] */
+  double ms[16];            /* whole-run time of each candidate, in the order tried */
+  int32_t cand_split[16][2];
+  uint64_t cand_budget[16];
+} dnaln_tune_report;
+
+int dnaln_hip_batch_tune(struct dnaln_hip_batch* b, dnaln_tune_report* report);
+
 #ifdef __cplusplus
 }
 #endif

@@ -92,3 +92,36 @@ def test_map_empty_batch():
     r = sets.reads(spec, g).subset(0, 0)
     recs, pool, stats = _context(spec).map(r)
     assert len(recs) == 0 and stats.reads == 0
+
+
+@pytest.mark.parametrize("name", ["edge_uniform_bw16", "edge_repeats_params", "edge_twins_sr_bw5"])
+@pytest.mark.parametrize("split", [(0, 0), (0, -1), (-1, 0), (-1, -1), (60, 120)])
+def test_every_length_bucketing_gives_reference_records(name, split):
+    """Whichever reads chain one per lane in the fused LDS kernel (and in which shape) and which
+    one per wave through global memory, the records are the reference's."""
+    spec = sets.SETS[name]
+    g = sets.genome(spec)
+    r = sets.reads(spec, g)
+    print(f"set {name} split {split}", flush=True)
+    b = _context(spec).batch(mode=abi.MODE_BATCHED, len_split=split, dir_budget=1 << 24)
+    b.upload(api.prepare(r))
+    b.run()
+    recs, pool, _ = b.download()
+    b.close()
+    _compare(name, recs, pool, g)
+
+
+def test_autotuned_batch_gives_reference_records():
+    spec = sets.SETS["edge_uniform_bw64"]
+    g = sets.genome(spec)
+    r = sets.reads(spec, g)
+    b = _context(spec).batch(mode=abi.MODE_BATCHED)
+    b.upload(api.prepare(r))
+    rep = b.tune()
+    print("tune", rep, flush=True)
+    assert 1 <= len(rep["tried"]) <= 16 and 0 <= rep["chosen"] < len(rep["tried"])
+    assert min(t["ms"] for t in rep["tried"]) == rep["tried"][rep["chosen"]]["ms"]
+    b.run()
+    recs, pool, _ = b.download()
+    b.close()
+    _compare("edge_uniform_bw64", recs, pool, g)
