@@ -82,7 +82,7 @@ def build_host() -> Path:
     tmp = out.with_suffix(".so.tmp")
     hip = LIB / "libdnaln_hip.so"
     link = ["-L", str(LIB), "-ldnaln_hip", f"-Wl,-rpath,$ORIGIN"] if hip.exists() else []
-    _run(["g++", "-shared", "-fPIC", *objs, "-o", tmp, *link, "-lm"])
+    _run(["g++", "-shared", "-fPIC", *objs, "-o", tmp, *link, "-lz", "-lm"])
     os.replace(tmp, out)
     mains = sorted((CSRC / "host").glob("*_main.cpp"))
     if mains and hip.exists():
@@ -90,7 +90,14 @@ def build_host() -> Path:
         for m in mains:
             exe = BIN / ("dnaln-amd" if m.stem == "cli_main" else m.stem.replace("_main", ""))
             _run([*HOST_CXX, m, "-o", exe, "-L", str(LIB), "-ldnaln_host", "-ldnaln_hip",
-                  "-Wl,-rpath,$ORIGIN/../lib", "-lm"])
+                  "-Wl,-rpath,$ORIGIN/../lib", "-lz", "-lm"])
+    # Test drivers of the C++ interface (tests/cpp/*.cpp -> tests/cpp/bin/).
+    checks = sorted((ROOT / "tests" / "cpp").glob("*.cpp"))
+    if checks and hip.exists():
+        (ROOT / "tests" / "cpp" / "bin").mkdir(parents=True, exist_ok=True)
+        for m in checks:
+            _run([*HOST_CXX, m, "-o", ROOT / "tests" / "cpp" / "bin" / m.stem, "-L", str(LIB), "-ldnaln_host",
+                  "-ldnaln_hip", "-Wl,-rpath,$ORIGIN/../../../dnaln_amd/lib", "-lz", "-lm"])
     return out
 
 

@@ -1347,23 +1347,25 @@ int dnaln_hip_batch_download(dnaln_hip_batch* b, dnaln_record* out, uint32_t* ci
   if (b->n) DN_HIP(hipMemcpy(out, b->rec.p, b->n * sizeof(dnaln_record), hipMemcpyDeviceToHost));
   if (b->pool_used) DN_HIP(hipMemcpy(cigar_pool, b->pool.p, b->pool_used * 4, hipMemcpyDeviceToHost));
   if (stats) {
+    // Band cells as the reference counts them (src/mapper.cpp:139, 186): every candidate of a
+    // read with two or more in the rank pass, and the chosen one in the traceback pass.
     dnaln_map_stats s = b->stats;
-    uint64_t mapped = 0, cells = 0;
     const uint64_t W = 2 * (uint64_t)b->ctx->p.sc.bw + 1;
-    uint64_t cand = 0;
+    uint64_t mapped = 0, cand = 0, cells = 0;
     for (uint64_t i = 0; i < b->n; ++i) {
+      const uint64_t nc = (uint64_t)std::max(out[i].n_cand, 0), len = (uint64_t)b->lens_h[i];
       mapped += out[i].contig >= 0;
-      cand += (uint64_t)out[i].n_cand;
+      cand += nc;
+      cells += ((nc >= 2 ? nc : 0) + (nc >= 1 ? 1 : 0)) * len * W;
     }
     s.candidates = cand;
-    (void)cells;
     s.mapped = mapped;
+    s.cells = cells;
     s.ms_seed = b->ms[0];
     s.ms_chain = b->ms[1];
     s.ms_rank = b->ms[2];
     s.ms_traceback = b->ms[3];
     s.ms_total = b->ms[4];
-    (void)W;
     *stats = s;
   }
   return DNALN_OK;

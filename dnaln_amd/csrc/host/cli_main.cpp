@@ -1,0 +1,182 @@
+// cli_main.cpp — `dnaln-amd map`: the reference's `dnaln map` (src/main.cpp:12-75) on one MI355X.
+//
+// Same arguments, defaults, validation and SAM output; reads go to the GPU in batches of
+// --batch reads (larger than the reference's 8192 by default, since a batch fills the GPU).
+#include <chrono>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <set>
+#include <string>
+#include <vector>
+
+#include "host/dnaln_amd.hpp"
+
+using namespace dnaln_amd;
+
+namespace {
+
+[[noreturn]] void die(const std::string& msg) {
+  std::fprintf(stderr, "[dnaln-amd] error: %s\n", msg.c_str());
+  std::exit(1);
+}
+
+void log_msg(const char* fmt, ...) __attribute__((format(printf, 1, 2)));
+void log_msg(const char* fmt, ...) {
+  std::fprintf(stderr, "[dnaln-amd] ");
+  va_list ap;
+  va_start(ap, fmt);
+  std::vfprintf(stderr, fmt, ap);
+  va_end(ap);
+  std::fputc('\n', stderr);
+}
+
+double now() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// Options as src/cli.hpp:12-70 parses them: value options take the next argument, flags
+// stand alone, anything else is positional; unknown options are errors.
+struct Args {
+  std::vector<std::string> pos;
+  std::map<std::string, std::string> opt;
+  Args(int argc, char** argv, const std::set<std::string>& values, const std::set<std::string>& flags) {
+    for (int i = 0; i < argc; ++i) {
+      const std::string a = argv[i];
+      if (a.size() > 1 && a[0] == '-') {
+        if (values.count(a)) {
+          if (i + 1 >= argc) die("option " + a + " needs a value");
+          opt[a] = argv[++i];
+        } else if (flags.count(a)) {
+          opt[a] = "";
+        } else {
+          die("unknown option " + a);
+        }
+      } else {
+        pos.push_back(a);
+      }
+    }
+  }
+  bool has(const std::string& k) const { return opt.count(k) != 0; }
+  long long num(const std::string& k, long long def) const {
+    const auto it = opt.find(k);
+    if (it == opt.end()) return def;
+    char* end = nullptr;
+    const long long v = std::strtoll(it->second.c_str(), &end, 10);
+    if (it->second.empty() || *end) die("option " + k + " expects an integer, got '" + it->second + "'");
+    return v;
+  }
+};
+
+const char* kHelp =
+    "Usage: dnaln-amd map [options] <ref.fa[.gz]> <reads.fq[.gz]>  > out.sam\n\n"
+    "  -o FILE        write SAM to FILE instead of stdout\n"
+    "  --batch INT    reads per GPU batch [1048576]\n"
+    "  -k INT         minimizer k-mer size [15]\n"
+    "  -w INT         minimizer window size [10]\n"
+    "  --bw INT       band half-width of banded Smith-Waterman [16]\n"
+    "  -A/-B INT      match score / mismatch penalty [2/4]\n"
+    "  -O/-E INT      gap open / extension penalty; gap of length L costs O+L*E [4/2]\n"
+    "  --max-occ INT  ignore minimizers with more occurrences [500]\n"
+    "  --max-cand INT candidate loci aligned per read [10]\n"
+    "  --min-chain INT minimum chain score [20]\n"
+    "  --min-score INT minimum alignment score to report [40]\n"
+    "  --scalar, --no-prefetch   accepted for compatibility (no effect on the GPU)\n"
+    "  --plain        plain GPU form (one launch sequence per 65,536-read chunk)\n"
+    "  --tune         run the autotuner on the first batch\n"
+    "  --device INT   HIP device [0]\n";
+
+int cmd_map(int argc, char** argv) {
+  const std::set<std::string> values = {"-k", "-w", "--bw", "-A", "-B", "-O", "-E", "--max-occ", "--min-score",
+                                        "--max-cand", "--min-chain", "-o", "--batch", "--device"};
+  const Args a(argc, argv, values, {"--scalar", "--no-prefetch", "--plain", "--tune", "-h", "--help"});
+  if (a.pos.size() != 2 || a.has("-h") || a.has("--help")) {
+    std::fputs(kHelp, stderr);
+    return a.has("-h") || a.has("--help") ? 0 : 1;
+  }
+  MapParams p;
+  p.k = (int)a.num("-k", p.k);
+  p.w = (int)a.num("-w", p.w);
+  p.sc.bw = (int)a.num("--bw", p.sc.bw);
+  p.sc.match = (int)a.num("-A", p.sc.match);
+  p.sc.mismatch = (int)a.num("-B", p.sc.mismatch);
+  p.sc.gap_open = (int)a.num("-O", p.sc.gap_open);
+  p.sc.gap_ext = (int)a.num("-E", p.sc.gap_ext);
+  p.max_occ = (uint32_t)a.num("--max-occ", p.max_occ);
+  p.max_cand = (int)a.num("--max-cand", p.max_cand);
+  p.ch.min_score = (int)a.num("--min-chain", p.ch.min_score);
+  p.min_aln_score = (int)a.num("--min-score", p.min_aln_score);
+  p.batched = !a.has("--plain");
+  p.tune = a.has("--tune");
+  p.device = (int)a.num("--device", 0);
+  // src/cli.hpp:93-95
+  if (p.sc.bw < 1 || p.sc.bw > 1000) die("--bw must be in [1, 1000]");
+  if (p.sc.match < 1 || p.sc.mismatch < 0 || p.sc.gap_open < 0 || p.sc.gap_ext < 1)
+    die("scores must satisfy A >= 1, B >= 0, O >= 0, E >= 1");
+  const long long batch = a.num("--batch", 1 << 20);
+  if (batch < 1) die("--batch must be positive");
+  std::string cmdline = "dnaln map";
+  for (int i = 0; i < argc; ++i) cmdline += std::string(" ") + argv[i];
+
+  const double t_total = now();
+  double t = now();
+  const Reference ref = Reference::load(a.pos[0]);
+  log_msg("reference: %zu contig(s), %zu bp (loaded in %.2f s)", ref.contigs.size(), ref.seq.size(), now() - t);
+  t = now();
+  MinimizerIndex idx;
+  idx.build(ref, p.k, p.w);
+  log_msg("index: k=%d w=%d, %zu distinct minimizers, %zu positions, %.1f MB, built in %.2f s", p.k, p.w,
+          idx.n_keys(), idx.n_positions(), idx.memory_bytes() / 1048576.0, now() - t);
+
+  FILE* out = stdout;
+  if (a.has("-o")) {
+    out = std::fopen(a.opt.at("-o").c_str(), "w");
+    if (!out) die("cannot open '" + a.opt.at("-o") + "' for writing");
+  }
+  Mapper mapper(ref, idx, p);
+  mapper.write_sam_header(out, cmdline);
+  log_msg("alignment kernel: %s (%s form)", mapper.kernel_name().c_str(), p.batched ? "batched" : "plain");
+
+  SeqReader reader(a.pos[1]);
+  std::vector<SeqRecord> reads;
+  std::vector<MapRecord> recs;
+  std::string buf;
+  const double t_map = now();
+  size_t n;
+  while ((n = reader.next_batch(reads, (size_t)batch)) > 0) {
+    mapper.map(reads.data(), n, recs);
+    buf.clear();
+    for (size_t i = 0; i < n; ++i) mapper.append_sam(reads[i], recs[i], buf);
+    std::fwrite(buf.data(), 1, buf.size(), out);
+  }
+  if (out != stdout) std::fclose(out);
+  else std::fflush(out);
+  const double sec = now() - t_map;
+  const MapStats& st = mapper.stats();
+  log_msg("mapped %llu / %llu reads (%.2f%%) in %.3f s -> %.0f reads/s (excl. index build)",
+          (unsigned long long)st.mapped, (unsigned long long)st.reads, st.reads ? 100.0 * st.mapped / st.reads : 0.0,
+          sec, st.reads / sec);
+  log_msg("device time: seed+chain %.3f s, alignment %.3f s (%.2f GCUPS); total wall %.2f s", st.t_seed, st.t_align,
+          st.t_align > 0 ? st.cells / st.t_align * 1e-9 : 0.0, now() - t_total);
+  return 0;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  if (argc < 2 || !std::strcmp(argv[1], "-h") || !std::strcmp(argv[1], "--help")) {
+    std::fputs("Usage: dnaln-amd map [options] <ref.fa> <reads.fq>\n", stderr);
+    return argc < 2 ? 1 : 0;
+  }
+  if (std::strcmp(argv[1], "map") != 0) {
+    std::fprintf(stderr, "[dnaln-amd] error: unknown command '%s' (this build provides `map`)\n", argv[1]);
+    return 1;
+  }
+  try {
+    return cmd_map(argc - 2, argv + 2);
+  } catch (const Error& e) {
+    die(e.what());
+  }
+}

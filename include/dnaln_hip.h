@@ -112,9 +112,10 @@ typedef struct {
   uint64_t cigar_off;
 } dnaln_record;
 
-/* MapStats (src/mapper.hpp:46-49) plus device timings of the last batch. anchors counts the
- * reads that took the global-memory seed path; global_path_reads counts those of them that the
- * batched form's LDS path handed over. */
+/* MapStats (src/mapper.hpp:46-49) plus device timings of the last batch (ms). cells counts
+ * band cells as the reference does (rank pass + traceback pass). anchors counts only the
+ * anchors of reads on the global-memory seed path (the LDS path keeps its anchors on chip);
+ * global_path_reads counts those reads in the batched form. */
 typedef struct {
   uint64_t reads, mapped, anchors, candidates, cells, rank_tasks, tb_tasks;
   double ms_seed, ms_chain, ms_rank, ms_traceback, ms_total;

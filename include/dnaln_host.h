@@ -4,7 +4,6 @@
  *                       occurrences by (hash, pos), place keys with linear probing at load <= 0.5.
  *                       Produces the view dnaln_hip_create uploads.
  *   dnaln_sam_format    Mapper::append_sam (src/mapper.cpp:231-270) for one record.
- *   dnaln_sam_header    Mapper::write_sam_header (src/mapper.cpp:225-229).
  *   dnaln_fnv1a         fnv1a (src/common.hpp:136-140), the equal-best tie-break hash.
  *   dnaln_encode        encode (src/common.hpp:36-38).
  */
