@@ -59,8 +59,8 @@ def time_form(ctx, rb, mode: int, steps: int, warmup: int, tune: bool = False, *
     for _ in range(warmup):
         b.run()
         b.poll()
-    fam = np.zeros(5)
-    launches = np.zeros(5)
+    fam = np.zeros(len(api.Batch.FAMILIES))
+    launches = np.zeros(len(api.Batch.FAMILIES))
     t0 = time.perf_counter()
     for _ in range(steps):
         b.run()
@@ -108,6 +108,112 @@ def cpu_baseline(w: synth.Workload, g: synth.Genome, reads: synth.Reads, per_pro
             "per_process_reads_per_s": total / procs / (sum(secs) / len(secs))}
 
 
+def copy_calibration(nbytes: int) -> None:
+    """One device-to-device hipMemcpy of nbytes: reads and writes exactly nbytes of HBM."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    src, dst = C.c_void_p(), C.c_void_p()
+    for p in (src, dst):
+        if hip.hipMalloc(C.byref(p), C.c_size_t(nbytes)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    if hip.hipMemset(src, 1, C.c_size_t(nbytes)) != 0 or hip.hipDeviceSynchronize() != 0:
+        raise RuntimeError("hipMemset failed")
+    if hip.hipMemcpy(dst, src, C.c_size_t(nbytes), 3) != 0 or hip.hipDeviceSynchronize() != 0:
+        raise RuntimeError("hipMemcpy failed")
+    hip.hipFree(src)
+    hip.hipFree(dst)
+
+
+def _pmc_rows(path: str) -> list[dict]:
+    import csv
+    with open(path, newline="") as fp:
+        return list(csv.DictReader(fp))
+
+
+def summarize_pmc(fetch_csv: str, write_csv: str, calib: int) -> int:
+    """HBM bytes per launch of the traceback fill kernel from FETCH_SIZE / WRITE_SIZE (KiB per
+    dispatch), each scaled by what the same counter reads for the calibration copy."""
+    out = {}
+    for counter, path in (("FETCH_SIZE", fetch_csv), ("WRITE_SIZE", write_csv)):
+        rows = [r for r in _pmc_rows(path) if r.get("Counter_Name") == counter]
+        per = {}
+        for r in rows:
+            per.setdefault(r["Kernel_Name"], []).append(float(r["Counter_Value"]))
+        copy = [v for k, vs in per.items() if "copyBuffer" in k for v in vs]
+        fill = [v for k, vs in per.items() if k.startswith("void (anonymous namespace)::k_sw_packed<") and ", true>" in k
+                for v in vs]
+        walk = [v for k, vs in per.items() if "k_walk<" in k for v in vs]
+        out[counter] = {"copy_kib": max(copy) if copy else None, "fill_kib_per_launch": sum(fill) / len(fill) if fill else None,
+                        "walk_kib_per_launch": sum(walk) / len(walk) if walk else None, "fill_launches": len(fill)}
+    res = {"counters": out}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        c = out[counter]
+        if c["copy_kib"] and calib:
+            c["scale"] = calib / (c["copy_kib"] * 1024.0)
+    f, w = out["FETCH_SIZE"], out["WRITE_SIZE"]
+    if f.get("scale") and w.get("scale") and f["fill_kib_per_launch"] is not None and w["fill_kib_per_launch"] is not None:
+        res["k_sw_packed_tb_bytes_per_launch"] = f["fill_kib_per_launch"] * 1024 * f["scale"] + \
+            w["fill_kib_per_launch"] * 1024 * w["scale"]
+        res["k_sw_packed_tb_read_bytes"] = f["fill_kib_per_launch"] * 1024 * f["scale"]
+        res["k_sw_packed_tb_write_bytes"] = w["fill_kib_per_launch"] * 1024 * w["scale"]
+        if f["walk_kib_per_launch"] is not None and w["walk_kib_per_launch"] is not None:
+            res["k_walk_bytes_per_launch"] = f["walk_kib_per_launch"] * 1024 * f["scale"] + \
+                w["walk_kib_per_launch"] * 1024 * w["scale"]
+    res["copy_bytes"] = calib
+    (ROOT / "profiles").mkdir(exist_ok=True)
+    (ROOT / "profiles" / "traffic.json").write_text(json.dumps(res, indent=1) + "\n")
+    print(json.dumps(res, indent=1))
+    return 0
+
+
+def serve_sweep(seconds: float, levels: list[float], slots: int) -> dict:
+    """Config 5: the 120 Mbp repeat-genome index resident; Poisson arrivals of requests of 1..4096
+    reads (log-uniform); reads of 100..300 bp drawn from a pool; offered load as a fraction of the
+    batched form's peak reads/s on the same pool. Latency is arrival to completion of a request."""
+    w = synth.WORKLOADS["serve_repeats120m"]
+    t = time.time()
+    g = synth.workload_genome(w)
+    pool = synth.workload_reads(w)
+    rb = api.prepare(pool)
+    ctx = api.Context(g, bw=w.bw)
+    print(f"[bench] serving pool of {len(rb)} reads ready in {time.time() - t:.1f}s", file=sys.stderr, flush=True)
+    b = ctx.batch(mode=abi.MODE_BATCHED)
+    b.upload(rb)
+    b.run()
+    b.poll()
+    t = time.perf_counter()
+    b.run()
+    b.poll()
+    peak = len(rb) / (time.perf_counter() - t)
+    b.close()
+    mean_size = 4096 / np.log(4097.0)
+    srv = api.Server(ctx, slots=slots)
+    rng = np.random.default_rng(123)
+    out = []
+    for i, load in enumerate(levels):
+        sizes, arrive = synth.request_arrivals(load * peak / mean_size, seconds, 1000 + i)
+        starts = np.array([rng.integers(0, len(rb) - s + 1) for s in sizes], np.uint64)
+        st0 = srv.stats()
+        lat, wall, _, _ = srv.run_trace(rb, starts, arrive, req_count=sizes.astype(np.uint64))
+        st = srv.stats()
+        batches = st["batches"] - st0["batches"]
+        row = {"load": load, "requests": int(len(sizes)), "offered_reads_per_s": float(sizes.sum() / seconds),
+               "achieved_reads_per_s": float(sizes.sum() / wall) if wall > 0 else None,
+               "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
+               "batches": int(batches), "mean_reads_per_batch": float((st["reads"] - st0["reads"]) / max(batches, 1))}
+        out.append(row)
+        print(f"[bench] serve load {load:.1f}: {row['requests']} requests, p50 {row['p50_ms']:.2f} ms "
+              f"p99 {row['p99_ms']:.2f} ms, {row['achieved_reads_per_s']:.0f} reads/s", file=sys.stderr, flush=True)
+    st = srv.stats()
+    srv.close()
+    ctx.close()
+    return {"workload": w.name, "genome_bp": int(len(g.codes)), "read_len": "100-300", "bw": w.bw,
+            "request_reads": "1-4096 log-uniform", "arrivals": "Poisson", "seconds_per_level": seconds,
+            "slots": slots, "peak_batched_reads_per_s": peak, "levels": out,
+            "server": {k: st[k] for k in ("batches", "reads", "max_inflight_batches", "max_inflight_bytes",
+                                          "budget_bytes", "deferred_by_budget")}}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -122,7 +228,21 @@ def main() -> int:
     ap.add_argument("--len-split", default="0,0", help="batched form: read-length bucket limits (0 default, -1 off)")
     ap.add_argument("--dir-budget", type=int, default=0, help="bytes of traceback directions per SW launch")
     ap.add_argument("--no-tune", action="store_true", help="batched form: skip the autotuner (it runs before timing)")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving sweep (config 5)")
+    ap.add_argument("--serve-seconds", type=float, default=1.0, help="seconds of arrivals per load level")
+    ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
+                    help="after the run, copy BYTES device-to-device once (a copy kernel of known traffic for "
+                         "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
+    ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="read two rocprofv3 --pmc counter_collection.csv files (FETCH_SIZE, WRITE_SIZE) of a "
+                         "--calibrate-copy BYTES run (pass the same BYTES), write profiles/traffic.json and print a "
+                         "summary; no GPU needed")
     a = ap.parse_args()
+    if a.summarize_pmc:
+        if not a.calibrate_copy:
+            ap.error("--summarize-pmc needs --calibrate-copy BYTES (the size the profiled run copied)")
+        return summarize_pmc(*a.summarize_pmc, a.calibrate_copy)
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported (reads shard "
                                    "trivially across GPUs, but no multi-GPU path is built)"}))
@@ -148,7 +268,7 @@ def main() -> int:
     recs, pool = batched["recs"], batched["pool"]
     rank_cells, tb_cells = cells_from_records(recs, reads.lens, W)
     fam = batched["family_ms"]
-    tb_ms, rank_ms = fam[3], fam[2]
+    tb_ms, walk_ms, rank_ms = fam[3], fam[4], fam[2]
     tb_launches = max(batched["launches"][3], 1)
 
     parity = {"records": len(recs), "mapped": int((recs["contig"] >= 0).sum())}
@@ -164,13 +284,16 @@ def main() -> int:
         else:
             parity["reference_chunk_digests"] = "not available for this read count"
 
-    # Roofline of the dominant kernel: the traceback fill (packed int16 VALU), time from HIP events.
+    # Roofline of the dominant kernel: the traceback fill (packed int16 VALU), timed with HIP
+    # events around its launches on the batch stream.
     tb_ops = tb_cells * OPS_PER_CELL["traceback"]
     achieved = tb_ops / (tb_ms * 1e-3) if tb_ms > 0 else 0.0
-    roofline = {"bound": "valu-int16", "kernel": f"k_sw_packed<{W},traceback> + k_walk<{W}>",
+    roofline = {"bound": "valu-int16", "kernel": f"k_sw_packed<{W}, true>",
                 "achieved": achieved / 1e12, "peak": P16_PEAK / 1e12, "unit": "Tops/s (packed int16)",
                 "frac": achieved / P16_PEAK, "traffic": None,
                 "ops_per_launch": tb_ops / tb_launches, "ms_per_launch": tb_ms / tb_launches,
+                "launches": int(tb_launches), "walk_ms_per_launch": walk_ms / max(batched["launches"][4], 1),
+                "fill_plus_walk_frac": (tb_ops / ((tb_ms + walk_ms) * 1e-3) / P16_PEAK) if tb_ms + walk_ms > 0 else None,
                 "score_fill_frac": (rank_cells * OPS_PER_CELL["score"] / (rank_ms * 1e-3) / P16_PEAK) if rank_ms > 0 else None}
     tpath = ROOT / "profiles" / "traffic.json"
     if tpath.exists():
@@ -185,14 +308,16 @@ def main() -> int:
         "plain": {"reads_per_s": plain["reads_per_s"], "ms_per_step": plain["ms_per_step"], "chunk_reads": 65536}
         if plain else None,
         "batched_over_plain": batched["reads_per_s"] / plain["reads_per_s"] if plain else None,
-        "gcups": (rank_cells + tb_cells) / ((rank_ms + tb_ms) * 1e-3) / 1e9 if rank_ms + tb_ms > 0 else None,
+        "gcups": (rank_cells + tb_cells) / ((rank_ms + tb_ms + walk_ms) * 1e-3) / 1e9 if rank_ms + tb_ms > 0 else None,
         "gcups_per_step_wall": (rank_cells + tb_cells) / (batched["ms_per_step"] * 1e-3) / 1e9,
         "cells": {"rank": rank_cells, "traceback": tb_cells},
-        "stage_ms": dict(zip(["seed", "chain", "rank_fill", "traceback_fill_walk", "run"], fam)),
-        "plain_stage_ms": dict(zip(["seed", "chain", "rank_fill", "traceback_fill_walk", "run"], plain["family_ms"]))
+        "stage_ms": dict(zip(api.Batch.FAMILIES, fam)),
+        "plain_stage_ms": dict(zip(api.Batch.FAMILIES, plain["family_ms"]))
         if plain else None,
         "autotune": {"len_split": batched["tune"]["len_split"], "dir_budget": batched["tune"]["dir_budget"],
-                     "tried": [(t["len_split"], t["dir_budget"] >> 20, round(t["ms"], 2)) for t in batched["tune"]["tried"]]}
+                     "walk_kind": batched["tune"]["walk_kind"],
+                     "tried": [(t["len_split"], t["dir_budget"] >> 20, t["walk_kind"], round(t["ms"], 2))
+                               for t in batched["tune"]["tried"]]}
         if batched["tune"] else None,
         "batched_stats": {k: int(getattr(batched["stats"], k)) for k in
                           ("anchors", "candidates", "rank_tasks", "tb_tasks", "global_path_reads")},
@@ -204,6 +329,10 @@ def main() -> int:
         if cb:
             out["gpu_over_cpu"] = batched["reads_per_s"] / cb["value"]
     ctx.close()
+    if not a.no_serve:
+        out["serving"] = serve_sweep(a.serve_seconds, [round(0.1 * x, 1) for x in range(1, 11)], a.serve_slots)
+    if a.calibrate_copy:
+        copy_calibration(a.calibrate_copy)
     print(json.dumps(out))
     return 0
 

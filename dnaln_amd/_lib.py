@@ -48,5 +48,6 @@ def host() -> C.CDLL:
         lib.synth_write_fastq.restype = i32
         from . import abi
         abi.declare(lib, abi.HOST_SYMBOLS)
+        abi.declare(lib, abi.SERVE_SYMBOLS)
         _host = lib
     return _host

@@ -48,12 +48,14 @@ class MapStats(C.Structure):
 
 class BatchConfig(C.Structure):
     _fields_ = [("mode", i32), ("chunk_reads", u32), ("max_reads", u64), ("max_bases", u64),
-                ("anchors_per_read", u32), ("tune", i32), ("len_split", i32 * 2), ("dir_budget", u64)]
+                ("anchors_per_read", u32), ("tune", i32), ("len_split", i32 * 2), ("dir_budget", u64),
+                ("walk_kind", i32), ("reserved", i32)]
 
 
 class TuneReport(C.Structure):
-    _fields_ = [("len_split", i32 * 2), ("dir_budget", u64), ("n_tried", i32), ("chosen", i32),
-                ("ms", dbl * 16), ("cand_split", (i32 * 2) * 16), ("cand_budget", u64 * 16)]
+    _fields_ = [("len_split", i32 * 2), ("dir_budget", u64), ("walk_kind", i32), ("n_tried", i32), ("chosen", i32),
+                ("reserved", i32), ("ms", dbl * 16), ("cand_split", (i32 * 2) * 16), ("cand_budget", u64 * 16),
+                ("cand_walk", i32 * 16)]
 
 
 TASK_DTYPE = np.dtype([("q_off", "<u8"), ("t_off", "<u8"), ("qlen", "<i4"), ("tlen", "<i4")])
@@ -63,7 +65,7 @@ RECORD_DTYPE = np.dtype([("contig", "<i4"), ("rev", "u1"), ("mapq", "u1"), ("res
                          ("score", "<i4"), ("nm", "<i4"), ("n_cand", "<i4"), ("cigar_len", "<u4"),
                          ("cigar_off", "<u8")])
 assert TASK_DTYPE.itemsize == 24 and RESULT_DTYPE.itemsize == 40 and RECORD_DTYPE.itemsize == 40
-assert C.sizeof(MapParams) == 80 and C.sizeof(BatchConfig) == 48 and C.sizeof(MapStats) == 104
+assert C.sizeof(MapParams) == 80 and C.sizeof(BatchConfig) == 56 and C.sizeof(MapStats) == 104
 
 # Every symbol include/dnaln_hip.h declares, with (restype, argtypes).
 HIP_SYMBOLS = {
@@ -83,6 +85,33 @@ HIP_SYMBOLS = {
     "dnaln_hip_batch_device_bytes": (u64, [vp, vp]),
     "dnaln_hip_batch_timings": (i32, [vp, vp, vp]),
     "dnaln_hip_batch_tune": (i32, [vp, vp]),
+    "dnaln_hip_device_memory": (i32, [vp, vp, vp]),
+    "dnaln_hip_batch_reserve": (i32, [vp, u64, u64, i32, u32]),
+    "dnaln_hip_context_params": (i32, [vp, vp]),
+}
+
+class ServeConfig(C.Structure):
+    _fields_ = [("slots", i32), ("mode", i32), ("hbm_budget", u64), ("max_batch_reads", u64),
+                ("max_batch_bases", u64), ("dir_budget", u64), ("max_read_len", i32), ("reserved", i32)]
+
+
+class ServeStats(C.Structure):
+    _fields_ = [(n, u64) for n in ("submitted", "completed", "batches", "reads", "queued_requests",
+                                   "inflight_batches", "max_inflight_batches", "inflight_bytes",
+                                   "max_inflight_bytes", "budget_bytes", "deferred_by_budget")]
+
+
+SERVE_SYMBOLS = {
+    "dnaln_serve_default_config": (None, [vp]),
+    "dnaln_serve_create": (i32, [vp, vp, vp]),
+    "dnaln_serve_destroy": (None, [vp]),
+    "dnaln_serve_batch_bytes": (u64, [vp, u64, u64, i32]),
+    "dnaln_serve_submit": (i32, [vp, u64, vp, vp, vp, vp, u64]),
+    "dnaln_serve_progress": (i32, [vp]),
+    "dnaln_serve_take": (i32, [vp, vp, vp, u64, vp, u64, vp, vp]),
+    "dnaln_serve_stats_get": (i32, [vp, vp]),
+    "dnaln_serve_last_error": (C.c_char_p, [vp]),
+    "dnaln_serve_run_trace": (i32, [vp, vp, vp, vp, vp, u64, vp, vp, vp, u64, vp, vp, vp, u64, vp]),
 }
 
 HOST_SYMBOLS = {

@@ -175,21 +175,23 @@ class Context:
         return out, cig, stats
 
     def batch(self, mode: int = abi.MODE_PLAIN, chunk_reads: int = 65536, max_reads: int = 0, max_bases: int = 0,
-              tune: bool = False, len_split: tuple[int, int] = (0, 0), dir_budget: int = 0) -> "Batch":
-        return Batch(self, mode, chunk_reads, max_reads, max_bases, tune, len_split, dir_budget)
+              tune: bool = False, len_split: tuple[int, int] = (0, 0), dir_budget: int = 0,
+              walk_kind: int = 0) -> "Batch":
+        return Batch(self, mode, chunk_reads, max_reads, max_bases, tune, len_split, dir_budget, walk_kind)
 
 
 class Batch:
     """A device-resident batch (dnaln_hip_batch_*): upload once, run many times, download."""
 
     def __init__(self, ctx: Context, mode: int, chunk_reads: int, max_reads: int, max_bases: int, tune: bool,
-                 len_split: tuple[int, int] = (0, 0), dir_budget: int = 0):
+                 len_split: tuple[int, int] = (0, 0), dir_budget: int = 0, walk_kind: int = 0):
         self.ctx = ctx
         cfg = abi.BatchConfig()
         _lib.hip().dnaln_hip_default_batch_config(C.byref(cfg))
         cfg.mode, cfg.chunk_reads, cfg.max_reads, cfg.max_bases, cfg.tune = mode, chunk_reads, max_reads, max_bases, int(tune)
         cfg.len_split[0], cfg.len_split[1] = len_split
         cfg.dir_budget = dir_budget
+        cfg.walk_kind = walk_kind
         self.cfg = cfg
         self.b = C.c_void_p()
         _check(_lib.hip().dnaln_hip_batch_create(ctx.ctx, C.byref(cfg), C.byref(self.b)), "dnaln_hip_batch_create")
@@ -223,10 +225,12 @@ class Batch:
                "dnaln_hip_batch_download")
         return out, cig, stats
 
+    FAMILIES = ("seed", "chain", "rank_fill", "traceback_fill", "traceback_walk", "run")
+
     def timings(self) -> tuple[list[float], list[int]]:
-        """(ms, launches) per family: seed, chain, rank fill, traceback fill + walk, whole run."""
-        ms = (C.c_double * 5)()
-        ln = (C.c_uint64 * 5)()
+        """(ms, launches) per family: seed, chain, rank fill, traceback fill, traceback walk, whole run."""
+        ms = (C.c_double * len(self.FAMILIES))()
+        ln = (C.c_uint64 * len(self.FAMILIES))()
         _check(_lib.hip().dnaln_hip_batch_timings(self.b, ms, ln), "dnaln_hip_batch_timings")
         return list(ms), list(ln)
 
@@ -235,9 +239,9 @@ class Batch:
         r = abi.TuneReport()
         _check(_lib.hip().dnaln_hip_batch_tune(self.b, C.byref(r)), "dnaln_hip_batch_tune")
         tried = [{"len_split": (r.cand_split[i][0], r.cand_split[i][1]), "dir_budget": int(r.cand_budget[i]),
-                  "ms": r.ms[i]} for i in range(r.n_tried)]
-        return {"len_split": (r.len_split[0], r.len_split[1]), "dir_budget": int(r.dir_budget), "chosen": r.chosen,
-                "tried": tried}
+                  "walk_kind": r.cand_walk[i], "ms": r.ms[i]} for i in range(r.n_tried)]
+        return {"len_split": (r.len_split[0], r.len_split[1]), "dir_budget": int(r.dir_budget),
+                "walk_kind": r.walk_kind, "chosen": r.chosen, "tried": tried}
 
     def close(self) -> None:
         if self.b:
@@ -271,3 +275,89 @@ def record_lines(names: list[str], recs: np.ndarray, pool: np.ndarray, contig_na
         out.append("\t".join([names[i], str(f[0]), f[1], str(f[2]), str(f[3]), f[4],
                               "*" if f[5] is None else str(f[5]), "*" if f[6] is None else str(f[6])]))
     return out
+
+
+class Server:
+    """Continuous batching on the resident index (dnaln_serve_*): requests are queued, packed into
+    batches as slots and the HBM budget allow, and run concurrently on the slots' streams."""
+
+    def __init__(self, ctx: Context, slots: int = 4, mode: int = abi.MODE_BATCHED, hbm_budget: int = 0,
+                 max_batch_reads: int = 1 << 18, max_batch_bases: int = 1 << 27, dir_budget: int = 0):
+        self.ctx = ctx
+        cfg = abi.ServeConfig()
+        _lib.host().dnaln_serve_default_config(C.byref(cfg))
+        cfg.slots, cfg.mode, cfg.hbm_budget = slots, mode, hbm_budget
+        cfg.max_batch_reads, cfg.max_batch_bases, cfg.dir_budget = max_batch_reads, max_batch_bases, dir_budget
+        self.s = C.c_void_p()
+        _check(_lib.host().dnaln_serve_create(ctx.ctx, C.byref(cfg), C.byref(self.s)), "dnaln_serve_create")
+
+    def submit(self, rid: int, rb: ReadBatch) -> None:
+        _check(_lib.host().dnaln_serve_submit(self.s, rid, _p(rb.codes), _p(rb.offs), _p(rb.lens), _p(rb.fnv), len(rb)),
+               "dnaln_serve_submit")
+
+    def progress(self) -> int:
+        rc = _lib.host().dnaln_serve_progress(self.s)
+        if rc < 0:
+            raise DnalnError(rc, "dnaln_serve_progress: " + _lib.host().dnaln_serve_last_error(self.s).decode())
+        return rc
+
+    def take(self):
+        """(id, records, cigar pool) of one finished request, or None."""
+        lib = _lib.host()
+        rid, n, nc = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        rc = lib.dnaln_serve_take(self.s, C.byref(rid), None, 0, None, 0, C.byref(n), C.byref(nc))
+        if rc == abi.E_PENDING:
+            return None
+        _check(rc, "dnaln_serve_take")
+        recs = np.zeros(n.value, abi.RECORD_DTYPE)
+        pool = np.zeros(max(nc.value, 1), np.uint32)
+        _check(lib.dnaln_serve_take(self.s, C.byref(rid), _p(recs), len(recs), _p(pool), len(pool), C.byref(n),
+                                    C.byref(nc)), "dnaln_serve_take")
+        return rid.value, recs, pool
+
+    def stats(self) -> dict:
+        st = abi.ServeStats()
+        _check(_lib.host().dnaln_serve_stats_get(self.s, C.byref(st)), "dnaln_serve_stats_get")
+        return {k: int(getattr(st, k)) for k, _ in abi.ServeStats._fields_}
+
+    def batch_bytes(self, n: int, bases: int, maxlen: int) -> int:
+        return int(_lib.host().dnaln_serve_batch_bytes(self.s, n, bases, maxlen))
+
+    def run_trace(self, rb: ReadBatch, req_first: np.ndarray, arrival_s: np.ndarray, want_records: bool = False,
+                  req_count: np.ndarray | None = None):
+        """Replays arrivals in real time; returns (latency_s per request, wall_s, records, cigar pool).
+        Request r holds reads [req_first[r], req_first[r] + req_count[r]); without req_count, req_first
+        has one more entry and requests are consecutive ranges."""
+        n_req = len(arrival_s)
+        req_first = np.ascontiguousarray(req_first, np.uint64)
+        if req_count is None:
+            req_count = np.diff(req_first)
+            req_first = req_first[:-1].copy()
+        req_count = np.ascontiguousarray(req_count, np.uint64)
+        arrival_s = np.ascontiguousarray(arrival_s, np.float64)
+        lat = np.zeros(n_req, np.float64)
+        wall = C.c_double()
+        recs = np.zeros(len(rb), abi.RECORD_DTYPE) if want_records else None
+        pool = None
+        if want_records:  # CIGAR slots: 2 * len + 4 words per read of every request
+            csum = np.concatenate([[0], np.cumsum(2 * rb.lens.astype(np.int64) + 4)])
+            a, c = req_first.astype(np.int64), req_count.astype(np.int64)
+            pool = np.zeros(int((csum[a + c] - csum[a]).sum()) + 1, np.uint32)
+        _check(_lib.host().dnaln_serve_run_trace(self.s, _p(rb.codes), _p(rb.offs), _p(rb.lens), _p(rb.fnv), len(rb),
+                                                  _p(req_first), _p(req_count), _p(arrival_s), n_req, _p(lat),
+                                                  _p(recs) if want_records else None,
+                                                  _p(pool) if want_records else None,
+                                                  len(pool) if want_records else 0, C.byref(wall)),
+               "dnaln_serve_run_trace")
+        return lat, wall.value, recs, pool
+
+    def close(self) -> None:
+        if self.s:
+            _lib.host().dnaln_serve_destroy(self.s)
+            self.s = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -82,7 +82,7 @@ def build_host() -> Path:
     tmp = out.with_suffix(".so.tmp")
     hip = LIB / "libdnaln_hip.so"
     link = ["-L", str(LIB), "-ldnaln_hip", f"-Wl,-rpath,$ORIGIN"] if hip.exists() else []
-    _run(["g++", "-shared", "-fPIC", *objs, "-o", tmp, *link, "-lz", "-lm"])
+    _run(["g++", "-shared", "-fPIC", "-pthread", *objs, "-o", tmp, *link, "-lz", "-lm"])
     os.replace(tmp, out)
     mains = sorted((CSRC / "host").glob("*_main.cpp"))
     if mains and hip.exists():

@@ -67,6 +67,22 @@ DN_HD uint32_t perm_b32(uint32_t s0, uint32_t s1, uint32_t sel) {
 #endif
 }
 
+// 16 aligned bytes, loaded or stored with one 16-byte access.
+struct alignas(16) Quad {
+  uint32_t x, y, z, w;
+};
+
+// Store of data the kernel itself never reads again (traceback flags, read back by the walk
+// much later): non-temporal on the device, so it streams past the caches that hold the
+// sequences.
+DN_HD void store_stream(uint64_t* p, uint64_t v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  __builtin_nontemporal_store(v, p);
+#else
+  *p = v;
+#endif
+}
+
 // Packed pairs of int16 in one 32-bit word: lane pair (A = bits 0..15, B = bits 16..31).
 #if defined(__HIP_DEVICE_COMPILE__)
 typedef short pk16 __attribute__((ext_vector_type(2)));

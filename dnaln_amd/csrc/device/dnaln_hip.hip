@@ -78,6 +78,15 @@ struct DevBuf {
   }
 };
 
+// Receiver of timing spans (event pairs on the launching stream) around kernel launches.
+struct SpanSink {
+  virtual int begin(int family, size_t& idx) = 0;
+  virtual int end(size_t idx) = 0;
+};
+
+// Timing families: 0 seed, 1 chain, 2 rank fill, 3 traceback fill, 4 traceback walk, 5 run.
+constexpr int kFamilies = 6;
+
 // Traceback directions per SW launch: enough fill waves to occupy every SIMD several times at
 // W = 129 and 1000-bp reads (8.7 MB per wave), within an eighth of the free HBM.
 uint64_t default_dir_budget() {
@@ -94,11 +103,12 @@ struct SwWork {
   DevBuf dir, rows, keys, keys2, ids, ids2, woff, tmp;
   std::vector<uint64_t> woff_h;
   uint64_t dir_budget = 1ull << 30;  // bytes of direction words per launch
+  int walk_kind = 0;                 // launch_walk_bw
 };
 
 // Layouts shared with other languages' bindings.
 static_assert(sizeof(dnaln_map_params) == 80, "dnaln_map_params layout");
-static_assert(sizeof(dnaln_batch_config) == 48, "dnaln_batch_config layout");
+static_assert(sizeof(dnaln_batch_config) == 56, "dnaln_batch_config layout");
 static_assert(sizeof(dnaln_map_stats) == 104, "dnaln_map_stats layout");
 static_assert(sizeof(dnaln_record) == 40 && sizeof(dnaln_result) == 40, "record/result layout");
 
@@ -323,11 +333,71 @@ __global__ __launch_bounds__(kSwBlock) void k_sw_packed(SwArgs a) {
   sw_pair<W, TB>(a, ida, idb, va, vb, ql, slot, lane);
 }
 
+// Traceback with one thread per task (walk_task).
 template <int W>
 __global__ __launch_bounds__(256) void k_walk(SwArgs a) {
   const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= a.n) return;
   walk_task<W>(a, k);
+}
+
+// Traceback of the 128 tasks of one fill wave by one block, through an LDS window of `rows`
+// rows of the wave's flag words: the block loads rows [top - rows + 1, top] (top = the highest
+// row any unfinished walk is on) with coalesced 16-byte loads, every walk steps while its row
+// is inside, and the window moves down. Each slot is read once, instead of a cache line per
+// step per task. Thread t walks task wave * 128 + t, the task the fill ran in lane t % 64, half
+// t / 64.
+template <int W>
+__global__ __launch_bounds__(128) void k_walk_win(SwArgs a, int rows) {
+  constexpr int G = (W + 7) / 8;
+  extern __shared__ uint64_t win[];
+  __shared__ int s_top;
+  const int t = threadIdx.x, lane = t & 63, half = t >> 6;
+  const uint32_t wave = blockIdx.x;
+  const uint32_t k = wave * 128 + t;
+  bool active = false;
+  uint32_t id = 0;
+  int score = 0, i_end = 0, d_end = 0;
+  SwTask tk{};
+  Walk w{};
+  WalkSeq seq;
+  uint32_t* out = nullptr;
+  if (k < a.n) {
+    id = a.order ? a.order[k] : k;
+    SwResult& r = a.res[id];
+    score = r.score;
+    i_end = r.qe;
+    d_end = r.te;
+    if (score <= 0) {
+      r.qb = r.qe = r.tb = r.te = r.nm = r.n_cigar = r.status = 0;
+    } else {
+      tk = a.tasks[id];
+      out = a.cigar + (uint64_t)id * (uint64_t)a.cigar_cap;
+      w = walk_init(i_end, d_end, score);
+      active = true;
+    }
+  }
+  const uint64_t* slot = dir_slot(a, wave);
+  for (;;) {
+    if (t == 0) s_top = -1;
+    __syncthreads();
+    if (active && !w.done) atomicMax(&s_top, w.i);
+    __syncthreads();
+    const int top = s_top;
+    if (top < 0) break;
+    const int lo = max(0, top - rows + 1);
+    const uint64_t n_words = (uint64_t)(top - lo + 1) * G * 64;
+    const uint64_t* src = slot + (uint64_t)lo * G * 64;
+    for (uint64_t x = 2 * (uint64_t)t; x < n_words; x += 256) {
+      const uint4 v = *reinterpret_cast<const uint4*>(src + x);
+      *reinterpret_cast<uint4*>(win + x) = v;
+    }
+    __syncthreads();
+    while (active && !w.done && (w.i >= lo || w.i < 0))
+      walk_step<W>(w, half, tk, win + lane, a.sp, out, a.cigar_cap, seq, lo);
+    __syncthreads();
+  }
+  if (active) walk_finish(w, i_end, d_end, score, a.cigar_cap, a.res[id]);
 }
 
 __global__ void k_sw_generic(SwGenericArgs a) {
@@ -366,12 +436,22 @@ bool launch_packed_bw(int bw, const SwArgs& a, unsigned grid, hipStream_t st) {
   }
 }
 
-bool launch_walk_bw(int bw, const SwArgs& a, hipStream_t st) {
-  const unsigned grid = (a.n + 255) / 256;
+// Rows per LDS window of the walk: 16 KiB of flag words, at least 4 rows.
+int walk_rows(int W) { return std::max(4, 16384 / (((W + 7) / 8) * 512)); }
+
+// walk_kind 0: one thread per task (k_walk); 1: one block per fill wave through LDS windows
+// (k_walk_win).
+bool launch_walk_bw(int bw, const SwArgs& a, int walk_kind, hipStream_t st) {
+  const int W = 2 * bw + 1;
+  const int rows = walk_rows(W);
+  const size_t lds = (size_t)rows * ((W + 7) / 8) * 512;
   switch (bw) {
-#define DN_CASE(B)                                                                 \
-  case B:                                                                          \
-    hipLaunchKernelGGL((k_walk<2 * B + 1>), dim3(grid), dim3(256), 0, st, a);      \
+#define DN_CASE(B)                                                                                       \
+  case B:                                                                                                \
+    if (walk_kind == 1)                                                                                  \
+      hipLaunchKernelGGL((k_walk_win<2 * B + 1>), dim3((a.n + 127) / 128), dim3(128), lds, st, a, rows); \
+    else                                                                                                 \
+      hipLaunchKernelGGL((k_walk<2 * B + 1>), dim3((a.n + 255) / 256), dim3(256), 0, st, a);             \
     return true;
     DN_PACKED_BW_LIST(DN_CASE)
 #undef DN_CASE
@@ -401,6 +481,7 @@ struct dnaln_hip_ctx {
   DevBuf ref, coff, clen, table, pos, cost, lut;
   uint8_t* ref_base = nullptr;  // reference codes, kSeqGuard bytes into `ref`
   std::vector<uint64_t> h_coff;
+  std::vector<int32_t> h_cost;  // chain gap cost by drift, 0..band
   std::vector<uint32_t> h_clen;
   uint64_t ref_len = 0, mask = 0, n_pos = 0;
   int nc = 0, k = 0, w = 0, lut_len = 0;
@@ -460,7 +541,8 @@ int scan_u64(DevBuf& tmp, const uint64_t* in, uint64_t* out, uint32_t n, hipStre
 // tasks' length, and each wave's direction slot is sized by its longest query. Launches are cut
 // at wave boundaries so that one launch's direction words fit wk.dir_budget.
 int run_sw(const ScoreP& sp, const SwTask* tasks, uint32_t n, int qmin, int qmax, bool traceback, SwResult* res,
-           uint32_t* cig, int cap, SwWork& wk, hipStream_t st, uint64_t* launches) {
+           uint32_t* cig, int cap, SwWork& wk, hipStream_t st, uint64_t* launches, SpanSink* spans = nullptr,
+           int fill_family = 0) {
   if (n == 0) return DNALN_OK;
   const int W = 2 * sp.bw + 1;
   const bool packed = packed_bw_available(sp.bw) && packed_ok(sp, qmax);
@@ -545,13 +627,19 @@ int run_sw(const ScoreP& sp, const SwTask* tasks, uint32_t n, int qmin, int qmax
       a.woff_base = woff ? wk.woff_h[c.first] : 0;
       const unsigned lw = (m + 127) / 128;
       const unsigned grid = (lw + kSwBlock / 64 - 1) / (kSwBlock / 64);
+      size_t si = 0;
+      if (spans) DN_TRY(spans->begin(fill_family, si));
       const bool ok = traceback ? launch_packed_bw<true>(sp.bw, a, grid, st) : launch_packed_bw<false>(sp.bw, a, grid, st);
       if (!ok) return fail(DNALN_E_INTERNAL, "no packed kernel for this band width");
       DN_HIP(hipGetLastError());
-      if (launches) ++*launches;
+      if (spans) DN_TRY(spans->end(si));
+      if (launches) ++launches[fill_family];
       if (traceback) {
-        if (!launch_walk_bw(sp.bw, a, st)) return fail(DNALN_E_INTERNAL, "no walk kernel for this band width");
+        if (spans) DN_TRY(spans->begin(4, si));
+        if (!launch_walk_bw(sp.bw, a, wk.walk_kind, st)) return fail(DNALN_E_INTERNAL, "no walk kernel for this band width");
         DN_HIP(hipGetLastError());
+        if (spans) DN_TRY(spans->end(si));
+        if (launches) ++launches[4];
       }
     }
   } else {
@@ -574,9 +662,12 @@ int run_sw(const ScoreP& sp, const SwTask* tasks, uint32_t n, int qmin, int qmax
       g.rows = wk.rows.as<int32_t>();
       g.dir = traceback ? wk.dir.as<uint8_t>() : nullptr;
       g.dir_stride = stride;
+      size_t si = 0;
+      if (spans) DN_TRY(spans->begin(fill_family, si));
       hipLaunchKernelGGL(k_sw_generic, dim3((m + 127) / 128), dim3(128), 0, st, g);
       DN_HIP(hipGetLastError());
-      if (launches) ++*launches;
+      if (spans) DN_TRY(spans->end(si));
+      if (launches) ++launches[fill_family];
     }
   }
   return DNALN_OK;
@@ -594,7 +685,7 @@ int exclusive_scan(DevBuf& tmp, const T* in, uint64_t* out, uint32_t n, hipStrea
 }  // namespace
 
 // ------------------------------------------------------------------------------------ batches
-struct dnaln_hip_batch {
+struct dnaln_hip_batch : SpanSink {
   dnaln_hip_ctx* ctx = nullptr;
   dnaln_batch_config cfg{};
   hipStream_t st = nullptr;
@@ -617,15 +708,15 @@ struct dnaln_hip_batch {
   uint64_t pool_used = 0;
   dnaln_map_stats stats{};
   // Timing: event pairs recorded on the batch stream around each launch group, summed per
-  // family at poll time: 0 seed, 1 chain, 2 rank fill, 3 traceback fill + walk, 4 whole run.
+  // family (kFamilies) at poll time.
   struct Span {
     hipEvent_t a = nullptr, b = nullptr;
     int family = 0;
   };
   std::vector<Span> spans;
   size_t n_spans = 0;
-  double ms[5] = {0, 0, 0, 0, 0};
-  uint64_t launches[5] = {0, 0, 0, 0, 0};
+  double ms[kFamilies] = {};
+  uint64_t launches[kFamilies] = {};
   hipEvent_t done = nullptr;
   bool pending = false;
   int last_rc = DNALN_OK;  // status of the last run; poll and download report it
@@ -648,6 +739,8 @@ struct dnaln_hip_batch {
     DN_HIP(hipEventRecord(spans[idx].b, st));
     return DNALN_OK;
   }
+  int begin(int family, size_t& idx) override { return span_begin(family, idx); }
+  int end(size_t idx) override { return span_end(idx); }
 };
 
 namespace {
@@ -826,6 +919,7 @@ int run_chunk(dnaln_hip_batch* b, uint64_t r0, uint32_t n, bool first_chunk) {
   sa.max_occ = p.max_occ;
   ChainArgs ca{};
   ca.cost = c->cost.as<int32_t>();
+  pack_gap_cost(ca, c->h_cost.data(), ca.band);
   ca.k = c->k;
   ca.max_gap = p.ch.max_gap;
   ca.band = p.ch.band;
@@ -911,10 +1005,8 @@ int run_chunk(dnaln_hip_batch* b, uint64_t r0, uint32_t n, bool first_chunk) {
   if (n)
     hipLaunchKernelGGL(k_rank_tasks, dim3(sgrid), dim3(kSeedBlock), 0, st, rv, g, b->cands.as<Cand>(),
                        b->ncand.as<int32_t>(), max_cand, b->roff.as<uint64_t>(), b->rtasks.as<SwTask>(), n);
-  DN_TRY(b->span_begin(2, sp_i));
   DN_TRY(run_sw(sp, b->rtasks.as<SwTask>(), (uint32_t)nr, qmin, b->maxlen, false, b->rres.as<SwResult>(),
-                nullptr, 0, b->sw, st, &b->launches[2]));
-  DN_TRY(b->span_end(sp_i));
+                nullptr, 0, b->sw, st, b->launches, b, 2));
   b->stats.rank_tasks += nr;
 
   // 4. select + traceback pass
@@ -935,10 +1027,8 @@ int run_chunk(dnaln_hip_batch* b, uint64_t r0, uint32_t n, bool first_chunk) {
   if (n)
     hipLaunchKernelGGL(k_tb_tasks, dim3(sgrid), dim3(kSeedBlock), 0, st, rv, g, b->cands.as<Cand>(), max_cand,
                        b->chosen.as<int32_t>(), b->tboff.as<uint64_t>(), b->ttasks.as<SwTask>(), n);
-  DN_TRY(b->span_begin(3, sp_i));
   DN_TRY(run_sw(sp, b->ttasks.as<SwTask>(), (uint32_t)nt, qmin, b->maxlen, true, b->tres.as<SwResult>(),
-                b->tcig.as<uint32_t>(), cap, b->sw, st, &b->launches[3]));
-  DN_TRY(b->span_end(sp_i));
+                b->tcig.as<uint32_t>(), cap, b->sw, st, b->launches, b, 3));
   b->stats.tb_tasks += nt;
 
   // 5. records and CIGARs
@@ -1037,6 +1127,7 @@ int dnaln_hip_create(const dnaln_map_params* params, const uint8_t* ref_codes, u
   std::vector<int32_t> cost(c->p.ch.band + 1, 0);
   for (int dd = 1; dd <= c->p.ch.band; ++dd)
     cost[dd] = (int)(0.01 * c->k * (double)dd + 0.5 * (63 - __builtin_clzll((uint64_t)dd)));
+  c->h_cost = cost;
   DN_TRY(c->cost.reserve(cost.size() * 4));
   DN_HIP(hipMemcpy(c->cost.p, cost.data(), cost.size() * 4, hipMemcpyHostToDevice));
   // MAPQ by score difference (src/mapper.cpp:212-214).
@@ -1150,6 +1241,8 @@ int dnaln_hip_batch_create(dnaln_hip_ctx* c, const dnaln_batch_config* cfg, dnal
   if (b->cfg.chunk_reads == 0) b->cfg.chunk_reads = 65536;
   DN_HIP(hipSetDevice(c->device));
   b->sw.dir_budget = b->cfg.dir_budget ? std::max<uint64_t>(b->cfg.dir_budget, 1 << 20) : default_dir_budget();
+  if (b->cfg.walk_kind < 0 || b->cfg.walk_kind > 1) return fail(DNALN_E_INVALID, "walk_kind must be 0 or 1");
+  b->sw.walk_kind = b->cfg.walk_kind;
   DN_HIP(hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking));
   DN_HIP(hipEventCreate(&b->done));
   DN_TRY(b->err.reserve(64));
@@ -1239,7 +1332,7 @@ int dnaln_hip_batch_run(dnaln_hip_batch* b) {
   for (auto& l : b->launches) l = 0;
   DN_HIP(hipMemsetAsync(b->err.p, 0, 4, b->st));
   size_t run_span = 0;
-  DN_TRY(b->span_begin(4, run_span));
+  DN_TRY(b->span_begin(5, run_span));
   const uint64_t chunk = b->cfg.mode == DNALN_MODE_PLAIN ? b->cfg.chunk_reads : std::max<uint64_t>(b->n, 1);
   b->last_rc = DNALN_OK;
   for (uint64_t r0 = 0; r0 < b->n || (r0 == 0 && b->n == 0); r0 += chunk) {
@@ -1264,50 +1357,65 @@ int dnaln_hip_batch_tune(dnaln_hip_batch* b, dnaln_tune_report* rep) {
   if (!b) return fail(DNALN_E_INVALID, "null batch");
   if (b->cfg.mode != DNALN_MODE_BATCHED) return fail(DNALN_E_INVALID, "the autotuner applies to the batched form");
   b->tuned = true;
-  const uint64_t big = default_dir_budget();
-  std::vector<uint64_t> budgets{big};
-  if (big != (1ull << 30)) budgets.push_back(1ull << 30);
-  std::vector<std::pair<std::pair<int, int>, uint64_t>> cand;
-  for (const auto& sp : {std::make_pair(0, 0), std::make_pair(0, -1), std::make_pair(-1, 0), std::make_pair(-1, -1)})
-    for (const uint64_t bud : budgets) cand.push_back({sp, bud});
   dnaln_tune_report r{};
-  double best = 1e300;
-  int best_i = 0;
-  std::vector<std::pair<int, int>> seen;
-  for (size_t i = 0; i < cand.size() && r.n_tried < 16; ++i) {
-    set_splits(b, cand[i].first.first, cand[i].first.second);
-    b->sw.dir_budget = cand[i].second;
-    const std::pair<int, int> eff{b->n_short > 0 ? b->split[0] : 0, b->n_mid > b->n_short ? b->split[1] : 0};
-    bool dup = false;
-    for (size_t x = 0; x < seen.size(); ++x)
-      dup = dup || (seen[x] == eff && r.cand_budget[x] == cand[i].second);
-    if (dup) continue;
-    seen.push_back(eff);
+  // One candidate: apply, run twice (the first run grows the workspace), keep the second time.
+  auto trial = [&](int s0, int s1, uint64_t budget, int walk, bool& skipped) -> int {
+    set_splits(b, s0, s1);
+    b->sw.dir_budget = budget;
+    b->sw.walk_kind = walk;
+    const int e0 = b->n_short > 0 ? b->split[0] : 0, e1 = b->n_mid > b->n_short ? b->split[1] : 0;
+    skipped = false;
+    for (int x = 0; x < r.n_tried; ++x)
+      if (r.cand_split[x][0] == e0 && r.cand_split[x][1] == e1 && r.cand_budget[x] == budget && r.cand_walk[x] == walk)
+        skipped = true;
+    if (skipped || r.n_tried >= 16) {
+      skipped = true;
+      return DNALN_OK;
+    }
     double ms = 0;
-    for (int rep_i = 0; rep_i < 2; ++rep_i) {  // the first run grows the workspace
+    for (int rep_i = 0; rep_i < 2; ++rep_i) {
       DN_TRY(dnaln_hip_batch_run(b));
       DN_TRY(dnaln_hip_batch_poll(b, 1));
-      ms = b->ms[4];
+      ms = b->ms[5];
     }
     const int t = r.n_tried++;
     r.ms[t] = ms;
-    r.cand_split[t][0] = eff.first;
-    r.cand_split[t][1] = eff.second;
-    r.cand_budget[t] = cand[i].second;
-    if (ms < best) {
-      best = ms;
-      best_i = t;
-    }
-  }
-  r.chosen = best_i;
-  r.len_split[0] = r.cand_split[best_i][0];
-  r.len_split[1] = r.cand_split[best_i][1];
-  r.dir_budget = r.cand_budget[best_i];
+    r.cand_split[t][0] = e0;
+    r.cand_split[t][1] = e1;
+    r.cand_budget[t] = budget;
+    r.cand_walk[t] = walk;
+    return DNALN_OK;
+  };
+  const uint64_t big = default_dir_budget();
+  std::vector<uint64_t> budgets{big};
+  if (big != (1ull << 30)) budgets.push_back(1ull << 30);
+  bool skipped = false;
+  // Stage 1: read-length bucketing x direction budget, with the per-thread walk.
+  for (const auto& sp : {std::make_pair(0, 0), std::make_pair(0, -1), std::make_pair(-1, 0), std::make_pair(-1, -1)})
+    for (const uint64_t bud : budgets) DN_TRY(trial(sp.first, sp.second, bud, 0, skipped));
+  auto best_of = [&]() {
+    int bi = 0;
+    for (int x = 1; x < r.n_tried; ++x)
+      if (r.ms[x] < r.ms[bi]) bi = x;
+    return bi;
+  };
+  // Stage 2: the other walk on the best setting.
+  int bi = best_of();
+  DN_TRY(trial(r.cand_split[bi][0] > 0 ? r.cand_split[bi][0] : -1, r.cand_split[bi][1] > 0 ? r.cand_split[bi][1] : -1,
+               r.cand_budget[bi], 1, skipped));
+  bi = best_of();
+  r.chosen = bi;
+  r.len_split[0] = r.cand_split[bi][0];
+  r.len_split[1] = r.cand_split[bi][1];
+  r.dir_budget = r.cand_budget[bi];
+  r.walk_kind = r.cand_walk[bi];
   b->cfg.len_split[0] = r.len_split[0] > 0 ? r.len_split[0] : -1;
   b->cfg.len_split[1] = r.len_split[1] > 0 ? r.len_split[1] : -1;
   b->cfg.dir_budget = r.dir_budget;
+  b->cfg.walk_kind = r.walk_kind;
   set_splits(b, b->cfg.len_split[0], b->cfg.len_split[1]);
   b->sw.dir_budget = r.dir_budget;
+  b->sw.walk_kind = r.walk_kind;
   if (rep) *rep = r;
   return DNALN_OK;
 }
@@ -1364,8 +1472,8 @@ int dnaln_hip_batch_download(dnaln_hip_batch* b, dnaln_record* out, uint32_t* ci
     s.ms_seed = b->ms[0];
     s.ms_chain = b->ms[1];
     s.ms_rank = b->ms[2];
-    s.ms_traceback = b->ms[3];
-    s.ms_total = b->ms[4];
+    s.ms_traceback = b->ms[3] + b->ms[4];
+    s.ms_total = b->ms[5];
     *stats = s;
   }
   return DNALN_OK;
@@ -1379,11 +1487,99 @@ uint64_t dnaln_hip_batch_device_bytes(dnaln_hip_ctx* c, const dnaln_batch_config
   return bases + n * per_read + n * (uint64_t)cfg->anchors_per_read * 48;
 }
 
-int dnaln_hip_batch_timings(dnaln_hip_batch* b, double* ms5, uint64_t* launches5) {
-  if (!b || !ms5) return fail(DNALN_E_INVALID, "null argument");
-  for (int i = 0; i < 5; ++i) ms5[i] = b->ms[i];
-  if (launches5)
-    for (int i = 0; i < 5; ++i) launches5[i] = b->launches[i];
+int dnaln_hip_batch_timings(dnaln_hip_batch* b, double* ms, uint64_t* launches) {
+  static_assert(kFamilies == DNALN_TIMING_FAMILIES, "timing families");
+  if (!b || !ms) return fail(DNALN_E_INVALID, "null argument");
+  for (int i = 0; i < kFamilies; ++i) ms[i] = b->ms[i];
+  if (launches)
+    for (int i = 0; i < kFamilies; ++i) launches[i] = b->launches[i];
+  return DNALN_OK;
+}
+
+int dnaln_hip_batch_reserve(dnaln_hip_batch* b, uint64_t n, uint64_t bases, int32_t maxlen,
+                            uint32_t anchors_per_read) {
+  if (!b || maxlen < 0) return fail(DNALN_E_INVALID, "bad argument");
+  dnaln_hip_ctx* c = b->ctx;
+  DN_HIP(hipSetDevice(c->device));
+  const uint64_t mc = (uint64_t)std::max(c->p.max_cand, 1);
+  const uint64_t cap = 2 * (uint64_t)maxlen + 4;
+  const uint64_t na = n * anchors_per_read;
+  const uint64_t tasks = n * mc;
+  // inputs and outputs
+  DN_TRY(b->codes.reserve(bases + 2 * kReadGuard));
+  DN_TRY(b->offs.reserve(n * 8 + 8));
+  DN_TRY(b->lens.reserve(n * 4 + 4));
+  DN_TRY(b->fnv.reserve(n * 4 + 4));
+  DN_TRY(b->rec.reserve(n * sizeof(dnaln_record) + sizeof(dnaln_record)));
+  DN_TRY(b->pool.reserve(n * (cap + 2) * 4 + 4));
+  // seeds and chains
+  DN_TRY(b->cands.reserve(n * mc * sizeof(Cand)));
+  DN_TRY(b->ncand.reserve(n * 4 + 4));
+  DN_TRY(b->lids.reserve(n * 4));
+  DN_TRY(b->lkeys.reserve(n * 4));
+  DN_TRY(b->lorder.reserve(n * 4));
+  DN_TRY(b->ovf.reserve(n * 4 + 8));
+  DN_TRY(b->acnt.reserve(n * 4 + 4));
+  DN_TRY(b->aoff.reserve(n * 8 + 8));
+  DN_TRY(b->keys.reserve(na * 8 + 8));
+  DN_TRY(b->f.reserve(na * 4 + 4));
+  DN_TRY(b->prev.reserve(na * 4 + 4));
+  DN_TRY(b->order.reserve(na * 8 + 8));
+  DN_TRY(b->chains.reserve(na * sizeof(ChainRec) + sizeof(ChainRec)));
+  // rank and traceback passes
+  DN_TRY(b->rcnt.reserve(n * 4 + 4));
+  DN_TRY(b->roff.reserve(n * 8 + 8));
+  DN_TRY(b->rtasks.reserve(tasks * sizeof(SwTask) + sizeof(SwTask)));
+  DN_TRY(b->rres.reserve(tasks * sizeof(SwResult) + sizeof(SwResult)));
+  DN_TRY(b->chosen.reserve(n * 4 + 4));
+  DN_TRY(b->second.reserve(n * 4 + 4));
+  DN_TRY(b->tbflag.reserve(n * 4 + 4));
+  DN_TRY(b->tboff.reserve(n * 8 + 8));
+  DN_TRY(b->ttasks.reserve(n * sizeof(SwTask) + sizeof(SwTask)));
+  DN_TRY(b->tres.reserve(n * sizeof(SwResult) + sizeof(SwResult)));
+  DN_TRY(b->tcig.reserve(n * cap * 4 + 4));
+  DN_TRY(b->ciglen.reserve(n * 4 + 4));
+  DN_TRY(b->cigoff.reserve(n * 8 + 8));
+  // SW workspace: length sort, per-wave slot offsets, directions of one launch
+  DN_TRY(b->sw.keys.reserve(tasks * 4));
+  DN_TRY(b->sw.keys2.reserve(tasks * 4));
+  DN_TRY(b->sw.ids.reserve(tasks * 4));
+  DN_TRY(b->sw.ids2.reserve(tasks * 4));
+  const uint64_t waves = (tasks + 127) / 128;
+  DN_TRY(b->sw.woff.reserve((waves + 2) * 16));
+  const int W = 2 * c->p.sc.bw + 1;
+  const uint64_t dir = std::min<uint64_t>(b->sw.dir_budget, ((n + 127) / 128 + 1) * (uint64_t)std::max(maxlen, 1) *
+                                                                ((W + 7) / 8) * 512);
+  DN_TRY(b->sw.dir.reserve(dir));
+  // scan and sort temporaries for the largest counts
+  size_t t1 = 0, t2 = 0, t3 = 0;
+  DN_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, t1, (const uint32_t*)nullptr, (uint64_t*)nullptr, (uint32_t)tasks,
+                                          b->st));
+  DN_HIP(hipcub::DeviceRadixSort::SortPairsDescending(nullptr, t2, (const uint32_t*)nullptr, (uint32_t*)nullptr,
+                                                      (const uint32_t*)nullptr, (uint32_t*)nullptr, (uint32_t)tasks, 0,
+                                                      16, b->st));
+  DN_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, t3, (const uint64_t*)nullptr, (uint64_t*)nullptr,
+                                          (uint32_t)(waves + 1), b->st));
+  DN_TRY(b->scan_tmp.reserve(t1));
+  DN_TRY(b->sw.tmp.reserve(std::max(t2, t3)));
+  DN_TRY(b->ltmp.reserve(t2));
+  DN_HIP(hipStreamSynchronize(b->st));
+  return DNALN_OK;
+}
+
+int dnaln_hip_device_memory(dnaln_hip_ctx* c, uint64_t* free_bytes, uint64_t* total_bytes) {
+  if (!c || !free_bytes || !total_bytes) return fail(DNALN_E_INVALID, "null argument");
+  DN_HIP(hipSetDevice(c->device));
+  size_t f = 0, t = 0;
+  DN_HIP(hipMemGetInfo(&f, &t));
+  *free_bytes = f;
+  *total_bytes = t;
+  return DNALN_OK;
+}
+
+int dnaln_hip_context_params(dnaln_hip_ctx* c, dnaln_map_params* out) {
+  if (!c || !out) return fail(DNALN_E_INVALID, "null argument");
+  *out = c->p;
   return DNALN_OK;
 }
 

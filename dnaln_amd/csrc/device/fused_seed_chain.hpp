@@ -8,11 +8,11 @@
 //   * f (int16) and prev (int8) live beside them; anchor j of the thread sits at element
 //     j * stride, with the 64 threads of a wave interleaved, so LDS accesses are conflict-free;
 //   * chain extraction replays the reference order (f desc, index asc) by repeated argmax over a
-//     64-bit used mask, first to find the best chain score, then to collect the chains scoring
-//     >= pri_ratio * best into a register list kept in stable score order;
+//     64-bit used mask, in one pass that keeps the qualifying chains in a register list in
+//     stable score order (at most MAXCH) and remembers the best score that fell off it;
 //   * candidates are selected from that list exactly as src/mapper.cpp:44-62 does.
-// A read whose anchors exceed the LDS capacity, or whose candidate selection would need more
-// chains than the register list holds, returns -1; the caller routes it to the global-memory
+// A read whose anchors exceed the LDS capacity, or whose candidate selection would need a chain
+// that fell off the register list, returns -1; the caller routes it to the global-memory
 // path, which has no such limits.
 #pragma once
 
@@ -93,8 +93,8 @@ DN_HD T hash_kmer(T key, T mask) {
 // (the hash then stays below the all-ones sentinel) and uint64_t otherwise. Appends the
 // minimizers of an N-free read to `mins`; returns 1, or 0 when the read holds an N (the caller
 // then uses sketch_stream), or -1 when the list would exceed maxmin.
-template <int WT, class T>
-DN_HD int sketch_vh(ChunkReader rd, int len, int k, uint64_t* mins, int stride, int maxmin, int& nmin) {
+template <int WT, class T, class Reader>
+DN_HD int sketch_vh(Reader rd, int len, int k, uint64_t* mins, int stride, int maxmin, int& nmin) {
   const T mask = (T)((1ULL << (2 * k)) - 1);
   const int shift = 2 * (k - 1);
   const T kInvalid = ~(T)0;
@@ -164,49 +164,85 @@ DN_HD int sketch_vh(ChunkReader rd, int len, int k, uint64_t* mins, int stride, 
   return 1;
 }
 
-// Chain DP over sorted anchors in LDS (src/chain.cpp:21-45); predecessors are visited nearest
-// first in groups of four whose loads are issued together.
+// The bases of a read staged in LDS: the aligned 16-byte chunks around the read, stored as u64
+// words in the thread's (interleaved) anchor slots; byte j of the read sits at byte off + j.
+struct LdsSeq {
+  const uint64_t* words;
+  int stride;
+  uint32_t off;
+  DN_HD uint8_t operator()(int j) const {
+    const uint32_t x = off + (uint32_t)j;
+    return (uint8_t)(words[(x >> 3) * stride] >> (8 * (x & 7)));
+  }
+};
+
+// Loads the read's chunks into `slots` u64 slots of `dst` (stride apart), sixteen chunks at a
+// time with all loads of a batch issued before its first wait. False when the read does not
+// fit. Reads up to 15 bytes before and after the read: the read buffer carries guard bytes.
+DN_HD bool stage_read(const uint8_t* seq, int len, uint64_t* dst, int stride, int slots, LdsSeq& out) {
+  constexpr int kBatch = 16;
+  const uintptr_t a = (uintptr_t)seq;
+  const uint32_t off = (uint32_t)(a & 15);
+  const int nch = (int)((off + (uint32_t)len + 15) >> 4);
+  if (2 * nch > slots) return false;
+  const Quad* src = (const Quad*)(a - off);
+  for (int b0 = 0; b0 < nch; b0 += kBatch) {
+    Quad c[kBatch];
+#pragma unroll
+    for (int x = 0; x < kBatch; ++x)
+      if (b0 + x < nch) c[x] = src[b0 + x];
+#pragma unroll
+    for (int x = 0; x < kBatch; ++x)
+      if (b0 + x < nch) {
+        dst[(2 * (b0 + x)) * stride] = (uint64_t)c[x].y << 32 | c[x].x;
+        dst[(2 * (b0 + x) + 1) * stride] = (uint64_t)c[x].w << 32 | c[x].z;
+      }
+  }
+  out.words = dst;
+  out.stride = stride;
+  out.off = off;
+  return true;
+}
+
+// Chain DP over sorted anchors in LDS (src/chain.cpp:21-45). Predecessors are visited nearest
+// first in groups of four whose loads are issued together. The reference's early stops (a
+// strand change, a reference gap above max_gap) are monotone in j on sorted anchors, so each
+// predecessor of a group is tested on its own, without branches, and the scan ends after the
+// first group whose farthest predecessor is stopped. Arithmetic is 32-bit: rpos < 2^31,
+// qpos < 2^16, and drift is only used where 0 < dr <= max_gap.
 DN_HD void chain_dp_lds(const ChainArgs& a, const LdsView& v, int n, int k) {
+  const int max_gap = a.max_gap, band = a.band;
   for (int i = 0; i < n; ++i) {
     const uint64_t ki = v.keys[i * v.stride];
     const uint32_t si = key_strand(ki);
-    const int64_t ri = key_rpos(ki), qi = key_qpos(ki);
+    const int ri = (int)key_rpos(ki), qi = key_qpos(ki);
     int best = k, best_j = -1;
     const int jlo = i - a.lookback > 0 ? i - a.lookback : 0;
-    bool stop = false;
-    for (int j0 = i - 1; j0 >= jlo && !stop; j0 -= 4) {
+    for (int j0 = i - 1; j0 >= jlo; j0 -= 4) {
       uint64_t kj[4];
       int fj[4];
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        const int j = j0 - u;
-        kj[u] = j >= jlo ? v.keys[j * v.stride] : 0;
-        fj[u] = j >= jlo ? v.f[j * v.stride] : 0;
+        const int j = j0 - u >= jlo ? j0 - u : jlo;
+        kj[u] = v.keys[j * v.stride];
+        fj[u] = v.f[j * v.stride];
       }
+      bool alive = true;
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         const int j = j0 - u;
-        if (stop || j < jlo) continue;
-        if (key_strand(kj[u]) != si) {
-          stop = true;
-          continue;
-        }
-        const int64_t dr = ri - key_rpos(kj[u]);
-        if (dr > a.max_gap) {
-          stop = true;
-          continue;
-        }
-        const int64_t dq = qi - key_qpos(kj[u]);
-        if (dr <= 0 || dq <= 0) continue;
-        const int64_t dd = dr > dq ? dr - dq : dq - dr;
-        if (dd > a.band) continue;
-        const int64_t m = dq < dr ? dq : dr;
-        const int sc = fj[u] + (int)(m < k ? m : k) - a.cost[dd];
-        if (sc > best) {
-          best = sc;
-          best_j = j;
-        }
+        const int dr = ri - (int)key_rpos(kj[u]);
+        alive = j >= jlo && key_strand(kj[u]) == si && dr <= max_gap;
+        const int drc = dr < 0 ? 0 : dr;
+        const int dq = qi - key_qpos(kj[u]);
+        const int dd = drc > dq ? drc - dq : dq - drc;
+        const int m = dq < drc ? dq : drc;
+        const int sc = fj[u] + (m < k ? m : k) - gap_cost(a, dd <= band ? dd : band);
+        const bool take = alive && dr > 0 && dq > 0 && dd <= band && sc > best;
+        best = take ? sc : best;
+        best_j = take ? j : best_j;
       }
+      if (!alive) break;
     }
     v.f[i * v.stride] = (int16_t)best;
     v.prev[i * v.stride] = (int8_t)best_j;
@@ -215,25 +251,15 @@ DN_HD void chain_dp_lds(const ChainArgs& a, const LdsView& v, int n, int k) {
 
 template <int MAXCH>
 DN_HD int chain_extract_lds(const ChainArgs& a, const LdsView& v, int n, uint32_t r) {
-  // pass 1: best qualifying chain score
+  // One pass over the chains in extraction order: the qualifying ones go to a register list
+  // kept in stable score order (at most MAXCH); dropped_max is the best score that fell off.
   uint64_t used = 0;
-  int best_score = -1;
-  for (int end = next_end(v, n, used); end >= 0; end = next_end(v, n, used)) {
-    int cnt;
-    const ChainLite c = walk_chain(v, end, used, cnt);
-    if (c.score >= a.min_score && cnt >= a.min_anchors && c.score > best_score) best_score = c.score;
-  }
-  if (best_score < 0) return 0;
-  // pass 2: chains scoring >= pri_ratio * best, in stable score order, at most MAXCH kept
-  const double threshold = best_score * a.pri_ratio;
   ChainLite top[MAXCH];
-  int count = 0;
-  bool truncated = false;
-  used = 0;
+  int count = 0, dropped_max = -1;
   for (int end = next_end(v, n, used); end >= 0; end = next_end(v, n, used)) {
     int cnt;
     ChainLite cur = walk_chain(v, end, used, cnt);
-    if (cur.score < a.min_score || cnt < a.min_anchors || cur.score < threshold) continue;
+    if (cur.score < a.min_score || cnt < a.min_anchors) continue;
     bool inserted = false;
 #pragma unroll
     for (int x = 0; x < MAXCH; ++x) {
@@ -245,9 +271,11 @@ DN_HD int chain_extract_lds(const ChainArgs& a, const LdsView& v, int n, uint32_
         inserted = true;
       }
     }
-    if (count == MAXCH || !inserted) truncated = true;
+    if (count == MAXCH) dropped_max = cur.score > dropped_max ? cur.score : dropped_max;
     count = count < MAXCH ? count + 1 : MAXCH;
   }
+  if (count == 0) return 0;
+  const double threshold = top[0].score * a.pri_ratio;
 
   // candidates (src/mapper.cpp:44-62)
   Cand* cands = a.cands + (uint64_t)r * (uint64_t)a.max_cand;
@@ -258,7 +286,7 @@ DN_HD int chain_extract_lds(const ChainArgs& a, const LdsView& v, int n, uint32_
 #pragma unroll
   for (int x = 0; x < MAXCH; ++x) {
     if (done || x >= count) continue;
-    if (nc >= a.max_cand) {
+    if (top[x].score < threshold || nc >= a.max_cand) {  // the reference stops here
       done = true;
       continue;
     }
@@ -281,8 +309,8 @@ DN_HD int chain_extract_lds(const ChainArgs& a, const LdsView& v, int n, uint32_
     cands[nc++] = cd;
     chosen |= 1u << x;
   }
-  // The list ran out before max_cand candidates while chains were dropped from it.
-  if (truncated && !done && nc < a.max_cand) return -1;
+  // The list ran out while the reference would go on to a chain that fell off it.
+  if (!done && dropped_max >= threshold && nc < a.max_cand) return -1;
   return nc;
 }
 
@@ -299,9 +327,12 @@ DN_HD int seed_chain_lds(const SeedArgs& s, const ChainArgs& a, uint32_t r, LdsV
   const uint8_t* seq = s.codes + s.offs[r];
   int nmin = 0;
   int got = 0;
-  if (s.w == WT)
-    got = k <= 15 ? sketch_vh<WT, uint32_t>(ChunkReader(seq), len, k, mins, v.stride, maxmin, nmin)
-                  : sketch_vh<WT, uint64_t>(ChunkReader(seq), len, k, mins, v.stride, maxmin, nmin);
+  LdsSeq ls;
+  // The read's bases go to LDS first (into the anchor slots, free until phase 2), so the
+  // sketch waits for memory once instead of at every chunk boundary.
+  if (s.w == WT && stage_read(seq, len, v.keys, v.stride, cap, ls))
+    got = k <= 15 ? sketch_vh<WT, uint32_t>(ls, len, k, mins, v.stride, maxmin, nmin)
+                  : sketch_vh<WT, uint64_t>(ls, len, k, mins, v.stride, maxmin, nmin);
   if (got < 0) return -1;
   if (got == 0) {
     nmin = 0;

@@ -316,6 +316,8 @@ struct ChainArgs {
   uint64_t* order;         // per anchor scratch: extraction order
   ChainRec* chains;        // per anchor scratch: chains of the read
   const int32_t* cost;     // gap cost by |dr - dq|, 0..band
+  uint64_t cost_pk[4];     // the same, one byte per drift 0..31, when cost_packed
+  int cost_packed;         // band <= 31 and every cost in [0, 255]
   int k, max_gap, band, lookback, min_score, min_anchors;
   int max_cand, bw;
   const uint64_t* contig_off;
@@ -360,6 +362,29 @@ DN_HD void heap_sort(T* a, int n, Less less) {
   }
 }
 
+// Gap cost of drift dd (0 <= dd <= band): from the kernel-argument copy when it fits one, so
+// the chain DP's inner loop does not wait on memory.
+DN_HD int gap_cost(const ChainArgs& a, int64_t dd) {
+  if (a.cost_packed) {
+    const int x = (int)dd;
+    const uint64_t w = x < 16 ? (x < 8 ? a.cost_pk[0] : a.cost_pk[1]) : (x < 24 ? a.cost_pk[2] : a.cost_pk[3]);
+    return (int)((w >> (8 * (x & 7))) & 0xff);
+  }
+  return a.cost[dd];
+}
+
+// Packs cost[0..band] into a.cost_pk when band <= 31 and all costs fit a byte.
+inline void pack_gap_cost(ChainArgs& a, const int32_t* host_cost, int band) {
+  a.cost_packed = 0;
+  a.cost_pk[0] = a.cost_pk[1] = a.cost_pk[2] = a.cost_pk[3] = 0;
+  if (band > 31) return;
+  for (int d = 0; d <= band; ++d) {
+    if (host_cost[d] < 0 || host_cost[d] > 255) return;
+    a.cost_pk[d >> 3] |= (uint64_t)host_cost[d] << (8 * (d & 7));
+  }
+  a.cost_packed = 1;
+}
+
 DN_HD int contig_of(const uint64_t* off, int nc, uint64_t gpos) {  // src/reference.cpp:25-32
   int lo = 0, hi = nc;
   while (hi - lo > 1) {
@@ -402,7 +427,7 @@ DN_HD void chain_read(const ChainArgs& a, uint32_t li) {
         const int64_t dd = dr > dq ? dr - dq : dq - dr;
         if (dd > a.band) continue;
         const int64_t m = dq < dr ? dq : dr;
-        const int sc = f[j] + (int)(m < k ? m : k) - a.cost[dd];
+        const int sc = f[j] + (int)(m < k ? m : k) - gap_cost(a, dd);
         if (sc > best) {
           best = sc;
           best_j = j;

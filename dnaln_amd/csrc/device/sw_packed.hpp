@@ -137,23 +137,26 @@ struct Walk {
   bool done;
 };
 
-// The aligned dword last loaded from one sequence: the walk moves through the query and the
-// target one position at a time, so most bases come from a register.
-struct CachedDword {
+// The aligned 16 bytes last loaded from one sequence (one 16-byte load): the walk moves
+// through the query and the target one position at a time, so a load serves about 16 steps.
+// The sequence buffers carry guard bytes, so the aligned chunk around a base is readable.
+struct CachedChunk {
   uint64_t at = ~0ull;
-  uint32_t w = 0;
+  Quad c{};
   DN_HD uint8_t get(uint64_t addr) {
-    const uint64_t a = addr & ~(uint64_t)3;
+    const uint64_t a = addr & ~(uint64_t)15;
     if (a != at) {
-      w = *(const uint32_t*)(uintptr_t)a;
+      c = *(const Quad*)(uintptr_t)a;
       at = a;
     }
+    const uint32_t s = (uint32_t)(addr >> 2) & 3;
+    const uint32_t w = s == 0 ? c.x : (s == 1 ? c.y : (s == 2 ? c.z : c.w));
     return (uint8_t)(w >> (8 * (addr & 3)));
   }
 };
 
 struct WalkSeq {
-  CachedDword q, t;
+  CachedChunk q, t;
   DN_HD uint8_t qb(const SwTask& tk, int i) {
     if (i < 0 || i >= tk.qlen) return 4;
     if (tk.flags & 1) return comp_code(q.get(tk.q + (uint64_t)(tk.qlen - 1 - i)));
@@ -186,9 +189,11 @@ DN_HD void walk_flush(Walk& w, uint32_t* out, int cap) {
   w.pend_len = 0;
 }
 
+// One traceback step. slot_lane holds the lane's flag words from row row0 on (word of row i,
+// group g at ((i - row0) * G + g) * 64); a step reads only row w.i, and none when w.i < 0.
 template <int W>
 DN_HD void walk_step(Walk& w, int half, const SwTask& tk, const uint64_t* slot_lane, const ScoreP& sp,
-                     uint32_t* out, int cap, WalkSeq& seq) {
+                     uint32_t* out, int cap, WalkSeq& seq, int row0 = 0) {
   constexpr int G = (W + 7) / 8;
   const int oe = sp.gap_open + sp.gap_ext, ge = sp.gap_ext;
   if (w.state == 0 && (w.i < 0 || w.v <= 0)) {
@@ -204,7 +209,7 @@ DN_HD void walk_step(Walk& w, int half, const SwTask& tk, const uint64_t* slot_l
     return;
   }
   // The flag word and, speculatively, the two bases of the cell (mostly from registers).
-  const uint64_t word = slot_lane[(uint64_t)(w.i * G + (w.d >> 3)) * 64];
+  const uint64_t word = slot_lane[(uint64_t)((w.i - row0) * G + (w.d >> 3)) * 64];
   const uint8_t a = seq.qb(tk, w.i), c = seq.tb(tk, w.i + w.d);
   const int b = w.d & 7;
   if (w.state == 0) {
@@ -282,7 +287,7 @@ DN_HD void walk_finish(const Walk& w, int i_end, int d_end, int score, int cap, 
 // ------------------------------------------------------------------------------------ fill
 // One lane: tasks ida and idb (va / vb false for an empty half), rows 0..qmax-1 of the wave.
 // Score-only: writes the scores. With flags (TB): stores the flag words into the wave's slot
-// and writes the score and the best cell (row in qe, diagonal in te) for walk_task().
+// and writes the score and the best cell (row in qe, diagonal in te) for the walk.
 template <int W, bool TB>
 DN_HD void sw_pair(const SwArgs& a, uint32_t ida, uint32_t idb, bool va, bool vb, int qmax, uint64_t* slot,
                    int lane) {
@@ -360,7 +365,7 @@ DN_HD void sw_pair(const SwArgs& a, uint32_t ida, uint32_t idb, bool va, bool vb
         acc12 |= p12 & bit;
         acc34 |= p34 & bit;
         if ((d & 7) == 7 || d == W - 1) {
-          slot_lane[(uint64_t)(i * G + (d >> 3)) * 64] = (uint64_t)acc12 | (uint64_t)acc34 << 32;
+          store_stream(&slot_lane[(uint64_t)(i * G + (d >> 3)) * 64], (uint64_t)acc12 | (uint64_t)acc34 << 32);
           acc12 = acc34 = 0;
         }
         rowmax = pk_max(rowmax, h);
@@ -432,7 +437,7 @@ DN_HD void sw_pair(const SwArgs& a, uint32_t ida, uint32_t idb, bool va, bool vb
 
 // Traceback of one alignment from the end cell sw_pair<W, true> left in its result. k is the
 // launch-relative position of the task (the fill's wave k / 128 holds it, in lane k % 64, half
-// (k / 64) % 2), so the 64 threads of a walk wave read the words of one fill wave.
+// (k / 64) % 2): one thread per task reads its flag words straight from the slot.
 template <int W>
 DN_HD void walk_task(const SwArgs& a, uint32_t k) {
   const uint32_t id = a.order ? a.order[k] : k;

@@ -21,6 +21,60 @@ using namespace dn;
 
 namespace {
 
+// The walk of k_walk_win<W> for one fill wave, thread by thread: rows outside the loaded window
+// read as a poison value, so a step outside [lo, top] gives a wrong result.
+template <int W>
+void walk_wave_window(const SwArgs& a, uint32_t wave, int rows) {
+  constexpr int G = (W + 7) / 8;
+  struct Th {
+    bool active = false;
+    uint32_t id = 0;
+    int score = 0, i_end = 0, d_end = 0;
+    SwTask tk{};
+    Walk w{};
+    WalkSeq seq;
+    uint32_t* out = nullptr;
+  };
+  std::vector<Th> th(128);
+  for (int t = 0; t < 128; ++t) {
+    const uint32_t k = wave * 128 + t;
+    if (k >= a.n) continue;
+    Th& x = th[t];
+    x.id = a.order ? a.order[k] : k;
+    SwResult& r = a.res[x.id];
+    x.score = r.score;
+    x.i_end = r.qe;
+    x.d_end = r.te;
+    if (x.score <= 0) {
+      r.qb = r.qe = r.tb = r.te = r.nm = r.n_cigar = r.status = 0;
+      continue;
+    }
+    x.tk = a.tasks[x.id];
+    x.out = a.cigar + (uint64_t)x.id * (uint64_t)a.cigar_cap;
+    x.w = walk_init(x.i_end, x.d_end, x.score);
+    x.active = true;
+  }
+  const uint64_t* slot = dir_slot(a, wave);
+  std::vector<uint64_t> win((size_t)rows * G * 64);
+  for (;;) {
+    int top = -1;
+    for (const Th& x : th)
+      if (x.active && !x.w.done) top = std::max(top, x.w.i);
+    if (top < 0) break;
+    const int lo = std::max(0, top - rows + 1);
+    const uint64_t n_words = (uint64_t)(top - lo + 1) * G * 64;
+    std::fill(win.begin(), win.end(), 0x5555aaaa5555aaaaULL);
+    std::copy(slot + (uint64_t)lo * G * 64, slot + (uint64_t)lo * G * 64 + n_words, win.begin());
+    for (int t = 0; t < 128; ++t) {
+      Th& x = th[t];
+      while (x.active && !x.w.done && (x.w.i >= lo || x.w.i < 0))
+        walk_step<W>(x.w, t >> 6, x.tk, win.data() + (t & 63), a.sp, x.out, a.cigar_cap, x.seq, lo);
+    }
+  }
+  for (Th& x : th)
+    if (x.active) walk_finish(x.w, x.i_end, x.d_end, x.score, a.cigar_cap, a.res[x.id]);
+}
+
 template <int W, bool TB>
 void run_packed(const SwArgs& a0) {
   constexpr int G = (W + 7) / 8;
@@ -41,8 +95,15 @@ void run_packed(const SwArgs& a0) {
       sw_pair<W, TB>(a, ia, ib, ia < a.n, ib < a.n, qmax, TB ? a.dir + (uint64_t)wv * slot_words : nullptr, lane);
     }
   }
-  if (TB)
-    for (uint32_t k = 0; k < a.n; ++k) walk_task<W>(a, k);
+  if (TB) {
+    // Both walks: windows of 3 rows (fewer than the device's, so many window moves) on even
+    // fill waves, one thread per task on odd ones.
+    for (uint32_t wv = 0; wv < waves; ++wv) {
+      if (wv % 2 == 0) walk_wave_window<W>(a, wv, 3);
+      else
+        for (uint32_t k = wv * 128; k < a.n && k < wv * 128 + 128; ++k) walk_task<W>(a, k);
+    }
+  }
 }
 
 template <bool TB>
@@ -183,6 +244,7 @@ long emu_seed_chain(const uint8_t* codes, const uint64_t* offs, const int32_t* l
   sa.max_occ = (uint32_t)mp[2];
   ChainArgs ca{};
   ca.cost = cost;
+  pack_gap_cost(ca, cost, mp[8]);
   ca.k = mp[0];
   ca.bw = mp[3];
   ca.max_cand = mp[4];

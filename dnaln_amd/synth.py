@@ -144,7 +144,40 @@ WORKLOADS = {
     "uniform4m_uneven_1m_bw64": Workload("uniform4m_uneven_1m_bw64", UNIFORM_4M, 1_000_000, LEN_LOGUNIFORM, 50, 1000, 0.02, 0.005, 12, 11, 64),
     # 120 Mbp / 8 contigs, 10% segmental copies, 250 bp reads, bw 32.
     "repeats120m_250bp_4m": Workload("repeats120m_250bp_4m", REPEATS_120M, 4_000_000, LEN_FIXED, 250, 250, 0.01, 0.001, 3, 14, 32),
+    # Serving: the same genome; read pool of lengths uniform in 100..300 bp, 1% sub / 0.1% indel, bw 32.
+    "serve_repeats120m": Workload("serve_repeats120m", REPEATS_120M, 1_000_000, LEN_UNIFORM, 100, 300, 0.01, 0.001, 3, 15, 32),
 }
+
+
+def request_arrivals(rate_per_s: float, duration_s: float, seed: int, max_size: int = 4096):
+    """Poisson arrivals over duration_s with log-uniform request sizes floor(exp(U[0, ln(max_size + 1))))
+    (1..max_size reads). Returns (sizes, arrival_s)."""
+    rng = np.random.default_rng(seed)
+    n = max(1, int(rate_per_s * duration_s * 1.2) + 16)
+    gaps = rng.exponential(1.0 / rate_per_s, n)
+    arrive = np.cumsum(gaps)
+    keep = arrive <= duration_s
+    sizes = np.floor(np.exp(rng.uniform(0.0, np.log(max_size + 1), n))).astype(np.int64)
+    sizes = np.clip(sizes, 1, max_size)
+    return sizes[keep], arrive[keep]
+
+
+def request_trace(n_reads: int, rate_per_s: float, seed: int, max_size: int = 4096, duration_s: float | None = None):
+    """Poisson arrivals of requests with floor(exp(U[0, ln(max_size + 1)))) reads each (1..max_size),
+    cut from a read pool of n_reads in order. Returns (req_first, arrival_s): request r holds reads
+    [req_first[r], req_first[r+1]). Stops when the pool or duration_s runs out."""
+    rng = np.random.default_rng(seed)
+    first, arrive = [0], []
+    t = 0.0
+    while True:
+        size = int(np.floor(np.exp(rng.uniform(0.0, np.log(max_size + 1)))))
+        size = max(1, min(size, max_size))
+        t += rng.exponential(1.0 / rate_per_s)
+        if first[-1] + size > n_reads or (duration_s is not None and t > duration_s):
+            break
+        first.append(first[-1] + size)
+        arrive.append(t)
+    return np.array(first, np.uint64), np.array(arrive, np.float64)
 
 _GENOME_CACHE: dict = {}
 

@@ -133,6 +133,11 @@ int dnaln_hip_create(const dnaln_map_params* params, const uint8_t* ref_codes, u
                      struct dnaln_hip_ctx** out);
 void dnaln_hip_destroy(struct dnaln_hip_ctx* ctx);
 const char* dnaln_hip_last_error(void);
+/* Free and total device memory of the context's GPU, bytes (no reference counterpart; the
+ * serving loop sizes its HBM budget from it). */
+int dnaln_hip_device_memory(struct dnaln_hip_ctx* ctx, uint64_t* free_bytes, uint64_t* total_bytes);
+/* The context's parameters after validation (the chain band clamped to the SW band). */
+int dnaln_hip_context_params(struct dnaln_hip_ctx* ctx, dnaln_map_params* out);
 
 /* Aligner::align: n independent banded alignments; traceback = 0 computes the score only.
  * sp may be NULL (use the context's scoring). Results are bit-identical to the reference's
@@ -162,7 +167,11 @@ typedef struct {
   int32_t len_split[2];     /* batched form: read-length limits of the two LDS shapes of the fused
                                seed+chain kernel; longer reads take the global-memory path
                                (0: the largest lengths the shapes hold on average; -1: bucket off) */
-  uint64_t dir_budget;      /* bytes of traceback direction words per SW launch (0: 1 GiB) */
+  uint64_t dir_budget;      /* bytes of traceback direction words per SW launch
+                               (0: min(16 GiB, free HBM / 8)) */
+  int32_t walk_kind;        /* traceback walk: 0 one thread per task, 1 one block per fill wave
+                               through LDS windows of flag words */
+  int32_t reserved;
 } dnaln_batch_config;
 
 void dnaln_hip_default_batch_config(dnaln_batch_config* c);
@@ -173,6 +182,12 @@ void dnaln_hip_batch_destroy(struct dnaln_hip_batch* b);
 /* Asynchronous H2D copy of n reads into the batch (host buffers must stay valid until sync). */
 int dnaln_hip_batch_upload(struct dnaln_hip_batch* b, const uint8_t* read_codes, const uint64_t* offs,
                            const int32_t* lens, const uint32_t* name_fnv1a, uint64_t n);
+/* Sizes the batch's device buffers for batches of up to n reads, `bases` bases and reads of up
+ * to maxlen bases (anchors_per_read anchors per read on the global path), so that running
+ * such batches allocates nothing more in the common case. No reference counterpart: for
+ * serving slots, which must not stall the device with allocations. */
+int dnaln_hip_batch_reserve(struct dnaln_hip_batch* b, uint64_t n, uint64_t bases, int32_t maxlen,
+                            uint32_t anchors_per_read);
 /* Runs the pipeline on the resident batch, asynchronously where the mode allows. */
 int dnaln_hip_batch_run(struct dnaln_hip_batch* b);
 /* Waits for the batch (wait = 1) or reports DNALN_E_PENDING (wait = 0). */
@@ -182,10 +197,12 @@ int dnaln_hip_batch_download(struct dnaln_hip_batch* b, dnaln_record* out, uint3
                              dnaln_map_stats* stats);
 /* Device bytes a batch of this configuration holds (for admission by HBM budget). */
 uint64_t dnaln_hip_batch_device_bytes(struct dnaln_hip_ctx* ctx, const dnaln_batch_config* cfg);
-/* Time of the last run measured with HIP events on the batch stream, per kernel family
- * (ms): [0] seed, [1] chain, [2] rank fill, [3] traceback fill + walk, [4] whole run.
- * launches5 (may be NULL) receives the number of fill kernel launches per family. */
-int dnaln_hip_batch_timings(struct dnaln_hip_batch* b, double* ms5, uint64_t* launches5);
+/* Time of the last run measured with HIP events on the batch stream, per family (ms):
+ * [0] seed, [1] chain, [2] rank fill, [3] traceback fill, [4] traceback walk, [5] whole run.
+ * The fill and walk families time exactly their kernel launches. launches (may be NULL)
+ * receives the number of kernel launches of families 2-4. Arrays hold DNALN_TIMING_FAMILIES. */
+#define DNALN_TIMING_FAMILIES 6
+int dnaln_hip_batch_timings(struct dnaln_hip_batch* b, double* ms, uint64_t* launches);
 
 /* Autotuner of the batched form (no counterpart in the reference, whose CPU path has no such
  * choices). Runs the resident batch under each candidate setting and keeps the fastest:
@@ -195,11 +212,14 @@ int dnaln_hip_batch_timings(struct dnaln_hip_batch* b, double* ms5, uint64_t* la
 typedef struct {
   int32_t len_split[2];     /* chosen bucket limits (0 = bucket unused) */
   uint64_t dir_budget;      /* chosen direction budget, bytes */
+  int32_t walk_kind;        /* chosen walk */
   int32_t n_tried;
   int32_t chosen;           /* index into ms[] */
+  int32_t reserved;
   double ms[16];            /* whole-run time of each candidate, in the order tried */
   int32_t cand_split[16][2];
   uint64_t cand_budget[16];
+  int32_t cand_walk[16];
 } dnaln_tune_report;
 
 int dnaln_hip_batch_tune(struct dnaln_hip_batch* b, dnaln_tune_report* report);

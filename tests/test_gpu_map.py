@@ -125,3 +125,21 @@ def test_autotuned_batch_gives_reference_records():
     recs, pool, _ = b.download()
     b.close()
     _compare("edge_uniform_bw64", recs, pool, g)
+
+
+@pytest.mark.parametrize("name", ["uniform4m_150bp_10k", "edge_uniform_bw64", "edge_twins_sr_bw5", "edge_repeats_bw32"])
+@pytest.mark.parametrize("walk_kind", [0, 1])
+@pytest.mark.parametrize("mode", [abi.MODE_PLAIN, abi.MODE_BATCHED])
+def test_both_traceback_walks_give_reference_records(name, walk_kind, mode):
+    """One thread per task reading flag words from HBM, or one block per fill wave through LDS
+    windows: the same CIGARs, coordinates and NM."""
+    spec = sets.SETS[name]
+    g = sets.genome(spec)
+    r = sets.reads(spec, g)
+    print(f"set {name} walk {walk_kind} mode {mode}", flush=True)
+    b = _context(spec).batch(mode=mode, walk_kind=walk_kind, dir_budget=1 << 22)
+    b.upload(api.prepare(r))
+    b.run()
+    recs, pool, _ = b.download()
+    b.close()
+    _compare(name, recs, pool, g)

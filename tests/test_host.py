@@ -130,3 +130,24 @@ def test_fnv1a_many_matches_oracle():
     nb = np.frombuffer(blob, np.uint8)
     _lib.host().dnaln_fnv1a_many(nb.ctypes.data, offs.ctypes.data, lens.ctypes.data, len(names), out.ctypes.data)
     assert out.tolist() == [oracle.fnv1a(x) for x in names]
+
+
+def test_serving_library_exports_every_declared_symbol():
+    lib = _lib.host()
+    declared = _declared("dnaln_serve.h")
+    assert declared == set(abi.SERVE_SYMBOLS), declared ^ set(abi.SERVE_SYMBOLS)
+    for name in declared:
+        assert hasattr(lib, name), name
+    cfg = abi.ServeConfig()
+    lib.dnaln_serve_default_config(C.byref(cfg))
+    assert cfg.slots >= 1 and cfg.mode == abi.MODE_BATCHED and cfg.max_batch_reads >= 4096
+
+
+def test_request_trace_follows_the_serving_model():
+    first, arrive = synth.request_trace(200_000, 1000.0, 3)
+    sizes = np.diff(first.astype(np.int64))
+    assert sizes.min() >= 1 and sizes.max() <= 4096 and np.all(np.diff(arrive) >= 0)
+    # log-uniform sizes: mean (4097 - 1) / ln 4097 = 492
+    assert 380 < sizes.mean() < 620
+    # Poisson arrivals at 1000/s: mean gap 1 ms
+    assert 0.8e-3 < np.diff(arrive).mean() < 1.25e-3
