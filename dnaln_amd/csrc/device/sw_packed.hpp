@@ -137,26 +137,24 @@ struct Walk {
   bool done;
 };
 
-// The aligned 16 bytes last loaded from one sequence (one 16-byte load): the walk moves
-// through the query and the target one position at a time, so a load serves about 16 steps.
-// The sequence buffers carry guard bytes, so the aligned chunk around a base is readable.
-struct CachedChunk {
+// The aligned dword last loaded from one sequence: the walk moves through the query and the
+// target one position at a time, so most bases come from a register. (Measured on MI355X: a
+// 16-byte cache refilled every 16 steps walks about 40% slower than this one.)
+struct CachedDword {
   uint64_t at = ~0ull;
-  Quad c{};
+  uint32_t w = 0;
   DN_HD uint8_t get(uint64_t addr) {
-    const uint64_t a = addr & ~(uint64_t)15;
+    const uint64_t a = addr & ~(uint64_t)3;
     if (a != at) {
-      c = *(const Quad*)(uintptr_t)a;
+      w = *(const uint32_t*)(uintptr_t)a;
       at = a;
     }
-    const uint32_t s = (uint32_t)(addr >> 2) & 3;
-    const uint32_t w = s == 0 ? c.x : (s == 1 ? c.y : (s == 2 ? c.z : c.w));
     return (uint8_t)(w >> (8 * (addr & 3)));
   }
 };
 
 struct WalkSeq {
-  CachedChunk q, t;
+  CachedDword q, t;
   DN_HD uint8_t qb(const SwTask& tk, int i) {
     if (i < 0 || i >= tk.qlen) return 4;
     if (tk.flags & 1) return comp_code(q.get(tk.q + (uint64_t)(tk.qlen - 1 - i)));
@@ -303,8 +301,13 @@ DN_HD void sw_pair(const SwArgs& a, uint32_t ida, uint32_t idb, bool va, bool vb
   const pk16 oe = pk_splat(a.sp.gap_open + a.sp.gap_ext), ge = pk_splat(a.sp.gap_ext);
   const uint32_t tab0 = a.tab0, tab1 = a.tab1;
 
+  // Rows per step of the row loop: the row body is unrolled U times, so the band's target codes
+  // (T[d] = target position i + d) shift once per step instead of once per row. Only where the
+  // extra registers cost no occupancy: the traceback fill up to W = 33 (2 waves per SIMD either
+  // way); the score-only fill would drop from 3 to 2 waves, wider bands would spill more.
+  constexpr int U = TB && W <= 33 ? 4 : 1;
   pk16 H[W + 1], E[W + 1];
-  uint32_t T[W];
+  uint32_t T[W + U - 1];
 #pragma unroll
   for (int d = 0; d <= W; ++d) H[d] = E[d] = zero;
   // Target positions 0..W-1, four at a time.
@@ -334,63 +337,76 @@ DN_HD void sw_pair(const SwArgs& a, uint32_t ida, uint32_t idb, bool va, bool vb
   }
   uint64_t* slot_lane = slot + lane;
 
-  for (int i = 0; i < qmax; ++i) {
-    const int u = i & 3;
-    if (u == 0) {  // fetch the next block of four rows
-      nqa = va ? qword4(ta, i + 4) : 0u;
-      nqb = vb ? qword4(tb, i + 4) : 0u;
-      nta = va ? tword4(ta, i + 4 + W) : 0u;
-      ntb = vb ? tword4(tb, i + 4 + W) : 0u;
+  for (int i0 = 0; i0 < qmax; i0 += U) {
+    const int u0 = i0 & 3;
+    if (u0 == 0) {  // fetch the next block of four rows
+      nqa = va ? qword4(ta, i0 + 4) : 0u;
+      nqb = vb ? qword4(tb, i0 + 4) : 0u;
+      nta = va ? tword4(ta, i0 + 4 + W) : 0u;
+      ntb = vb ? tword4(tb, i0 + 4 + W) : 0u;
     }
-    const uint32_t qsel = qsel_from(byte_at(cqa, u), i < ta.qlen, ra) | qsel_from(byte_at(cqb, u), i < tb.qlen, rb) << 16;
+    // Target positions i0 + W .. i0 + W + U - 2, which rows i0 + 1 .. of this step reach.
+#pragma unroll
+    for (int v = 0; v + 1 < U; ++v) {
+      const int jn = i0 + W + v;
+      T[W + v] = tsel_from(byte_at(cta, u0 + v), jn >= ta.tlo && jn < ta.thi) |
+                 tsel_from(byte_at(ctb, u0 + v), jn >= tb.tlo && jn < tb.thi) << 16;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = i0 + u;
+      if (i >= qmax) break;
+      const uint32_t qsel = qsel_from(byte_at(cqa, u0 + u), i < ta.qlen, ra) |
+                            qsel_from(byte_at(cqb, u0 + u), i < tb.qlen, rb) << 16;
 
-    pk16 hdiag = H[0], f = zero, hleft = zero, rowmax = zero;
-    uint32_t acc12 = 0, acc34 = 0;
+      pk16 hdiag = H[0], f = zero, hleft = zero, rowmax = zero;
+      uint32_t acc12 = 0, acc34 = 0;
 #pragma unroll
-    for (int d = 0; d < W; ++d) {
-      const pk16 hup = H[d + 1], eup = E[d + 1];
-      const pk16 s = pk_from(perm_b32(tab0, tab1, T[d] ^ qsel));
-      const pk16 hd = pk_add(hdiag, s);
-      const pk16 eo = pk_sub(hup, oe), ee = pk_sub(eup, ge);
-      const pk16 e = pk_max(eo, ee);
-      const pk16 fo = pk_sub(hleft, oe), fe = pk_sub(f, ge);
-      f = pk_max(fo, fe);
-      const pk16 h = pk_max(pk_max(pk_max(hd, e), zero), f);
-      H[d] = h;
-      E[d] = e;
-      if (TB) {
-        const uint32_t p12 = perm_b32(pk_bits(pk_sub(eo, ee)), pk_bits(pk_sub(fo, fe)), 0x0B0A0908u);
-        const uint32_t p34 = perm_b32(pk_bits(pk_sub(hd, h)), pk_bits(pk_sub(e, h)), 0x0B0A0908u);
-        const uint32_t bit = 0x01010101u << (d & 7);
-        acc12 |= p12 & bit;
-        acc34 |= p34 & bit;
-        if ((d & 7) == 7 || d == W - 1) {
-          store_stream(&slot_lane[(uint64_t)(i * G + (d >> 3)) * 64], (uint64_t)acc12 | (uint64_t)acc34 << 32);
-          acc12 = acc34 = 0;
+      for (int d = 0; d < W; ++d) {
+        const pk16 hup = H[d + 1], eup = E[d + 1];
+        const pk16 s = pk_from(perm_b32(tab0, tab1, T[d + u] ^ qsel));
+        const pk16 hd = pk_add(hdiag, s);
+        const pk16 eo = pk_sub(hup, oe), ee = pk_sub(eup, ge);
+        const pk16 e = pk_max(eo, ee);
+        const pk16 fo = pk_sub(hleft, oe), fe = pk_sub(f, ge);
+        f = pk_max(fo, fe);
+        const pk16 h = pk_max(pk_max(pk_max(hd, e), zero), f);
+        H[d] = h;
+        E[d] = e;
+        if (TB) {
+          const uint32_t p12 = perm_b32(pk_bits(pk_sub(eo, ee)), pk_bits(pk_sub(fo, fe)), 0x0B0A0908u);
+          const uint32_t p34 = perm_b32(pk_bits(pk_sub(hd, h)), pk_bits(pk_sub(e, h)), 0x0B0A0908u);
+          const uint32_t bit = 0x01010101u << (d & 7);
+          acc12 |= p12 & bit;
+          acc34 |= p34 & bit;
+          if ((d & 7) == 7 || d == W - 1) {
+            store_stream(&slot_lane[(uint64_t)(i * G + (d >> 3)) * 64], (uint64_t)acc12 | (uint64_t)acc34 << 32);
+            acc12 = acc34 = 0;
+          }
+          rowmax = pk_max(rowmax, h);
+        } else {
+          best = pk_max(best, h);
         }
-        rowmax = pk_max(rowmax, h);
-      } else {
-        best = pk_max(best, h);
+        hleft = h;
+        hdiag = hup;
       }
-      hleft = h;
-      hdiag = hup;
-    }
-    if (TB) {
-      const uint32_t m = pk_signmask(pk_sub(best, rowmax));  // halves whose row max beats best
-      if (m) {
+      if (TB) {
+        const uint32_t m = pk_signmask(pk_sub(best, rowmax));  // halves whose row max beats best
+        if (m) {
 #pragma unroll
-        for (int d = 0; d < (TB ? W : 1); ++d) BH[d] = pk_from(bfi(m, pk_bits(H[d]), pk_bits(BH[d])));
-        best = pk_max(best, rowmax);
-        best_i = bfi(m, (uint32_t)i * 0x10001u, best_i);
+          for (int d = 0; d < (TB ? W : 1); ++d) BH[d] = pk_from(bfi(m, pk_bits(H[d]), pk_bits(BH[d])));
+          best = pk_max(best, rowmax);
+          best_i = bfi(m, (uint32_t)i * 0x10001u, best_i);
+        }
       }
     }
-    // Shift the band one position and bring in target position i + W.
+    // Shift the band U positions and bring in target position i0 + W + U - 1.
 #pragma unroll
-    for (int d = 0; d + 1 < W; ++d) T[d] = T[d + 1];
-    const int jn = i + W;
-    T[W - 1] = tsel_from(byte_at(cta, u), jn >= ta.tlo && jn < ta.thi) |
-               tsel_from(byte_at(ctb, u), jn >= tb.tlo && jn < tb.thi) << 16;
-    if (u == 3) {
+    for (int d = 0; d + 1 < W; ++d) T[d] = T[d + U];
+    const int jn = i0 + W + U - 1;
+    T[W - 1] = tsel_from(byte_at(cta, u0 + U - 1), jn >= ta.tlo && jn < ta.thi) |
+               tsel_from(byte_at(ctb, u0 + U - 1), jn >= tb.tlo && jn < tb.thi) << 16;
+    if (u0 + U == 4) {
       cqa = nqa;
       cqb = nqb;
       cta = nta;
