@@ -80,6 +80,7 @@ struct DevBuf {
 
 // Receiver of timing spans (event pairs on the launching stream) around kernel launches.
 struct SpanSink {
+  virtual ~SpanSink() = default;
   virtual int begin(int family, size_t& idx) = 0;
   virtual int end(size_t idx) = 0;
 };
@@ -525,7 +526,7 @@ __global__ void k_len_keys(const SwTask* tasks, uint32_t n, uint32_t* keys, uint
 __global__ void k_wave_words(const uint32_t* sorted_len, uint32_t waves, uint64_t row_words, uint64_t* words) {
   const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
   if (w > waves) return;
-  words[w] = w < waves ? (uint64_t)max(sorted_len[(uint64_t)w * 128], 1u) * row_words : 0;
+  words[w] = w < waves ? fill_rows((int)sorted_len[(uint64_t)w * 128]) * row_words : 0;
 }
 
 int scan_u64(DevBuf& tmp, const uint64_t* in, uint64_t* out, uint32_t n, hipStream_t st) {
@@ -553,7 +554,7 @@ int run_sw(const ScoreP& sp, const SwTask* tasks, uint32_t n, int qmin, int qmax
     score_table(sp, a.tab0, a.tab1);
     a.cigar_cap = cap;
     const uint64_t row_words = (uint64_t)((W + 7) / 8) * 64;
-    const uint64_t slot_words = (uint64_t)std::max(qmax, 1) * row_words;
+    const uint64_t slot_words = fill_rows(qmax) * row_words;
     const uint32_t waves = (n + 127) / 128;
     const uint32_t* order = nullptr;
     const uint64_t* woff = nullptr;

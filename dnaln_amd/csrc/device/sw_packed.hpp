@@ -14,8 +14,9 @@
 // (the band loop is fully unrolled for a compile-time W). Substitution scores come from one
 // v_xor_b32 + one v_perm_b32 per cell and pair (see score_table()).
 //
-// Sequence bytes are streamed with one aligned dword load per stream per 4 rows (v_alignbyte),
-// so every device sequence buffer carries guard bytes on both sides (kSeqGuard).
+// Sequence bytes are streamed with one 8-byte load per stream per 4 rows, a block ahead
+// (BlockStreams), from addresses kept within a few bytes of each sequence; every device sequence
+// buffer carries guard bytes on both sides.
 //
 // Traceback storage: 4 flag bits per cell and alignment. Flags are the sign bits of four packed
 // differences, gathered by v_perm_b32 into byte masks and accumulated 8 cells to a byte:
@@ -29,7 +30,7 @@
 namespace dn {
 
 // Selector encodings (per 16-bit half): query [4|q, 8|q] or [0xF0, 0xF0] for N; target [t, t] or
-// [0x30, 0x30] for N (qsel_from / tsel_from below). x = qsel ^ tsel indexes the table below: low
+// [0x30, 0x30] for N (BlockStreams::next, tsel_from below). x = qsel ^ tsel indexes the table below: low
 // byte 4|(q^t) picks the score, high byte 8|(q^t) replicates its sign; any N gives a selector
 // >= 13, i.e. 0xFFFF = -1 (ambig 1).
 
@@ -99,18 +100,6 @@ DN_HD uint32_t load4(uint64_t base, int64_t j) {
 #endif
 }
 
-// Query codes of rows r0 .. r0+3 as bytes 0..3 (reverse-complemented reads walk backwards;
-// the complement is applied later). Bytes of rows >= qlen are undefined and masked by the caller.
-DN_HD uint32_t qword4(const SwTask& t, int r0) {
-  if (t.flags & 1) {
-    int j = t.qlen - 4 - r0;
-    j = j < -4 ? -4 : j;
-    return perm_b32(0, load4(t.q, j), 0x00010203u);  // byte reverse
-  }
-  const int last = t.qlen > 0 ? t.qlen - 1 : 0;
-  return load4(t.q, r0 < last ? r0 : last);
-}
-
 // Target codes at window positions j0 .. j0+3; positions outside [tlo, thi) are masked by the
 // caller. The address is clamped so it stays next to the valid range.
 DN_HD uint32_t tword4(const SwTask& t, int j0) {
@@ -121,11 +110,119 @@ DN_HD uint32_t tword4(const SwTask& t, int j0) {
 
 DN_HD uint32_t byte_at(uint32_t w, int u) { return (w >> (8 * u)) & 0xffu; }
 
-// Selector halves with validity, complement for reverse-complemented queries, and N.
-DN_HD uint32_t qsel_from(uint32_t c, bool valid, bool rev) {
-  if (!valid || c > 3) return 0xF0F0u;
-  return rev ? 0x0B07u - c * 0x0101u : 0x0804u + c * 0x0101u;
+// The 8 bytes at the 4-aligned address a, as two dwords (one global_load_dwordx2).
+DN_HD void load8(uint64_t a, uint32_t& lo, uint32_t& hi) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  typedef const __attribute__((address_space(1))) uint32_t* gptr;
+  const gptr p = (gptr)a;
+#else
+  const uint32_t* p = (const uint32_t*)(uintptr_t)a;
+#endif
+  lo = p[0];
+  hi = p[1];
 }
+
+// Bytes sh .. sh+3 (sh = bits 1:0) of hi:lo.
+DN_HD uint32_t align4(uint32_t lo, uint32_t hi, uint32_t sh) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_alignbyte(hi, lo, sh);
+#else
+  return (uint32_t)((((uint64_t)hi << 32) | lo) >> (8 * (sh & 3)));
+#endif
+}
+
+// Byte masks over a word's four bytes: bytes k..3, and bytes 0..k-1 (k clamped to 0..4).
+DN_HD uint32_t bytes_from(int k) { return k <= 0 ? ~0u : k >= 4 ? 0u : ~0u << (8 * k); }
+DN_HD uint32_t bytes_below(int k) { return ~bytes_from(k); }
+
+// Rows per fill slot: the traceback fill runs its rows in steps of up to 4, so a slot holds
+// the wave's longest query rounded up to a multiple of 4 (rows past a query are never read).
+DN_HD uint64_t fill_rows(int qmax) { return (uint64_t)(((qmax > 1 ? qmax : 1) + 3) & ~3); }
+
+// The sequence streams of one lane's two alignments (h = 0: A, 1: B), read one block of four
+// rows ahead. For block r0 (a multiple of 4) a query stream needs the codes of rows r0..r0+3
+// (q[r0..r0+3], or q[qlen-4-r0 .. qlen-1-r0] reversed) and a target stream the codes of
+// positions r0+W .. r0+W+3, which enter the band during those rows. Each block is one 8-byte
+// load from a 4-aligned address, issued a block before its bytes are used, so the fill never
+// waits on it; the byte offset within the 8 bytes is the same for every block of a stream.
+// Offsets are clamped to the range where a block with any valid row or position reads its
+// bytes: loads stay within a few bytes of the sequence, and a clamped block is fully masked.
+template <int W>
+struct BlockStreams {
+  uint64_t q[2], t[2];
+  int32_t qlen[2];
+  uint32_t tlim[2];                             // tlo | thi << 16
+  uint32_t qlo[2], qhi[2], tlo_w[2], thi_w[2];  // raw bytes of the next block
+  uint32_t sh;  // per byte (query A, B, target A, B): bits 1:0 the byte offset; bit 7 of the
+                // query bytes: reverse-complement
+
+  DN_HD bool rev(int h) const { return (sh >> (8 * h + 7)) & 1; }
+  DN_HD int tlo(int h) const { return (int)(tlim[h] & 0xffff); }
+  DN_HD int thi(int h) const { return (int)(tlim[h] >> 16); }
+  DN_HD int qoff(int h, int r0) const {
+    const int o = rev(h) ? qlen[h] - 4 - r0 : r0;
+    const int lo = -3, hi = qlen[h] - 1;
+    return o < lo ? lo : o > hi ? hi : o;
+  }
+  DN_HD int toff(int h, int r0) const {
+    int o = r0 + W;
+    o = o < tlo(h) - 3 ? tlo(h) - 3 : o;
+    return o > thi(h) - 1 ? thi(h) - 1 : o;
+  }
+  DN_HD void fetch(int r0) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      load8((q[h] + (uint64_t)(int64_t)qoff(h, r0)) & ~(uint64_t)3, qlo[h], qhi[h]);
+      load8((t[h] + (uint64_t)(int64_t)toff(h, r0)) & ~(uint64_t)3, tlo_w[h], thi_w[h]);
+    }
+  }
+  // tlo and thi lie in [0, 32768) (thi <= qlen + 2*bw).
+  DN_HD void init(const SwTask& a, const SwTask& b) {
+    const SwTask* tk[2] = {&a, &b};
+    sh = 0;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      q[h] = tk[h]->q;
+      t[h] = tk[h]->t;
+      qlen[h] = tk[h]->qlen;
+      tlim[h] = (uint32_t)tk[h]->tlo | (uint32_t)tk[h]->thi << 16;
+      const bool r = tk[h]->flags & 1;
+      const uint32_t sq = (uint32_t)(q[h] + (r ? (uint64_t)(int64_t)(qlen[h] - 4) : 0)) & 3;
+      const uint32_t st = (uint32_t)(t[h] + W) & 3;
+      sh |= (sq | (r ? 0x80u : 0u)) << (8 * h) | st << (16 + 8 * h);
+    }
+    fetch(0);
+  }
+  // Block r0 from the fetched bytes, then fetches block r0 + 4. qc: the query codes of rows
+  // r0..r0+3 (byte u = row r0+u; A in qc[0], B in qc[1]) as 0..3, complemented for reversed
+  // queries, and 0xF0 for N or a row past the query. ts[u]: the target selector of position
+  // r0+W+u (both halves).
+  DN_HD void next(int r0, uint32_t (&qc)[2], uint32_t (&ts)[4]) {
+    uint32_t tt[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const uint32_t qw = align4(qlo[h], qhi[h], sh >> (8 * h));
+      const uint32_t tw = align4(tlo_w[h], thi_w[h], sh >> (16 + 8 * h));
+      // Reversed queries read backwards and complement (c ^ 3 swaps A/T and C/G; N = 4 becomes 7).
+      const uint32_t c = rev(h) ? perm_b32(0, qw, 0x00010203u) ^ 0x03030303u : qw;
+      qc[h] = bfi(bytes_from(qlen[h] - r0), 0xF0F0F0F0u, perm_b32(0xF0F0F0F0u, 0x03020100u, c));
+      const int j0 = r0 + W;
+      const uint32_t tbad = bytes_below(tlo(h) - j0) | bytes_from(thi(h) - j0);
+      tt[h] = bfi(tbad, 0x30303030u, perm_b32(0x30303030u, 0x03020100u, tw));
+    }
+    fetch(r0 + 4);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) ts[u] = perm_b32(tt[1], tt[0], (uint32_t)u * 0x0101u | (uint32_t)(u + 4) * 0x01010000u);
+  }
+};
+
+// Query selector of row u of a block (both halves): bytes [4|cA, 8|cA, 4|cB, 8|cB]; the N code
+// 0xF0 stays >= 13.
+DN_HD uint32_t qsel_row(const uint32_t (&qc)[2], int u) {
+  return perm_b32(qc[1], qc[0], (uint32_t)u * 0x01010101u + 0x04040000u) | 0x08040804u;
+}
+
+// Target selector half with validity and N.
 DN_HD uint32_t tsel_from(uint32_t c, bool valid) { return (!valid || c > 3) ? 0x3030u : c * 0x0101u; }
 
 // ------------------------------------------------------------------------------------ walk
@@ -290,12 +387,11 @@ template <int W, bool TB>
 DN_HD void sw_pair(const SwArgs& a, uint32_t ida, uint32_t idb, bool va, bool vb, int qmax, uint64_t* slot,
                    int lane) {
   constexpr int G = (W + 7) / 8;
-  SwTask ta, tb;
-  if (va) ta = a.tasks[ida];
-  else ta.qlen = ta.tlo = ta.thi = 0, ta.flags = 0, ta.q = ta.t = 0;
-  if (vb) tb = a.tasks[idb];
-  else tb.qlen = tb.tlo = tb.thi = 0, tb.flags = 0, tb.q = tb.t = 0;
-  const bool ra = ta.flags & 1, rb = tb.flags & 1;
+  // An empty half reads task 0's sequences (every launch has a task 0) with no valid row or
+  // target position, so its loads stay in bounds and its cells score as N.
+  SwTask ta = a.tasks[va ? ida : 0], tb = a.tasks[vb ? idb : 0];
+  if (!va) ta.qlen = ta.tlo = ta.thi = 0, ta.flags = 0;
+  if (!vb) tb.qlen = tb.tlo = tb.thi = 0, tb.flags = 0;
 
   const pk16 zero = pk_splat(0);
   const pk16 oe = pk_splat(a.sp.gap_open + a.sp.gap_ext), ge = pk_splat(a.sp.gap_ext);
@@ -305,6 +401,8 @@ DN_HD void sw_pair(const SwArgs& a, uint32_t ida, uint32_t idb, bool va, bool vb
   // (T[d] = target position i + d) shift once per step instead of once per row. Only where the
   // extra registers cost no occupancy: the traceback fill up to W = 33 (2 waves per SIMD either
   // way); the score-only fill would drop from 3 to 2 waves, wider bands would spill more.
+  // A step always runs all U rows; rows past qmax score below the best and store their flags
+  // into the slot's padding rows (fill_rows).
   constexpr int U = TB && W <= 33 ? 4 : 1;
   pk16 H[W + 1], E[W + 1];
   uint32_t T[W + U - 1];
@@ -313,7 +411,7 @@ DN_HD void sw_pair(const SwArgs& a, uint32_t ida, uint32_t idb, bool va, bool vb
   // Target positions 0..W-1, four at a time.
 #pragma unroll
   for (int d0 = 0; d0 < W; d0 += 4) {
-    const uint32_t wa = va ? tword4(ta, d0) : 0u, wb = vb ? tword4(tb, d0) : 0u;
+    const uint32_t wa = tword4(ta, d0), wb = tword4(tb, d0);
 #pragma unroll
     for (int u = 0; u < 4 && d0 + u < W; ++u) {
       const int j = d0 + u;
@@ -321,12 +419,12 @@ DN_HD void sw_pair(const SwArgs& a, uint32_t ida, uint32_t idb, bool va, bool vb
              tsel_from(byte_at(wb, u), j >= tb.tlo && j < tb.thi) << 16;
     }
   }
-  // Streams, per block of four rows r = 4b .. 4b+3: the query bytes of those rows, and the
-  // target bytes that enter the band after each of them (positions r + W); the next block's
-  // words are fetched at the start of the current one.
-  uint32_t cqa = va ? qword4(ta, 0) : 0u, cqb = vb ? qword4(tb, 0) : 0u;
-  uint32_t cta = va ? tword4(ta, W) : 0u, ctb = vb ? tword4(tb, W) : 0u;
-  uint32_t nqa = 0, nqb = 0, nta = 0, ntb = 0;
+  // Per block of four rows: the query codes of its rows (QC) and the target selectors of the
+  // positions entering the band during them (TS, consumed U per step).
+  BlockStreams<W> bs;
+  bs.init(ta, tb);
+  uint32_t QC[2], TS[4];
+  bs.next(0, QC, TS);
 
   pk16 best = zero;
   pk16 BH[TB ? W : 1];
@@ -338,26 +436,13 @@ DN_HD void sw_pair(const SwArgs& a, uint32_t ida, uint32_t idb, bool va, bool vb
   uint64_t* slot_lane = slot + lane;
 
   for (int i0 = 0; i0 < qmax; i0 += U) {
-    const int u0 = i0 & 3;
-    if (u0 == 0) {  // fetch the next block of four rows
-      nqa = va ? qword4(ta, i0 + 4) : 0u;
-      nqb = vb ? qword4(tb, i0 + 4) : 0u;
-      nta = va ? tword4(ta, i0 + 4 + W) : 0u;
-      ntb = vb ? tword4(tb, i0 + 4 + W) : 0u;
-    }
     // Target positions i0 + W .. i0 + W + U - 2, which rows i0 + 1 .. of this step reach.
 #pragma unroll
-    for (int v = 0; v + 1 < U; ++v) {
-      const int jn = i0 + W + v;
-      T[W + v] = tsel_from(byte_at(cta, u0 + v), jn >= ta.tlo && jn < ta.thi) |
-                 tsel_from(byte_at(ctb, u0 + v), jn >= tb.tlo && jn < tb.thi) << 16;
-    }
+    for (int v = 0; v + 1 < U; ++v) T[W + v] = TS[v];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int i = i0 + u;
-      if (i >= qmax) break;
-      const uint32_t qsel = qsel_from(byte_at(cqa, u0 + u), i < ta.qlen, ra) |
-                            qsel_from(byte_at(cqb, u0 + u), i < tb.qlen, rb) << 16;
+      const uint32_t qsel = qsel_row(QC, (i0 & 3) + u);
 
       pk16 hdiag = H[0], f = zero, hleft = zero, rowmax = zero;
       uint32_t acc12 = 0, acc34 = 0;
@@ -400,18 +485,16 @@ DN_HD void sw_pair(const SwArgs& a, uint32_t ida, uint32_t idb, bool va, bool vb
         }
       }
     }
-    // Shift the band U positions and bring in target position i0 + W + U - 1.
+    // Shift the band U positions and bring in target position i0 + W + U - 1; move the block's
+    // remaining target selectors down by U.
 #pragma unroll
     for (int d = 0; d + 1 < W; ++d) T[d] = T[d + U];
-    const int jn = i0 + W + U - 1;
-    T[W - 1] = tsel_from(byte_at(cta, u0 + U - 1), jn >= ta.tlo && jn < ta.thi) |
-               tsel_from(byte_at(ctb, u0 + U - 1), jn >= tb.tlo && jn < tb.thi) << 16;
-    if (u0 + U == 4) {
-      cqa = nqa;
-      cqb = nqb;
-      cta = nta;
-      ctb = ntb;
-    }
+    T[W - 1] = TS[U - 1];
+#pragma unroll
+    for (int v = 0; v + U < 4; ++v) TS[v] = TS[v + U];
+    // The next block's selectors, from bytes fetched a block ago (the stores of a whole block
+    // lie between that load and this use).
+    if (((i0 + U) & 3) == 0) bs.next(i0 + U, QC, TS);
   }
 
   const int sa = pk_lo(best), sb = pk_hi(best);

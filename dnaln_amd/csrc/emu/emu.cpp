@@ -80,7 +80,7 @@ void run_packed(const SwArgs& a0) {
   constexpr int G = (W + 7) / 8;
   int qall = 1;
   for (uint32_t k = 0; k < a0.n; ++k) qall = std::max(qall, a0.tasks[k].qlen);
-  const uint64_t slot_words = (uint64_t)qall * G * 64;
+  const uint64_t slot_words = fill_rows(qall) * G * 64;
   const uint32_t waves = (a0.n + 127) / 128;
   std::vector<uint64_t> dir(TB ? (size_t)waves * slot_words : 1, 0xdeadbeefdeadbeefULL);
   SwArgs a = a0;
@@ -149,6 +149,7 @@ int emu_align_ex(const int* sp6, const uint8_t* pool, const int64_t* q_off, cons
   uint64_t pool_len = 0;
   for (size_t i = 0; i < n; ++i)
     pool_len = std::max<uint64_t>(pool_len, std::max<uint64_t>(q_off[i] + q_len[i], t_off[i] + t_len[i]));
+  if (pool_len > (1ull << 40)) return -1;  // offsets beyond any host pool
   std::vector<uint8_t> guarded(pool_len + 2 * kSeqGuard, 0xEE);
   std::memcpy(guarded.data() + kSeqGuard, pool, pool_len);
   pool = guarded.data() + kSeqGuard;
