@@ -111,4 +111,8 @@ DN_HD uint32_t pk_signmask(pk16 a) { return perm_b32(0, pk_bits(a), 0x09090808u)
 // Bitfield insert: m ? a : b, bitwise.
 DN_HD uint32_t bfi(uint32_t m, uint32_t a, uint32_t b) { return (a & m) | (b & ~m); }
 
+// Byte masks over a word's four bytes: bytes k..3, and bytes 0..k-1 (k clamped to 0..4).
+DN_HD uint32_t bytes_from(int k) { return k <= 0 ? ~0u : k >= 4 ? 0u : ~0u << (8 * k); }
+DN_HD uint32_t bytes_below(int k) { return ~bytes_from(k); }
+
 }  // namespace dn

@@ -213,30 +213,28 @@ __global__ __launch_bounds__(256) void k_chain_wave(ChainArgs a) {
 }
 
 
-// One read per thread; the read's anchor keys and minimizer list sit in LDS, interleaved by
-// thread. Per thread: `cap` u64 key slots, then `mslots` u64 slots holding the minimizer list
-// during the probes and f (int16) and prev (int8) during the DP. The fallback sketch's ring
-// (w u64 hashes + w u32 positions) uses the key slots before any anchor is written.
+// One read per thread; the read's anchor words sit in LDS, `cap` u64 slots per thread,
+// interleaved by thread. The minimizer list goes to the block's part of `mins` (maxmin u64 per
+// thread, interleaved: written by the sketch, read back by the probes). The fallback sketch's
+// ring (w u64 hashes + w u32 positions) uses the anchor slots before any anchor is written.
 template <int WT>
-__global__ __launch_bounds__(kFusedBlock) void k_seed_chain_fused(SeedArgs s, ChainArgs c, int cap, int mslots,
-                                                                  int maxmin, uint32_t* ovf, uint32_t* novf) {
+__global__ __launch_bounds__(kFusedBlock) void k_seed_chain_fused(SeedArgs s, ChainArgs c, int cap, int maxmin,
+                                                                  uint64_t* mins, uint32_t* ovf, uint32_t* novf) {
   extern __shared__ uint64_t lds[];
   const int t = threadIdx.x;
   const uint32_t i = blockIdx.x * kFusedBlock + t;
   if (i >= s.n_reads) return;
   uint64_t* keys = lds;
-  uint64_t* mins = lds + (size_t)cap * kFusedBlock;
   Ring ring;
   ring.h = keys + t;
   ring.ps = reinterpret_cast<uint32_t*>(keys + (size_t)s.w * kFusedBlock) + t;
   ring.stride = kFusedBlock;
   LdsView v;
   v.keys = keys + t;
-  v.f = reinterpret_cast<int16_t*>(mins) + t;
-  v.prev = reinterpret_cast<int8_t*>(mins) + (size_t)2 * cap * kFusedBlock + t;
   v.stride = kFusedBlock;
   const uint32_t r = read_id(s.list, i);
-  const int nc = seed_chain_lds<8, 10>(s, c, r, v, mins + t, maxmin, ring, cap);
+  uint64_t* my_mins = mins + (size_t)blockIdx.x * maxmin * kFusedBlock + t;
+  const int nc = seed_chain_lds<8, 10>(s, c, r, v, my_mins, kFusedBlock, maxmin, ring, cap);
   if (nc < 0) {
     ovf[atomicAdd(novf, 1u)] = r;
     c.n_cand[r] = 0;
@@ -699,6 +697,7 @@ struct dnaln_hip_batch : SpanSink {
   std::vector<int32_t> lens_h;       // host copy of the read lengths (bucket counts)
   bool tuned = false;
   DevBuf lids, lkeys, lorder, ltmp;  // length-sorted read order
+  DevBuf fmins;                      // minimizer lists of the fused kernel
   // workspace
   DevBuf acnt, aoff, keys, f, prev, order, chains, ring_h, ring_ps;
   DevBuf cands, ncand, rcnt, roff, rtasks, rres, chosen, second, tbflag, tboff, ttasks, tres, tcig;
@@ -830,10 +829,10 @@ int seed_chain_global(dnaln_hip_batch* b, SeedArgs sa, ChainArgs ca, const uint3
   return DNALN_OK;
 }
 
-// LDS shape of the fused seed + chain kernel: anchor slots, minimizer-list entries and the
-// u64 slots that hold the list (and later f and prev), per read.
+// Shape of the fused seed + chain kernel: anchor slots (LDS) and minimizer-list entries (global
+// scratch) per read.
 struct FusedShape {
-  int cap = 0, maxmin = 0, mslots = 0;
+  int cap = 0, maxmin = 0;
   size_t lds = 0;
 };
 
@@ -851,10 +850,14 @@ FusedShape fused_shape(int k, int w, int maxlen) {
   FusedShape f;
   f.cap = maxlen <= 200 ? 32 : 64;
   f.maxmin = std::min(64, std::max(16, fused_list_need(k, maxlen)));
-  f.mslots = std::max(f.maxmin, (3 * f.cap + 7) / 8);
-  f.lds = (size_t)kFusedBlock * 8 * (f.cap + f.mslots);
-  if (12 * w > 8 * f.cap || f.lds > 64 * 1024) f.cap = 0;
+  f.lds = (size_t)kFusedBlock * 8 * f.cap;
+  if (12 * w > 8 * f.cap || f.lds > 64 * 1024 || maxlen >= 16384) f.cap = 0;
   return f;
+}
+
+// Minimizer scratch of a fused launch over n reads (u64 entries).
+uint64_t fused_mins_words(const FusedShape& fs, uint64_t n) {
+  return (n + kFusedBlock - 1) / kFusedBlock * kFusedBlock * (uint64_t)fs.maxmin;
 }
 
 // Default length buckets: the longest reads whose expected minimizer list fits the 32-entry
@@ -873,11 +876,11 @@ void default_splits(int k, int w, int out[2]) {
     }
 }
 
-int launch_fused(const FusedShape& fs, const SeedArgs& sa, const ChainArgs& ca, uint32_t* ovf, uint32_t* novf,
-                 hipStream_t st) {
+int launch_fused(const FusedShape& fs, const SeedArgs& sa, const ChainArgs& ca, uint64_t* mins, uint32_t* ovf,
+                 uint32_t* novf, hipStream_t st) {
   const unsigned grid = (sa.n_reads + kFusedBlock - 1) / kFusedBlock;
-  hipLaunchKernelGGL((k_seed_chain_fused<10>), dim3(grid), dim3(kFusedBlock), fs.lds, st, sa, ca, fs.cap, fs.mslots,
-                     fs.maxmin, ovf, novf);
+  hipLaunchKernelGGL((k_seed_chain_fused<10>), dim3(grid), dim3(kFusedBlock), fs.lds, st, sa, ca, fs.cap, fs.maxmin,
+                     mins, ovf, novf);
   DN_HIP(hipGetLastError());
   return DNALN_OK;
 }
@@ -969,9 +972,14 @@ int run_chunk(dnaln_hip_batch* b, uint64_t r0, uint32_t n, bool first_chunk) {
     for (int x = 0; x < 2; ++x) {
       if (hi[x] <= lo[x]) continue;
       const FusedShape fs = fused_shape(c->k, c->w, std::min(b->split[x], b->maxlen));
+      DN_TRY(b->fmins.reserve(fused_mins_words(fs, hi[x] - lo[x]) * 8));
+    }
+    for (int x = 0; x < 2; ++x) {
+      if (hi[x] <= lo[x]) continue;
+      const FusedShape fs = fused_shape(c->k, c->w, std::min(b->split[x], b->maxlen));
       sa.list = lorder ? lorder + lo[x] : nullptr;
       sa.n_reads = (uint32_t)(hi[x] - lo[x]);
-      DN_TRY(launch_fused(fs, sa, ca, ovf, novf_d, st));
+      DN_TRY(launch_fused(fs, sa, ca, b->fmins.as<uint64_t>(), ovf, novf_d, st));
     }
     DN_TRY(b->span_end(sp_i));
     uint32_t novf = 0;
@@ -1520,6 +1528,11 @@ int dnaln_hip_batch_reserve(dnaln_hip_batch* b, uint64_t n, uint64_t bases, int3
   DN_TRY(b->lkeys.reserve(n * 4));
   DN_TRY(b->lorder.reserve(n * 4));
   DN_TRY(b->ovf.reserve(n * 4 + 8));
+  {
+    FusedShape fs;
+    fs.maxmin = 64;
+    DN_TRY(b->fmins.reserve(fused_mins_words(fs, n) * 8));
+  }
   DN_TRY(b->acnt.reserve(n * 4 + 4));
   DN_TRY(b->aoff.reserve(n * 8 + 8));
   DN_TRY(b->keys.reserve(na * 8 + 8));

@@ -20,12 +20,30 @@
 
 namespace dn {
 
+// The read's anchors in LDS, one u64 word each: anchor j of the thread at keys[j * stride], the
+// 64 threads of a wave interleaved. A word is the anchor key (strand << 63 | rpos << 32 | qpos)
+// with the chain DP's f and prev packed into the low dword above a 14-bit qpos: qpos bits 0..13,
+// f bits 14..24 (1 <= f <= cap * k <= 64 * 28 < 2^11), prev + 1 bits 25..31 (prev in -1..63).
+// f and prev are 0 until the DP writes them, so sorting the words sorts the keys.
 struct LdsView {
   uint64_t* keys;
-  int16_t* f;
-  int8_t* prev;
   int stride;
+  DN_HD uint64_t word(int j) const { return keys[j * stride]; }
+  // Low dword of anchor j (qpos, f, prev).
+  DN_HD void set_low(int j, uint32_t lo) const {
+#if defined(__HIP_DEVICE_COMPILE__)
+    reinterpret_cast<uint32_t*>(keys + j * stride)[0] = lo;
+#else
+    keys[j * stride] = (keys[j * stride] & ~0xffffffffull) | lo;
+#endif
+  }
 };
+DN_HD int w_qpos(uint64_t x) { return (int)(x & 0x3fffu); }
+DN_HD int w_f(uint64_t x) { return (int)((x >> 14) & 0x7ffu); }
+DN_HD int w_prev(uint64_t x) { return (int)((x >> 25) & 0x7fu) - 1; }
+DN_HD uint32_t w_low(int qpos, int f, int prev) {
+  return (uint32_t)qpos | (uint32_t)f << 14 | (uint32_t)(prev + 1) << 25;
+}
 
 struct ChainLite {
   int32_t score;
@@ -38,7 +56,7 @@ DN_HD int next_end(const LdsView& v, int n, uint64_t used) {
   int best = -1, bf = -1;
   for (int i = 0; i < n; ++i) {
     if ((used >> i) & 1) continue;
-    const int fi = v.f[i * v.stride];
+    const int fi = w_f(v.word(i));
     if (fi > bf) {
       bf = fi;
       best = i;
@@ -51,21 +69,23 @@ DN_HD int next_end(const LdsView& v, int n, uint64_t used) {
 DN_HD ChainLite walk_chain(const LdsView& v, int end, uint64_t& used, int& cnt) {
   int i = end;
   cnt = 0;
-  int64_t dmin = INT64_MAX, dmax = INT64_MIN;
+  int32_t dmin = INT32_MAX, dmax = INT32_MIN;
+  int f_stop = 0;
   while (i >= 0 && !((used >> i) & 1)) {
     used |= 1ull << i;
     ++cnt;
-    const uint64_t k = v.keys[i * v.stride];
-    const int64_t d = (int64_t)key_rpos(k) - key_qpos(k);
+    const uint64_t x = v.word(i);
+    const int32_t d = (int32_t)key_rpos(x) - w_qpos(x);
     dmin = d < dmin ? d : dmin;
     dmax = d > dmax ? d : dmax;
-    i = v.prev[i * v.stride];
+    i = w_prev(x);
   }
+  if (i >= 0) f_stop = w_f(v.word(i));
+  const uint64_t ke = v.word(end);
   ChainLite c;
-  c.score = v.f[end * v.stride] - (i >= 0 ? v.f[i * v.stride] : 0);
-  c.dmin = (int32_t)dmin;
-  c.dmax = (int32_t)dmax;
-  const uint64_t ke = v.keys[end * v.stride];
+  c.score = w_f(ke) - f_stop;
+  c.dmin = dmin;
+  c.dmax = dmax;
   c.rpos_strand = key_rpos(ke) | key_strand(ke) << 31;
   return c;
 }
@@ -86,24 +106,48 @@ DN_HD T hash_kmer(T key, T mask) {
   key = (key + (key << 31)) & mask;
   return key;
 }
+// The 32-bit form, with shift amounts offset by z, a value that is 0 at run time but unknown to
+// the compiler: it then keeps every a + (a << s) as one v_lshl_add_u32 instead of turning the
+// shift-add chains into v_mul_lo_u32, which issues at a quarter of the rate.
+DN_HD uint32_t hash_kmer32(uint32_t key, uint32_t mask, uint32_t z) {
+  key = (~key + (key << (21 + z))) & mask;
+  key = key ^ key >> 24;
+  key = ((key + (key << (3 + z))) + (key << (8 + z))) & mask;
+  key = key ^ key >> 14;
+  key = ((key + (key << (2 + z))) + (key << (4 + z))) & mask;
+  key = key ^ key >> 28;
+  key = (key + (key << (31 + z))) & mask;
+  return key;
+}
+template <class T>
+DN_HD T hash_k(T key, T mask, uint32_t) {
+  return hash_kmer<T>(key, mask);
+}
+template <>
+DN_HD uint32_t hash_k<uint32_t>(uint32_t key, uint32_t mask, uint32_t z) {
+  return hash_kmer32(key, mask, z);
+}
 
 // The reference's streamed van Herk / Gil-Werman window minimum (src/minimizer.cpp:5-68) with
-// the window length fixed at compile time: block and suffix minima are registers and every lane
-// of a wave runs the same block step, so the sketch does not diverge. T is uint32_t for k <= 15
-// (the hash then stays below the all-ones sentinel) and uint64_t otherwise. Appends the
-// minimizers of an N-free read to `mins`; returns 1, or 0 when the read holds an N (the caller
-// then uses sketch_stream), or -1 when the list would exceed maxmin.
+// the window length fixed at compile time, for a read without N: block and suffix minima are
+// registers, and the step of every k-mer is the same branch-free code on every lane (emission is
+// a predicated store). The minimizer of a window is its leftmost smallest hash, as the
+// reference's tie rules give: strict < within a block prefix, <= within a suffix, and the suffix
+// (the left part) wins a tie between the two. T is uint32_t for k <= 15 (the hash then stays
+// below the all-ones sentinel) and uint64_t otherwise. Appends the minimizers to `mins`; returns
+// 1, or -1 when the list would exceed maxmin. The reader is called for up to WT - 1 positions
+// past the read; what it returns there is never used.
 template <int WT, class T, class Reader>
 DN_HD int sketch_vh(Reader rd, int len, int k, uint64_t* mins, int stride, int maxmin, int& nmin) {
   const T mask = (T)((1ULL << (2 * k)) - 1);
   const int shift = 2 * (k - 1);
   const T kInvalid = ~(T)0;
+  const uint32_t z = (uint32_t)k >> 8;  // 0 (k <= 28)
   T fwd = 0, rev = 0;
   nmin = 0;
   if (len < k) return 1;
   for (int i = 0; i < k - 1; ++i) {
-    const uint8_t c = rd(i);
-    if (c > 3) return 0;
+    const uint32_t c = rd(i);
     fwd = ((fwd << 2) | c) & mask;
     rev = (rev >> 2) | ((T)(3 - c) << shift);
   }
@@ -112,44 +156,43 @@ DN_HD int sketch_vh(Reader rd, int len, int k, uint64_t* mins, int stride, int m
 #pragma unroll
   for (int j = 0; j < WT; ++j) bh[j] = sh[j] = kInvalid, bp[j] = sp[j] = 0;
   T ph = kInvalid;
-  uint32_t pp = 0;
-  int64_t last = -1;
+  uint32_t pp = 0, last = ~0u;
+  int n = 0;
   const int nk = len - k + 1;
   for (int x0 = 0; x0 < nk; x0 += WT) {
 #pragma unroll
     for (int j = 0; j < WT; ++j) {
       const int x = x0 + j;
-      if (x < nk) {
-        const uint8_t c = rd(x + k - 1);
-        if (c > 3) return 0;
-        fwd = ((fwd << 2) | c) & mask;
-        rev = (rev >> 2) | ((T)(3 - c) << shift);
-        const uint32_t strand = rev < fwd;
-        const T h = fwd != rev ? hash_kmer<T>(strand ? rev : fwd, mask) : kInvalid;
-        const uint32_t ps = (uint32_t)x << 1 | strand;
-        bh[j] = h;
-        bp[j] = ps;
-        if (j == 0 || h < ph) {
-          ph = h;
-          pp = ps;
-        }
-        if (x + 1 >= WT) {
-          T mh = ph;
-          uint32_t mps = pp;
-          if (j + 1 < WT && !(ph < sh[j + 1])) {
-            mh = sh[j + 1];
-            mps = sp[j + 1];
-          }
-          if (mh != kInvalid) {
-            const uint32_t p = mps >> 1;
-            if ((int64_t)p != last) {
-              if (nmin >= maxmin) return -1;
-              mins[(nmin++) * stride] = pack_min(mh, p, mps & 1);
-            }
-            last = p;
-          }
-        }
+      const uint32_t c = rd(x + k - 1);  // past the read (x >= nk) the k-mer is never used
+      fwd = ((fwd << 2) | c) & mask;
+      rev = (rev >> 2) | ((T)(3 - c) << shift);
+      const bool strand = rev < fwd;
+      const T h = fwd != rev ? hash_k<T>(strand ? rev : fwd, mask, z) : kInvalid;
+      const uint32_t ps = (uint32_t)x << 1 | (uint32_t)strand;
+      bh[j] = h;
+      bp[j] = ps;
+      if (j == 0) {
+        ph = h;
+        pp = ps;
+      } else {
+        const bool lt = h < ph;
+        ph = lt ? h : ph;
+        pp = lt ? ps : pp;
       }
+      // Window [x - WT + 1, x]: the suffix of the previous block, then this block's prefix.
+      T mh = ph;
+      uint32_t mps = pp;
+      if (j + 1 < WT) {
+        const bool suf = !(ph < sh[j + 1]);
+        mh = suf ? sh[j + 1] : ph;
+        mps = suf ? sp[j + 1] : pp;
+      }
+      const bool live = x + 1 >= WT && x < nk && mh != kInvalid;
+      const uint32_t p = mps >> 1;
+      const bool emit = live && p != last;
+      if (emit && n < maxmin) mins[n * stride] = pack_min(mh, p, mps & 1);
+      n += emit ? 1 : 0;
+      last = live ? p : last;
     }
     // block complete: suffix minima, right to left (ties keep the left k-mer)
     sh[WT - 1] = bh[WT - 1];
@@ -161,7 +204,8 @@ DN_HD int sketch_vh(Reader rd, int len, int k, uint64_t* mins, int stride, int m
       sp[r] = left ? bp[r] : sp[r + 1];
     }
   }
-  return 1;
+  nmin = n;
+  return n > maxmin ? -1 : 1;
 }
 
 // The bases of a read staged in LDS: the aligned 16-byte chunks around the read, stored as u64
@@ -176,16 +220,19 @@ struct LdsSeq {
   }
 };
 
-// Loads the read's chunks into `slots` u64 slots of `dst` (stride apart), sixteen chunks at a
-// time with all loads of a batch issued before its first wait. False when the read does not
-// fit. Reads up to 15 bytes before and after the read: the read buffer carries guard bytes.
-DN_HD bool stage_read(const uint8_t* seq, int len, uint64_t* dst, int stride, int slots, LdsSeq& out) {
+// Loads the read's chunks into the u64 slots of `dst` (stride apart), sixteen chunks at a time
+// with all loads of a batch issued before its first wait, and checks the read for N (codes
+// above 3) on the way. True when the read is staged and holds no N; false when it holds an N or
+// the slots cannot hold it plus `tail` bytes that a reader may look at past its end. Reads up
+// to 15 bytes before and after the read: the read buffer carries guard bytes.
+DN_HD bool stage_read(const uint8_t* seq, int len, int tail, uint64_t* dst, int stride, int slots, LdsSeq& out) {
   constexpr int kBatch = 16;
   const uintptr_t a = (uintptr_t)seq;
   const uint32_t off = (uint32_t)(a & 15);
   const int nch = (int)((off + (uint32_t)len + 15) >> 4);
-  if (2 * nch > slots) return false;
+  if ((int)((off + (uint32_t)(len + tail) + 7) >> 3) > slots || 2 * nch > slots) return false;
   const Quad* src = (const Quad*)(a - off);
+  uint32_t bad = 0;
   for (int b0 = 0; b0 < nch; b0 += kBatch) {
     Quad c[kBatch];
 #pragma unroll
@@ -196,12 +243,17 @@ DN_HD bool stage_read(const uint8_t* seq, int len, uint64_t* dst, int stride, in
       if (b0 + x < nch) {
         dst[(2 * (b0 + x)) * stride] = (uint64_t)c[x].y << 32 | c[x].x;
         dst[(2 * (b0 + x) + 1) * stride] = (uint64_t)c[x].w << 32 | c[x].z;
+        // Bytes of the read in each dword (the first and last chunks hold bytes of neighbours).
+        const int q = 16 * (b0 + x) - (int)off;  // read position of the chunk's byte 0
+        const uint32_t dw[4] = {c[x].x, c[x].y, c[x].z, c[x].w};
+#pragma unroll
+        for (int d = 0; d < 4; ++d) bad |= dw[d] & bytes_from(-(q + 4 * d)) & bytes_below(len - (q + 4 * d));
       }
   }
   out.words = dst;
   out.stride = stride;
   out.off = off;
-  return true;
+  return (bad & 0xFCFCFCFCu) == 0;
 }
 
 // Chain DP over sorted anchors in LDS (src/chain.cpp:21-45). Predecessors are visited nearest
@@ -213,20 +265,15 @@ DN_HD bool stage_read(const uint8_t* seq, int len, uint64_t* dst, int stride, in
 DN_HD void chain_dp_lds(const ChainArgs& a, const LdsView& v, int n, int k) {
   const int max_gap = a.max_gap, band = a.band;
   for (int i = 0; i < n; ++i) {
-    const uint64_t ki = v.keys[i * v.stride];
+    const uint64_t ki = v.word(i);
     const uint32_t si = key_strand(ki);
-    const int ri = (int)key_rpos(ki), qi = key_qpos(ki);
+    const int ri = (int)key_rpos(ki), qi = w_qpos(ki);
     int best = k, best_j = -1;
     const int jlo = i - a.lookback > 0 ? i - a.lookback : 0;
     for (int j0 = i - 1; j0 >= jlo; j0 -= 4) {
       uint64_t kj[4];
-      int fj[4];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int j = j0 - u >= jlo ? j0 - u : jlo;
-        kj[u] = v.keys[j * v.stride];
-        fj[u] = v.f[j * v.stride];
-      }
+      for (int u = 0; u < 4; ++u) kj[u] = v.word(j0 - u >= jlo ? j0 - u : jlo);
       bool alive = true;
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
@@ -234,18 +281,17 @@ DN_HD void chain_dp_lds(const ChainArgs& a, const LdsView& v, int n, int k) {
         const int dr = ri - (int)key_rpos(kj[u]);
         alive = j >= jlo && key_strand(kj[u]) == si && dr <= max_gap;
         const int drc = dr < 0 ? 0 : dr;
-        const int dq = qi - key_qpos(kj[u]);
+        const int dq = qi - w_qpos(kj[u]);
         const int dd = drc > dq ? drc - dq : dq - drc;
         const int m = dq < drc ? dq : drc;
-        const int sc = fj[u] + (m < k ? m : k) - gap_cost(a, dd <= band ? dd : band);
+        const int sc = w_f(kj[u]) + (m < k ? m : k) - gap_cost(a, dd <= band ? dd : band);
         const bool take = alive && dr > 0 && dq > 0 && dd <= band && sc > best;
         best = take ? sc : best;
         best_j = take ? j : best_j;
       }
       if (!alive) break;
     }
-    v.f[i * v.stride] = (int16_t)best;
-    v.prev[i * v.stride] = (int8_t)best_j;
+    v.set_low(i, w_low(qi, best, best_j));
   }
 }
 
@@ -314,32 +360,32 @@ DN_HD int chain_extract_lds(const ChainArgs& a, const LdsView& v, int n, uint32_
   return nc;
 }
 
-// Phases per read (one thread): 1. minimizers into an LDS list; 2. probes, four minimizers at a
-// time, anchors into LDS; 3. sort, DP, extraction, candidates. `mins` holds maxmin entries and
-// is dead once the probes are done, so v.f and v.prev may alias it. `ring` (the fallback sketch
-// for reads with N or a window length other than WT) may alias the anchor keys, which are free
-// until phase 2. Returns the candidate count or -1 (the read does not fit; global path).
+// Phases per read (one thread): 1. minimizers into a list (`mins`, maxmin entries mstride
+// apart: the thread's column of a global scratch array); 2. probes, four minimizers at a time,
+// anchors into LDS; 3. sort, DP, extraction, candidates. `ring` (the fallback sketch for reads
+// with N or a window length other than WT) may alias the anchor words, which are free until
+// phase 2. Returns the candidate count or -1 (the read does not fit; global path).
 template <int MAXCH, int WT>
-DN_HD int seed_chain_lds(const SeedArgs& s, const ChainArgs& a, uint32_t r, LdsView v, uint64_t* mins, int maxmin,
-                         Ring ring, int cap) {
+DN_HD int seed_chain_lds(const SeedArgs& s, const ChainArgs& a, uint32_t r, LdsView v, uint64_t* mins, int mstride,
+                         int maxmin, Ring ring, int cap) {
   const int len = s.lens[r], k = s.k;
-  if (len >= 65536 || 2 * k > 46) return -1;
+  if (len >= 16384 || 2 * k > 46) return -1;
   const uint8_t* seq = s.codes + s.offs[r];
   int nmin = 0;
   int got = 0;
   LdsSeq ls;
   // The read's bases go to LDS first (into the anchor slots, free until phase 2), so the
   // sketch waits for memory once instead of at every chunk boundary.
-  if (s.w == WT && stage_read(seq, len, v.keys, v.stride, cap, ls))
-    got = k <= 15 ? sketch_vh<WT, uint32_t>(ls, len, k, mins, v.stride, maxmin, nmin)
-                  : sketch_vh<WT, uint64_t>(ls, len, k, mins, v.stride, maxmin, nmin);
+  if (s.w == WT && stage_read(seq, len, WT, v.keys, v.stride, cap, ls))
+    got = k <= 15 ? sketch_vh<WT, uint32_t>(ls, len, k, mins, mstride, maxmin, nmin)
+                  : sketch_vh<WT, uint64_t>(ls, len, k, mins, mstride, maxmin, nmin);
   if (got < 0) return -1;
   if (got == 0) {
     nmin = 0;
     bool full = false;
     sketch_stream(ChunkReader(seq), len, k, s.w, ring, [&](uint64_t h, uint32_t p, uint32_t st) {
       if (nmin >= maxmin) full = true;
-      else mins[(nmin++) * v.stride] = pack_min(h, p, st);
+      else mins[(nmin++) * mstride] = pack_min(h, p, st);
     });
     if (full) return -1;
   }
@@ -352,7 +398,7 @@ DN_HD int seed_chain_lds(const SeedArgs& s, const ChainArgs& a, uint32_t r, LdsV
     Bucket b[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      e[u] = m0 + u < nmin ? mins[(m0 + u) * v.stride] : 0;
+      e[u] = m0 + u < nmin ? mins[(m0 + u) * mstride] : 0;
       if (m0 + u < nmin) b[u] = s.table[(e[u] >> 17) & s.mask];
     }
     uint32_t off[4], cnt[4], first[4];
@@ -390,15 +436,15 @@ DN_HD int seed_chain_lds(const SeedArgs& s, const ChainArgs& a, uint32_t r, LdsV
   }
   if (overflow) return -1;
   const int nrev = cap - 1 - hi, n = lo + nrev;
-  for (int x = 0; x < nrev; ++x) v.keys[(lo + x) * v.stride] = v.keys[(hi + 1 + x) * v.stride];
+  for (int x = 0; x < nrev; ++x) v.keys[(lo + x) * v.stride] = v.word(hi + 1 + x);
   if (n == 0) return 0;
 
   // 3. sort (nearly sorted: insertion sort), DP, extraction, candidates
   for (int i = 1; i < n; ++i) {
-    const uint64_t x = v.keys[i * v.stride];
+    const uint64_t x = v.word(i);
     int j = i - 1;
-    while (j >= 0 && x < v.keys[j * v.stride]) {
-      v.keys[(j + 1) * v.stride] = v.keys[j * v.stride];
+    while (j >= 0 && x < v.word(j)) {
+      v.keys[(j + 1) * v.stride] = v.word(j);
       --j;
     }
     v.keys[(j + 1) * v.stride] = x;

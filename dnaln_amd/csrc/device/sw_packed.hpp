@@ -131,10 +131,6 @@ DN_HD uint32_t align4(uint32_t lo, uint32_t hi, uint32_t sh) {
 #endif
 }
 
-// Byte masks over a word's four bytes: bytes k..3, and bytes 0..k-1 (k clamped to 0..4).
-DN_HD uint32_t bytes_from(int k) { return k <= 0 ? ~0u : k >= 4 ? 0u : ~0u << (8 * k); }
-DN_HD uint32_t bytes_below(int k) { return ~bytes_from(k); }
-
 // Rows per fill slot: the traceback fill runs its rows in steps of up to 4, so a slot holds
 // the wave's longest query rounded up to a multiple of 4 (rows past a query are never read).
 DN_HD uint64_t fill_rows(int qmax) { return (uint64_t)(((qmax > 1 ? qmax : 1) + 3) & ~3); }

@@ -264,8 +264,6 @@ long emu_seed_chain(const uint8_t* codes, const uint64_t* offs, const int32_t* l
   std::vector<uint32_t> rps(sa.w);
   Ring ring{rh.data(), rps.data(), 1};
   std::vector<uint64_t> keys(cap), mins(maxmin), gkeys, order;
-  std::vector<int16_t> f16(cap);
-  std::vector<int8_t> p8(cap);
   std::vector<int32_t> f32, prev;
   std::vector<ChainRec> chains;
   long differ = 0;
@@ -306,8 +304,8 @@ long emu_seed_chain(const uint8_t* codes, const uint64_t* offs, const int32_t* l
     // LDS path
     ChainArgs l = ca;
     l.cands = c2.data();
-    LdsView v{keys.data(), f16.data(), p8.data(), 1};
-    const int nc = seed_chain_lds<8, 10>(sa, l, (uint32_t)r, v, mins.data(), maxmin, ring, cap);
+    LdsView v{keys.data(), 1};
+    const int nc = seed_chain_lds<8, 10>(sa, l, (uint32_t)r, v, mins.data(), 1, maxmin, ring, cap);
     if (nc < 0) {
       ++stats[0];
       continue;

@@ -110,10 +110,11 @@ struct dnaln_server {
 
   uint64_t batch_bytes(uint64_t n, uint64_t bases, int32_t maxlen) const {
     // Inputs, per-read state, candidates and tasks, CIGAR slots, anchors of reads on the
-    // global path, and the traceback directions of one SW launch (4 bytes per group of 8
-    // band cells per row per alignment), bounded by the per-launch direction budget.
+    // global path, minimizer lists of the fused kernel, and the traceback directions of one SW
+    // launch (4 bytes per group of 8 band cells per row per alignment), bounded by the
+    // per-launch direction budget.
     const uint64_t G = (uint64_t)(2 * bw + 1 + 7) / 8;
-    const uint64_t per_read = 16 + sizeof(dnaln_record) + 64 + (uint64_t)(max_cand + 1) * 96 + 64 * 48;
+    const uint64_t per_read = 16 + sizeof(dnaln_record) + 64 + (uint64_t)(max_cand + 1) * 96 + 64 * 48 + 64 * 8;
     const uint64_t dirs = std::min<uint64_t>(bases * 4 * G + (uint64_t)maxlen * G * 512, cfg.dir_budget);
     return bases + 256 + n * per_read + n * (2 * (uint64_t)maxlen + 4) * 4 + dirs;
   }
