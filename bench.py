@@ -194,13 +194,16 @@ def serve_sweep(seconds: float, levels: list[float], slots: int) -> dict:
         sizes, arrive = synth.request_arrivals(load * peak / mean_size, seconds, 1000 + i)
         starts = np.array([rng.integers(0, len(rb) - s + 1) for s in sizes], np.uint64)
         st0 = srv.stats()
+        al0 = api.device_allocs()
         lat, wall, _, _ = srv.run_trace(rb, starts, arrive, req_count=sizes.astype(np.uint64))
+        al1 = api.device_allocs()
         st = srv.stats()
         batches = st["batches"] - st0["batches"]
         row = {"load": load, "requests": int(len(sizes)), "offered_reads_per_s": float(sizes.sum() / seconds),
                "achieved_reads_per_s": float(sizes.sum() / wall) if wall > 0 else None,
                "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
-               "batches": int(batches), "mean_reads_per_batch": float((st["reads"] - st0["reads"]) / max(batches, 1))}
+               "batches": int(batches), "mean_reads_per_batch": float((st["reads"] - st0["reads"]) / max(batches, 1)),
+               "device_allocs": al1[0] - al0[0], "device_alloc_bytes": al1[1] - al0[1]}
         out.append(row)
         print(f"[bench] serve load {load:.1f}: {row['requests']} requests, p50 {row['p50_ms']:.2f} ms "
               f"p99 {row['p99_ms']:.2f} ms, {row['achieved_reads_per_s']:.0f} reads/s", file=sys.stderr, flush=True)

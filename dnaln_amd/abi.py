@@ -88,6 +88,7 @@ HIP_SYMBOLS = {
     "dnaln_hip_device_memory": (i32, [vp, vp, vp]),
     "dnaln_hip_batch_reserve": (i32, [vp, u64, u64, i32, u32]),
     "dnaln_hip_context_params": (i32, [vp, vp]),
+    "dnaln_hip_alloc_counters": (None, [vp, vp]),
 }
 
 class ServeConfig(C.Structure):

@@ -42,6 +42,13 @@ def encode(seq_ascii: np.ndarray) -> np.ndarray:
     return _ENCODE[seq_ascii]
 
 
+def device_allocs() -> tuple[int, int]:
+    """Device allocations the library has made in this process: (count, bytes)."""
+    ev, by = C.c_uint64(), C.c_uint64()
+    _lib.hip().dnaln_hip_alloc_counters(C.byref(ev), C.byref(by))
+    return int(ev.value), int(by.value)
+
+
 def map_params(**kw) -> abi.MapParams:
     p = abi.MapParams()
     _lib.hip().dnaln_hip_default_params(C.byref(p))

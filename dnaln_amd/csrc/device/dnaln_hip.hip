@@ -9,6 +9,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -50,6 +51,9 @@ int fail(int code, const std::string& msg) {
   } while (0)
 
 // Grow-only device buffer.
+// Device allocations made by DevBuf in this process (count, bytes).
+std::atomic<uint64_t> g_alloc_events{0}, g_alloc_bytes{0};
+
 struct DevBuf {
   void* p = nullptr;
   size_t cap = 0;
@@ -70,6 +74,8 @@ struct DevBuf {
       return fail(DNALN_E_NOMEM, "hipMalloc of " + std::to_string(want) + " bytes failed");
     }
     cap = want;
+    g_alloc_events.fetch_add(1, std::memory_order_relaxed);
+    g_alloc_bytes.fetch_add(want, std::memory_order_relaxed);
     return DNALN_OK;
   }
   template <class T>
@@ -335,9 +341,11 @@ __global__ __launch_bounds__(kSwBlock) void k_sw_packed(SwArgs a) {
 // Traceback with one thread per task (walk_task).
 template <int W>
 __global__ __launch_bounds__(256) void k_walk(SwArgs a) {
+  __shared__ uint32_t seqbuf[32 * 256];
   const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= a.n) return;
-  walk_task<W>(a, k);
+  WalkSeq seq(seqbuf + threadIdx.x, 256);
+  walk_task<W>(a, k, seq);
 }
 
 // Traceback of the 128 tasks of one fill wave by one block, through an LDS window of `rows`
@@ -351,6 +359,7 @@ __global__ __launch_bounds__(128) void k_walk_win(SwArgs a, int rows) {
   constexpr int G = (W + 7) / 8;
   extern __shared__ uint64_t win[];
   __shared__ int s_top;
+  __shared__ uint32_t seqbuf[32 * 128];
   const int t = threadIdx.x, lane = t & 63, half = t >> 6;
   const uint32_t wave = blockIdx.x;
   const uint32_t k = wave * 128 + t;
@@ -359,7 +368,7 @@ __global__ __launch_bounds__(128) void k_walk_win(SwArgs a, int rows) {
   int score = 0, i_end = 0, d_end = 0;
   SwTask tk{};
   Walk w{};
-  WalkSeq seq;
+  WalkSeq seq(seqbuf + t, 128);
   uint32_t* out = nullptr;
   if (k < a.n) {
     id = a.order ? a.order[k] : k;
@@ -1595,6 +1604,11 @@ int dnaln_hip_context_params(dnaln_hip_ctx* c, dnaln_map_params* out) {
   if (!c || !out) return fail(DNALN_E_INVALID, "null argument");
   *out = c->p;
   return DNALN_OK;
+}
+
+void dnaln_hip_alloc_counters(uint64_t* events, uint64_t* bytes) {
+  if (events) *events = g_alloc_events.load(std::memory_order_relaxed);
+  if (bytes) *bytes = g_alloc_bytes.load(std::memory_order_relaxed);
 }
 
 int dnaln_hip_map_batch(dnaln_hip_ctx* c, const uint8_t* read_codes, const uint64_t* offs, const int32_t* lens,

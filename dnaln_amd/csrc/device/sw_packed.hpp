@@ -230,24 +230,57 @@ struct Walk {
   bool done;
 };
 
-// The aligned dword last loaded from one sequence: the walk moves through the query and the
-// target one position at a time, so most bases come from a register. (Measured on MI355X: a
-// 16-byte cache refilled every 16 steps walks about 40% slower than this one.)
-struct CachedDword {
+// 16 bytes at the 16-aligned address a (one global_load_dwordx4).
+DN_HD Quad load16(uint64_t a) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  typedef const __attribute__((address_space(1))) Quad* gquad;
+  return *(gquad)a;
+#else
+  return *(const Quad*)(uintptr_t)a;
+#endif
+}
+
+// The 64 aligned bytes of one sequence around the walk's position, in memory private to the
+// walking thread (LDS on the device): dword x of the window at buf[x * stride], the threads of a
+// block interleaved, so every byte read is free of bank conflicts. The walk moves one position
+// per step, so a window is refilled (four 16-byte loads) about every 64 steps. (Measured on
+// MI355X: a dword cached in a register, refilled every 4 steps, makes the walk 4x slower; with a
+// wave's worth of streams per SIMD those refills miss L2.) Windows reach up to 63 bytes before
+// and after the bytes read: every sequence buffer carries guard bytes.
+struct SeqWindow {
+  uint32_t* buf = nullptr;
+  int stride = 1;
   uint64_t at = ~0ull;
-  uint32_t w = 0;
   DN_HD uint8_t get(uint64_t addr) {
-    const uint64_t a = addr & ~(uint64_t)3;
+    const uint64_t a = addr & ~(uint64_t)63;
     if (a != at) {
-      w = *(const uint32_t*)(uintptr_t)a;
+      Quad c[4];
+#pragma unroll
+      for (int x = 0; x < 4; ++x) c[x] = load16(a + 16 * x);
+#pragma unroll
+      for (int x = 0; x < 4; ++x) {
+        buf[(4 * x) * stride] = c[x].x;
+        buf[(4 * x + 1) * stride] = c[x].y;
+        buf[(4 * x + 2) * stride] = c[x].z;
+        buf[(4 * x + 3) * stride] = c[x].w;
+      }
       at = a;
     }
-    return (uint8_t)(w >> (8 * (addr & 3)));
+    const uint32_t x = (uint32_t)(addr & 63);
+    return (uint8_t)(buf[(x >> 2) * stride] >> (8 * (x & 3)));
   }
 };
 
+// The query and target windows of one walk: 32 dwords at buf, stride apart.
 struct WalkSeq {
-  CachedDword q, t;
+  SeqWindow q, t;
+  WalkSeq() = default;
+  DN_HD WalkSeq(uint32_t* buf, int stride) {
+    q.buf = buf;
+    q.stride = stride;
+    t.buf = buf + 16 * stride;
+    t.stride = stride;
+  }
   DN_HD uint8_t qb(const SwTask& tk, int i) {
     if (i < 0 || i >= tk.qlen) return 4;
     if (tk.flags & 1) return comp_code(q.get(tk.q + (uint64_t)(tk.qlen - 1 - i)));
@@ -534,7 +567,7 @@ DN_HD void sw_pair(const SwArgs& a, uint32_t ida, uint32_t idb, bool va, bool vb
 // launch-relative position of the task (the fill's wave k / 128 holds it, in lane k % 64, half
 // (k / 64) % 2): one thread per task reads its flag words straight from the slot.
 template <int W>
-DN_HD void walk_task(const SwArgs& a, uint32_t k) {
+DN_HD void walk_task(const SwArgs& a, uint32_t k, WalkSeq& seq) {
   const uint32_t id = a.order ? a.order[k] : k;
   SwResult& r = a.res[id];
   const int score = r.score, i_end = r.qe, d_end = r.te;
@@ -547,7 +580,7 @@ DN_HD void walk_task(const SwArgs& a, uint32_t k) {
   const uint64_t* slot_lane = dir_slot(a, wave) + lane;
   uint32_t* out = a.cigar + (uint64_t)id * (uint64_t)a.cigar_cap;
   Walk w = walk_init(i_end, d_end, score);
-  WalkSeq seq;
+  seq.q.at = seq.t.at = ~0ull;
   while (!w.done) walk_step<W>(w, (int)half, tk, slot_lane, a.sp, out, a.cigar_cap, seq);
   walk_finish(w, i_end, d_end, score, a.cigar_cap, r);
 }

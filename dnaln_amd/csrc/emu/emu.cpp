@@ -32,10 +32,12 @@ void walk_wave_window(const SwArgs& a, uint32_t wave, int rows) {
     int score = 0, i_end = 0, d_end = 0;
     SwTask tk{};
     Walk w{};
+    uint32_t buf[32];
     WalkSeq seq;
     uint32_t* out = nullptr;
   };
   std::vector<Th> th(128);
+  for (Th& x : th) x.seq = WalkSeq(x.buf, 1);
   for (int t = 0; t < 128; ++t) {
     const uint32_t k = wave * 128 + t;
     if (k >= a.n) continue;
@@ -101,7 +103,11 @@ void run_packed(const SwArgs& a0) {
     for (uint32_t wv = 0; wv < waves; ++wv) {
       if (wv % 2 == 0) walk_wave_window<W>(a, wv, 3);
       else
-        for (uint32_t k = wv * 128; k < a.n && k < wv * 128 + 128; ++k) walk_task<W>(a, k);
+        for (uint32_t k = wv * 128; k < a.n && k < wv * 128 + 128; ++k) {
+          uint32_t buf[32];
+          WalkSeq seq(buf, 1);
+          walk_task<W>(a, k, seq);
+        }
     }
   }
 }

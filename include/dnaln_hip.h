@@ -138,6 +138,9 @@ const char* dnaln_hip_last_error(void);
 int dnaln_hip_device_memory(struct dnaln_hip_ctx* ctx, uint64_t* free_bytes, uint64_t* total_bytes);
 /* The context's parameters after validation (the chain band clamped to the SW band). */
 int dnaln_hip_context_params(struct dnaln_hip_ctx* ctx, dnaln_map_params* out);
+/* Device allocations the library has made in this process (count and bytes), for callers that
+ * check that a warmed-up path allocates nothing (no reference counterpart). */
+void dnaln_hip_alloc_counters(uint64_t* events, uint64_t* bytes);
 
 /* Aligner::align: n independent banded alignments; traceback = 0 computes the score only.
  * sp may be NULL (use the context's scoring). Results are bit-identical to the reference's

@@ -58,10 +58,14 @@ def test_each_request_same_alone_and_in_the_mix(name):
     mix.close()
     mix = api.Server(ctx, slots=4, max_batch_reads=1024, hbm_budget=2 * budget_probe + 1)
     arrivals = np.zeros(n_req)
+    allocs = api.device_allocs()
     lat, wall, recs, pool = mix.run_trace(rb, first, arrivals, want_records=True)
+    allocs_after = api.device_allocs()
     st = mix.stats()
     mix.close()
-    print("stats", st, "wall", wall, flush=True)
+    print("stats", st, "wall", wall, "device allocations while serving", allocs_after[0] - allocs[0], flush=True)
+    # slots are sized when the server starts: serving allocates no device memory
+    assert allocs_after == allocs
     assert st["completed"] == n_req and st["max_inflight_bytes"] <= st["budget_bytes"]
     assert st["max_inflight_batches"] >= 2  # batches overlapped
     assert np.all(lat >= 0)
