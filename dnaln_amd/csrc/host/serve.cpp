@@ -19,14 +19,23 @@ double now_s() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
+// A request's reads: offsets relative to `codes`. Either its own copy of the caller's arrays
+// (dnaln_serve_submit) or a view of arrays that outlive it (the trace replay).
 struct Request {
   uint64_t id = 0;
-  std::vector<uint8_t> codes;
-  std::vector<uint64_t> offs;
-  std::vector<int32_t> lens;
-  std::vector<uint32_t> fnv;
+  std::vector<uint8_t> own_codes;
+  std::vector<uint64_t> own_offs;
+  std::vector<int32_t> own_lens;
+  std::vector<uint32_t> own_fnv;
+  const uint8_t* codes = nullptr;
+  const uint64_t* offs = nullptr;
+  const int32_t* lens = nullptr;
+  const uint32_t* fnv = nullptr;
+  uint64_t n = 0;
+  uint64_t code_bytes = 0;  // bytes of codes the reads span
   uint64_t bases = 0;
   int32_t maxlen = 0;
+  bool view = false;
 };
 
 struct Done {
@@ -36,8 +45,10 @@ struct Done {
   std::vector<uint32_t> cig;
 };
 
-// A batch slot and its worker thread. The main thread hands a batch over (state 1) and takes
-// it back finished (state 2); only the worker touches the device batch in between.
+// A batch slot and its worker thread. The main thread hands a batch of requests over (state 1)
+// and takes it back finished (state 2); in between only the worker touches the slot: it gathers
+// the requests into one upload, runs the batch, and splits the records back into requests, so
+// the copying of every batch runs on its own thread.
 struct Slot {
   dnaln_hip_batch* b = nullptr;
   std::mutex m;
@@ -46,10 +57,10 @@ struct Slot {
   bool quit = false;
   int rc = 0;
   std::string err;
-  double t_done = 0;
   std::thread worker;
   uint64_t est = 0;
-  std::vector<uint64_t> req_id, req_first, req_n;
+  std::vector<Request> reqs;  // the batch
+  std::vector<Done> out;      // its requests, finished
   std::vector<uint8_t> codes;
   std::vector<uint64_t> offs;
   std::vector<int32_t> lens;
@@ -57,12 +68,58 @@ struct Slot {
   std::vector<dnaln_record> rec;
   std::vector<uint32_t> cig;
 
+  void gather() {
+    uint64_t n = 0, nb = 0;
+    for (const Request& q : reqs) n += q.n, nb += q.code_bytes;
+    codes.resize(nb);
+    offs.resize(n);
+    lens.resize(n);
+    fnv.resize(n);
+    uint64_t r0 = 0, c0 = 0;
+    for (const Request& q : reqs) {
+      if (q.code_bytes) std::memcpy(codes.data() + c0, q.codes, q.code_bytes);
+      for (uint64_t i = 0; i < q.n; ++i) offs[r0 + i] = c0 + q.offs[i];
+      if (q.n) {
+        std::memcpy(lens.data() + r0, q.lens, q.n * sizeof(int32_t));
+        std::memcpy(fnv.data() + r0, q.fnv, q.n * sizeof(uint32_t));
+      }
+      r0 += q.n;
+      c0 += q.code_bytes;
+    }
+  }
+
+  void split() {
+    out.clear();
+    out.resize(reqs.size());
+    uint64_t r0 = 0;
+    for (size_t x = 0; x < reqs.size(); ++x) {
+      Done& d = out[x];
+      d.id = reqs[x].id;
+      const uint64_t n = reqs[x].n;
+      d.rec.assign(rec.begin() + r0, rec.begin() + r0 + n);
+      uint64_t nc = 0;
+      for (const dnaln_record& m : d.rec) nc += m.cigar_len;
+      d.cig.resize(nc);
+      uint64_t o = 0;
+      for (dnaln_record& m : d.rec) {
+        std::copy(cig.begin() + m.cigar_off, cig.begin() + m.cigar_off + m.cigar_len, d.cig.begin() + o);
+        m.cigar_off = o;
+        o += m.cigar_len;
+      }
+      r0 += n;
+    }
+    const double t = now_s();
+    for (Done& d : out) d.t_done = t;
+    reqs.clear();
+  }
+
   void work() {
     for (;;) {
       std::unique_lock<std::mutex> lk(m);
       cv.wait(lk, [&] { return state == 1 || quit; });
       if (state != 1) return;
       lk.unlock();
+      gather();
       int r = dnaln_hip_batch_upload(b, codes.data(), offs.data(), lens.data(), fnv.data(), lens.size());
       if (r == DNALN_OK) r = dnaln_hip_batch_run(b);
       if (r == DNALN_OK) r = dnaln_hip_batch_poll(b, 1);
@@ -74,10 +131,10 @@ struct Slot {
         r = dnaln_hip_batch_download(b, rec.data(), cig.data(), cig.size(), nullptr);
       }
       const std::string e = r == DNALN_OK ? std::string() : std::string(dnaln_hip_last_error());
+      if (r == DNALN_OK) split();
       lk.lock();
       rc = r;
       err = e;
-      t_done = now_s();
       state = 2;
     }
   }
@@ -132,24 +189,30 @@ struct dnaln_server {
       st.inflight_bytes -= s.est;
       if (s.rc != DNALN_OK) {
         last_err = s.err;
+        s.reqs.clear();
         return s.rc;
       }
-      for (size_t r = 0; r < s.req_id.size(); ++r) {
-        Done out;
-        out.id = s.req_id[r];
-        out.t_done = s.t_done;
-        out.rec.assign(s.rec.begin() + s.req_first[r], s.rec.begin() + s.req_first[r] + s.req_n[r]);
-        for (dnaln_record& m : out.rec) {
-          const uint64_t o = out.cig.size();
-          out.cig.insert(out.cig.end(), s.cig.begin() + m.cigar_off, s.cig.begin() + m.cigar_off + m.cigar_len);
-          m.cigar_off = o;
-        }
-        done.push_back(std::move(out));
+      for (Done& d : s.out) {
+        done.push_back(std::move(d));
         ++finished;
         ++st.completed;
       }
+      s.out.clear();
     }
     return finished;
+  }
+
+  // Drops the queued requests that are views of a caller's arrays and waits until no batch is
+  // running, so that nothing refers to those arrays any more (a trace replay ending early).
+  void drop_views() {
+    std::deque<Request> keep;
+    for (Request& q : queue)
+      if (!q.view) keep.push_back(std::move(q));
+    queue.swap(keep);
+    while (st.inflight_batches > 0) {
+      (void)collect();
+      std::this_thread::yield();
+    }
   }
 
   int admit() {
@@ -166,7 +229,7 @@ struct dnaln_server {
       uint64_t est = 0;
       while (k < queue.size()) {
         const Request& q = queue[k];
-        const uint64_t n2 = n + q.lens.size(), b2 = bases + q.bases;
+        const uint64_t n2 = n + q.n, b2 = bases + q.bases;
         const int32_t m2 = std::max(maxlen, q.maxlen);
         if (k > 0 && (n2 > cfg.max_batch_reads || b2 > cfg.max_batch_bases)) break;
         const uint64_t e2 = batch_bytes(n2, b2, m2);
@@ -181,23 +244,11 @@ struct dnaln_server {
         ++st.deferred_by_budget;
         return DNALN_OK;
       }
-      s.codes.clear();
-      s.offs.clear();
-      s.lens.clear();
-      s.fnv.clear();
-      s.req_id.clear();
-      s.req_first.clear();
-      s.req_n.clear();
+      s.reqs.clear();
+      uint64_t reads = 0;
       for (uint64_t r = 0; r < k; ++r) {
-        Request& q = queue.front();
-        s.req_id.push_back(q.id);
-        s.req_first.push_back(s.lens.size());
-        s.req_n.push_back(q.lens.size());
-        const uint64_t base = s.codes.size();
-        s.codes.insert(s.codes.end(), q.codes.begin(), q.codes.end());
-        for (uint64_t o : q.offs) s.offs.push_back(base + o);
-        s.lens.insert(s.lens.end(), q.lens.begin(), q.lens.end());
-        s.fnv.insert(s.fnv.end(), q.fnv.begin(), q.fnv.end());
+        reads += queue.front().n;
+        s.reqs.push_back(std::move(queue.front()));
         queue.pop_front();
       }
       s.est = est;
@@ -207,7 +258,7 @@ struct dnaln_server {
       }
       s.cv.notify_one();
       st.batches += 1;
-      st.reads += s.lens.size();
+      st.reads += reads;
       st.inflight_batches += 1;
       st.inflight_bytes += est;
       st.max_inflight_batches = std::max(st.max_inflight_batches, st.inflight_batches);
@@ -276,8 +327,11 @@ uint64_t dnaln_serve_batch_bytes(const dnaln_server* s, uint64_t n, uint64_t bas
   return s ? s->batch_bytes(n, bases, maxlen) : 0;
 }
 
-int dnaln_serve_submit(dnaln_server* s, uint64_t id, const uint8_t* codes, const uint64_t* offs, const int32_t* lens,
-                       const uint32_t* fnv, uint64_t n) {
+namespace {
+
+// Checks a request and queues it, as a copy of the caller's arrays or as a view of them.
+int submit_request(dnaln_server* s, uint64_t id, const uint8_t* codes, const uint64_t* offs, const int32_t* lens,
+                   const uint32_t* fnv, uint64_t n, bool copy) {
   if (!s || (n && (!codes || !offs || !lens || !fnv))) return DNALN_E_INVALID;
   Request q;
   q.id = id;
@@ -291,13 +345,34 @@ int dnaln_serve_submit(dnaln_server* s, uint64_t id, const uint8_t* codes, const
   if (n > s->cfg.max_batch_reads || q.bases > s->cfg.max_batch_bases ||
       s->batch_bytes(n, q.bases, q.maxlen) > s->st.budget_bytes)
     return DNALN_E_CAPACITY;
-  q.codes.assign(codes, codes + end);
-  q.offs.assign(offs, offs + n);
-  q.lens.assign(lens, lens + n);
-  q.fnv.assign(fnv, fnv + n);
+  q.n = n;
+  q.code_bytes = end;
+  if (copy) {
+    q.own_codes.assign(codes, codes + end);
+    q.own_offs.assign(offs, offs + n);
+    q.own_lens.assign(lens, lens + n);
+    q.own_fnv.assign(fnv, fnv + n);
+    q.codes = q.own_codes.data();
+    q.offs = q.own_offs.data();
+    q.lens = q.own_lens.data();
+    q.fnv = q.own_fnv.data();
+  } else {
+    q.codes = codes;
+    q.offs = offs;
+    q.lens = lens;
+    q.fnv = fnv;
+    q.view = true;
+  }
   s->queue.push_back(std::move(q));
   s->st.submitted += 1;
   return DNALN_OK;
+}
+
+}  // namespace
+
+int dnaln_serve_submit(dnaln_server* s, uint64_t id, const uint8_t* codes, const uint64_t* offs, const int32_t* lens,
+                       const uint32_t* fnv, uint64_t n) {
+  return submit_request(s, id, codes, offs, lens, fnv, n, true);
 }
 
 int dnaln_serve_progress(dnaln_server* s) {
@@ -343,7 +418,16 @@ int dnaln_serve_run_trace(dnaln_server* s, const uint8_t* codes, const uint64_t*
     return DNALN_E_INVALID;
   for (uint64_t r = 0; r < n_req; ++r)
     if (req_first[r] + req_count[r] > n_reads || (r && arrival_s[r] < arrival_s[r - 1])) return DNALN_E_INVALID;
+  // Request offsets relative to each request's first base, for all requests up front: the
+  // requests are queued as views of these arrays and of the caller's, which outlive the replay.
   std::vector<uint64_t> local_offs;
+  std::vector<uint64_t> local_at(n_req + 1, 0);
+  for (uint64_t r = 0; r < n_req; ++r) local_at[r + 1] = local_at[r] + req_count[r];
+  local_offs.resize(local_at[n_req]);
+  for (uint64_t r = 0; r < n_req; ++r) {
+    const uint64_t a = req_first[r], n = req_count[r], base = n ? offs[a] : 0;
+    for (uint64_t i = 0; i < n; ++i) local_offs[local_at[r] + i] = offs[a + i] - base;
+  }
   uint64_t next = 0, finished = 0, cig_used = 0;
   const double t0 = now_s();
   double t_last = t0;
@@ -351,16 +435,20 @@ int dnaln_serve_run_trace(dnaln_server* s, const uint8_t* codes, const uint64_t*
     const double t = now_s() - t0;
     while (next < n_req && arrival_s[next] <= t) {
       const uint64_t a = req_first[next], n = req_count[next];
-      // A request's reads as their own buffer: offsets relative to its first base.
       const uint64_t base = n ? offs[a] : 0;
-      local_offs.resize(n);
-      for (uint64_t i = 0; i < n; ++i) local_offs[i] = offs[a + i] - base;
-      const int rc = dnaln_serve_submit(s, next, codes + base, local_offs.data(), lens + a, fnv + a, n);
-      if (rc != DNALN_OK) return rc;
+      const int rc = submit_request(s, next, codes + base, local_offs.data() + local_at[next], lens + a, fnv + a, n,
+                                    false);
+      if (rc != DNALN_OK) {
+        s->drop_views();
+        return rc;
+      }
       ++next;
     }
     const int rc = dnaln_serve_progress(s);
-    if (rc < 0) return rc;
+    if (rc < 0) {
+      s->drop_views();
+      return rc;
+    }
     while (!s->done.empty()) {
       Done& d = s->done.front();
       const uint64_t r = d.id;
@@ -370,7 +458,10 @@ int dnaln_serve_run_trace(dnaln_server* s, const uint8_t* codes, const uint64_t*
         const uint64_t a = req_first[r];
         for (size_t i = 0; i < d.rec.size(); ++i) {
           dnaln_record m = d.rec[i];
-          if (cig_used + m.cigar_len > cigar_cap) return DNALN_E_CAPACITY;
+          if (cig_used + m.cigar_len > cigar_cap) {
+            s->drop_views();
+            return DNALN_E_CAPACITY;
+          }
           std::copy(d.cig.begin() + m.cigar_off, d.cig.begin() + m.cigar_off + m.cigar_len, cigar_pool + cig_used);
           m.cigar_off = cig_used;
           cig_used += m.cigar_len;
