@@ -72,6 +72,17 @@ struct alignas(16) Quad {
   uint32_t x, y, z, w;
 };
 
+// 16 bytes at the 16-aligned global address a (one global_load_dwordx4; an address held as an
+// integer would otherwise be read with a generic flat load).
+DN_HD Quad load16(uint64_t a) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  typedef const __attribute__((address_space(1))) Quad* gquad;
+  return *(gquad)a;
+#else
+  return *(const Quad*)(uintptr_t)a;
+#endif
+}
+
 // Store of data the kernel itself never reads again (traceback flags, read back by the walk
 // much later): non-temporal on the device, so it streams past the caches that hold the
 // sequences.

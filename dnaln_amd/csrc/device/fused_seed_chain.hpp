@@ -231,13 +231,13 @@ DN_HD bool stage_read(const uint8_t* seq, int len, int tail, uint64_t* dst, int 
   const uint32_t off = (uint32_t)(a & 15);
   const int nch = (int)((off + (uint32_t)len + 15) >> 4);
   if ((int)((off + (uint32_t)(len + tail) + 7) >> 3) > slots || 2 * nch > slots) return false;
-  const Quad* src = (const Quad*)(a - off);
+  const uint64_t src = (uint64_t)(a - off);
   uint32_t bad = 0;
   for (int b0 = 0; b0 < nch; b0 += kBatch) {
     Quad c[kBatch];
 #pragma unroll
     for (int x = 0; x < kBatch; ++x)
-      if (b0 + x < nch) c[x] = src[b0 + x];
+      if (b0 + x < nch) c[x] = load16(src + 16 * (uint64_t)(b0 + x));
 #pragma unroll
     for (int x = 0; x < kBatch; ++x)
       if (b0 + x < nch) {
@@ -256,20 +256,27 @@ DN_HD bool stage_read(const uint8_t* seq, int len, int tail, uint64_t* dst, int 
   return (bad & 0xFCFCFCFCu) == 0;
 }
 
-// Chain DP over sorted anchors in LDS (src/chain.cpp:21-45). Predecessors are visited nearest
-// first in groups of four whose loads are issued together. The reference's early stops (a
-// strand change, a reference gap above max_gap) are monotone in j on sorted anchors, so each
-// predecessor of a group is tested on its own, without branches, and the scan ends after the
-// first group whose farthest predecessor is stopped. Arithmetic is 32-bit: rpos < 2^31,
-// qpos < 2^16, and drift is only used where 0 < dr <= max_gap.
-DN_HD void chain_dp_lds(const ChainArgs& a, const LdsView& v, int n, int k) {
-  const int max_gap = a.max_gap, band = a.band;
+// Chain DP over sorted anchors in LDS (src/chain.cpp:21-45). The anchors are sorted by strand
+// first, so the predecessors that the reference reaches before its strand break are the same-
+// strand ones: j >= the strand's first anchor (nf, the forward-strand count, for the reverse
+// strand). Within a strand rpos is sorted, so dr = ri - rj is the difference of the words' high
+// halves (the strand bits cancel) and never negative, and the reference's stop at dr > max_gap
+// is monotone in j. Predecessors are visited nearest first in groups of four whose loads are
+// issued together; each is tested on its own, without branches, and the scan ends after the
+// first group whose farthest predecessor is stopped. NIB: gap costs from the four-bit table
+// (drift < 16) and cost_at_band (drift == band), valid when cost_nib_ok; a drift above the band
+// is never taken, so its cost does not matter. Arithmetic is 32-bit: rpos < 2^31, qpos < 2^14.
+template <bool NIB>
+DN_HD void chain_dp_lds(const ChainArgs& a, const LdsView& v, int n, int nf, int k) {
+  const uint32_t max_gap = (uint32_t)a.max_gap;
+  const int band = a.band;
   for (int i = 0; i < n; ++i) {
     const uint64_t ki = v.word(i);
-    const uint32_t si = key_strand(ki);
-    const int ri = (int)key_rpos(ki), qi = w_qpos(ki);
+    const uint32_t hi = (uint32_t)(ki >> 32);
+    const int qi = w_qpos(ki);
     int best = k, best_j = -1;
-    const int jlo = i - a.lookback > 0 ? i - a.lookback : 0;
+    const int s0 = i < nf ? 0 : nf;
+    const int jlo = i - a.lookback > s0 ? i - a.lookback : s0;
     for (int j0 = i - 1; j0 >= jlo; j0 -= 4) {
       uint64_t kj[4];
 #pragma unroll
@@ -278,14 +285,22 @@ DN_HD void chain_dp_lds(const ChainArgs& a, const LdsView& v, int n, int k) {
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         const int j = j0 - u;
-        const int dr = ri - (int)key_rpos(kj[u]);
-        alive = j >= jlo && key_strand(kj[u]) == si && dr <= max_gap;
-        const int drc = dr < 0 ? 0 : dr;
+        const uint32_t dr = hi - (uint32_t)(kj[u] >> 32);
+        alive = j >= jlo && dr <= max_gap;
         const int dq = qi - w_qpos(kj[u]);
-        const int dd = drc > dq ? drc - dq : dq - drc;
-        const int m = dq < drc ? dq : drc;
-        const int sc = w_f(kj[u]) + (m < k ? m : k) - gap_cost(a, dd <= band ? dd : band);
-        const bool take = alive && dr > 0 && dq > 0 && dd <= band && sc > best;
+        const uint32_t t = dr - (uint32_t)dq;                      // dr - dq, modulo 2^32
+        const uint32_t dd = (int32_t)t < 0 ? 0u - t : t;           // |dr - dq| where it matters
+        int cost;
+        if (NIB) {
+          const int c = (int)((a.cost_nib >> (4 * (dd & 15))) & 15u);
+          cost = dd == (uint32_t)band ? a.cost_at_band : c;
+        } else {
+          cost = gap_cost(a, dd <= (uint32_t)band ? (int)dd : band);
+        }
+        const int drk = dr < (uint32_t)k ? (int)dr : k;
+        const int gain = dq < drk ? dq : drk;
+        const int sc = w_f(kj[u]) + gain - cost;
+        const bool take = alive && dr > 0 && dq > 0 && dd <= (uint32_t)band && sc > best;
         best = take ? sc : best;
         best_j = take ? j : best_j;
       }
@@ -390,20 +405,21 @@ DN_HD int seed_chain_lds(const SeedArgs& s, const ChainArgs& a, uint32_t r, LdsV
     if (full) return -1;
   }
 
-  // 2. probes, four minimizers in flight
+  // 2. probes, PG minimizers in flight
+  constexpr int PG = 8;
   int lo = 0, hi = cap - 1;
   bool overflow = false;
-  for (int m0 = 0; m0 < nmin && !overflow; m0 += 4) {
-    uint64_t e[4];
-    Bucket b[4];
+  for (int m0 = 0; m0 < nmin && !overflow; m0 += PG) {
+    uint64_t e[PG];
+    Bucket b[PG];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < PG; ++u) {
       e[u] = m0 + u < nmin ? mins[(m0 + u) * mstride] : 0;
       if (m0 + u < nmin) b[u] = s.table[(e[u] >> 17) & s.mask];
     }
-    uint32_t off[4], cnt[4], first[4];
+    uint32_t off[PG], cnt[PG], first[PG];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < PG; ++u) {
       off[u] = cnt[u] = first[u] = 0;
       if (m0 + u >= nmin) continue;
       const uint64_t h = e[u] >> 17;
@@ -417,7 +433,7 @@ DN_HD int seed_chain_lds(const SeedArgs& s, const ChainArgs& a, uint32_t r, LdsV
       if (cnt[u]) first[u] = s.pos[off[u]];
     }
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < PG; ++u) {
       if (!cnt[u] || overflow) continue;
       if ((int)cnt[u] > hi - lo + 1) {
         overflow = true;
@@ -449,7 +465,8 @@ DN_HD int seed_chain_lds(const SeedArgs& s, const ChainArgs& a, uint32_t r, LdsV
     }
     v.keys[(j + 1) * v.stride] = x;
   }
-  chain_dp_lds(a, v, n, k);
+  if (a.cost_nib_ok) chain_dp_lds<true>(a, v, n, lo, k);
+  else chain_dp_lds<false>(a, v, n, lo, k);
   return chain_extract_lds<MAXCH>(a, v, n, r);
 }
 

@@ -318,6 +318,9 @@ struct ChainArgs {
   const int32_t* cost;     // gap cost by |dr - dq|, 0..band
   uint64_t cost_pk[4];     // the same, one byte per drift 0..31, when cost_packed
   int cost_packed;         // band <= 31 and every cost in [0, 255]
+  uint64_t cost_nib;       // the same, four bits per drift 0..15, when cost_nib_ok
+  int cost_at_band;        // cost[band]
+  int cost_nib_ok;         // band <= 16 and every cost in [0, 15]
   int k, max_gap, band, lookback, min_score, min_anchors;
   int max_cand, bw;
   const uint64_t* contig_off;
@@ -373,8 +376,16 @@ DN_HD int gap_cost(const ChainArgs& a, int64_t dd) {
   return a.cost[dd];
 }
 
-// Packs cost[0..band] into a.cost_pk when band <= 31 and all costs fit a byte.
+// Packs cost[0..band] into a.cost_pk when band <= 31 and all costs fit a byte, and into
+// a.cost_nib (drifts 0..15) and a.cost_at_band when band <= 16 and all costs fit four bits.
 inline void pack_gap_cost(ChainArgs& a, const int32_t* host_cost, int band) {
+  a.cost_nib = 0;
+  a.cost_at_band = band >= 0 ? host_cost[band] : 0;
+  a.cost_nib_ok = band >= 0 && band <= 16;
+  for (int d = 0; d <= band && d <= 16; ++d) {
+    if (host_cost[d] < 0 || host_cost[d] > 15) a.cost_nib_ok = 0;
+    else if (d < 16) a.cost_nib |= (uint64_t)host_cost[d] << (4 * d);
+  }
   a.cost_packed = 0;
   a.cost_pk[0] = a.cost_pk[1] = a.cost_pk[2] = a.cost_pk[3] = 0;
   if (band > 31) return;

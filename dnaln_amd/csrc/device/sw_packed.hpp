@@ -230,16 +230,6 @@ struct Walk {
   bool done;
 };
 
-// 16 bytes at the 16-aligned address a (one global_load_dwordx4).
-DN_HD Quad load16(uint64_t a) {
-#if defined(__HIP_DEVICE_COMPILE__)
-  typedef const __attribute__((address_space(1))) Quad* gquad;
-  return *(gquad)a;
-#else
-  return *(const Quad*)(uintptr_t)a;
-#endif
-}
-
 // The 64 aligned bytes of one sequence around the walk's position, in memory private to the
 // walking thread (LDS on the device): dword x of the window at buf[x * stride], the threads of a
 // block interleaved, so every byte read is free of bank conflicts. The walk moves one position

@@ -108,14 +108,15 @@ def test_emulated_packed_kernel_reverse_queries_and_clipped_windows(emu):
             assert cig[offs[i]:offs[i + 1]].tolist() == exp[6], (bw, i)
 
 
-def _emu_seed_chain(emu, spec, g, r, cap, maxmin):
+def _emu_seed_chain(emu, spec, g, r, cap, maxmin, band=None):
     from dnaln_amd import api
     mp = sets.map_params(spec)
     k, w, bw = mp["k"], mp["w"], mp["bw"]
     idx = api.Index(g, k, w)
     tab, pos = idx.table(), idx.positions()
     rb = api.prepare(r)
-    band = min(16, bw)  # ChainParams::band, clamped to the SW band (src/mapper.cpp:9)
+    if band is None:
+        band = min(16, bw)  # ChainParams::band, clamped to the SW band (src/mapper.cpp:9)
     cost = np.array([0] + [int(0.01 * k * dd + 0.5 * (dd.bit_length() - 1)) for dd in range(1, band + 1)], np.int32)
     m = np.array([k, w, mp["max_occ"], bw, mp["max_cand"], mp["min_chain"], 50, 5000, band, 2], np.int32)
     st = np.zeros(3, np.uint64)
@@ -139,5 +140,16 @@ def test_emulated_fused_seed_chain_matches_global_path(emu, name, cap, maxmin):
     g = sets.genome(spec)
     r = sets.reads(spec, g)
     differ, declined = _emu_seed_chain(emu, spec, g, r, cap, maxmin)
+    assert differ == 0
+    assert declined < len(r)
+
+
+def test_emulated_fused_seed_chain_with_a_chain_band_above_16(emu):
+    """A chain band above 16 (a band option of the reference's ChainParams) takes the fused DP's
+    byte-table gap costs instead of the four-bit table; candidates still match the global path."""
+    spec = sets.SETS["edge_uniform_linear_bw21"]
+    g = sets.genome(spec)
+    r = sets.reads(spec, g)
+    differ, declined = _emu_seed_chain(emu, spec, g, r, 64, 56, band=21)
     assert differ == 0
     assert declined < len(r)
