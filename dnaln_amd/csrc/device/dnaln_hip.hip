@@ -341,7 +341,7 @@ __global__ __launch_bounds__(kSwBlock) void k_sw_packed(SwArgs a) {
 // Traceback with one thread per task (walk_task).
 template <int W>
 __global__ __launch_bounds__(256) void k_walk(SwArgs a) {
-  __shared__ uint32_t seqbuf[32 * 256];
+  __shared__ uint32_t seqbuf[kWalkSeqDwords * 256];
   const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= a.n) return;
   WalkSeq seq(seqbuf + threadIdx.x, 256);
@@ -359,7 +359,7 @@ __global__ __launch_bounds__(128) void k_walk_win(SwArgs a, int rows) {
   constexpr int G = (W + 7) / 8;
   extern __shared__ uint64_t win[];
   __shared__ int s_top;
-  __shared__ uint32_t seqbuf[32 * 128];
+  __shared__ uint32_t seqbuf[kWalkSeqDwords * 128];
   const int t = threadIdx.x, lane = t & 63, half = t >> 6;
   const uint32_t wave = blockIdx.x;
   const uint32_t k = wave * 128 + t;

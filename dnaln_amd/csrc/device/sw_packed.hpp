@@ -230,25 +230,31 @@ struct Walk {
   bool done;
 };
 
-// The 64 aligned bytes of one sequence around the walk's position, in memory private to the
-// walking thread (LDS on the device): dword x of the window at buf[x * stride], the threads of a
-// block interleaved, so every byte read is free of bank conflicts. The walk moves one position
-// per step, so a window is refilled (four 16-byte loads) about every 64 steps. (Measured on
-// MI355X: a dword cached in a register, refilled every 4 steps, makes the walk 4x slower; with a
-// wave's worth of streams per SIMD those refills miss L2.) Windows reach up to 63 bytes before
-// and after the bytes read: every sequence buffer carries guard bytes.
+// The kSeqWindow aligned bytes of one sequence around the walk's position, in memory private to
+// the walking thread (LDS on the device): dword x of the window at buf[x * stride], the threads
+// of a block interleaved, so every byte read is free of bank conflicts. The walk moves one
+// position per step, so a window is refilled (16-byte loads) every kSeqWindow steps. Measured on
+// MI355X (2M x 150 bp, W = 33): a dword cached in a register and refilled every 4 steps walks in
+// 2.7 ms, since with a wave's worth of streams per SIMD its refills miss L2; 64-byte windows in
+// 1.8 ms, 32 bytes in 1.2 ms, 16 bytes in 1.24 ms (the walk is latency-bound, so the LDS each
+// thread holds, through occupancy, matters more than how often it refills); prefetching the
+// next 64 bytes into registers cost more occupancy than it saved (2.6 ms). Windows reach up to
+// kSeqWindow - 1 bytes before and after the bytes read: every sequence buffer carries guard
+// bytes.
+constexpr int kSeqWindow = 32;  // bytes
+
 struct SeqWindow {
   uint32_t* buf = nullptr;
   int stride = 1;
   uint64_t at = ~0ull;
   DN_HD uint8_t get(uint64_t addr) {
-    const uint64_t a = addr & ~(uint64_t)63;
+    const uint64_t a = addr & ~(uint64_t)(kSeqWindow - 1);
     if (a != at) {
-      Quad c[4];
+      Quad c[kSeqWindow / 16];
 #pragma unroll
-      for (int x = 0; x < 4; ++x) c[x] = load16(a + 16 * x);
+      for (int x = 0; x < kSeqWindow / 16; ++x) c[x] = load16(a + 16 * x);
 #pragma unroll
-      for (int x = 0; x < 4; ++x) {
+      for (int x = 0; x < kSeqWindow / 16; ++x) {
         buf[(4 * x) * stride] = c[x].x;
         buf[(4 * x + 1) * stride] = c[x].y;
         buf[(4 * x + 2) * stride] = c[x].z;
@@ -256,19 +262,20 @@ struct SeqWindow {
       }
       at = a;
     }
-    const uint32_t x = (uint32_t)(addr & 63);
+    const uint32_t x = (uint32_t)(addr & (kSeqWindow - 1));
     return (uint8_t)(buf[(x >> 2) * stride] >> (8 * (x & 3)));
   }
 };
 
-// The query and target windows of one walk: 32 dwords at buf, stride apart.
+// The query and target windows of one walk: kWalkSeqDwords dwords at buf, stride apart.
+constexpr int kWalkSeqDwords = 2 * kSeqWindow / 4;
 struct WalkSeq {
   SeqWindow q, t;
   WalkSeq() = default;
   DN_HD WalkSeq(uint32_t* buf, int stride) {
     q.buf = buf;
     q.stride = stride;
-    t.buf = buf + 16 * stride;
+    t.buf = buf + (kSeqWindow / 4) * stride;
     t.stride = stride;
   }
   DN_HD uint8_t qb(const SwTask& tk, int i) {

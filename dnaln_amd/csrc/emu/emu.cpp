@@ -32,7 +32,7 @@ void walk_wave_window(const SwArgs& a, uint32_t wave, int rows) {
     int score = 0, i_end = 0, d_end = 0;
     SwTask tk{};
     Walk w{};
-    uint32_t buf[32];
+    uint32_t buf[kWalkSeqDwords];
     WalkSeq seq;
     uint32_t* out = nullptr;
   };
@@ -104,7 +104,7 @@ void run_packed(const SwArgs& a0) {
       if (wv % 2 == 0) walk_wave_window<W>(a, wv, 3);
       else
         for (uint32_t k = wv * 128; k < a.n && k < wv * 128 + 128; ++k) {
-          uint32_t buf[32];
+          uint32_t buf[kWalkSeqDwords];
           WalkSeq seq(buf, 1);
           walk_task<W>(a, k, seq);
         }
