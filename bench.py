@@ -143,8 +143,10 @@ def summarize_pmc(fetch_csv: str, write_csv: str, calib: int) -> int:
         fill = [v for k, vs in per.items() if k.startswith("void (anonymous namespace)::k_sw_packed<") and ", true>" in k
                 for v in vs]
         walk = [v for k, vs in per.items() if "k_walk<" in k for v in vs]
+        fused = [v for k, vs in per.items() if "k_seed_chain_fused<" in k for v in vs]
         out[counter] = {"copy_kib": max(copy) if copy else None, "fill_kib_per_launch": sum(fill) / len(fill) if fill else None,
-                        "walk_kib_per_launch": sum(walk) / len(walk) if walk else None, "fill_launches": len(fill)}
+                        "walk_kib_per_launch": sum(walk) / len(walk) if walk else None, "fill_launches": len(fill),
+                        "fused_kib_per_launch": sum(fused) / len(fused) if fused else None}
     res = {"counters": out}
     for counter in ("FETCH_SIZE", "WRITE_SIZE"):
         c = out[counter]
@@ -156,9 +158,10 @@ def summarize_pmc(fetch_csv: str, write_csv: str, calib: int) -> int:
             w["fill_kib_per_launch"] * 1024 * w["scale"]
         res["k_sw_packed_tb_read_bytes"] = f["fill_kib_per_launch"] * 1024 * f["scale"]
         res["k_sw_packed_tb_write_bytes"] = w["fill_kib_per_launch"] * 1024 * w["scale"]
-        if f["walk_kib_per_launch"] is not None and w["walk_kib_per_launch"] is not None:
-            res["k_walk_bytes_per_launch"] = f["walk_kib_per_launch"] * 1024 * f["scale"] + \
-                w["walk_kib_per_launch"] * 1024 * w["scale"]
+        for key, name in (("walk_kib_per_launch", "k_walk_bytes_per_launch"),
+                          ("fused_kib_per_launch", "k_seed_chain_fused_bytes_per_launch")):
+            if f[key] is not None and w[key] is not None:
+                res[name] = f[key] * 1024 * f["scale"] + w[key] * 1024 * w["scale"]
     res["copy_bytes"] = calib
     (ROOT / "profiles").mkdir(exist_ok=True)
     (ROOT / "profiles" / "traffic.json").write_text(json.dumps(res, indent=1) + "\n")
@@ -299,8 +302,19 @@ def main() -> int:
                 "fill_plus_walk_frac": (tb_ops / ((tb_ms + walk_ms) * 1e-3) / P16_PEAK) if tb_ms + walk_ms > 0 else None,
                 "score_fill_frac": (rank_cells * OPS_PER_CELL["score"] / (rank_ms * 1e-3) / P16_PEAK) if rank_ms > 0 else None}
     tpath = ROOT / "profiles" / "traffic.json"
+    hbm = None
     if tpath.exists():
-        roofline["traffic"] = json.loads(tpath.read_text()).get("k_sw_packed_tb_bytes_per_launch")
+        tr = json.loads(tpath.read_text())
+        roofline["traffic"] = tr.get("k_sw_packed_tb_bytes_per_launch")
+        # HBM-side view of the two memory-bound kernels: counted bytes per launch (from the
+        # committed counter runs) over this run's measured time per launch.
+        seed_ms, n_seed = fam[0], max(batched["launches"][0], 1)
+        hbm = {"peak_gbps": HBM_PEAK / 1e9}
+        for name, key, ms in (("k_seed_chain_fused", "k_seed_chain_fused_bytes_per_launch", seed_ms / n_seed),
+                              ("k_walk", "k_walk_bytes_per_launch", walk_ms / max(batched["launches"][4], 1))):
+            if tr.get(key) and ms > 0:
+                gbps = tr[key] / (ms * 1e-3) / 1e9
+                hbm[name] = {"bytes_per_launch": tr[key], "ms_per_launch": ms, "gbps": gbps, "frac": gbps * 1e9 / HBM_PEAK}
 
     out = {
         "metric": METRIC, "value": batched["reads_per_s"], "unit": "reads/s", "n_gpus": 1, "steps": a.steps,
@@ -315,6 +329,7 @@ def main() -> int:
         "gcups_per_step_wall": (rank_cells + tb_cells) / (batched["ms_per_step"] * 1e-3) / 1e9,
         "cells": {"rank": rank_cells, "traceback": tb_cells},
         "stage_ms": dict(zip(api.Batch.FAMILIES, fam)),
+        "hbm": hbm,
         "plain_stage_ms": dict(zip(api.Batch.FAMILIES, plain["family_ms"]))
         if plain else None,
         "autotune": {"len_split": batched["tune"]["len_split"], "dir_budget": batched["tune"]["dir_budget"],

@@ -414,6 +414,15 @@ __global__ void k_sw_generic(SwGenericArgs a) {
   sw_generic_one(a, a.order ? a.order[tid] : tid, tid);
 }
 
+// The device copy of the index: a bucket whose minimizer occurs once holds that occurrence
+// itself in `offset`, so a probe of a unique minimizer is one random access instead of two.
+__global__ void k_inline_unique(Bucket* table, uint64_t n, const uint32_t* pos) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const Bucket b = table[i];
+  if (b.key != ~0ull && b.count == 1) table[i].offset = pos[b.offset];
+}
+
 __global__ void k_pool_tasks(const dnaln_task* in, uint64_t pool, int bw, SwTask* out, uint32_t n) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -927,6 +936,7 @@ int run_chunk(dnaln_hip_batch* b, uint64_t r0, uint32_t n, bool first_chunk) {
   sa.table = c->table.as<Bucket>();
   sa.mask = c->mask;
   sa.pos = c->pos.as<uint32_t>();
+  sa.pos_inline = 1;
   sa.k = c->k;
   sa.w = c->w;
   sa.max_occ = p.max_occ;
@@ -1141,6 +1151,12 @@ int dnaln_hip_create(const dnaln_map_params* params, const uint8_t* ref_codes, u
   DN_HIP(hipMemcpy(c->table.p, index->table, index->n_buckets * sizeof(dnaln_bucket), hipMemcpyHostToDevice));
   DN_TRY(c->pos.reserve(index->n_pos * 4 + 4));
   if (index->n_pos) DN_HIP(hipMemcpy(c->pos.p, index->pos, index->n_pos * 4, hipMemcpyHostToDevice));
+  if (index->n_buckets) {
+    hipLaunchKernelGGL(k_inline_unique, dim3((unsigned)((index->n_buckets + 255) / 256)), dim3(256), 0, nullptr,
+                       c->table.as<Bucket>(), (uint64_t)index->n_buckets, c->pos.as<uint32_t>());
+    DN_HIP(hipGetLastError());
+    DN_HIP(hipDeviceSynchronize());
+  }
   // Chain gap cost by drift dd (src/chain.cpp:36), evaluated in double exactly as the reference.
   std::vector<int32_t> cost(c->p.ch.band + 1, 0);
   for (int dd = 1; dd <= c->p.ch.band; ++dd)

@@ -430,7 +430,7 @@ DN_HD int seed_chain_lds(const SeedArgs& s, const ChainArgs& a, uint32_t r, LdsV
         probe_from(s.table, s.mask, h, ((h & s.mask) + 1) & s.mask, off[u], cnt[u]);
       }
       if (cnt[u] > s.max_occ) cnt[u] = 0;
-      if (cnt[u]) first[u] = s.pos[off[u]];
+      if (cnt[u]) first[u] = first_pos(s, off[u], cnt[u]);
     }
 #pragma unroll
     for (int u = 0; u < PG; ++u) {

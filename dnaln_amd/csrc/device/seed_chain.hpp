@@ -33,6 +33,7 @@ struct SeedArgs {
   const Bucket* table;
   uint64_t mask;
   const uint32_t* pos;
+  int pos_inline;           // a bucket with count 1 holds its occurrence itself in `offset`
   int k, w;
   uint32_t max_occ;
   uint32_t* anchor_count;   // seed_count output, by list position
@@ -53,21 +54,9 @@ DN_HD uint64_t hash64(uint64_t key, uint64_t mask) {  // src/minimizer.hpp:21-30
   return key;
 }
 
-DN_HD void probe(const Bucket* table, uint64_t mask, uint64_t h, uint32_t& off, uint32_t& cnt) {
-  uint64_t i = h & mask;
-  for (;;) {
-    const Bucket b = table[i];
-    if (b.key == h) {
-      off = b.offset;
-      cnt = b.count;
-      return;
-    }
-    if (b.key == ~0ULL) {
-      off = cnt = 0;
-      return;
-    }
-    i = (i + 1) & mask;
-  }
+// First occurrence of a minimizer whose bucket gave (off, cnt >= 1).
+DN_HD uint32_t first_pos(const SeedArgs& s, uint32_t off, uint32_t cnt) {
+  return s.pos_inline && cnt == 1 ? off : s.pos[off];
 }
 
 // Ring of the last w k-mers: hash and (pos << 1 | strand), at ring[slot * stride].
@@ -233,7 +222,7 @@ struct ProbePipe {
         ms2 = ms1;
         off2 = off;
         cnt2 = cnt;
-        first2 = NeedPos ? s.pos[off] : 0u;
+        first2 = NeedPos ? first_pos(s, off, cnt) : 0u;
         v2 = true;
       }
       v1 = false;

@@ -243,9 +243,14 @@ long emu_seed_chain(const uint8_t* codes, const uint64_t* offs, const int32_t* l
   sa.codes = gcodes;
   sa.offs = offs;
   sa.lens = lens;
-  sa.table = static_cast<const Bucket*>(table);
+  // The device's index layout: unique minimizers hold their occurrence in the bucket.
+  std::vector<Bucket> tab(static_cast<const Bucket*>(table), static_cast<const Bucket*>(table) + n_buckets);
+  for (Bucket& b : tab)
+    if (b.key != ~0ull && b.count == 1) b.offset = pos[b.offset];
+  sa.table = tab.data();
   sa.mask = n_buckets - 1;
   sa.pos = pos;
+  sa.pos_inline = 1;
   sa.k = mp[0];
   sa.w = mp[1];
   sa.max_occ = (uint32_t)mp[2];
