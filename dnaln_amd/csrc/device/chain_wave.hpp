@@ -86,9 +86,14 @@ __device__ __forceinline__ void chain_wave_read(const ChainArgs& a, uint32_t r, 
   // 1. anchors in (strand, rpos, qpos) order
   wave_sort(s.keys, n, lane);
 
-  // 2. DP (src/chain.cpp:21-45)
+  // 2. DP (src/chain.cpp:21-45). 32-bit arithmetic: rpos < 2^31 and qpos < 2^31, and
+  // dr, dq and their difference are only used where 0 < dr <= max_gap and 0 < dq. With
+  // a.cost_nib_ok the gap cost comes from the four-bit table in the kernel arguments (scalar
+  // registers), which keeps a memory round trip off the serial chain of steps.
   const int k = a.k;
-  const int64_t max_gap = a.max_gap, band = a.band;
+  const uint32_t max_gap = (uint32_t)a.max_gap;
+  const int band = a.band;
+  const bool nib = a.cost_nib_ok;
   constexpr int kBias = 1 << 20;
   uint64_t kw = 0;  // key of anchor i-1-lane
   int fw = 0;       // f of anchor i-1-lane
@@ -96,18 +101,22 @@ __device__ __forceinline__ void chain_wave_read(const ChainArgs& a, uint32_t r, 
   for (int i = 0; i < n; ++i) {
     const uint64_t ki = knext;
     if (i + 1 < n) knext = s.keys[i + 1];
-    const uint32_t si = key_strand(ki);
-    const int64_t ri = key_rpos(ki), qi = key_qpos(ki);
+    const uint32_t hi_i = (uint32_t)(ki >> 32);
+    const int qi = key_qpos(ki);
     const int j = i - 1 - lane;
     uint32_t cand = 0;
     if (lane < a.lookback && j >= 0) {
-      const int64_t dr = ri - key_rpos(kw);
-      const int64_t dq = qi - key_qpos(kw);
-      const int64_t dd = dr > dq ? dr - dq : dq - dr;
-      const bool alive = key_strand(kw) == si && dr <= max_gap;
-      if (alive && dr > 0 && dq > 0 && dd <= band) {
-        const int64_t m = dq < dr ? dq : dr;
-        const int sc = fw + (int)(m < k ? m : k) - cost[dd];
+      const uint32_t hj = (uint32_t)(kw >> 32);
+      const uint32_t dr = hi_i - hj;  // same strand: the strand bits cancel
+      const int dq = qi - key_qpos(kw);
+      const uint32_t t = dr - (uint32_t)dq;
+      const uint32_t dd = (int32_t)t < 0 ? 0u - t : t;
+      const bool alive = ((hi_i ^ hj) >> 31) == 0 && dr <= max_gap;
+      if (alive && dr > 0 && dq > 0 && dd <= (uint32_t)band) {
+        const int m = dq < (int)dr ? dq : (int)dr;
+        const int c = nib ? (dd == (uint32_t)band ? a.cost_at_band : (int)((a.cost_nib >> (4 * (dd & 15))) & 15u))
+                          : cost[dd];
+        const int sc = fw + (m < k ? m : k) - c;
         cand = (uint32_t)(sc + kBias) << 6 | (uint32_t)(63 - lane);
       }
     }
