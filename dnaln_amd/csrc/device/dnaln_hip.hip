@@ -335,7 +335,13 @@ __global__ __launch_bounds__(kSwBlock) void k_sw_packed(SwArgs a) {
   int ql = max(va ? a.tasks[ida].qlen : 0, vb ? a.tasks[idb].qlen : 0);
   ql = wave_max(ql);
   uint64_t* slot = TB ? dir_slot(a, wave) : nullptr;
-  sw_pair<W, TB>(a, ida, idb, va, vb, ql, slot, lane);
+  if (TB && W > kBestRowInRegs) {
+    extern __shared__ uint16_t bh_rows[];  // per wave of the block: W x 64 lanes x {A, B}
+    uint16_t* bh = bh_rows + (size_t)(threadIdx.x >> 6) * best_row_lds_words(W) * 2 + 2 * lane;
+    sw_pair<W, TB>(a, ida, idb, va, vb, ql, slot, lane, bh, 64);
+  } else {
+    sw_pair<W, TB>(a, ida, idb, va, vb, ql, slot, lane);
+  }
 }
 
 // Traceback with one thread per task (walk_task).
@@ -439,12 +445,20 @@ __global__ void k_pool_tasks(const dnaln_task* in, uint64_t pool, int bw, SwTask
   out[i] = s;
 }
 
+// Threads per block of the packed fill: four waves, or one where each wave's best row sits in
+// LDS (so a block stays within 64 KiB of LDS).
+constexpr int sw_block(int W, bool TB) { return TB && W > kBestRowInRegs ? 64 : kSwBlock; }
+
+// grid: blocks of kSwBlock threads.
 template <bool TB>
 bool launch_packed_bw(int bw, const SwArgs& a, unsigned grid, hipStream_t st) {
   switch (bw) {
 #define DN_CASE(B)                                                                    \
   case B:                                                                             \
-    hipLaunchKernelGGL((k_sw_packed<2 * B + 1, TB>), dim3(grid), dim3(kSwBlock), 0, st, a); \
+    hipLaunchKernelGGL((k_sw_packed<2 * B + 1, TB>), dim3(grid * (kSwBlock / sw_block(2 * B + 1, TB))),  \
+                       dim3(sw_block(2 * B + 1, TB)),                                                     \
+                       TB ? (size_t)best_row_lds_words(2 * B + 1) * 4 * (sw_block(2 * B + 1, TB) / 64) : 0, \
+                       st, a);                                                                            \
     return true;
     DN_PACKED_BW_LIST(DN_CASE)
 #undef DN_CASE

@@ -409,9 +409,15 @@ DN_HD void walk_finish(const Walk& w, int i_end, int d_end, int score, int cap, 
 // One lane: tasks ida and idb (va / vb false for an empty half), rows 0..qmax-1 of the wave.
 // Score-only: writes the scores. With flags (TB): stores the flag words into the wave's slot
 // and writes the score and the best cell (row in qe, diagonal in te) for the walk.
+// Traceback fills wider than this keep the saved best row in LDS (bh_lds) instead of VGPRs.
+constexpr int kBestRowInRegs = 33;
+
+// Best-row words per fill wave when the row lives in LDS (one u32 per band cell and lane).
+DN_HD int best_row_lds_words(int W) { return W > kBestRowInRegs ? W * 64 : 0; }
+
 template <int W, bool TB>
 DN_HD void sw_pair(const SwArgs& a, uint32_t ida, uint32_t idb, bool va, bool vb, int qmax, uint64_t* slot,
-                   int lane) {
+                   int lane, uint16_t* bh_lds = nullptr, int bh_stride = 0) {
   constexpr int G = (W + 7) / 8;
   // An empty half reads task 0's sequences (every launch has a task 0) with no valid row or
   // target position, so its loads stay in bounds and its cells score as N.
@@ -452,12 +458,19 @@ DN_HD void sw_pair(const SwArgs& a, uint32_t ida, uint32_t idb, bool va, bool vb
   uint32_t QC[2], TS[4];
   bs.next(0, QC, TS);
 
+  // The saved best row: in VGPRs up to W = kBestRowInRegs; wider, in LDS as half-words
+  // (alignment A at bh_lds[2 * d * bh_stride], B beside it), which keeps W registers free for
+  // occupancy. It changes only on rows whose maximum beats the best so far.
+  constexpr bool kBhLds = TB && W > kBestRowInRegs;
   pk16 best = zero;
-  pk16 BH[TB ? W : 1];
+  pk16 BH[TB && !kBhLds ? W : 1];
   uint32_t best_i = 0xffffffffu;
   if (TB) {
 #pragma unroll
-    for (int d = 0; d < (TB ? W : 1); ++d) BH[d] = zero;
+    for (int d = 0; d < W; ++d) {
+      if (kBhLds) bh_lds[2 * d * bh_stride] = bh_lds[2 * d * bh_stride + 1] = 0;
+      else BH[kBhLds ? 0 : d] = zero;
+    }
   }
   uint64_t* slot_lane = slot + lane;
 
@@ -504,8 +517,20 @@ DN_HD void sw_pair(const SwArgs& a, uint32_t ida, uint32_t idb, bool va, bool vb
       if (TB) {
         const uint32_t m = pk_signmask(pk_sub(best, rowmax));  // halves whose row max beats best
         if (m) {
+          if (kBhLds) {
+            if (m & 0xffffu) {
 #pragma unroll
-          for (int d = 0; d < (TB ? W : 1); ++d) BH[d] = pk_from(bfi(m, pk_bits(H[d]), pk_bits(BH[d])));
+              for (int d = 0; d < W; ++d) bh_lds[2 * d * bh_stride] = (uint16_t)pk_bits(H[d]);
+            }
+            if (m >> 16) {
+#pragma unroll
+              for (int d = 0; d < W; ++d) bh_lds[2 * d * bh_stride + 1] = (uint16_t)(pk_bits(H[d]) >> 16);
+            }
+          } else {
+#pragma unroll
+            for (int d = 0; d < (TB && !kBhLds ? W : 1); ++d)
+              BH[d] = pk_from(bfi(m, pk_bits(H[d]), pk_bits(BH[d])));
+          }
           best = pk_max(best, rowmax);
           best_i = bfi(m, (uint32_t)i * 0x10001u, best_i);
         }
@@ -540,9 +565,11 @@ DN_HD void sw_pair(const SwArgs& a, uint32_t ida, uint32_t idb, bool va, bool vb
   // Best cell: first d of the saved best row that attains the score.
   int da = -1, db = -1;
 #pragma unroll
-  for (int d = (TB ? W : 1) - 1; d >= 0; --d) {
-    if (pk_lo(BH[d]) == sa) da = d;
-    if (pk_hi(BH[d]) == sb) db = d;
+  for (int d = W - 1; d >= 0; --d) {
+    const int ha = kBhLds ? (int16_t)bh_lds[2 * d * bh_stride] : pk_lo(BH[kBhLds ? 0 : d]);
+    const int hb = kBhLds ? (int16_t)bh_lds[2 * d * bh_stride + 1] : pk_hi(BH[kBhLds ? 0 : d]);
+    if (ha == sa) da = d;
+    if (hb == sb) db = d;
   }
   if (va) {
     SwResult& r = a.res[ida];

@@ -94,7 +94,9 @@ void run_packed(const SwArgs& a0) {
     for (uint32_t k = base; k < a.n && k < base + 128; ++k) qmax = std::max(qmax, a.tasks[k].qlen);
     for (int lane = 0; lane < 64; ++lane) {
       const uint32_t ia = base + lane, ib = base + 64 + lane;
-      sw_pair<W, TB>(a, ia, ib, ia < a.n, ib < a.n, qmax, TB ? a.dir + (uint64_t)wv * slot_words : nullptr, lane);
+      uint16_t bh[2 * W];
+      sw_pair<W, TB>(a, ia < a.n ? ia : 0, ib < a.n ? ib : 0, ia < a.n, ib < a.n, qmax,
+                     TB ? a.dir + (uint64_t)wv * slot_words : nullptr, lane, bh, 1);
     }
   }
   if (TB) {
