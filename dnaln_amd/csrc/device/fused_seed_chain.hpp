@@ -376,7 +376,7 @@ DN_HD int chain_extract_lds(const ChainArgs& a, const LdsView& v, int n, uint32_
 }
 
 // Phases per read (one thread): 1. minimizers into a list (`mins`, maxmin entries mstride
-// apart: the thread's column of a global scratch array); 2. probes, four minimizers at a time,
+// apart: the thread's column of a global scratch array); 2. probes, eight minimizers at a time,
 // anchors into LDS; 3. sort, DP, extraction, candidates. `ring` (the fallback sketch for reads
 // with N or a window length other than WT) may alias the anchor words, which are free until
 // phase 2. Returns the candidate count or -1 (the read does not fit; global path).
