@@ -338,7 +338,7 @@ def main() -> int:
                                for t in batched["tune"]["tried"]]}
         if batched["tune"] else None,
         "batched_stats": {k: int(getattr(batched["stats"], k)) for k in
-                          ("anchors", "candidates", "rank_tasks", "tb_tasks", "global_path_reads")},
+                          ("anchors", "candidates", "rank_tasks", "tb_tasks", "global_path_reads", "fused_retry_reads")},
         "parity": parity, "roofline": roofline,
     }
     if not a.no_cpu:

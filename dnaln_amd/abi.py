@@ -43,7 +43,7 @@ class Task(C.Structure):
 class MapStats(C.Structure):
     _fields_ = [(n, u64) for n in ("reads", "mapped", "anchors", "candidates", "cells", "rank_tasks", "tb_tasks")] + \
                [(n, dbl) for n in ("ms_seed", "ms_chain", "ms_rank", "ms_traceback", "ms_total")] + \
-               [("global_path_reads", u64)]
+               [("global_path_reads", u64), ("fused_retry_reads", u64)]
 
 
 class BatchConfig(C.Structure):
@@ -65,7 +65,7 @@ RECORD_DTYPE = np.dtype([("contig", "<i4"), ("rev", "u1"), ("mapq", "u1"), ("res
                          ("score", "<i4"), ("nm", "<i4"), ("n_cand", "<i4"), ("cigar_len", "<u4"),
                          ("cigar_off", "<u8")])
 assert TASK_DTYPE.itemsize == 24 and RESULT_DTYPE.itemsize == 40 and RECORD_DTYPE.itemsize == 40
-assert C.sizeof(MapParams) == 80 and C.sizeof(BatchConfig) == 56 and C.sizeof(MapStats) == 104
+assert C.sizeof(MapParams) == 80 and C.sizeof(BatchConfig) == 56 and C.sizeof(MapStats) == 112
 
 # Every symbol include/dnaln_hip.h declares, with (restype, argtypes).
 HIP_SYMBOLS = {

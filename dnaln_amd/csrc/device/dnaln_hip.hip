@@ -116,7 +116,7 @@ struct SwWork {
 // Layouts shared with other languages' bindings.
 static_assert(sizeof(dnaln_map_params) == 80, "dnaln_map_params layout");
 static_assert(sizeof(dnaln_batch_config) == 56, "dnaln_batch_config layout");
-static_assert(sizeof(dnaln_map_stats) == 104, "dnaln_map_stats layout");
+static_assert(sizeof(dnaln_map_stats) == 112, "dnaln_map_stats layout");
 static_assert(sizeof(dnaln_record) == 40 && sizeof(dnaln_result) == 40, "record/result layout");
 
 constexpr int kSeedBlock = 128;
@@ -730,6 +730,7 @@ struct dnaln_hip_batch : SpanSink {
   bool tuned = false;
   DevBuf lids, lkeys, lorder, ltmp;  // length-sorted read order
   DevBuf fmins;                      // minimizer lists of the fused kernel
+  DevBuf ovf2;                       // reads the fused kernel's second pass declined
   // workspace
   DevBuf acnt, aoff, keys, f, prev, order, chains, ring_h, ring_ps;
   DevBuf cands, ncand, rcnt, roff, rtasks, rres, chosen, second, tbflag, tboff, ttasks, tres, tcig;
@@ -887,6 +888,17 @@ FusedShape fused_shape(int k, int w, int maxlen) {
   return f;
 }
 
+// The shape of the second fused pass over the reads a 32-anchor shape declined: 64 anchors and
+// 64 list entries (cap == 0 where it cannot be used).
+FusedShape fused_retry_shape(int k, int w, int maxlen) {
+  FusedShape f;
+  f.cap = 64;
+  f.maxmin = 64;
+  f.lds = (size_t)kFusedBlock * 8 * f.cap;
+  if (12 * w > 8 * f.cap || f.lds > 64 * 1024 || maxlen >= 16384) f.cap = 0;
+  return f;
+}
+
 // Minimizer scratch of a fused launch over n reads (u64 entries).
 uint64_t fused_mins_words(const FusedShape& fs, uint64_t n) {
   return (n + kFusedBlock - 1) / kFusedBlock * kFusedBlock * (uint64_t)fs.maxmin;
@@ -1014,10 +1026,30 @@ int run_chunk(dnaln_hip_batch* b, uint64_t r0, uint32_t n, bool first_chunk) {
       sa.n_reads = (uint32_t)(hi[x] - lo[x]);
       DN_TRY(launch_fused(fs, sa, ca, b->fmins.as<uint64_t>(), ovf, novf_d, st));
     }
-    DN_TRY(b->span_end(sp_i));
     uint32_t novf = 0;
     DN_HIP(hipMemcpyAsync(&novf, novf_d, 4, hipMemcpyDeviceToHost, st));
     DN_HIP(hipStreamSynchronize(st));
+    // Reads a 32-anchor shape declined (more minimizers or anchors than its slots) get a second
+    // fused pass in the 64-anchor shape; only what that declines too takes the global path.
+    bool small = false;
+    for (int x = 0; x < 2; ++x)
+      if (hi[x] > lo[x]) small = small || fused_shape(c->k, c->w, std::min(b->split[x], b->maxlen)).cap < 64;
+    const FusedShape retry = fused_retry_shape(c->k, c->w, b->maxlen);
+    if (novf && small && retry.cap) {
+      DN_TRY(b->ovf2.reserve((uint64_t)n * 4 + 8));
+      uint32_t* ovf2 = b->ovf2.as<uint32_t>();
+      uint32_t* novf2_d = ovf2 + n;
+      DN_HIP(hipMemsetAsync(novf2_d, 0, 4, st));
+      DN_TRY(b->fmins.reserve(fused_mins_words(retry, novf) * 8));
+      sa.list = ovf;
+      sa.n_reads = novf;
+      DN_TRY(launch_fused(retry, sa, ca, b->fmins.as<uint64_t>(), ovf2, novf2_d, st));
+      b->stats.fused_retry_reads += novf;
+      DN_HIP(hipMemcpyAsync(&novf, novf2_d, 4, hipMemcpyDeviceToHost, st));
+      DN_HIP(hipStreamSynchronize(st));
+      ovf = ovf2;
+    }
+    DN_TRY(b->span_end(sp_i));
     const uint32_t* list = ovf;
     uint32_t nl = novf;
     if (n > nb) {
@@ -1567,6 +1599,7 @@ int dnaln_hip_batch_reserve(dnaln_hip_batch* b, uint64_t n, uint64_t bases, int3
   DN_TRY(b->lkeys.reserve(n * 4));
   DN_TRY(b->lorder.reserve(n * 4));
   DN_TRY(b->ovf.reserve(n * 4 + 8));
+  DN_TRY(b->ovf2.reserve(n * 4 + 8));
   {
     FusedShape fs;
     fs.maxmin = 64;

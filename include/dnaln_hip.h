@@ -115,11 +115,12 @@ typedef struct {
 /* MapStats (src/mapper.hpp:46-49) plus device timings of the last batch (ms). cells counts
  * band cells as the reference does (rank pass + traceback pass). anchors counts only the
  * anchors of reads on the global-memory seed path (the LDS path keeps its anchors on chip);
- * global_path_reads counts those reads in the batched form. */
+ * global_path_reads counts those reads in the batched form, and fused_retry_reads the reads
+ * that the LDS path's 32-anchor shape declined and its 64-anchor shape took again. */
 typedef struct {
   uint64_t reads, mapped, anchors, candidates, cells, rank_tasks, tb_tasks;
   double ms_seed, ms_chain, ms_rank, ms_traceback, ms_total;
-  uint64_t global_path_reads;
+  uint64_t global_path_reads, fused_retry_reads;
 } dnaln_map_stats;
 
 /* Fills the reference's defaults (k 15, w 10, max_occ 500, 2/4/4/2/1 bw 16, chain

@@ -54,7 +54,8 @@ def test_batched_form_matches_reference_records(name):
     recs, pool, stats = b.download()
     b.close()
     _compare(name, recs, pool, g)
-    assert stats.global_path_reads <= len(r)
+    assert stats.global_path_reads <= len(r) and stats.fused_retry_reads <= len(r)
+    print(f"global path {stats.global_path_reads}, second fused pass {stats.fused_retry_reads}", flush=True)
 
 
 @pytest.mark.parametrize("name", ["uniform4m_150bp_10k", "edge_repeats_bw32", "edge_twins_bw16"])
