@@ -884,7 +884,7 @@ FusedShape fused_shape(int k, int w, int maxlen) {
   f.cap = maxlen <= 200 ? 32 : 64;
   f.maxmin = std::min(64, std::max(16, fused_list_need(k, maxlen)));
   f.lds = (size_t)kFusedBlock * 8 * f.cap;
-  if (12 * w > 8 * f.cap || f.lds > 64 * 1024 || maxlen >= 16384) f.cap = 0;
+  if (12 * w > 8 * f.cap || f.lds > 64 * 1024 || maxlen >= kFusedMaxLen) f.cap = 0;
   return f;
 }
 
@@ -895,7 +895,7 @@ FusedShape fused_retry_shape(int k, int w, int maxlen) {
   f.cap = 64;
   f.maxmin = 64;
   f.lds = (size_t)kFusedBlock * 8 * f.cap;
-  if (12 * w > 8 * f.cap || f.lds > 64 * 1024 || maxlen >= 16384) f.cap = 0;
+  if (12 * w > 8 * f.cap || f.lds > 64 * 1024 || maxlen >= kFusedMaxLen) f.cap = 0;
   return f;
 }
 

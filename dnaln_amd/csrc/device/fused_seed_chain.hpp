@@ -22,9 +22,11 @@ namespace dn {
 
 // The read's anchors in LDS, one u64 word each: anchor j of the thread at keys[j * stride], the
 // 64 threads of a wave interleaved. A word is the anchor key (strand << 63 | rpos << 32 | qpos)
-// with the chain DP's f and prev packed into the low dword above a 14-bit qpos: qpos bits 0..13,
-// f bits 14..24 (1 <= f <= cap * k <= 64 * 28 < 2^11), prev + 1 bits 25..31 (prev in -1..63).
-// f and prev are 0 until the DP writes them, so sorting the words sorts the keys.
+// with the chain DP's results packed into the low dword above a 12-bit qpos (reads < 4096 bp):
+// qpos bits 0..11; bits 12..13 the DP's headroom dm = (max f over the strand's anchors up to
+// this one) - f, 3 meaning 3 or more; f bits 14..24 (1 <= f <= cap * k <= 64 * 28 < 2^11);
+// prev + 1 bits 25..31 (prev in -1..63). The DP fields are 0 until the DP writes them, so
+// sorting the words sorts the keys.
 struct LdsView {
   uint64_t* keys;
   int stride;
@@ -38,11 +40,13 @@ struct LdsView {
 #endif
   }
 };
-DN_HD int w_qpos(uint64_t x) { return (int)(x & 0x3fffu); }
+constexpr int kFusedMaxLen = 4096;  // reads the fused path takes (12-bit qpos)
+DN_HD int w_qpos(uint64_t x) { return (int)(x & 0xfffu); }
+DN_HD int w_dm(uint64_t x) { return (int)((x >> 12) & 3u); }
 DN_HD int w_f(uint64_t x) { return (int)((x >> 14) & 0x7ffu); }
 DN_HD int w_prev(uint64_t x) { return (int)((x >> 25) & 0x7fu) - 1; }
-DN_HD uint32_t w_low(int qpos, int f, int prev) {
-  return (uint32_t)qpos | (uint32_t)f << 14 | (uint32_t)(prev + 1) << 25;
+DN_HD uint32_t w_low(int qpos, int dm, int f, int prev) {
+  return (uint32_t)qpos | (uint32_t)dm << 12 | (uint32_t)f << 14 | (uint32_t)(prev + 1) << 25;
 }
 
 struct ChainLite {
@@ -263,13 +267,18 @@ DN_HD bool stage_read(const uint8_t* seq, int len, int tail, uint64_t* dst, int 
 // halves (the strand bits cancel) and never negative, and the reference's stop at dr > max_gap
 // is monotone in j. Predecessors are visited nearest first in groups of four whose loads are
 // issued together; each is tested on its own, without branches, and the scan ends after the
-// first group whose farthest predecessor is stopped. NIB: gap costs from the four-bit table
-// (drift < 16) and cost_at_band (drift == band), valid when cost_nib_ok; a drift above the band
-// is never taken, so its cost does not matter. Arithmetic is 32-bit: rpos < 2^31, qpos < 2^14.
+// first group whose farthest predecessor is stopped. It also ends once no predecessor left can
+// beat the best: a predecessor's score is at most f[j] + k (gain <= k, cost >= 0), and the
+// farthest anchor of a group carries dm, so max f over the anchors before it is known; along a
+// colinear chain f grows, so the scan usually ends after one group. The result is the
+// reference's either way. NIB: gap costs from the four-bit table (drift < 16) and cost_at_band
+// (drift == band), valid when cost_nib_ok; a drift above the band is never taken, so its cost
+// does not matter. Arithmetic is 32-bit: rpos < 2^31, qpos < 2^12.
 template <bool NIB>
 DN_HD void chain_dp_lds(const ChainArgs& a, const LdsView& v, int n, int nf, int k) {
   const uint32_t max_gap = (uint32_t)a.max_gap;
   const int band = a.band;
+  int fmax = 0;  // max f over this strand's anchors so far
   for (int i = 0; i < n; ++i) {
     const uint64_t ki = v.word(i);
     const uint32_t hi = (uint32_t)(ki >> 32);
@@ -305,8 +314,11 @@ DN_HD void chain_dp_lds(const ChainArgs& a, const LdsView& v, int n, int nf, int
         best_j = take ? j : best_j;
       }
       if (!alive) break;
+      const int dm = w_dm(kj[3]);  // the group's farthest anchor (or jlo's, where the scan ends)
+      if (dm < 3 && w_f(kj[3]) + dm + k <= best) break;
     }
-    v.set_low(i, w_low(qi, best, best_j));
+    fmax = (i == 0 || i == nf || best > fmax) ? best : fmax;
+    v.set_low(i, w_low(qi, fmax - best < 3 ? fmax - best : 3, best, best_j));
   }
 }
 
@@ -384,7 +396,7 @@ template <int MAXCH, int WT>
 DN_HD int seed_chain_lds(const SeedArgs& s, const ChainArgs& a, uint32_t r, LdsView v, uint64_t* mins, int mstride,
                          int maxmin, Ring ring, int cap) {
   const int len = s.lens[r], k = s.k;
-  if (len >= 16384 || 2 * k > 46) return -1;
+  if (len >= kFusedMaxLen || 2 * k > 46) return -1;
   const uint8_t* seq = s.codes + s.offs[r];
   int nmin = 0;
   int got = 0;
@@ -440,13 +452,22 @@ DN_HD int seed_chain_lds(const SeedArgs& s, const ChainArgs& a, uint32_t r, LdsV
         continue;
       }
       const uint32_t mpos = (uint32_t)(e[u] >> 1) & 0xffffu, mstrand = (uint32_t)(e[u] & 1);
-      for (uint32_t j = 0; j < cnt[u]; ++j) {
-        const uint32_t val = j == 0 ? first[u] : s.pos[off[u] + j];
+      const auto put = [&](uint32_t val) {
         const uint32_t st = (val & 1) ^ mstrand;
         const int32_t qpos = st ? len - (int32_t)mpos - k : (int32_t)mpos;
         const uint64_t key = anchor_key(st, val >> 1, qpos);
         if (st) v.keys[(hi--) * v.stride] = key;
         else v.keys[(lo++) * v.stride] = key;
+      };
+      put(first[u]);
+      // Further occurrences four loads at a time (indices clamped into the slice).
+      for (uint32_t j0 = 1; j0 < cnt[u]; j0 += 4) {
+        uint32_t w[4];
+#pragma unroll
+        for (int x = 0; x < 4; ++x) w[x] = s.pos[off[u] + (j0 + x < cnt[u] ? j0 + x : cnt[u] - 1)];
+#pragma unroll
+        for (int x = 0; x < 4; ++x)
+          if (j0 + x < cnt[u]) put(w[x]);
       }
     }
   }

@@ -264,13 +264,23 @@ DN_HD void seed_emit_read(const SeedArgs& a, uint32_t i, Ring ring) {
   int64_t lo = 0, hi = (int64_t)a.anchor_count[i] - 1;
   const int k = a.k;
   auto on = [&](uint32_t mpos, uint32_t mstrand, uint32_t off, uint32_t cnt, uint32_t first) {
-    for (uint32_t j = 0; j < cnt; ++j) {
-      const uint32_t v = j == 0 ? first : a.pos[off + j];
+    const auto put = [&](uint32_t v) {
       const uint32_t st = (v & 1) ^ mstrand;
       const int32_t qpos = st ? len - (int32_t)mpos - k : (int32_t)mpos;
       const uint64_t key = anchor_key(st, v >> 1, qpos);
       if (st) out[hi--] = key;
       else out[lo++] = key;
+    };
+    put(first);
+    // Further occurrences eight loads at a time (indices clamped into the slice), so a
+    // repeated minimizer's hundreds of occurrences do not cost a round trip each.
+    for (uint32_t j0 = 1; j0 < cnt; j0 += 8) {
+      uint32_t w[8];
+#pragma unroll
+      for (int x = 0; x < 8; ++x) w[x] = a.pos[off + (j0 + x < cnt ? j0 + x : cnt - 1)];
+#pragma unroll
+      for (int x = 0; x < 8; ++x)
+        if (j0 + x < cnt) put(w[x]);
     }
   };
   ProbePipe<true, decltype(on)> pipe(a, on);
