@@ -94,124 +94,6 @@ DN_HD ChainLite walk_chain(const LdsView& v, int end, uint64_t& used, int& cnt) 
   return c;
 }
 
-// Minimizer list entry: hash << 17 | pos << 1 | strand (2k <= 46 bits, reads < 65536 bp).
-DN_HD uint64_t pack_min(uint64_t h, uint32_t pos, uint32_t strand) { return h << 17 | (uint64_t)pos << 1 | strand; }
-
-// hash64 (src/minimizer.hpp:21-30) in the k-mer's own word type. Every step either masks to 2k
-// bits or shifts a masked value right, so for 2k <= 32 the 32-bit form gives the same hash.
-template <class T>
-DN_HD T hash_kmer(T key, T mask) {
-  key = (~key + (key << 21)) & mask;
-  key = key ^ key >> 24;
-  key = ((key + (key << 3)) + (key << 8)) & mask;
-  key = key ^ key >> 14;
-  key = ((key + (key << 2)) + (key << 4)) & mask;
-  key = key ^ key >> 28;
-  key = (key + (key << 31)) & mask;
-  return key;
-}
-// The 32-bit form, with shift amounts offset by z, a value that is 0 at run time but unknown to
-// the compiler: it then keeps every a + (a << s) as one v_lshl_add_u32 instead of turning the
-// shift-add chains into v_mul_lo_u32, which issues at a quarter of the rate.
-DN_HD uint32_t hash_kmer32(uint32_t key, uint32_t mask, uint32_t z) {
-  key = (~key + (key << (21 + z))) & mask;
-  key = key ^ key >> 24;
-  key = ((key + (key << (3 + z))) + (key << (8 + z))) & mask;
-  key = key ^ key >> 14;
-  key = ((key + (key << (2 + z))) + (key << (4 + z))) & mask;
-  key = key ^ key >> 28;
-  key = (key + (key << (31 + z))) & mask;
-  return key;
-}
-template <class T>
-DN_HD T hash_k(T key, T mask, uint32_t) {
-  return hash_kmer<T>(key, mask);
-}
-template <>
-DN_HD uint32_t hash_k<uint32_t>(uint32_t key, uint32_t mask, uint32_t z) {
-  return hash_kmer32(key, mask, z);
-}
-
-// The reference's streamed van Herk / Gil-Werman window minimum (src/minimizer.cpp:5-68) with
-// the window length fixed at compile time, for a read without N: block and suffix minima are
-// registers, and the step of every k-mer is the same branch-free code on every lane (emission is
-// a predicated store). The minimizer of a window is its leftmost smallest hash, as the
-// reference's tie rules give: strict < within a block prefix, <= within a suffix, and the suffix
-// (the left part) wins a tie between the two. T is uint32_t for k <= 15 (the hash then stays
-// below the all-ones sentinel) and uint64_t otherwise. Appends the minimizers to `mins`; returns
-// 1, or -1 when the list would exceed maxmin. The reader is called for up to WT - 1 positions
-// past the read; what it returns there is never used.
-template <int WT, class T, class Reader>
-DN_HD int sketch_vh(Reader rd, int len, int k, uint64_t* mins, int stride, int maxmin, int& nmin) {
-  const T mask = (T)((1ULL << (2 * k)) - 1);
-  const int shift = 2 * (k - 1);
-  const T kInvalid = ~(T)0;
-  const uint32_t z = (uint32_t)k >> 8;  // 0 (k <= 28)
-  T fwd = 0, rev = 0;
-  nmin = 0;
-  if (len < k) return 1;
-  for (int i = 0; i < k - 1; ++i) {
-    const uint32_t c = rd(i);
-    fwd = ((fwd << 2) | c) & mask;
-    rev = (rev >> 2) | ((T)(3 - c) << shift);
-  }
-  T bh[WT], sh[WT];
-  uint32_t bp[WT], sp[WT];
-#pragma unroll
-  for (int j = 0; j < WT; ++j) bh[j] = sh[j] = kInvalid, bp[j] = sp[j] = 0;
-  T ph = kInvalid;
-  uint32_t pp = 0, last = ~0u;
-  int n = 0;
-  const int nk = len - k + 1;
-  for (int x0 = 0; x0 < nk; x0 += WT) {
-#pragma unroll
-    for (int j = 0; j < WT; ++j) {
-      const int x = x0 + j;
-      const uint32_t c = rd(x + k - 1);  // past the read (x >= nk) the k-mer is never used
-      fwd = ((fwd << 2) | c) & mask;
-      rev = (rev >> 2) | ((T)(3 - c) << shift);
-      const bool strand = rev < fwd;
-      const T h = fwd != rev ? hash_k<T>(strand ? rev : fwd, mask, z) : kInvalid;
-      const uint32_t ps = (uint32_t)x << 1 | (uint32_t)strand;
-      bh[j] = h;
-      bp[j] = ps;
-      if (j == 0) {
-        ph = h;
-        pp = ps;
-      } else {
-        const bool lt = h < ph;
-        ph = lt ? h : ph;
-        pp = lt ? ps : pp;
-      }
-      // Window [x - WT + 1, x]: the suffix of the previous block, then this block's prefix.
-      T mh = ph;
-      uint32_t mps = pp;
-      if (j + 1 < WT) {
-        const bool suf = !(ph < sh[j + 1]);
-        mh = suf ? sh[j + 1] : ph;
-        mps = suf ? sp[j + 1] : pp;
-      }
-      const bool live = x + 1 >= WT && x < nk && mh != kInvalid;
-      const uint32_t p = mps >> 1;
-      const bool emit = live && p != last;
-      if (emit && n < maxmin) mins[n * stride] = pack_min(mh, p, mps & 1);
-      n += emit ? 1 : 0;
-      last = live ? p : last;
-    }
-    // block complete: suffix minima, right to left (ties keep the left k-mer)
-    sh[WT - 1] = bh[WT - 1];
-    sp[WT - 1] = bp[WT - 1];
-#pragma unroll
-    for (int r = WT - 2; r >= 0; --r) {
-      const bool left = bh[r] <= sh[r + 1];
-      sh[r] = left ? bh[r] : sh[r + 1];
-      sp[r] = left ? bp[r] : sp[r + 1];
-    }
-  }
-  nmin = n;
-  return n > maxmin ? -1 : 1;
-}
-
 // The bases of a read staged in LDS: the aligned 16-byte chunks around the read, stored as u64
 // words in the thread's (interleaved) anchor slots; byte j of the read sits at byte off + j.
 struct LdsSeq {
@@ -404,8 +286,17 @@ DN_HD int seed_chain_lds(const SeedArgs& s, const ChainArgs& a, uint32_t r, LdsV
   // The read's bases go to LDS first (into the anchor slots, free until phase 2), so the
   // sketch waits for memory once instead of at every chunk boundary.
   if (s.w == WT && stage_read(seq, len, WT, v.keys, v.stride, cap, ls))
-    got = k <= 15 ? sketch_vh<WT, uint32_t>(ls, len, k, mins, mstride, maxmin, nmin)
-                  : sketch_vh<WT, uint64_t>(ls, len, k, mins, mstride, maxmin, nmin);
+  {
+    int m = 0;
+    auto put = [&](uint64_t h, uint32_t p, uint32_t st) {
+      if (m < maxmin) mins[m * mstride] = pack_min(h, p, st);
+      ++m;
+    };
+    if (k <= 15) sketch_vh<WT, uint32_t>(ls, len, k, put);
+    else sketch_vh<WT, uint64_t>(ls, len, k, put);
+    nmin = m;
+    got = m > maxmin ? -1 : 1;
+  }
   if (got < 0) return -1;
   if (got == 0) {
     nmin = 0;

@@ -16,6 +16,7 @@
 #pragma once
 
 #include "common.hpp"
+#include "sketch_vh.hpp"
 
 namespace dn {
 
@@ -243,13 +244,26 @@ struct ProbePipe {
   }
 };
 
+// The minimizers of read r, in position order: the register sketch for window length 10 and an
+// N-free read (one branch-free step per k-mer), else the streamed sketch with its ring.
+template <class Emit>
+DN_HD void sketch_read(const SeedArgs& a, uint32_t r, Ring ring, Emit&& emit) {
+  const uint8_t* seq = a.codes + a.offs[r];
+  const int len = a.lens[r];
+  if (a.w == 10 && !read_has_n(seq, len)) {
+    if (a.k <= 15) sketch_vh<10, uint32_t>(ChunkReader(seq), len, a.k, emit);
+    else sketch_vh<10, uint64_t>(ChunkReader(seq), len, a.k, emit);
+  } else {
+    sketch_stream(ChunkReader(seq), len, a.k, a.w, ring, emit);
+  }
+}
+
 DN_HD uint32_t seed_count_read(const SeedArgs& a, uint32_t i, Ring ring) {
   const uint32_t r = read_id(a.list, i);
   uint32_t total = 0;
   auto on = [&](uint32_t, uint32_t, uint32_t, uint32_t cnt, uint32_t) { total += cnt; };
   ProbePipe<false, decltype(on)> pipe(a, on);
-  sketch_stream(ChunkReader(a.codes + a.offs[r]), a.lens[r], a.k, a.w, ring,
-                [&](uint64_t h, uint32_t p, uint32_t st) { pipe.push(h, p, st); });
+  sketch_read(a, r, ring, [&](uint64_t h, uint32_t p, uint32_t st) { pipe.push(h, p, st); });
   pipe.drain();
   return total;
 }
@@ -284,8 +298,7 @@ DN_HD void seed_emit_read(const SeedArgs& a, uint32_t i, Ring ring) {
     }
   };
   ProbePipe<true, decltype(on)> pipe(a, on);
-  sketch_stream(ChunkReader(a.codes + a.offs[r]), len, k, a.w, ring,
-                [&](uint64_t h, uint32_t p, uint32_t st) { pipe.push(h, p, st); });
+  sketch_read(a, r, ring, [&](uint64_t h, uint32_t p, uint32_t st) { pipe.push(h, p, st); });
   pipe.drain();
 }
 
