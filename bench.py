@@ -178,8 +178,11 @@ def serve_sweep(seconds: float, levels: list[float], slots: int) -> dict:
     g = synth.workload_genome(w)
     pool = synth.workload_reads(w)
     rb = api.prepare(pool)
-    ctx = api.Context(g, bw=w.bw)
-    print(f"[bench] serving pool of {len(rb)} reads ready in {time.time() - t:.1f}s", file=sys.stderr, flush=True)
+    tc = time.perf_counter()
+    ctx = api.Context(g, "gpu", bw=w.bw)  # the index built on the device from the uploaded codes
+    start = {"context_s": time.perf_counter() - tc, "index_build_ms": ctx.index_info()["build_ms"]}
+    print(f"[bench] serving pool of {len(rb)} reads ready in {time.time() - t:.1f}s (context with a GPU-built "
+          f"index in {start['context_s']:.2f} s)", file=sys.stderr, flush=True)
     b = ctx.batch(mode=abi.MODE_BATCHED)
     b.upload(rb)
     b.run()
@@ -215,9 +218,43 @@ def serve_sweep(seconds: float, levels: list[float], slots: int) -> dict:
     ctx.close()
     return {"workload": w.name, "genome_bp": int(len(g.codes)), "read_len": "100-300", "bw": w.bw,
             "request_reads": "1-4096 log-uniform", "arrivals": "Poisson", "seconds_per_level": seconds,
-            "slots": slots, "peak_batched_reads_per_s": peak, "levels": out,
+            "slots": slots, "peak_batched_reads_per_s": peak, "start": start, "levels": out,
             "server": {k: st[k] for k in ("batches", "reads", "max_inflight_batches", "max_inflight_bytes",
                                           "budget_bytes", "deferred_by_budget")}}
+
+
+def index_block(g: synth.Genome, k: int = 15, w: int = 10) -> tuple[api.Index, dict]:
+    """MinimizerIndex (src/index.cpp:10-45) built on the host and on the GPU, checked equal
+    (bucket count, key count, pos[], the set of buckets), and its file written and read back.
+    Returns the host index (the context uploads it) and the timings."""
+    t = time.perf_counter()
+    host = api.Index(g, k, w)
+    host_s = time.perf_counter() - t
+    t = time.perf_counter()
+    gpu = api.Index(g, k, w, build="gpu")
+    gpu_s = time.perf_counter() - t
+    empty = np.uint64(0xFFFFFFFFFFFFFFFF)
+    th, tg = host.table(), gpu.table()
+    same = (gpu.n_buckets, gpu.n_keys, gpu.max_count) == (host.n_buckets, host.n_keys, host.max_count) \
+        and np.array_equal(gpu.positions(), host.positions()) \
+        and np.array_equal(np.sort(th[th["key"] != empty], order="key"), np.sort(tg[tg["key"] != empty], order="key"))
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "ref.dnx")
+        t = time.perf_counter()
+        host.save(path)
+        save_s = time.perf_counter() - t
+        t = time.perf_counter()
+        loaded = api.Index.load(path, g)
+        load_s = time.perf_counter() - t
+        size = os.path.getsize(path)
+    same_file = np.array_equal(loaded.table(), th) and np.array_equal(loaded.positions(), host.positions())
+    blk = {"genome_bp": int(len(g.codes)), "keys": int(host.n_keys), "positions": int(host.n_pos),
+           "host_build_s": host_s, "gpu_build_ms": gpu.build_ms, "gpu_build_and_download_s": gpu_s,
+           "gpu_equals_host": bool(same), "file_bytes": size, "file_save_s": save_s, "file_load_s": load_s,
+           "file_round_trip_equal": bool(same_file)}
+    print(f"[bench] index: host {host_s:.2f} s, GPU {gpu.build_ms:.1f} ms (equal: {same}), file load {load_s:.2f} s",
+          file=sys.stderr, flush=True)
+    return host, blk
 
 
 def main() -> int:
@@ -260,7 +297,8 @@ def main() -> int:
     reads = synth.workload_reads(w, a.reads or None)
     rb = api.prepare(reads)
     print(f"[bench] {w.name}: {len(reads)} reads generated in {time.time() - t:.1f}s", file=sys.stderr, flush=True)
-    ctx = api.Context(g, bw=w.bw)
+    host_index, ix_blk = index_block(g)
+    ctx = api.Context(g, host_index, bw=w.bw)
     W = 2 * w.bw + 1
 
     plain = time_form(ctx, rb, abi.MODE_PLAIN, a.plain_steps, 1) if a.plain_steps > 0 else None
@@ -339,7 +377,7 @@ def main() -> int:
         if batched["tune"] else None,
         "batched_stats": {k: int(getattr(batched["stats"], k)) for k in
                           ("anchors", "candidates", "rank_tasks", "tb_tasks", "global_path_reads", "fused_retry_reads")},
-        "parity": parity, "roofline": roofline,
+        "parity": parity, "roofline": roofline, "index": ix_blk,
     }
     if not a.no_cpu:
         cb = cpu_baseline(w, g, reads, a.cpu_reads_per_proc)

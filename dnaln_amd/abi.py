@@ -7,7 +7,7 @@ import numpy as np
 
 i32, u32, u64, i64, dbl, vp = C.c_int32, C.c_uint32, C.c_uint64, C.c_int64, C.c_double, C.c_void_p
 
-OK, E_INVALID, E_CAPACITY, E_DEVICE, E_NOMEM, E_INTERNAL, E_PENDING = 0, -1, -2, -3, -4, -5, -6
+OK, E_INVALID, E_CAPACITY, E_DEVICE, E_NOMEM, E_INTERNAL, E_PENDING, E_IO = 0, -1, -2, -3, -4, -5, -6, -7
 MODE_PLAIN, MODE_BATCHED = 0, 1
 
 
@@ -89,6 +89,11 @@ HIP_SYMBOLS = {
     "dnaln_hip_batch_reserve": (i32, [vp, u64, u64, i32, u32]),
     "dnaln_hip_context_params": (i32, [vp, vp]),
     "dnaln_hip_alloc_counters": (None, [vp, vp]),
+    "dnaln_hip_index_build": (i32, [vp, u64, vp, i32, i32, i32, i32, vp]),
+    "dnaln_hip_index_info": (i32, [vp, vp, vp, vp, vp, vp]),
+    "dnaln_hip_index_download": (i32, [vp, vp, vp]),
+    "dnaln_hip_index_free": (None, [vp]),
+    "dnaln_hip_context_index": (i32, [vp, vp, vp, vp]),
 }
 
 class ServeConfig(C.Structure):
@@ -118,6 +123,10 @@ SERVE_SYMBOLS = {
 HOST_SYMBOLS = {
     "dnaln_index_build": (i32, [vp, vp, i32, i32, i32, vp]),
     "dnaln_index_free": (None, [vp]),
+    "dnaln_index_build_gpu": (i32, [vp, vp, i32, i32, i32, i32, vp, vp]),
+    "dnaln_index_save": (i32, [vp, vp, vp, i32, C.c_char_p]),
+    "dnaln_index_load": (i32, [C.c_char_p, vp, vp, i32, vp]),
+    "dnaln_reference_digest": (u64, [vp, vp, i32]),
     "dnaln_fnv1a": (u32, [C.c_char_p, C.c_size_t]),
     "dnaln_fnv1a_many": (None, [vp, vp, vp, u64, vp]),
     "dnaln_encode": (None, [C.c_char_p, C.c_size_t, vp]),

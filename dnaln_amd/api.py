@@ -65,14 +65,54 @@ def map_params(**kw) -> abi.MapParams:
 
 
 class Index:
-    """MinimizerIndex built on the host (src/index.cpp:10-45), uploaded by Context."""
+    """MinimizerIndex (src/index.cpp:10-45) in host memory, uploaded by Context.
 
-    def __init__(self, genome: Genome, k: int = 15, w: int = 10):
+    build="host" sketches and sorts on the CPU (dnaln_index_build); build="gpu" builds the same
+    index on GPU `device` and downloads it (dnaln_index_build_gpu; k <= 16), with its device wall
+    time in `build_ms`. Index.load reads a file written by save(), checked against the genome.
+    """
+
+    def __init__(self, genome: Genome, k: int = 15, w: int = 10, build: str = "host", device: int = 0,
+                 _empty: bool = False):
         self._contigs = contig_array(genome)
+        self.genome = genome
         self.idx = abi.Index()
-        _check(_lib.host().dnaln_index_build(_p(genome.codes), C.cast(self._contigs, C.c_void_p), len(self._contigs),
-                                             k, w, C.byref(self.idx)), "dnaln_index_build")
         self.k, self.w = k, w
+        self.build_ms = 0.0
+        if _empty:
+            return
+        if build == "host":
+            _check(_lib.host().dnaln_index_build(_p(genome.codes), C.cast(self._contigs, C.c_void_p),
+                                                 len(self._contigs), k, w, C.byref(self.idx)), "dnaln_index_build")
+        elif build == "gpu":
+            ms = C.c_double(0)
+            _check(_lib.host().dnaln_index_build_gpu(_p(genome.codes), C.cast(self._contigs, C.c_void_p),
+                                                     len(self._contigs), k, w, device, C.byref(self.idx),
+                                                     C.byref(ms)), "dnaln_index_build_gpu")
+            self.build_ms = ms.value
+        else:
+            raise ValueError(f"build must be 'host' or 'gpu', not {build!r}")
+
+    def save(self, path: str) -> None:
+        _check(_lib.host().dnaln_index_save(C.byref(self.idx), _p(self.genome.codes),
+                                            C.cast(self._contigs, C.c_void_p), len(self._contigs),
+                                            str(path).encode()), f"dnaln_index_save({path})")
+
+    @classmethod
+    def load(cls, path: str, genome: Genome) -> "Index":
+        ix = cls(genome, _empty=True)
+        _check(_lib.host().dnaln_index_load(str(path).encode(), _p(genome.codes), C.cast(ix._contigs, C.c_void_p),
+                                            len(ix._contigs), C.byref(ix.idx)), f"dnaln_index_load({path})")
+        ix.k, ix.w = ix.idx.view.k, ix.idx.view.w
+        return ix
+
+    @property
+    def max_count(self) -> int:
+        return self.idx.max_count
+
+    @property
+    def n_buckets(self) -> int:
+        return self.idx.view.n_buckets
 
     @property
     def n_keys(self) -> int:
@@ -130,16 +170,29 @@ def prepare(reads: Reads) -> ReadBatch:
 class Context:
     """Resident reference + index + parameters on one GPU (dnaln_hip_create)."""
 
-    def __init__(self, genome: Genome, index: Index | None = None, device: int = 0, **params):
+    def __init__(self, genome: Genome, index: Index | str | None = None, device: int = 0, **params):
+        """index: an Index (uploaded), None (built on the host, then uploaded), or "gpu" (built
+        on the device from the uploaded codes; nothing crosses PCIe but the codes)."""
         self.params = map_params(**params)
         self.genome = genome
-        self.index = index or Index(genome, self.params.k, self.params.w)
+        gpu = isinstance(index, str)
+        if gpu and index != "gpu":
+            raise ValueError(f"index must be an Index, None or 'gpu', not {index!r}")
+        self.index = None if gpu else (index or Index(genome, self.params.k, self.params.w))
         self._contigs = contig_array(genome)
         self.ctx = C.c_void_p()
         lib = _lib.hip()
+        view = None if gpu else C.byref(self.index.idx.view)
         _check(lib.dnaln_hip_create(C.byref(self.params), _p(genome.codes), len(genome.codes),
-                                    C.cast(self._contigs, C.c_void_p), genome.n_contigs, C.byref(self.index.idx.view),
+                                    C.cast(self._contigs, C.c_void_p), genome.n_contigs, view,
                                     device, C.byref(self.ctx)), "dnaln_hip_create")
+
+    def index_info(self) -> dict:
+        """Bucket and position counts of the resident index, and its GPU build time (0 if uploaded)."""
+        nb, npos, ms = C.c_uint64(0), C.c_uint64(0), C.c_double(0)
+        _check(_lib.hip().dnaln_hip_context_index(self.ctx, C.byref(nb), C.byref(npos), C.byref(ms)),
+               "dnaln_hip_context_index")
+        return {"n_buckets": nb.value, "n_pos": npos.value, "build_ms": ms.value}
 
     def close(self) -> None:
         if self.ctx:

@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -25,6 +26,7 @@
 #include "sw_packed.hpp"
 #include "fused_seed_chain.hpp"
 #include "chain_wave.hpp"
+#include "index_build.hpp"
 
 using namespace dn;
 
@@ -429,6 +431,72 @@ __global__ void k_inline_unique(Bucket* table, uint64_t n, const uint32_t* pos) 
   if (b.key != ~0ull && b.count == 1) table[i].offset = pos[b.offset];
 }
 
+// ---- index build on the device (src/index.cpp:10-45): segments sketched one per thread, the
+// occurrences radix-sorted by (hash, pos), each key placed by one thread with atomic linear
+// probing. Probe chains only ever grow, so every key stays reachable from its home slot.
+__global__ void k_ix_count(const uint8_t* ref, const IndexSeg* segs, uint32_t nseg, int k, int w, uint64_t* rh,
+                           uint32_t* rps, uint64_t* counts) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > nseg) return;
+  uint64_t n = 0;
+  if (i < nseg)
+    sketch_segment(ref, segs[i], k, w, Ring{rh + i, rps + i, (int)nseg}, [&](uint64_t, uint32_t, uint32_t) { ++n; });
+  counts[i] = n;  // entry nseg is 0, so the exclusive scan ends with the total
+}
+
+__global__ void k_ix_emit(const uint8_t* ref, const IndexSeg* segs, uint32_t nseg, int k, int w, uint64_t* rh,
+                          uint32_t* rps, const uint64_t* offs, uint64_t* keys) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nseg) return;
+  const IndexSeg g = segs[i];
+  uint64_t* out = keys + offs[i];
+  sketch_segment(ref, g, k, w, Ring{rh + i, rps + i, (int)nseg},
+                 [&](uint64_t h, uint32_t p, uint32_t st) { *out++ = occ_key(h, g.base, p, st); });
+}
+
+// flags[i] = 1 where a new hash starts (entry n is 0); pos[i] = the occurrence's position word.
+__global__ void k_ix_flags(const uint64_t* keys, uint64_t n, uint64_t* flags, uint32_t* pos) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > n) return;
+  if (i == n) {
+    flags[i] = 0;
+    return;
+  }
+  const uint64_t k = keys[i];
+  flags[i] = i == 0 || (keys[i - 1] >> 32) != (k >> 32);
+  pos[i] = (uint32_t)k;
+}
+
+// run_start[r] = first occurrence of the r-th distinct hash; run_start[n_keys] = n.
+__global__ void k_ix_starts(const uint64_t* flags, const uint64_t* ridx, uint64_t n, uint32_t* run_start) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > n) return;
+  if (i == n) run_start[ridx[n]] = (uint32_t)n;
+  else if (flags[i]) run_start[ridx[i]] = (uint32_t)i;
+}
+
+__global__ void k_ix_clear(Bucket* table, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) table[i] = Bucket{~0ull, 0, 0};
+}
+
+// One key per thread; the load factor stays <= 0.5, so every probe ends.
+__global__ void k_ix_insert(const uint64_t* keys, const uint32_t* run_start, uint64_t n_keys, Bucket* table,
+                            uint64_t mask, uint32_t* max_count) {
+  const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n_keys) return;
+  const uint32_t off = run_start[r], cnt = run_start[r + 1] - off;
+  const unsigned long long h = keys[off] >> 32;
+  for (uint64_t s = h & mask;; s = (s + 1) & mask) {
+    if (atomicCAS(reinterpret_cast<unsigned long long*>(&table[s].key), ~0ull, h) == ~0ull) {
+      table[s].offset = off;
+      table[s].count = cnt;
+      break;
+    }
+  }
+  atomicMax(max_count, cnt);
+}
+
 __global__ void k_pool_tasks(const dnaln_task* in, uint64_t pool, int bw, SwTask* out, uint32_t n) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -505,8 +573,16 @@ bool packed_bw_available(int bw) {
 }  // namespace
 
 // ------------------------------------------------------------------------------------ context
+// Sizes of a device index, and the wall time of its GPU build (0 when uploaded from the host).
+struct IndexInfo {
+  uint64_t n_buckets = 0, n_pos = 0, n_keys = 0;
+  uint32_t max_count = 0;
+  double ms = 0;
+};
+
 struct dnaln_hip_ctx {
   dnaln_map_params p{};
+  IndexInfo ix;
   int device = 0;
   hipStream_t stream = nullptr;
   DevBuf ref, coff, clen, table, pos, cost, lut;
@@ -520,6 +596,9 @@ struct dnaln_hip_ctx {
   DevBuf a_pool, a_in, a_tasks, a_res, a_cig;
   SwWork a_sw;
   dnaln_hip_batch* batch = nullptr;  // used by dnaln_hip_map_batch
+  ~dnaln_hip_ctx() {
+    if (stream) (void)hipStreamDestroy(stream);  // also when create fails part-way
+  }
 };
 
 namespace {
@@ -1136,7 +1215,115 @@ int run_chunk(dnaln_hip_batch* b, uint64_t r0, uint32_t n, bool first_chunk) {
   return DNALN_OK;
 }
 
+// k-mer starts per index-build thread; the overlap each thread re-reads is 2(w - 1) + k - 1 bases.
+constexpr uint32_t kIndexSegment = 1024;
+
+// Builds MinimizerIndex (src/index.cpp:10-45) for the reference codes at `ref` (device memory
+// with guard bytes on both sides) into `table` (the host build's form: no inlined occurrences)
+// and `pos`, on stream st. The bucket count and pos[] equal the host build's; a key may sit in
+// another slot of its probe chain, so lookups return the same slices.
+int build_index_dev(const uint8_t* ref, const uint64_t* coff, const uint32_t* clen, int nc, int k, int w,
+                    hipStream_t st, DevBuf& table, DevBuf& pos, IndexInfo& info) {
+  if (k < 4 || k > 16)
+    return fail(DNALN_E_INVALID, "the GPU index build needs k in [4, 16] (the hash fills the upper word of a "
+                                 "64-bit sort key); use a host-built index");
+  if (w < 1 || w > 255) return fail(DNALN_E_INVALID, "w must be in [1, 255]");
+  const auto t0 = std::chrono::steady_clock::now();
+  std::vector<IndexSeg> segs;
+  for (int c = 0; c < nc; ++c) {
+    if (clen[c] < (uint32_t)k) continue;
+    const uint32_t nk = clen[c] - (uint32_t)k + 1;
+    for (uint32_t s = 0; s < nk; s += kIndexSegment)
+      segs.push_back(IndexSeg{coff[c], clen[c], s, std::min(nk, s + kIndexSegment), 0});
+  }
+  const uint32_t nseg = (uint32_t)segs.size();
+  DevBuf dsegs, ring_h, ring_p, counts, offs, keys, keys2, flags, ridx, starts, tmp, mx;
+  uint64_t n_occ = 0, n_keys = 0;
+  uint32_t max_count = 0;
+  if (nseg) {
+    DN_TRY(dsegs.reserve(nseg * sizeof(IndexSeg)));
+    DN_HIP(hipMemcpyAsync(dsegs.p, segs.data(), nseg * sizeof(IndexSeg), hipMemcpyHostToDevice, st));
+    DN_TRY(ring_h.reserve((uint64_t)nseg * w * 8));
+    DN_TRY(ring_p.reserve((uint64_t)nseg * w * 4));
+    DN_TRY(counts.reserve((nseg + 1ull) * 8));
+    DN_TRY(offs.reserve((nseg + 1ull) * 8));
+    hipLaunchKernelGGL(k_ix_count, dim3((nseg + 1 + 255) / 256), dim3(256), 0, st, ref, dsegs.as<IndexSeg>(), nseg, k,
+                       w, ring_h.as<uint64_t>(), ring_p.as<uint32_t>(), counts.as<uint64_t>());
+    DN_HIP(hipGetLastError());
+    DN_TRY(scan_u64(tmp, counts.as<uint64_t>(), offs.as<uint64_t>(), nseg + 1, st));
+    DN_HIP(hipMemcpyAsync(&n_occ, offs.as<uint64_t>() + nseg, 8, hipMemcpyDeviceToHost, st));
+    DN_HIP(hipStreamSynchronize(st));
+  }
+  if (n_occ >= (1ull << 31)) return fail(DNALN_E_INVALID, "too many minimizer occurrences for the GPU index build");
+  DN_TRY(pos.reserve(n_occ * 4 + 4));
+  if (n_occ) {
+    DN_TRY(keys.reserve(n_occ * 8));
+    DN_TRY(keys2.reserve(n_occ * 8));
+    hipLaunchKernelGGL(k_ix_emit, dim3((nseg + 255) / 256), dim3(256), 0, st, ref, dsegs.as<IndexSeg>(), nseg, k, w,
+                       ring_h.as<uint64_t>(), ring_p.as<uint32_t>(), offs.as<uint64_t>(), keys.as<uint64_t>());
+    DN_HIP(hipGetLastError());
+    size_t bytes = 0;
+    DN_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, bytes, keys.as<uint64_t>(), keys2.as<uint64_t>(), (int)n_occ, 0,
+                                             32 + 2 * k, st));
+    DN_TRY(tmp.reserve(bytes));
+    DN_HIP(hipcub::DeviceRadixSort::SortKeys(tmp.p, bytes, keys.as<uint64_t>(), keys2.as<uint64_t>(), (int)n_occ, 0,
+                                             32 + 2 * k, st));
+    DN_TRY(flags.reserve((n_occ + 1) * 8));
+    DN_TRY(ridx.reserve((n_occ + 1) * 8));
+    const unsigned g = (unsigned)((n_occ + 1 + 255) / 256);
+    hipLaunchKernelGGL(k_ix_flags, dim3(g), dim3(256), 0, st, keys2.as<uint64_t>(), n_occ, flags.as<uint64_t>(),
+                       pos.as<uint32_t>());
+    DN_HIP(hipGetLastError());
+    DN_TRY(scan_u64(tmp, flags.as<uint64_t>(), ridx.as<uint64_t>(), (uint32_t)(n_occ + 1), st));
+    DN_HIP(hipMemcpyAsync(&n_keys, ridx.as<uint64_t>() + n_occ, 8, hipMemcpyDeviceToHost, st));
+    DN_HIP(hipStreamSynchronize(st));
+    DN_TRY(starts.reserve((n_keys + 1) * 4));
+    hipLaunchKernelGGL(k_ix_starts, dim3(g), dim3(256), 0, st, flags.as<uint64_t>(), ridx.as<uint64_t>(), n_occ,
+                       starts.as<uint32_t>());
+    DN_HIP(hipGetLastError());
+  }
+  uint64_t cap = 16;  // src/index.cpp:28-30
+  while (cap < n_keys * 2) cap <<= 1;
+  DN_TRY(table.reserve(cap * sizeof(Bucket)));
+  hipLaunchKernelGGL(k_ix_clear, dim3((unsigned)((cap + 255) / 256)), dim3(256), 0, st, table.as<Bucket>(), cap);
+  DN_HIP(hipGetLastError());
+  if (n_keys) {
+    DN_TRY(mx.reserve(4));
+    DN_HIP(hipMemsetAsync(mx.p, 0, 4, st));
+    hipLaunchKernelGGL(k_ix_insert, dim3((unsigned)((n_keys + 255) / 256)), dim3(256), 0, st, keys2.as<uint64_t>(),
+                       starts.as<uint32_t>(), n_keys, table.as<Bucket>(), cap - 1, mx.as<uint32_t>());
+    DN_HIP(hipGetLastError());
+    DN_HIP(hipMemcpyAsync(&max_count, mx.p, 4, hipMemcpyDeviceToHost, st));
+  }
+  DN_HIP(hipStreamSynchronize(st));
+  info.n_buckets = cap;
+  info.n_pos = n_occ;
+  info.n_keys = n_keys;
+  info.max_count = max_count;
+  info.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return DNALN_OK;
+}
+
+// Reference codes in device memory between two guard regions of N (kSeqGuard bytes each): the
+// chunked readers may touch bytes just outside a contig.
+int upload_reference(DevBuf& buf, const uint8_t* codes, uint64_t len, hipStream_t st, uint8_t** base) {
+  DN_TRY(buf.reserve(len + 2 * kSeqGuard));
+  DN_HIP(hipMemsetAsync(buf.p, 4, len + 2 * kSeqGuard, st));
+  *base = buf.as<uint8_t>() + kSeqGuard;
+  if (len) DN_HIP(hipMemcpyAsync(*base, codes, len, hipMemcpyHostToDevice, st));
+  DN_HIP(hipStreamSynchronize(st));
+  return DNALN_OK;
+}
+
 }  // namespace
+
+// A GPU-built index held on its device until downloaded (dnaln_hip_index_build).
+struct dnaln_hip_index {
+  int device = 0;
+  DevBuf table, pos;
+  IndexInfo info;
+};
+static_assert(sizeof(Bucket) == sizeof(dnaln_bucket), "device bucket layout");
 
 extern "C" {
 
@@ -1157,11 +1344,12 @@ const char* dnaln_hip_last_error(void) { return g_err.c_str(); }
 int dnaln_hip_create(const dnaln_map_params* params, const uint8_t* ref_codes, uint64_t ref_len,
                      const dnaln_contig* contigs, int32_t n_contigs, const dnaln_index_view* index, int32_t device,
                      dnaln_hip_ctx** out) {
-  if (!params || !out || !index || (!ref_codes && ref_len) || n_contigs < 1 || !contigs)
+  if (!params || !out || (!ref_codes && ref_len) || n_contigs < 1 || !contigs)
     return fail(DNALN_E_INVALID, "null argument");
   DN_TRY(validate(*params));
-  if (index->k != params->k || index->w != params->w) return fail(DNALN_E_INVALID, "index k/w differ from params");
-  if (index->n_buckets == 0 || (index->n_buckets & (index->n_buckets - 1)))
+  if (index && (index->k != params->k || index->w != params->w))
+    return fail(DNALN_E_INVALID, "index k/w differ from params");
+  if (index && (index->n_buckets == 0 || (index->n_buckets & (index->n_buckets - 1))))
     return fail(DNALN_E_INVALID, "index bucket count must be a power of two");
   if (ref_len >= (1ull << 31)) return fail(DNALN_E_INVALID, "reference is too large (limit: 2^31 bp)");
   std::unique_ptr<dnaln_hip_ctx> c(new dnaln_hip_ctx);
@@ -1176,10 +1364,7 @@ int dnaln_hip_create(const dnaln_map_params* params, const uint8_t* ref_codes, u
   c->w = params->w;
   // The packed kernel streams aligned dwords and may touch bytes just outside a window: the
   // reference sits between two guard regions of kSeqGuard bytes (never used as sequence).
-  DN_TRY(c->ref.reserve(ref_len + 2 * kSeqGuard));
-  DN_HIP(hipMemset(c->ref.p, 4, ref_len + 2 * kSeqGuard));
-  c->ref_base = c->ref.as<uint8_t>() + kSeqGuard;
-  if (ref_len) DN_HIP(hipMemcpy(c->ref_base, ref_codes, ref_len, hipMemcpyHostToDevice));
+  DN_TRY(upload_reference(c->ref, ref_codes, ref_len, c->stream, &c->ref_base));
   c->h_coff.resize(n_contigs);
   c->h_clen.resize(n_contigs);
   for (int i = 0; i < n_contigs; ++i) {
@@ -1191,18 +1376,23 @@ int dnaln_hip_create(const dnaln_map_params* params, const uint8_t* ref_codes, u
   DN_TRY(c->clen.reserve(n_contigs * 4));
   DN_HIP(hipMemcpy(c->coff.p, c->h_coff.data(), n_contigs * 8, hipMemcpyHostToDevice));
   DN_HIP(hipMemcpy(c->clen.p, c->h_clen.data(), n_contigs * 4, hipMemcpyHostToDevice));
-  c->mask = index->n_buckets - 1;
-  c->n_pos = index->n_pos;
-  DN_TRY(c->table.reserve(index->n_buckets * sizeof(dnaln_bucket)));
-  DN_HIP(hipMemcpy(c->table.p, index->table, index->n_buckets * sizeof(dnaln_bucket), hipMemcpyHostToDevice));
-  DN_TRY(c->pos.reserve(index->n_pos * 4 + 4));
-  if (index->n_pos) DN_HIP(hipMemcpy(c->pos.p, index->pos, index->n_pos * 4, hipMemcpyHostToDevice));
-  if (index->n_buckets) {
-    hipLaunchKernelGGL(k_inline_unique, dim3((unsigned)((index->n_buckets + 255) / 256)), dim3(256), 0, nullptr,
-                       c->table.as<Bucket>(), (uint64_t)index->n_buckets, c->pos.as<uint32_t>());
-    DN_HIP(hipGetLastError());
-    DN_HIP(hipDeviceSynchronize());
+  if (index) {
+    c->ix.n_buckets = index->n_buckets;
+    c->ix.n_pos = index->n_pos;
+    DN_TRY(c->table.reserve(index->n_buckets * sizeof(dnaln_bucket)));
+    DN_HIP(hipMemcpy(c->table.p, index->table, index->n_buckets * sizeof(dnaln_bucket), hipMemcpyHostToDevice));
+    DN_TRY(c->pos.reserve(index->n_pos * 4 + 4));
+    if (index->n_pos) DN_HIP(hipMemcpy(c->pos.p, index->pos, index->n_pos * 4, hipMemcpyHostToDevice));
+  } else {
+    DN_TRY(build_index_dev(c->ref_base, c->h_coff.data(), c->h_clen.data(), n_contigs, c->k, c->w, c->stream, c->table,
+                           c->pos, c->ix));
   }
+  c->mask = c->ix.n_buckets - 1;
+  c->n_pos = c->ix.n_pos;
+  hipLaunchKernelGGL(k_inline_unique, dim3((unsigned)((c->ix.n_buckets + 255) / 256)), dim3(256), 0, c->stream,
+                     c->table.as<Bucket>(), c->ix.n_buckets, c->pos.as<uint32_t>());
+  DN_HIP(hipGetLastError());
+  DN_HIP(hipStreamSynchronize(c->stream));
   // Chain gap cost by drift dd (src/chain.cpp:36), evaluated in double exactly as the reference.
   std::vector<int32_t> cost(c->p.ch.band + 1, 0);
   for (int dd = 1; dd <= c->p.ch.band; ++dd)
@@ -1230,7 +1420,6 @@ void dnaln_hip_destroy(dnaln_hip_ctx* c) {
   (void)hipSetDevice(c->device);
   (void)hipDeviceSynchronize();
   if (c->batch) dnaln_hip_batch_destroy(c->batch);
-  if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
 
@@ -1661,6 +1850,71 @@ int dnaln_hip_device_memory(dnaln_hip_ctx* c, uint64_t* free_bytes, uint64_t* to
   *free_bytes = f;
   *total_bytes = t;
   return DNALN_OK;
+}
+
+int dnaln_hip_context_index(dnaln_hip_ctx* c, uint64_t* n_buckets, uint64_t* n_pos, double* build_ms) {
+  if (!c) return fail(DNALN_E_INVALID, "null argument");
+  if (n_buckets) *n_buckets = c->ix.n_buckets;
+  if (n_pos) *n_pos = c->ix.n_pos;
+  if (build_ms) *build_ms = c->ix.ms;
+  return DNALN_OK;
+}
+
+int dnaln_hip_index_build(const uint8_t* ref_codes, uint64_t ref_len, const dnaln_contig* contigs, int32_t n_contigs,
+                          int32_t k, int32_t w, int32_t device, dnaln_hip_index** out) {
+  if (!out || !contigs || n_contigs < 1 || (!ref_codes && ref_len)) return fail(DNALN_E_INVALID, "null argument");
+  *out = nullptr;
+  if (ref_len >= (1ull << 31)) return fail(DNALN_E_INVALID, "reference is too large (limit: 2^31 bp)");
+  std::vector<uint64_t> coff(n_contigs);
+  std::vector<uint32_t> clen(n_contigs);
+  for (int i = 0; i < n_contigs; ++i) {
+    if (contigs[i].offset + contigs[i].len > ref_len) return fail(DNALN_E_INVALID, "contig outside the reference");
+    coff[i] = contigs[i].offset;
+    clen[i] = (uint32_t)contigs[i].len;
+  }
+  std::unique_ptr<dnaln_hip_index> ix(new dnaln_hip_index);
+  ix->device = device;
+  DN_HIP(hipSetDevice(device));
+  hipStream_t st = nullptr;
+  DN_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  int rc;
+  {
+    DevBuf ref;
+    uint8_t* base = nullptr;
+    rc = upload_reference(ref, ref_codes, ref_len, st, &base);
+    if (rc == DNALN_OK)
+      rc = build_index_dev(base, coff.data(), clen.data(), n_contigs, k, w, st, ix->table, ix->pos, ix->info);
+    (void)hipStreamSynchronize(st);
+  }
+  (void)hipStreamDestroy(st);
+  if (rc != DNALN_OK) return rc;
+  *out = ix.release();
+  return DNALN_OK;
+}
+
+int dnaln_hip_index_info(const dnaln_hip_index* ix, uint64_t* n_buckets, uint64_t* n_pos, uint64_t* n_keys,
+                         uint32_t* max_count, double* build_ms) {
+  if (!ix) return fail(DNALN_E_INVALID, "null argument");
+  if (n_buckets) *n_buckets = ix->info.n_buckets;
+  if (n_pos) *n_pos = ix->info.n_pos;
+  if (n_keys) *n_keys = ix->info.n_keys;
+  if (max_count) *max_count = ix->info.max_count;
+  if (build_ms) *build_ms = ix->info.ms;
+  return DNALN_OK;
+}
+
+int dnaln_hip_index_download(const dnaln_hip_index* ix, dnaln_bucket* table, uint32_t* pos) {
+  if (!ix || !table || (!pos && ix->info.n_pos)) return fail(DNALN_E_INVALID, "null argument");
+  DN_HIP(hipSetDevice(ix->device));
+  DN_HIP(hipMemcpy(table, ix->table.p, ix->info.n_buckets * sizeof(dnaln_bucket), hipMemcpyDeviceToHost));
+  if (ix->info.n_pos) DN_HIP(hipMemcpy(pos, ix->pos.p, ix->info.n_pos * 4, hipMemcpyDeviceToHost));
+  return DNALN_OK;
+}
+
+void dnaln_hip_index_free(dnaln_hip_index* ix) {
+  if (!ix) return;
+  (void)hipSetDevice(ix->device);
+  delete ix;
 }
 
 int dnaln_hip_context_params(dnaln_hip_ctx* c, dnaln_map_params* out) {

@@ -14,6 +14,7 @@
 
 #include "device/bw_list.hpp"
 #include "device/fused_seed_chain.hpp"
+#include "device/index_build.hpp"
 #include "device/sw_generic.hpp"
 #include "device/sw_packed.hpp"
 
@@ -333,6 +334,34 @@ long emu_seed_chain(const uint8_t* codes, const uint64_t* offs, const int32_t* l
     differ += !same;
   }
   return differ;
+}
+
+// The GPU index build's occurrence list (k_ix_count / k_ix_emit): every contig cut into
+// segments of `seg` k-mer starts, each sketched on its own by sketch_segment. Writes the
+// unsorted occurrence keys (hash << 32 | pos << 1 | strand) and returns their number, or minus
+// the number needed when cap is short.
+long emu_index_occ(const uint8_t* ref, const int64_t* coff, const int64_t* clen, int nc, int k, int w, uint32_t seg,
+                   uint64_t* out, size_t cap) {
+  uint64_t total = 0;
+  for (int c = 0; c < nc; ++c) total = std::max<uint64_t>(total, (uint64_t)(coff[c] + clen[c]));
+  std::vector<uint8_t> guarded(total + 2 * kSeqGuard, 4);
+  std::memcpy(guarded.data() + kSeqGuard, ref, total);
+  const uint8_t* base = guarded.data() + kSeqGuard;
+  std::vector<uint64_t> rh(w);
+  std::vector<uint32_t> rps(w);
+  size_t n = 0;
+  for (int c = 0; c < nc; ++c) {
+    if (clen[c] < k) continue;
+    const uint32_t nk = (uint32_t)clen[c] - (uint32_t)k + 1;
+    for (uint32_t s = 0; s < nk; s += seg) {
+      const IndexSeg g{(uint64_t)coff[c], (uint32_t)clen[c], s, std::min(nk, s + seg), 0};
+      sketch_segment(base, g, k, w, Ring{rh.data(), rps.data(), 1}, [&](uint64_t h, uint32_t p, uint32_t st) {
+        if (n < cap) out[n] = occ_key(h, g.base, p, st);
+        ++n;
+      });
+    }
+  }
+  return n <= cap ? (long)n : -(long)n;
 }
 
 }  // extern "C"

@@ -1,4 +1,5 @@
-// cli_main.cpp — `dnaln-amd map`: the reference's `dnaln map` (src/main.cpp:12-75) on one MI355X.
+// cli_main.cpp — `dnaln-amd map`: the reference's `dnaln map` (src/main.cpp:12-75) on one MI355X,
+// and `dnaln-amd index`, which writes the index to a file that `map --index` loads.
 //
 // Same arguments, defaults, validation and SAM output; reads go to the GPU in batches of
 // --batch reads (larger than the reference's 8192 by default, since a batch fills the GPU).
@@ -86,12 +87,47 @@ const char* kHelp =
     "  --scalar, --no-prefetch   accepted for compatibility (no effect on the GPU)\n"
     "  --plain        plain GPU form (one launch sequence per 65,536-read chunk)\n"
     "  --tune         run the autotuner on the first batch\n"
+    "  --device INT   HIP device [0]\n"
+    "  --index FILE   load the index from FILE (written by `dnaln-amd index`) instead of building it\n"
+    "  --gpu-index    build the index on the GPU instead of the host (k <= 16)\n";
+
+const char* kIndexHelp =
+    "Usage: dnaln-amd index [options] <ref.fa[.gz]> -o <out.dnx>\n\n"
+    "  -k INT         minimizer k-mer size [15]\n"
+    "  -w INT         minimizer window size [10]\n"
+    "  --gpu          build on the GPU (k <= 16)\n"
     "  --device INT   HIP device [0]\n";
+
+// Builds the index of a reference and writes it to a file that `map --index` loads (the
+// reference rebuilds its index on every run, src/main.cpp:34-36).
+int cmd_index(int argc, char** argv) {
+  const Args a(argc, argv, {"-k", "-w", "-o", "--device"}, {"--gpu", "-h", "--help"});
+  if (a.pos.size() != 1 || !a.has("-o") || a.has("-h") || a.has("--help")) {
+    std::fputs(kIndexHelp, stderr);
+    return a.has("-h") || a.has("--help") ? 0 : 1;
+  }
+  const int k = (int)a.num("-k", 15), w = (int)a.num("-w", 10);
+  double t = now();
+  const Reference ref = Reference::load(a.pos[0]);
+  log_msg("reference: %zu contig(s), %zu bp (loaded in %.2f s)", ref.contigs.size(), ref.seq.size(), now() - t);
+  t = now();
+  MinimizerIndex idx;
+  if (a.has("--gpu")) idx.build_gpu(ref, k, w, (int)a.num("--device", 0));
+  else idx.build(ref, k, w);
+  log_msg("index: k=%d w=%d, %zu distinct minimizers, %zu positions, %.1f MB, built on the %s in %.2f s", k, w,
+          idx.n_keys(), idx.n_positions(), idx.memory_bytes() / 1048576.0, a.has("--gpu") ? "GPU" : "host",
+          now() - t);
+  t = now();
+  idx.save(ref, a.opt.at("-o"));
+  log_msg("wrote %s in %.2f s", a.opt.at("-o").c_str(), now() - t);
+  return 0;
+}
 
 int cmd_map(int argc, char** argv) {
   const std::set<std::string> values = {"-k", "-w", "--bw", "-A", "-B", "-O", "-E", "--max-occ", "--min-score",
-                                        "--max-cand", "--min-chain", "-o", "--batch", "--device"};
-  const Args a(argc, argv, values, {"--scalar", "--no-prefetch", "--plain", "--tune", "-h", "--help"});
+                                        "--max-cand", "--min-chain", "-o", "--batch", "--device", "--index"};
+  const Args a(argc, argv, values,
+               {"--scalar", "--no-prefetch", "--plain", "--tune", "--gpu-index", "-h", "--help"});
   if (a.pos.size() != 2 || a.has("-h") || a.has("--help")) {
     std::fputs(kHelp, stderr);
     return a.has("-h") || a.has("--help") ? 0 : 1;
@@ -126,9 +162,21 @@ int cmd_map(int argc, char** argv) {
   log_msg("reference: %zu contig(s), %zu bp (loaded in %.2f s)", ref.contigs.size(), ref.seq.size(), now() - t);
   t = now();
   MinimizerIndex idx;
-  idx.build(ref, p.k, p.w);
-  log_msg("index: k=%d w=%d, %zu distinct minimizers, %zu positions, %.1f MB, built in %.2f s", p.k, p.w,
-          idx.n_keys(), idx.n_positions(), idx.memory_bytes() / 1048576.0, now() - t);
+  const char* how = "built";
+  if (a.has("--index")) {
+    idx.load(ref, a.opt.at("--index"));
+    if (idx.k() != p.k || idx.w() != p.w)
+      die("index file has k=" + std::to_string(idx.k()) + " w=" + std::to_string(idx.w()) +
+          "; pass the same -k / -w");
+    how = "loaded";
+  } else if (a.has("--gpu-index")) {
+    idx.build_gpu(ref, p.k, p.w, p.device);
+    how = "built on the GPU";
+  } else {
+    idx.build(ref, p.k, p.w);
+  }
+  log_msg("index: k=%d w=%d, %zu distinct minimizers, %zu positions, %.1f MB, %s in %.2f s", p.k, p.w,
+          idx.n_keys(), idx.n_positions(), idx.memory_bytes() / 1048576.0, how, now() - t);
 
   FILE* out = stdout;
   if (a.has("-o")) {
@@ -167,15 +215,19 @@ int cmd_map(int argc, char** argv) {
 
 int main(int argc, char** argv) {
   if (argc < 2 || !std::strcmp(argv[1], "-h") || !std::strcmp(argv[1], "--help")) {
-    std::fputs("Usage: dnaln-amd map [options] <ref.fa> <reads.fq>\n", stderr);
+    std::fputs("Usage: dnaln-amd map [options] <ref.fa> <reads.fq>\n"
+               "       dnaln-amd index [options] <ref.fa> -o <out.dnx>\n",
+               stderr);
     return argc < 2 ? 1 : 0;
   }
-  if (std::strcmp(argv[1], "map") != 0) {
-    std::fprintf(stderr, "[dnaln-amd] error: unknown command '%s' (this build provides `map`)\n", argv[1]);
+  const bool map = !std::strcmp(argv[1], "map"), index = !std::strcmp(argv[1], "index");
+  if (!map && !index) {
+    std::fprintf(stderr, "[dnaln-amd] error: unknown command '%s' (this build provides `map` and `index`)\n",
+                 argv[1]);
     return 1;
   }
   try {
-    return cmd_map(argc - 2, argv + 2);
+    return map ? cmd_map(argc - 2, argv + 2) : cmd_index(argc - 2, argv + 2);
   } catch (const Error& e) {
     die(e.what());
   }

@@ -218,6 +218,41 @@ void MinimizerIndex::build(const Reference& ref, int k, int w) {
   built_ = true;
 }
 
+namespace {
+std::vector<dnaln_contig> contig_table(const Reference& ref) {
+  std::vector<dnaln_contig> cs;
+  for (const Contig& c : ref.contigs) cs.push_back({c.offset, c.len, 0});
+  return cs;
+}
+}  // namespace
+
+double MinimizerIndex::build_gpu(const Reference& ref, int k, int w, int device) {
+  if (built_) dnaln_index_free(&idx_);
+  built_ = false;
+  const std::vector<dnaln_contig> cs = contig_table(ref);
+  double ms = 0;
+  if (dnaln_index_build_gpu(ref.seq.data(), cs.data(), (int32_t)cs.size(), k, w, device, &idx_, &ms) != DNALN_OK)
+    throw Error(std::string("GPU index build failed: ") + dnaln_hip_last_error());
+  built_ = true;
+  return ms;
+}
+
+void MinimizerIndex::save(const Reference& ref, const std::string& path) const {
+  if (!built_) throw Error("no index to save");
+  const std::vector<dnaln_contig> cs = contig_table(ref);
+  if (dnaln_index_save(&idx_, ref.seq.data(), cs.data(), (int32_t)cs.size(), path.c_str()) != DNALN_OK)
+    throw Error("cannot write index file '" + path + "'");
+}
+
+void MinimizerIndex::load(const Reference& ref, const std::string& path) {
+  if (built_) dnaln_index_free(&idx_);
+  built_ = false;
+  const std::vector<dnaln_contig> cs = contig_table(ref);
+  if (dnaln_index_load(path.c_str(), ref.seq.data(), cs.data(), (int32_t)cs.size(), &idx_) != DNALN_OK)
+    throw Error("index file '" + path + "' is unreadable, corrupt, or was built from another reference");
+  built_ = true;
+}
+
 const uint32_t* MinimizerIndex::lookup(uint64_t h, uint32_t& n) const {
   n = 0;
   if (!built_) return nullptr;

@@ -137,6 +137,12 @@ class MinimizerIndex {
   MinimizerIndex(const MinimizerIndex&) = delete;
   MinimizerIndex& operator=(const MinimizerIndex&) = delete;
   void build(const Reference& ref, int k, int w);
+  // The same index built on HIP device `device` (k <= 16) and downloaded; returns the device
+  // build's wall time in ms.
+  double build_gpu(const Reference& ref, int k, int w, int device);
+  // The index file of include/dnaln_host.h, checked against `ref` on load.
+  void save(const Reference& ref, const std::string& path) const;
+  void load(const Reference& ref, const std::string& path);
   // Occurrences (global_pos << 1 | strand) of hash h, ascending; n = 0 when absent.
   const uint32_t* lookup(uint64_t h, uint32_t& n) const;
   int k() const { return idx_.view.k; }

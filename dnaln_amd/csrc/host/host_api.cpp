@@ -2,6 +2,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <vector>
 
 #include "device/seed_chain.hpp"
@@ -96,6 +97,162 @@ void dnaln_index_free(dnaln_index* idx) {
   delete[] idx->view.table;
   delete[] idx->view.pos;
   std::memset(idx, 0, sizeof *idx);
+}
+
+int dnaln_index_build_gpu(const uint8_t* ref, const dnaln_contig* contigs, int32_t n_contigs, int32_t k, int32_t w,
+                          int32_t device, dnaln_index* out, double* build_ms) {
+  if (!out || !contigs || n_contigs < 1) return DNALN_E_INVALID;
+  std::memset(out, 0, sizeof *out);
+  uint64_t total = 0;
+  for (int c = 0; c < n_contigs; ++c) total = std::max<uint64_t>(total, contigs[c].offset + contigs[c].len);
+  dnaln_hip_index* ix = nullptr;
+  int rc = dnaln_hip_index_build(ref, total, contigs, n_contigs, k, w, device, &ix);
+  if (rc != DNALN_OK) return rc;
+  uint64_t nb = 0, np = 0, nk = 0;
+  uint32_t mc = 0;
+  double ms = 0;
+  dnaln_hip_index_info(ix, &nb, &np, &nk, &mc, &ms);
+  dnaln_bucket* table = new dnaln_bucket[nb];
+  uint32_t* pos = new uint32_t[np + 1];
+  rc = dnaln_hip_index_download(ix, table, pos);
+  dnaln_hip_index_free(ix);
+  if (rc != DNALN_OK) {
+    delete[] table;
+    delete[] pos;
+    return rc;
+  }
+  out->view.table = table;
+  out->view.n_buckets = nb;
+  out->view.pos = pos;
+  out->view.n_pos = np;
+  out->view.k = k;
+  out->view.w = w;
+  out->n_keys = nk;
+  out->max_count = mc;
+  if (build_ms) *build_ms = ms;
+  return DNALN_OK;
+}
+
+// A 64-bit multiply-xorshift digest over 8-byte words (not cryptographic: it catches an index
+// file paired with other codes or contigs, not a forgery).
+uint64_t dnaln_reference_digest(const uint8_t* ref, const dnaln_contig* contigs, int32_t n_contigs) {
+  uint64_t h = 0x243f6a8885a308d3ull ^ (uint64_t)n_contigs;
+  auto mix = [&h](uint64_t v) {
+    h ^= v;
+    h *= 0x9fb21c651e98df25ull;
+    h ^= h >> 29;
+  };
+  for (int c = 0; c < n_contigs; ++c) {
+    mix(contigs[c].offset);
+    mix(contigs[c].len);
+  }
+  for (int c = 0; c < n_contigs; ++c) {
+    const uint8_t* p = ref + contigs[c].offset;
+    const uint64_t n = contigs[c].len;
+    uint64_t i = 0;
+    for (; i + 8 <= n; i += 8) {
+      uint64_t v;
+      std::memcpy(&v, p + i, 8);
+      mix(v);
+    }
+    uint64_t tail = 0;
+    for (uint64_t j = 0; i + j < n; ++j) tail |= (uint64_t)p[i + j] << (8 * j);
+    mix(tail ^ (n - i) << 56);
+  }
+  return h;
+}
+
+namespace {
+
+struct IndexFileHeader {
+  char magic[8];
+  uint32_t version;
+  int32_t k, w;
+  uint32_t max_count;
+  uint64_t n_buckets, n_pos, n_keys, digest, reserved;
+};
+static_assert(sizeof(IndexFileHeader) == 64, "index file header is 64 bytes");
+constexpr char kIndexMagic[8] = {'D', 'N', 'A', 'L', 'N', 'I', 'X', '1'};
+
+struct File {
+  std::FILE* f;
+  ~File() {
+    if (f) std::fclose(f);
+  }
+};
+
+}  // namespace
+
+int dnaln_index_save(const dnaln_index* idx, const uint8_t* ref, const dnaln_contig* contigs, int32_t n_contigs,
+                     const char* path) {
+  if (!idx || !path || !contigs || n_contigs < 1 || !idx->view.table) return DNALN_E_INVALID;
+  IndexFileHeader h{};
+  std::memcpy(h.magic, kIndexMagic, 8);
+  h.version = 1;
+  h.k = idx->view.k;
+  h.w = idx->view.w;
+  h.max_count = idx->max_count;
+  h.n_buckets = idx->view.n_buckets;
+  h.n_pos = idx->view.n_pos;
+  h.n_keys = idx->n_keys;
+  h.digest = dnaln_reference_digest(ref, contigs, n_contigs);
+  File f{std::fopen(path, "wb")};
+  if (!f.f) return DNALN_E_IO;
+  if (std::fwrite(&h, sizeof h, 1, f.f) != 1) return DNALN_E_IO;
+  if (std::fwrite(idx->view.table, sizeof(dnaln_bucket), h.n_buckets, f.f) != h.n_buckets) return DNALN_E_IO;
+  if (h.n_pos && std::fwrite(idx->view.pos, 4, h.n_pos, f.f) != h.n_pos) return DNALN_E_IO;
+  if (std::fflush(f.f) != 0) return DNALN_E_IO;
+  return DNALN_OK;
+}
+
+int dnaln_index_load(const char* path, const uint8_t* ref, const dnaln_contig* contigs, int32_t n_contigs,
+                     dnaln_index* out) {
+  if (!out || !path || !contigs || n_contigs < 1) return DNALN_E_INVALID;
+  std::memset(out, 0, sizeof *out);
+  File f{std::fopen(path, "rb")};
+  if (!f.f) return DNALN_E_IO;
+  IndexFileHeader h{};
+  if (std::fread(&h, sizeof h, 1, f.f) != 1 || std::memcmp(h.magic, kIndexMagic, 8) != 0 || h.version != 1)
+    return DNALN_E_IO;
+  if (h.k < 4 || h.k > 28 || h.w < 1 || h.w > 255 || h.n_buckets < 16 || (h.n_buckets & (h.n_buckets - 1)) ||
+      h.n_buckets > (1ull << 40) || h.n_keys * 2 > h.n_buckets || h.n_pos >= (1ull << 32) || h.n_keys > h.n_pos)
+    return DNALN_E_IO;
+  if (std::fseek(f.f, 0, SEEK_END) != 0) return DNALN_E_IO;
+  const long long size = std::ftell(f.f);
+  if (size < 0 || (uint64_t)size != sizeof h + h.n_buckets * sizeof(dnaln_bucket) + h.n_pos * 4) return DNALN_E_IO;
+  if (h.digest != dnaln_reference_digest(ref, contigs, n_contigs)) return DNALN_E_IO;
+  if (std::fseek(f.f, (long)sizeof h, SEEK_SET) != 0) return DNALN_E_IO;
+  std::unique_ptr<dnaln_bucket[]> table(new dnaln_bucket[h.n_buckets]);
+  std::unique_ptr<uint32_t[]> pos(new uint32_t[h.n_pos + 1]);
+  pos[h.n_pos] = 0;
+  if (std::fread(table.get(), sizeof(dnaln_bucket), h.n_buckets, f.f) != h.n_buckets) return DNALN_E_IO;
+  if (h.n_pos && std::fread(pos.get(), 4, h.n_pos, f.f) != h.n_pos) return DNALN_E_IO;
+  // What the device kernels read stays in bounds: every slice inside pos[], every position
+  // inside the reference, at least half the buckets empty (so every probe ends); the stated key
+  // count and largest count match the table.
+  uint64_t total = 0;
+  for (int c = 0; c < n_contigs; ++c) total = std::max<uint64_t>(total, contigs[c].offset + contigs[c].len);
+  uint64_t keys = 0;
+  uint32_t max_count = 0;
+  for (uint64_t i = 0; i < h.n_buckets; ++i) {
+    const dnaln_bucket& b = table[i];
+    if (b.key == ~0ULL) continue;
+    if (b.count == 0 || (uint64_t)b.offset + b.count > h.n_pos) return DNALN_E_IO;
+    ++keys;
+    max_count = std::max(max_count, b.count);
+  }
+  if (keys != h.n_keys || max_count != h.max_count) return DNALN_E_IO;
+  for (uint64_t i = 0; i < h.n_pos; ++i)
+    if ((pos[i] >> 1) >= total) return DNALN_E_IO;
+  out->view.table = table.release();
+  out->view.pos = pos.release();
+  out->view.n_buckets = h.n_buckets;
+  out->view.n_pos = h.n_pos;
+  out->view.k = h.k;
+  out->view.w = h.w;
+  out->n_keys = h.n_keys;
+  out->max_count = h.max_count;
+  return DNALN_OK;
 }
 
 uint32_t dnaln_fnv1a(const char* s, size_t n) {

@@ -36,7 +36,8 @@ enum {
   DNALN_E_DEVICE = -3,    /* HIP runtime error */
   DNALN_E_NOMEM = -4,     /* device allocation failed */
   DNALN_E_INTERNAL = -5,  /* an internal consistency check failed */
-  DNALN_E_PENDING = -6    /* dnaln_hip_batch_poll: still running */
+  DNALN_E_PENDING = -6,   /* dnaln_hip_batch_poll: still running */
+  DNALN_E_IO = -7         /* index file: unreadable, truncated, corrupt, or built from other codes */
 };
 
 /* ScoreParams (src/align.hpp:34-40). A gap of length L costs gap_open + L * gap_ext. */
@@ -128,7 +129,9 @@ typedef struct {
 void dnaln_hip_default_params(dnaln_map_params* p);
 
 /* Uploads the reference codes (0..4) and the index once; they stay resident in HBM.
- * Validates the parameters as src/cli.hpp:93-95 and src/index.cpp:11-12 do. */
+ * Validates the parameters as src/cli.hpp:93-95 and src/index.cpp:11-12 do. With index NULL
+ * the index is built on the GPU from the uploaded codes (as dnaln_hip_index_build; k <= 16) in
+ * place of the host's MinimizerIndex::build before the Mapper is made (src/main.cpp:34-36). */
 int dnaln_hip_create(const dnaln_map_params* params, const uint8_t* ref_codes, uint64_t ref_len,
                      const dnaln_contig* contigs, int32_t n_contigs, const dnaln_index_view* index, int32_t device,
                      struct dnaln_hip_ctx** out);
@@ -142,6 +145,25 @@ int dnaln_hip_context_params(struct dnaln_hip_ctx* ctx, dnaln_map_params* out);
 /* Device allocations the library has made in this process (count and bytes), for callers that
  * check that a warmed-up path allocates nothing (no reference counterpart). */
 void dnaln_hip_alloc_counters(uint64_t* events, uint64_t* bytes);
+
+/* MinimizerIndex::build (src/index.cpp:10-45) on the GPU: every contig sketched in segments of
+ * 1,024 k-mer starts, one per thread; the occurrences radix-sorted by (hash, pos); each key placed
+ * by one thread with atomic linear probing. The bucket count and pos[] equal the host build's; a
+ * key may sit in a different slot of its probe chain, so every lookup (src/index.hpp:24-38)
+ * returns the same slice. Needs k <= 16. The result stays on the device until downloaded into
+ * host arrays of n_buckets buckets and n_pos positions (dnaln_hip_index_info). No reference
+ * counterpart for the handle: the reference builds its index on the host at start-up
+ * (src/main.cpp:34-36). */
+typedef struct dnaln_hip_index dnaln_hip_index;
+int dnaln_hip_index_build(const uint8_t* ref_codes, uint64_t ref_len, const dnaln_contig* contigs, int32_t n_contigs,
+                          int32_t k, int32_t w, int32_t device, dnaln_hip_index** out);
+int dnaln_hip_index_info(const dnaln_hip_index* ix, uint64_t* n_buckets, uint64_t* n_pos, uint64_t* n_keys,
+                         uint32_t* max_count, double* build_ms);
+int dnaln_hip_index_download(const dnaln_hip_index* ix, dnaln_bucket* table, uint32_t* pos);
+void dnaln_hip_index_free(dnaln_hip_index* ix);
+/* Sizes of the context's resident index, and the wall time of its GPU build in ms (0 when the
+ * index came from the host). */
+int dnaln_hip_context_index(struct dnaln_hip_ctx* ctx, uint64_t* n_buckets, uint64_t* n_pos, double* build_ms);
 
 /* Aligner::align: n independent banded alignments; traceback = 0 computes the score only.
  * sp may be NULL (use the context's scoring). Results are bit-identical to the reference's

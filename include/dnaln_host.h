@@ -3,6 +3,8 @@
  *   dnaln_index_build   MinimizerIndex::build (src/index.cpp:10-45): sketch every contig, sort
  *                       occurrences by (hash, pos), place keys with linear probing at load <= 0.5.
  *                       Produces the view dnaln_hip_create uploads.
+ *   dnaln_index_build_gpu, dnaln_index_save / dnaln_index_load
+ *                       the same index built on the GPU, and written to / read from a file.
  *   dnaln_sam_format    Mapper::append_sam (src/mapper.cpp:231-270) for one record.
  *   dnaln_fnv1a         fnv1a (src/common.hpp:136-140), the equal-best tie-break hash.
  *   dnaln_encode        encode (src/common.hpp:36-38).
@@ -28,6 +30,25 @@ typedef struct {
 int dnaln_index_build(const uint8_t* ref_codes, const dnaln_contig* contigs, int32_t n_contigs, int32_t k, int32_t w,
                       dnaln_index* out);
 void dnaln_index_free(dnaln_index* idx);
+
+/* The same index built on GPU `device` (dnaln_hip_index_build, k <= 16) and downloaded into
+ * host arrays: bucket count, pos[], key count and largest count equal dnaln_index_build's.
+ * *build_ms (optional) receives the device build's wall time. */
+int dnaln_index_build_gpu(const uint8_t* ref_codes, const dnaln_contig* contigs, int32_t n_contigs, int32_t k,
+                          int32_t w, int32_t device, dnaln_index* out, double* build_ms);
+
+/* Serialized index (no reference counterpart: src/main.cpp:34-36 rebuilds it on every run).
+ * Little-endian file: a 64-byte header (magic "DNALNIX1", version 1, k, w, largest count,
+ * bucket count, position count, key count, a 64-bit digest of the contigs' extents and codes),
+ * then the buckets (16 bytes each) and the positions (4 bytes each). Load checks the header,
+ * the exact file size, the digest against the codes given, and that every bucket's slice lies
+ * inside pos[]; any failure returns DNALN_E_IO. */
+int dnaln_index_save(const dnaln_index* idx, const uint8_t* ref_codes, const dnaln_contig* contigs,
+                     int32_t n_contigs, const char* path);
+int dnaln_index_load(const char* path, const uint8_t* ref_codes, const dnaln_contig* contigs, int32_t n_contigs,
+                     dnaln_index* out);
+/* The digest stored in an index file: contig offsets and lengths, then their codes. */
+uint64_t dnaln_reference_digest(const uint8_t* ref_codes, const dnaln_contig* contigs, int32_t n_contigs);
 
 uint32_t dnaln_fnv1a(const char* s, size_t n);
 void dnaln_fnv1a_many(const char* names, const uint64_t* offs, const int32_t* lens, uint64_t n, uint32_t* out);

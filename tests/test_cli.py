@@ -52,3 +52,25 @@ def test_refuses_to_map_without_a_gpu(tmp_path):
     assert res.returncode == 1
     assert "dnaln_hip_create" in res.stderr
     assert "index: k=15 w=10" in res.stderr  # the host part ran; the device part refused
+
+
+def test_index_command_writes_a_loadable_file(tmp_path):
+    """`dnaln-amd index` (host build) writes the index file; it loads against its reference and
+    equals the in-process host build; a different reference refuses it."""
+    from dnaln_amd import api
+    import numpy as np
+
+    g = synth.make_genome(0, 60_000, 3, 5)
+    synth.write_fasta(g, str(tmp_path / "ref.fa"))
+    r = _run("index", "-k", "13", "-w", "7", tmp_path / "ref.fa", "-o", tmp_path / "ref.dnx")
+    assert r.returncode == 0, r.stderr
+    assert "built on the host" in r.stderr
+    loaded = api.Index.load(tmp_path / "ref.dnx", g)
+    mine = api.Index(g, 13, 7)
+    assert (loaded.k, loaded.w) == (13, 7)
+    assert np.array_equal(loaded.table(), mine.table()) and np.array_equal(loaded.positions(), mine.positions())
+    r = _run("index", tmp_path / "ref.fa")
+    assert r.returncode == 1 and "Usage: dnaln-amd index" in r.stderr
+    other = synth.make_genome(0, 60_000, 3, 6)
+    with pytest.raises(api.DnalnError):
+        api.Index.load(tmp_path / "ref.dnx", other)

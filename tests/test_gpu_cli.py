@@ -67,3 +67,31 @@ def test_hip_aligner_matches_reference_kernel_vectors(tmp_path, si):
             v = [int(x) for x in line.split()]
             assert v[:6] == want[i].tolist(), (tb, i)
             assert v[6:] == cig[offs[i]:offs[i + 1]].tolist(), (tb, i)
+
+
+@pytest.mark.parametrize("how", ["gpu-build", "file"])
+def test_cli_with_gpu_built_or_loaded_index_gives_reference_sam(tmp_path, how):
+    """`map --gpu-index` and `index --gpu` + `map --index`: the same SAM body as the reference."""
+    name = "edge_repeats_params"
+    spec = sets.SETS[name]
+    g = sets.genome(spec)
+    r = sets.reads(spec, g)
+    synth.write_fasta(g, str(tmp_path / "ref.fa"))
+    synth.write_fastq(r, str(tmp_path / "reads.fq"))
+    flags = [str(x) for kv in spec["flags"].items() for x in kv]
+    extra = ["--gpu-index"]
+    if how == "file":
+        kw = [str(x) for kv in spec["flags"].items() if kv[0] in ("-k", "-w") for x in kv]
+        cmd = [str(EXE), "index", "--gpu", *kw, str(tmp_path / "ref.fa"), "-o", str(tmp_path / "ref.dnx")]
+        print(" ".join(cmd), flush=True)
+        res = subprocess.run(cmd, capture_output=True, text=True, timeout=100)
+        assert res.returncode == 0 and "built on the GPU" in res.stderr, res.stderr
+        extra = ["--index", str(tmp_path / "ref.dnx")]
+    cmd = [str(EXE), "map", *flags, *extra, "-o", str(tmp_path / "out.sam"), str(tmp_path / "ref.fa"),
+           str(tmp_path / "reads.fq")]
+    print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True, timeout=100)
+    assert res.returncode == 0, res.stderr
+    assert ("loaded in" if how == "file" else "built on the GPU in") in res.stderr
+    body = b"".join(l for l in (tmp_path / "out.sam").read_bytes().splitlines(keepends=True) if not l.startswith(b"@"))
+    assert hashlib.sha256(body).hexdigest() == golden_io.manifest()["sets"][name]["sam_body_sha256"]
