@@ -223,6 +223,29 @@ def serve_sweep(seconds: float, levels: list[float], slots: int) -> dict:
                                           "budget_bytes", "deferred_by_budget")}}
 
 
+def end_to_end(ctx: api.Context, reads: synth.Reads, threads: int, batch_bytes: int) -> dict:
+    """The workload as a FASTQ file (in the page cache) mapped to SAM text by dnaln_map_file: the
+    input stage (parse, encode, hash names), the device (upload from page-locked staging, run,
+    download records) and the output stage (SAM formatting, written to /dev/null) overlapped. The
+    second of two runs is reported. device_pipeline_reads_per_s is reads over the device thread's
+    busy time: encoded reads in page-locked memory to records on the host, copies included."""
+    with tempfile.TemporaryDirectory() as d:
+        fq = os.path.join(d, "reads.fq")
+        synth.write_fastq(reads, fq)
+        size = os.path.getsize(fq)
+        for _ in range(2):
+            st = ctx.map_file(fq, None, mode=abi.MODE_BATCHED, batch_bytes=batch_bytes, threads=threads)
+    n = st["reads"]
+    out = {"input": f"4-line FASTQ file in the page cache, {size} bytes", "output": "SAM text to /dev/null",
+           "threads": threads, "batch_bytes": batch_bytes, "reads": n, "batches": st["batches"],
+           "parallel_fastq": bool(st["parallel_fastq"]), "sam_bytes": st["sam_bytes"],
+           "reads_per_s": n / st["s_wall"], "device_pipeline_reads_per_s": n / st["s_device"],
+           "wall_s": st["s_wall"], "busy_s": {k: st["s_" + k] for k in ("input", "device", "format", "write")}}
+    print(f"[bench] end to end: {out['reads_per_s']:.0f} reads/s (device pipeline "
+          f"{out['device_pipeline_reads_per_s']:.0f} reads/s), busy s {out['busy_s']}", file=sys.stderr, flush=True)
+    return out
+
+
 def index_block(g: synth.Genome, k: int = 15, w: int = 10) -> tuple[api.Index, dict]:
     """MinimizerIndex (src/index.cpp:10-45) built on the host and on the GPU, checked equal
     (bucket count, key count, pos[], the set of buckets), and its file written and read back.
@@ -274,6 +297,9 @@ def main() -> int:
     ap.add_argument("--no-serve", action="store_true", help="skip the serving sweep (config 5)")
     ap.add_argument("--serve-seconds", type=float, default=1.0, help="seconds of arrivals per load level")
     ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--no-e2e", action="store_true", help="skip the FASTQ-to-SAM end-to-end run")
+    ap.add_argument("--e2e-threads", type=int, default=16, help="host threads of the end-to-end run")
+    ap.add_argument("--e2e-batch-bytes", type=int, default=256 << 20, help="input bytes per batch of that run")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="after the run, copy BYTES device-to-device once (a copy kernel of known traffic for "
                          "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -379,6 +405,8 @@ def main() -> int:
                           ("anchors", "candidates", "rank_tasks", "tb_tasks", "global_path_reads", "fused_retry_reads")},
         "parity": parity, "roofline": roofline, "index": ix_blk,
     }
+    if not a.no_e2e:
+        out["end_to_end"] = end_to_end(ctx, reads, a.e2e_threads, a.e2e_batch_bytes)
     if not a.no_cpu:
         cb = cpu_baseline(w, g, reads, a.cpu_reads_per_proc)
         out["cpu_baseline"] = cb

@@ -36,6 +36,12 @@ class Index(C.Structure):
     _fields_ = [("view", IndexView), ("n_keys", u64), ("max_count", u32)]
 
 
+class PipelineStats(C.Structure):
+    _fields_ = [(n, u64) for n in ("reads", "mapped", "bases", "sam_bytes", "batches")] + \
+               [(n, C.c_double) for n in ("s_wall", "s_input", "s_device", "s_format", "s_write")] + \
+               [("parallel_fastq", i32), ("reserved", i32)]
+
+
 class Task(C.Structure):
     _fields_ = [("q_off", u64), ("t_off", u64), ("qlen", i32), ("tlen", i32)]
 
@@ -94,6 +100,8 @@ HIP_SYMBOLS = {
     "dnaln_hip_index_download": (i32, [vp, vp, vp]),
     "dnaln_hip_index_free": (None, [vp]),
     "dnaln_hip_context_index": (i32, [vp, vp, vp, vp]),
+    "dnaln_hip_host_alloc": (i32, [u64, vp]),
+    "dnaln_hip_host_free": (None, [vp]),
 }
 
 class ServeConfig(C.Structure):
@@ -127,6 +135,9 @@ HOST_SYMBOLS = {
     "dnaln_index_save": (i32, [vp, vp, vp, i32, C.c_char_p]),
     "dnaln_index_load": (i32, [C.c_char_p, vp, vp, i32, vp]),
     "dnaln_reference_digest": (u64, [vp, vp, i32]),
+    "dnaln_map_file": (i32, [vp, vp, C.c_char_p, i32, vp, i32, u64, i32, vp]),
+    "dnaln_map_file_on": (i32, [vp, vp, C.c_char_p, i32, vp, i32, u64, i32, vp]),
+    "dnaln_host_last_error": (C.c_char_p, []),
     "dnaln_fnv1a": (u32, [C.c_char_p, C.c_size_t]),
     "dnaln_fnv1a_many": (None, [vp, vp, vp, u64, vp]),
     "dnaln_encode": (None, [C.c_char_p, C.c_size_t, vp]),

@@ -7,6 +7,7 @@ plays Mapper::map (src/mapper.hpp:54) + append_sam. Nothing here computes alignm
 from __future__ import annotations
 
 import ctypes as C
+import os
 from dataclasses import dataclass
 
 import numpy as np
@@ -195,6 +196,10 @@ class Context:
         return {"n_buckets": nb.value, "n_pos": npos.value, "build_ms": ms.value}
 
     def close(self) -> None:
+        for pair in getattr(self, "_file_batches", {}).values():
+            for b in pair:
+                b.close()
+        self._file_batches = {}
         if self.ctx:
             _lib.hip().dnaln_hip_destroy(self.ctx)
             self.ctx = C.c_void_p()
@@ -233,6 +238,30 @@ class Context:
         _check(_lib.hip().dnaln_hip_map_batch(self.ctx, _p(rb.codes), _p(rb.offs), _p(rb.lens), _p(rb.fnv), n,
                                               _p(out), _p(cig), cap, C.byref(stats)), "dnaln_hip_map_batch")
         return out, cig, stats
+
+    def map_file(self, reads_path: str, out_path: str | None = None, mode: int = abi.MODE_BATCHED,
+                 batch_bytes: int = 0, threads: int = 16, tune: bool = False) -> dict:
+        """FASTA/FASTQ file (plain or gzip) -> SAM records (no header) at out_path (None: discarded),
+        through the overlapped input / device / output pipeline (dnaln_map_file). Returns its stats."""
+        # two device batches per form, kept on the context: later calls run warm
+        if not hasattr(self, "_file_batches"):
+            self._file_batches = {}
+        key = (mode, bool(tune))
+        if key not in self._file_batches:
+            self._file_batches[key] = (self.batch(mode=mode, tune=tune), self.batch(mode=mode, tune=tune))
+        b0, b1 = self._file_batches[key]
+        names = (C.c_char_p * self.genome.n_contigs)(*[n.encode() for n in self.genome.names])
+        st = abi.PipelineStats()
+        fd = os.open(out_path or os.devnull, os.O_WRONLY | os.O_CREAT | os.O_TRUNC, 0o644)
+        try:
+            rc = _lib.host().dnaln_map_file_on(b0.b, b1.b, str(reads_path).encode(), fd, names,
+                                               self.genome.n_contigs, batch_bytes, threads, C.byref(st))
+        finally:
+            os.close(fd)
+        if rc != abi.OK:
+            msg = _lib.host().dnaln_host_last_error().decode(errors="replace")
+            raise DnalnError(rc, f"dnaln_map_file({reads_path}): {msg}")
+        return {f: getattr(st, f) for f, _ in abi.PipelineStats._fields_ if f != "reserved"}
 
     def batch(self, mode: int = abi.MODE_PLAIN, chunk_reads: int = 65536, max_reads: int = 0, max_bases: int = 0,
               tune: bool = False, len_split: tuple[int, int] = (0, 0), dir_budget: int = 0,

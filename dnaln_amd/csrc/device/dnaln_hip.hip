@@ -1852,6 +1852,21 @@ int dnaln_hip_device_memory(dnaln_hip_ctx* c, uint64_t* free_bytes, uint64_t* to
   return DNALN_OK;
 }
 
+int dnaln_hip_host_alloc(uint64_t bytes, void** out) {
+  if (!out) return fail(DNALN_E_INVALID, "null argument");
+  *out = nullptr;
+  if (hipHostMalloc(out, std::max<uint64_t>(bytes, 1), hipHostMallocDefault) != hipSuccess) {
+    (void)hipGetLastError();
+    *out = nullptr;
+    return fail(DNALN_E_NOMEM, "hipHostMalloc of " + std::to_string(bytes) + " bytes failed");
+  }
+  return DNALN_OK;
+}
+
+void dnaln_hip_host_free(void* p) {
+  if (p) (void)hipHostFree(p);
+}
+
 int dnaln_hip_context_index(dnaln_hip_ctx* c, uint64_t* n_buckets, uint64_t* n_pos, double* build_ms) {
   if (!c) return fail(DNALN_E_INVALID, "null argument");
   if (n_buckets) *n_buckets = c->ix.n_buckets;

@@ -3,6 +3,7 @@
 //
 // Same arguments, defaults, validation and SAM output; reads go to the GPU in batches of
 // --batch reads (larger than the reference's 8192 by default, since a batch fills the GPU).
+#include <algorithm>
 #include <chrono>
 #include <cstdarg>
 #include <cstdio>
@@ -74,7 +75,9 @@ struct Args {
 const char* kHelp =
     "Usage: dnaln-amd map [options] <ref.fa[.gz]> <reads.fq[.gz]>  > out.sam\n\n"
     "  -o FILE        write SAM to FILE instead of stdout\n"
-    "  --batch INT    reads per GPU batch [1048576]\n"
+    "  --batch INT    about INT reads per GPU batch: the input is cut into batches of INT x 256\n"
+    "                 bytes [1048576]\n"
+    "  --threads INT  host threads for parsing and SAM formatting [16]\n"
     "  -k INT         minimizer k-mer size [15]\n"
     "  -w INT         minimizer window size [10]\n"
     "  --bw INT       band half-width of banded Smith-Waterman [16]\n"
@@ -125,7 +128,8 @@ int cmd_index(int argc, char** argv) {
 
 int cmd_map(int argc, char** argv) {
   const std::set<std::string> values = {"-k", "-w", "--bw", "-A", "-B", "-O", "-E", "--max-occ", "--min-score",
-                                        "--max-cand", "--min-chain", "-o", "--batch", "--device", "--index"};
+                                        "--max-cand", "--min-chain", "-o", "--batch", "--device", "--index",
+                                        "--threads"};
   const Args a(argc, argv, values,
                {"--scalar", "--no-prefetch", "--plain", "--tune", "--gpu-index", "-h", "--help"});
   if (a.pos.size() != 2 || a.has("-h") || a.has("--help")) {
@@ -153,6 +157,8 @@ int cmd_map(int argc, char** argv) {
     die("scores must satisfy A >= 1, B >= 0, O >= 0, E >= 1");
   const long long batch = a.num("--batch", 1 << 20);
   if (batch < 1) die("--batch must be positive");
+  const long long threads = a.num("--threads", 16);
+  if (threads < 1 || threads > 64) die("--threads must be in [1, 64]");
   std::string cmdline = "dnaln map";
   for (int i = 0; i < argc; ++i) cmdline += std::string(" ") + argv[i];
 
@@ -187,27 +193,29 @@ int cmd_map(int argc, char** argv) {
   mapper.write_sam_header(out, cmdline);
   log_msg("alignment kernel: %s (%s form)", mapper.kernel_name().c_str(), p.batched ? "batched" : "plain");
 
-  SeqReader reader(a.pos[1]);
-  std::vector<SeqRecord> reads;
-  std::vector<MapRecord> recs;
-  std::string buf;
-  const double t_map = now();
-  size_t n;
-  while ((n = reader.next_batch(reads, (size_t)batch)) > 0) {
-    mapper.map(reads.data(), n, recs);
-    buf.clear();
-    for (size_t i = 0; i < n; ++i) mapper.append_sam(reads[i], recs[i], buf);
-    std::fwrite(buf.data(), 1, buf.size(), out);
+  // Input, device and output overlap on consecutive batches (dnaln_map_file): the records and
+  // their SAM text are those of Mapper::map + append_sam, batch after batch.
+  std::fflush(out);
+  std::vector<const char*> names;
+  for (const Contig& c : ref.contigs) names.push_back(c.name.c_str());
+  dnaln_batch_config cfg;
+  dnaln_hip_default_batch_config(&cfg);
+  cfg.mode = p.batched ? DNALN_MODE_BATCHED : DNALN_MODE_PLAIN;
+  cfg.tune = p.tune ? 1 : 0;
+  dnaln_pipeline_stats st{};
+  const int rc = dnaln_map_file(mapper.context(), &cfg, a.pos[1].c_str(), fileno(out), names.data(),
+                                (int32_t)names.size(), (uint64_t)batch * 256, (int32_t)threads, &st);
+  if (out != stdout) {
+    if (std::fclose(out) != 0) die("write error");
   }
-  if (out != stdout) std::fclose(out);
-  else std::fflush(out);
-  const double sec = now() - t_map;
-  const MapStats& st = mapper.stats();
+  if (rc != DNALN_OK) die(dnaln_host_last_error());
   log_msg("mapped %llu / %llu reads (%.2f%%) in %.3f s -> %.0f reads/s (excl. index build)",
           (unsigned long long)st.mapped, (unsigned long long)st.reads, st.reads ? 100.0 * st.mapped / st.reads : 0.0,
-          sec, st.reads / sec);
-  log_msg("device time: seed+chain %.3f s, alignment %.3f s (%.2f GCUPS); total wall %.2f s", st.t_seed, st.t_align,
-          st.t_align > 0 ? st.cells / st.t_align * 1e-9 : 0.0, now() - t_total);
+          st.s_wall, st.reads / std::max(st.s_wall, 1e-9));
+  log_msg("%llu batches, %s input; busy seconds: input %.3f, device %.3f, SAM formatting %.3f, write %.3f; total "
+          "wall %.2f s",
+          (unsigned long long)st.batches, st.parallel_fastq ? "4-line FASTQ (parallel)" : "sequential",
+          st.s_input, st.s_device, st.s_format, st.s_write, now() - t_total);
   return 0;
 }
 

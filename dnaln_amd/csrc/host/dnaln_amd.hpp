@@ -200,6 +200,9 @@ class Mapper {
   const MapStats& stats() const { return stats_; }
   std::string kernel_name() const { return "hip-gfx950"; }
 
+  // The device context (for dnaln_map_file and other C-ABI calls on the same resident index).
+  dnaln_hip_ctx* context() const { return ctx_; }
+
  private:
   const Reference& ref_;
   MapParams p_;

@@ -7,6 +7,7 @@
 
 #include "device/seed_chain.hpp"
 #include "dnaln_host.h"
+#include "host/sam_text.hpp"
 
 namespace {
 
@@ -17,21 +18,6 @@ uint8_t nt4(unsigned char c) {  // src/common.hpp:23-31
     case 'G': case 'g': return 2;
     case 'T': case 't': case 'U': case 'u': return 3;
     default: return 4;
-  }
-}
-
-char comp_char(char c) {  // src/common.cpp:6-14
-  switch (c) {
-    case 'A': return 'T';
-    case 'C': return 'G';
-    case 'G': return 'C';
-    case 'T': return 'A';
-    case 'a': return 't';
-    case 'c': return 'g';
-    case 'g': return 'c';
-    case 't': return 'a';
-    case 'n': return 'n';
-    default: return 'N';
   }
 }
 
@@ -268,55 +254,9 @@ void dnaln_encode(const char* s, size_t n, uint8_t* out) {
 long dnaln_sam_format(const char* name, size_t name_len, const char* seq, const char* qual, size_t len,
                       const dnaln_record* m, const uint32_t* cigar_pool, const char* contig_name, char* buf,
                       size_t cap) {
-  static const char kOps[] = "MIDNSHP=X";
-  const size_t need = name_len + 2 * len + 96 + (size_t)m->cigar_len * 12 + (contig_name ? std::strlen(contig_name) : 0);
-  if (need > cap) return -1;
-  char* o = buf;
-  std::memcpy(o, name, name_len);
-  o += name_len;
-  if (m->contig < 0) {
-    static const char kUnmapped[] = "\t4\t*\t0\t0\t*\t*\t0\t0\t";
-    std::memcpy(o, kUnmapped, sizeof kUnmapped - 1);
-    o += sizeof kUnmapped - 1;
-    if (len == 0) *o++ = '*';
-    else {
-      std::memcpy(o, seq, len);
-      o += len;
-    }
-    *o++ = '\t';
-    if (len == 0 || !qual) *o++ = '*';
-    else {
-      std::memcpy(o, qual, len);
-      o += len;
-    }
-    *o++ = '\n';
-    return (long)(o - buf);
-  }
-  o += std::sprintf(o, "\t%d\t%s\t%lld\t%d\t", m->rev ? 16 : 0, contig_name, (long long)(m->pos + 1), (int)m->mapq);
-  for (uint32_t c = 0; c < m->cigar_len; ++c) {
-    const uint32_t x = cigar_pool[m->cigar_off + c];
-    o += std::sprintf(o, "%u%c", x >> 4, kOps[x & 0xf]);
-  }
-  std::memcpy(o, "\t*\t0\t0\t", 7);
-  o += 7;
-  if (m->rev) {
-    for (size_t i = 0; i < len; ++i) *o++ = comp_char(seq[len - 1 - i]);
-    *o++ = '\t';
-    if (len == 0 || !qual) *o++ = '*';
-    else
-      for (size_t i = 0; i < len; ++i) *o++ = qual[len - 1 - i];
-  } else {
-    std::memcpy(o, seq, len);
-    o += len;
-    *o++ = '\t';
-    if (len == 0 || !qual) *o++ = '*';
-    else {
-      std::memcpy(o, qual, len);
-      o += len;
-    }
-  }
-  o += std::sprintf(o, "\tNM:i:%d\tAS:i:%d\n", m->nm, m->score);
-  return (long)(o - buf);
+  const size_t clen = contig_name ? std::strlen(contig_name) : 0;
+  if (dnaln_amd::sam_bound(name_len, len, m->cigar_len, clen) > cap) return -1;
+  return (long)(dnaln_amd::sam_record(buf, name, name_len, seq, qual, len, *m, cigar_pool, contig_name, clen) - buf);
 }
 
 }  // extern "C"

@@ -161,6 +161,10 @@ int dnaln_hip_index_info(const dnaln_hip_index* ix, uint64_t* n_buckets, uint64_
                          uint32_t* max_count, double* build_ms);
 int dnaln_hip_index_download(const dnaln_hip_index* ix, dnaln_bucket* table, uint32_t* pos);
 void dnaln_hip_index_free(dnaln_hip_index* ix);
+/* Page-locked host memory (hipHostMalloc) for staging reads and records, so that batch uploads
+ * and downloads run at full PCIe rate and overlap with the device (no reference counterpart). */
+int dnaln_hip_host_alloc(uint64_t bytes, void** out);
+void dnaln_hip_host_free(void* p);
 /* Sizes of the context's resident index, and the wall time of its GPU build in ms (0 when the
  * index came from the host). */
 int dnaln_hip_context_index(struct dnaln_hip_ctx* ctx, uint64_t* n_buckets, uint64_t* n_pos, double* build_ms);

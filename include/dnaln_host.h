@@ -5,6 +5,7 @@
  *                       Produces the view dnaln_hip_create uploads.
  *   dnaln_index_build_gpu, dnaln_index_save / dnaln_index_load
  *                       the same index built on the GPU, and written to / read from a file.
+ *   dnaln_map_file      reads file -> SAM records with input, device and output overlapped.
  *   dnaln_sam_format    Mapper::append_sam (src/mapper.cpp:231-270) for one record.
  *   dnaln_fnv1a         fnv1a (src/common.hpp:136-140), the equal-best tie-break hash.
  *   dnaln_encode        encode (src/common.hpp:36-38).
@@ -59,6 +60,35 @@ void dnaln_encode(const char* s, size_t n, uint8_t* out);
 long dnaln_sam_format(const char* name, size_t name_len, const char* seq, const char* qual, size_t len,
                       const dnaln_record* rec, const uint32_t* cigar_pool, const char* contig_name, char* buf,
                       size_t cap);
+
+/* Mapper::map + append_sam over a whole FASTA/FASTQ file, plain or gzip (the driver loop of
+ * src/main.cpp:47-61), as three stages that overlap on consecutive batches: input (parse,
+ * encode, name hashes into page-locked staging; plain 4-line FASTQ parsed by `threads` threads,
+ * anything else by the reference's sequential rules), device (two batches of configuration cfg
+ * in turn: one's copies overlap the other's kernels) and output (SAM text formatted by `threads`
+ * threads, written to out_fd in input order; no header). A batch holds about batch_bytes of
+ * input (0: 256 MiB). The records are those of dnaln_hip_map_batch + dnaln_sam_format. */
+typedef struct {
+  uint64_t reads, mapped, bases, sam_bytes, batches;
+  double s_wall;    /* the whole call */
+  double s_input;   /* busy time of each stage (the stages overlap) */
+  double s_device;
+  double s_format;
+  double s_write;
+  int32_t parallel_fastq; /* 1: every batch went through the parallel 4-line FASTQ parser */
+  int32_t reserved;
+} dnaln_pipeline_stats;
+
+int dnaln_map_file(struct dnaln_hip_ctx* ctx, const dnaln_batch_config* cfg, const char* reads_path, int out_fd,
+                   const char* const* contig_names, int32_t n_contigs, uint64_t batch_bytes, int32_t threads,
+                   dnaln_pipeline_stats* stats);
+/* The same on two batches the caller created on one context (dnaln_hip_batch_create) and keeps
+ * for later calls, which then allocate nothing on the device once warm. */
+int dnaln_map_file_on(struct dnaln_hip_batch* b0, struct dnaln_hip_batch* b1, const char* reads_path, int out_fd,
+                      const char* const* contig_names, int32_t n_contigs, uint64_t batch_bytes, int32_t threads,
+                      dnaln_pipeline_stats* stats);
+/* Message of the calling thread's last failed dnaln_map_file or dnaln_map_file_on. */
+const char* dnaln_host_last_error(void);
 
 #ifdef __cplusplus
 }
