@@ -299,7 +299,7 @@ def main() -> int:
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--no-e2e", action="store_true", help="skip the FASTQ-to-SAM end-to-end run")
     ap.add_argument("--e2e-threads", type=int, default=16, help="host threads of the end-to-end run")
-    ap.add_argument("--e2e-batch-bytes", type=int, default=256 << 20, help="input bytes per batch of that run")
+    ap.add_argument("--e2e-batch-bytes", type=int, default=64 << 20, help="input bytes per batch of that run")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="after the run, copy BYTES device-to-device once (a copy kernel of known traffic for "
                          "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")

@@ -76,7 +76,7 @@ const char* kHelp =
     "Usage: dnaln-amd map [options] <ref.fa[.gz]> <reads.fq[.gz]>  > out.sam\n\n"
     "  -o FILE        write SAM to FILE instead of stdout\n"
     "  --batch INT    about INT reads per GPU batch: the input is cut into batches of INT x 256\n"
-    "                 bytes [1048576]\n"
+    "                 bytes [262144]\n"
     "  --threads INT  host threads for parsing and SAM formatting [16]\n"
     "  -k INT         minimizer k-mer size [15]\n"
     "  -w INT         minimizer window size [10]\n"
@@ -155,7 +155,7 @@ int cmd_map(int argc, char** argv) {
   if (p.sc.bw < 1 || p.sc.bw > 1000) die("--bw must be in [1, 1000]");
   if (p.sc.match < 1 || p.sc.mismatch < 0 || p.sc.gap_open < 0 || p.sc.gap_ext < 1)
     die("scores must satisfy A >= 1, B >= 0, O >= 0, E >= 1");
-  const long long batch = a.num("--batch", 1 << 20);
+  const long long batch = a.num("--batch", 1 << 18);
   if (batch < 1) die("--batch must be positive");
   const long long threads = a.num("--threads", 16);
   if (threads < 1 || threads > 64) die("--threads must be in [1, 64]");

@@ -350,7 +350,7 @@ int dnaln_map_file_on(dnaln_hip_batch* b0, dnaln_hip_batch* b1, const char* read
   p.contig_names = contig_names;
   p.n_contigs = n_contigs;
   for (int i = 0; i < n_contigs; ++i) p.contig_len.push_back(std::strlen(contig_names[i]));
-  p.batch_bytes = std::max<uint64_t>(batch_bytes ? batch_bytes : (256ull << 20), 1 << 16);
+  p.batch_bytes = std::max<uint64_t>(batch_bytes ? batch_bytes : (64ull << 20), 1 << 16);
   p.threads = std::max(1, std::min(threads > 0 ? threads : 8, 64));
   p.out_fd = out_fd;
   try {

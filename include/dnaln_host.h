@@ -67,7 +67,7 @@ long dnaln_sam_format(const char* name, size_t name_len, const char* seq, const 
  * anything else by the reference's sequential rules), device (two batches of configuration cfg
  * in turn: one's copies overlap the other's kernels) and output (SAM text formatted by `threads`
  * threads, written to out_fd in input order; no header). A batch holds about batch_bytes of
- * input (0: 256 MiB). The records are those of dnaln_hip_map_batch + dnaln_sam_format. */
+ * input (0: 64 MiB). The records are those of dnaln_hip_map_batch + dnaln_sam_format. */
 typedef struct {
   uint64_t reads, mapped, bases, sam_bytes, batches;
   double s_wall;    /* the whole call */
