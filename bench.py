@@ -223,6 +223,23 @@ def serve_sweep(seconds: float, levels: list[float], slots: int) -> dict:
                                           "budget_bytes", "deferred_by_budget")}}
 
 
+def accuracy(ctx: api.Context, w: synth.Workload, g: synth.Genome, n: int, recs: np.ndarray, pool: np.ndarray,
+             sample: int) -> dict:
+    """The batched step's records scored against the generator's truth (src/evaluate.cpp:40-84),
+    and the true-locus kernel checks (src/bench.cpp:272-314) on a sample of the reads."""
+    t = time.perf_counter()
+    tr = synth.workload_reads(w, n, truth=True)  # the same reads, with their true alignments
+    e = api.evaluate(recs, pool, tr, g)
+    tc = api.truth_check(ctx, tr.subset(0, min(sample, len(tr))), g)
+    out = {"accuracy": e["accuracy"], "base_concordance": e["base_concordance"],
+           "mapped": e["mapped"] / max(e["n_truth"], 1),
+           "mapq60_error_rate": e["wrong_mapq60"] / max(e["mapped_mapq60"], 1),
+           "truth_check": tc}
+    print(f"[bench] accuracy {e['accuracy']:.4f}, base concordance {e['base_concordance']:.4f}, true-locus "
+          f"checks {tc} ({time.perf_counter() - t:.1f} s)", file=sys.stderr, flush=True)
+    return out
+
+
 def end_to_end(ctx: api.Context, reads: synth.Reads, threads: int, batch_bytes: int) -> dict:
     """The workload as a FASTQ file (in the page cache) mapped to SAM text by dnaln_map_file: the
     input stage (parse, encode, hash names), the device (upload from page-locked staging, run,
@@ -298,6 +315,7 @@ def main() -> int:
     ap.add_argument("--serve-seconds", type=float, default=1.0, help="seconds of arrivals per load level")
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--no-e2e", action="store_true", help="skip the FASTQ-to-SAM end-to-end run")
+    ap.add_argument("--no-accuracy", action="store_true", help="skip scoring the records against the truth")
     ap.add_argument("--e2e-threads", type=int, default=16, help="host threads of the end-to-end run")
     ap.add_argument("--e2e-batch-bytes", type=int, default=64 << 20, help="input bytes per batch of that run")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
@@ -405,6 +423,8 @@ def main() -> int:
                           ("anchors", "candidates", "rank_tasks", "tb_tasks", "global_path_reads", "fused_retry_reads")},
         "parity": parity, "roofline": roofline, "index": ix_blk,
     }
+    if not a.no_accuracy:
+        out["accuracy"] = accuracy(ctx, w, g, len(reads), recs, pool, 20000)
     if not a.no_e2e:
         out["end_to_end"] = end_to_end(ctx, reads, a.e2e_threads, a.e2e_batch_bytes)
     if not a.no_cpu:

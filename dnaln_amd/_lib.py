@@ -42,6 +42,8 @@ def host() -> C.CDLL:
         lib.synth_reads_capacity.restype = u64
         lib.synth_reads.argtypes = [vp, vp, vp, i32, vp, vp, vp, vp, vp, vp]
         lib.synth_reads.restype = C.c_int64
+        lib.synth_reads_truth.argtypes = [vp, vp, vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, u64]
+        lib.synth_reads_truth.restype = C.c_int64
         lib.synth_write_fasta.argtypes = [C.c_char_p, vp, vp, vp, i32]
         lib.synth_write_fasta.restype = i32
         lib.synth_write_fastq.argtypes = [C.c_char_p, vp, vp, vp, u64, u64]

@@ -42,6 +42,15 @@ class PipelineStats(C.Structure):
                [("parallel_fastq", i32), ("reserved", i32)]
 
 
+class EvalStats(C.Structure):
+    _fields_ = [(n, u64) for n in ("n_truth", "records", "mapped", "correct", "bases", "bases_ok")] + \
+               [("mq_mapped", u64 * 61), ("mq_wrong", u64 * 61)]
+
+
+class TruthStats(C.Structure):
+    _fields_ = [(n, u64) for n in ("n", "rescored_ok", "in_band", "optimal_ok", "majority_ok", "bases", "bases_ok")]
+
+
 class Task(C.Structure):
     _fields_ = [("q_off", u64), ("t_off", u64), ("qlen", i32), ("tlen", i32)]
 
@@ -138,6 +147,8 @@ HOST_SYMBOLS = {
     "dnaln_map_file": (i32, [vp, vp, C.c_char_p, i32, vp, i32, u64, i32, vp]),
     "dnaln_map_file_on": (i32, [vp, vp, C.c_char_p, i32, vp, i32, u64, i32, vp]),
     "dnaln_host_last_error": (C.c_char_p, []),
+    "dnaln_evaluate": (i32, [vp, vp, u64, vp, vp, vp, vp, vp, i32, vp]),
+    "dnaln_truth_check": (i32, [vp, vp, vp, u64, vp, vp, vp, vp, vp]),
     "dnaln_fnv1a": (u32, [C.c_char_p, C.c_size_t]),
     "dnaln_fnv1a_many": (None, [vp, vp, vp, u64, vp]),
     "dnaln_encode": (None, [C.c_char_p, C.c_size_t, vp]),

@@ -450,3 +450,81 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+# ------------------------------------------------------------------ accuracy against simulated truth
+def _truth_arrays(genome: Genome, reads: Reads):
+    """(contig, start within contig, strand) of each read's true locus; start -1 for random reads."""
+    pos = np.asarray(reads.truth_pos, np.int64)
+    c = np.searchsorted(genome.contig_off.astype(np.int64), np.maximum(pos, 0), side="right").astype(np.int32) - 1
+    start = np.where(pos >= 0, pos - genome.contig_off.astype(np.int64)[np.maximum(c, 0)], -1).astype(np.int64)
+    return np.ascontiguousarray(c, np.int32), np.ascontiguousarray(start), np.ascontiguousarray(reads.truth_rev,
+                                                                                                np.uint8)
+
+
+def evaluate(recs: np.ndarray, cigars: np.ndarray, reads: Reads, genome: Genome, tol: int = 10) -> dict:
+    """Evaluator (src/evaluate.hpp:27-52) over device records of `reads` (made with truth=True
+    for base concordance): accuracy = correct / reads in the truth set."""
+    c, start, rev = _truth_arrays(genome, reads)
+    recs = np.ascontiguousarray(recs, abi.RECORD_DTYPE)
+    cig = np.ascontiguousarray(cigars, np.uint32) if len(cigars) else np.zeros(1, np.uint32)
+    tc = reads.truth_cigar
+    st = abi.EvalStats()
+    _check(_lib.host().dnaln_evaluate(_p(recs), _p(cig), len(recs), _p(c), _p(start), _p(rev),
+                                      _p(tc) if tc is not None and len(tc) else None,
+                                      _p(reads.truth_cig_off) if tc is not None else None, tol, C.byref(st)),
+           "dnaln_evaluate")
+    mqm, mqw = np.array(st.mq_mapped[:], np.int64), np.array(st.mq_wrong[:], np.int64)
+    out = {k: int(getattr(st, k)) for k in ("n_truth", "records", "mapped", "correct", "bases", "bases_ok")}
+    out["accuracy"] = out["correct"] / out["n_truth"] if out["n_truth"] else 0.0
+    out["base_concordance"] = out["bases_ok"] / out["bases"] if out["bases"] else 0.0
+    for q in (60, 30, 10, 1):
+        out[f"mapped_mapq{q}"] = int(mqm[q:].sum())
+        out[f"wrong_mapq{q}"] = int(mqw[q:].sum())
+    return out
+
+
+def truth_tasks(genome: Genome, reads: Reads, bw: int, limit: int | None = None):
+    """Alignment tasks at each read's true locus (src/groundtruth.cpp:11-44): the read in
+    reference orientation against [true start - bw, true start + len + bw) of its contig, N
+    outside the contig. Returns (pool, tasks, read indices)."""
+    c, start, rev = _truth_arrays(genome, reads)
+    idx = np.flatnonzero(start >= 0)
+    if limit is not None:
+        idx = idx[:limit]
+    codes = encode(reads.seq)
+    comp = np.array([3, 2, 1, 0, 4], np.uint8)
+    parts, tasks = [], np.zeros(len(idx), abi.TASK_DTYPE)
+    off = 0
+    for k, i in enumerate(idx):
+        L = int(reads.lens[i])
+        q = codes[int(reads.offs[i]):int(reads.offs[i]) + L]
+        if rev[i]:
+            q = comp[q[::-1]]
+        co, cl = int(genome.contig_off[c[i]]), int(genome.contig_len[c[i]])
+        lo, wl = int(start[i]) - bw, L + 2 * bw
+        t = np.full(wl, 4, np.uint8)
+        a, b = max(lo, 0), min(lo + wl, cl)
+        if b > a:
+            t[a - lo:b - lo] = genome.codes[co + a:co + b]
+        tasks[k] = (off, off + L, L, wl)
+        parts += [q, t]
+        off += L + wl
+    pool = np.concatenate(parts) if parts else np.zeros(1, np.uint8)
+    return pool, tasks, idx
+
+
+def truth_check(ctx: "Context", reads: Reads, genome: Genome, limit: int | None = None) -> dict:
+    """The true-locus kernel checks of src/bench.cpp:272-314 on the device aligner's traceback
+    results: re-scoring, optimality against the true path, base placement."""
+    sp = ctx.params.sc
+    pool, tasks, idx = truth_tasks(genome, reads, sp.bw, limit)
+    res, cig = ctx.align(pool, tasks, traceback=True)
+    tc = [reads.truth_cigar[int(reads.truth_cig_off[i]):int(reads.truth_cig_off[i + 1])] for i in idx]
+    toff = np.zeros(len(idx) + 1, np.uint64)
+    toff[1:] = np.cumsum([len(x) for x in tc])
+    tcat = np.concatenate(tc).astype(np.uint32) if len(tc) and toff[-1] else np.zeros(1, np.uint32)
+    st = abi.TruthStats()
+    _check(_lib.host().dnaln_truth_check(C.byref(sp), _p(pool), _p(tasks), len(tasks), _p(res), _p(cig), _p(tcat),
+                                         _p(toff), C.byref(st)), "dnaln_truth_check")
+    return {f: int(getattr(st, f)) for f, _ in abi.TruthStats._fields_}

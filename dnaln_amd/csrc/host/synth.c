@@ -109,10 +109,20 @@ static const char kAcgt[4] = {'A', 'C', 'G', 'T'};
 /* Walks the genome from `start`, emitting `L` read bases with substitutions and indels. No indel
  * is placed before the first base, and an insertion never reaches the last base, so the true
  * alignment starts and ends with a match. Returns 0, or -1 when the walk runs off the contig. */
+/* Appends op (BAM code: 0 M, 1 I, 2 D) of length len to a CIGAR, merging with the last op. */
+static void cigar_push(uint32_t* cig, int* ncig, uint32_t op, uint32_t len) {
+  if (!cig) return;
+  if (*ncig > 0 && (cig[*ncig - 1] & 0xf) == op) cig[*ncig - 1] += len << 4;
+  else cig[(*ncig)++] = len << 4 | op;
+}
+
+/* Also records the true alignment (genome orientation) in cig when it is not NULL; the random
+ * draws are the same either way. */
 static int walk_read(synth_rng* r, const uint8_t* codes, uint64_t start, uint64_t limit, int L,
-                     const synth_read_params* p, char* out) {
+                     const synth_read_params* p, char* out, uint32_t* cig, int* ncig) {
   uint64_t rp = start;
   int n = 0, last_match = 1;
+  if (ncig) *ncig = 0;
   while (n < L) {
     if (rp >= limit) return -1;
     if (n > 0 && last_match && p->indel > 0 && synth_rng_uniform(r) < p->indel) {
@@ -121,8 +131,10 @@ static int walk_read(synth_rng* r, const uint8_t* codes, uint64_t start, uint64_
       if (synth_rng_uniform(r) < 0.5) {
         if (n + g >= L) continue;
         for (int i = 0; i < g; ++i) out[n++] = kAcgt[synth_rng_below(r, 4)];
+        cigar_push(cig, ncig, 1, (uint32_t)g);
       } else {
         rp += (uint64_t)g;
+        cigar_push(cig, ncig, 2, (uint32_t)g);
       }
       last_match = 0;
       continue;
@@ -130,6 +142,7 @@ static int walk_read(synth_rng* r, const uint8_t* codes, uint64_t start, uint64_
     uint8_t b = codes[rp];
     if (synth_rng_uniform(r) < p->sub) b = (uint8_t)((b + 1 + synth_rng_below(r, 3)) & 3);
     out[n++] = kAcgt[b & 3];
+    cigar_push(cig, ncig, 0, 1);
     last_match = 1;
     ++rp;
   }
@@ -149,7 +162,22 @@ static char comp_char(char c) {
 int64_t synth_reads(const uint8_t* codes, const uint64_t* contig_off, const uint32_t* contig_len, int n_contigs,
                     const synth_read_params* p, char* seq, uint64_t* offs, int32_t* lens, int64_t* truth_pos,
                     uint8_t* truth_rev) {
+  return synth_reads_truth(codes, contig_off, contig_len, n_contigs, p, seq, offs, lens, truth_pos, truth_rev, NULL,
+                           NULL, 0);
+}
+
+int64_t synth_reads_truth(const uint8_t* codes, const uint64_t* contig_off, const uint32_t* contig_len, int n_contigs,
+                          const synth_read_params* p, char* seq, uint64_t* offs, int32_t* lens, int64_t* truth_pos,
+                          uint8_t* truth_rev, uint32_t* truth_cigar, uint64_t* truth_cigar_off, uint64_t cigar_cap) {
   if (!codes || n_contigs < 1 || !p || !seq || !offs || !lens) return -1;
+  int32_t maxlen = 1;
+  for (int j = 0; j < 2; ++j) maxlen = (j ? p->len_b : p->len_a) > maxlen ? (j ? p->len_b : p->len_a) : maxlen;
+  /* scratch for one read's ops (at most 2L: a deletion may follow every base), copied out */
+  uint32_t* one = truth_cigar ? (uint32_t*)malloc((2 * (size_t)maxlen + 2) * sizeof(uint32_t)) : NULL;
+  if (truth_cigar && !one) return -1;
+  if ((truth_cigar != NULL) != (truth_cigar_off != NULL)) return -1;
+  uint64_t co = 0;
+  if (truth_cigar_off) truth_cigar_off[0] = 0;
   uint64_t total = 0;
   for (int c = 0; c < n_contigs; ++c) total += contig_len[c];
   synth_rng r;
@@ -163,7 +191,10 @@ int64_t synth_reads(const uint8_t* codes, const uint64_t* contig_off, const uint
       for (int j = 0; j < L; ++j) out[j] = kAcgt[synth_rng_below(&r, 4)];
     } else if (L > 0) {
       for (int attempt = 0;; ++attempt) {
-        if (attempt > 1000) return -1;
+        if (attempt > 1000) {
+          free(one);
+          return -1;
+        }
         int c;
         uint64_t start;
         if (p->end_prob > 0 && synth_rng_uniform(&r) < p->end_prob) {
@@ -181,8 +212,17 @@ int64_t synth_reads(const uint8_t* codes, const uint64_t* contig_off, const uint
           while (c + 1 < n_contigs && contig_off[c + 1] <= g) ++c;
           start = g;
         }
-        if (walk_read(&r, codes, start, contig_off[c] + contig_len[c], L, p, out) == 0) {
+        int ncig = 0;
+        if (walk_read(&r, codes, start, contig_off[c] + contig_len[c], L, p, out, one, &ncig) == 0) {
           pos = (int64_t)start;
+          if (truth_cigar) {
+            if (co + (uint64_t)ncig > cigar_cap) {
+              free(one);
+              return -2;
+            }
+            memcpy(truth_cigar + co, one, (size_t)ncig * sizeof(uint32_t));
+            co += (uint64_t)ncig;
+          }
           break;
         }
       }
@@ -204,8 +244,10 @@ int64_t synth_reads(const uint8_t* codes, const uint64_t* contig_off, const uint
     lens[i] = L;
     if (truth_pos) truth_pos[i] = pos;
     if (truth_rev) truth_rev[i] = rev;
+    if (truth_cigar_off) truth_cigar_off[i + 1] = co;
     o += (uint64_t)L;
   }
+  free(one);
   return (int64_t)o;
 }
 

@@ -61,6 +61,13 @@ uint64_t synth_reads_capacity(const synth_read_params* p);
 int64_t synth_reads(const uint8_t* codes, const uint64_t* contig_off, const uint32_t* contig_len, int n_contigs,
                     const synth_read_params* p, char* seq, uint64_t* offs, int32_t* lens, int64_t* truth_pos,
                     uint8_t* truth_rev);
+/* synth_reads plus each read's true alignment: truth_cigar_off[i] .. [i + 1] delimit read i's
+ * CIGAR (BAM encoding, M/I/D, genome orientation, starting at truth_pos[i]) in truth_cigar of
+ * cigar_cap words; reads with truth_pos -1 have none. Returns -2 when cigar_cap is short. The
+ * reads are the same as synth_reads gives. */
+int64_t synth_reads_truth(const uint8_t* codes, const uint64_t* contig_off, const uint32_t* contig_len, int n_contigs,
+                          const synth_read_params* p, char* seq, uint64_t* offs, int32_t* lens, int64_t* truth_pos,
+                          uint8_t* truth_rev, uint32_t* truth_cigar, uint64_t* truth_cigar_off, uint64_t cigar_cap);
 
 /* Writes a FASTA (80 columns) for the genome, contigs named chr1..chrN. Returns 0 or -1. */
 int synth_write_fasta(const char* path, const uint8_t* codes, const uint64_t* contig_off, const uint32_t* contig_len,

@@ -43,6 +43,8 @@ class Reads:
     truth_pos: np.ndarray      # int64 global start (-1 for random reads)
     truth_rev: np.ndarray      # uint8
     first_index: int = 0       # read i is named r<first_index + i>
+    truth_cigar: np.ndarray | None = None    # uint32 BAM CIGAR words, genome orientation (make_reads(truth=True))
+    truth_cig_off: np.ndarray | None = None  # uint64, n + 1: read i's words are [off[i], off[i + 1])
 
     def __len__(self) -> int:
         return len(self.lens)
@@ -61,8 +63,12 @@ class Reads:
     def subset(self, lo: int, hi: int) -> "Reads":
         base = int(self.offs[lo]) if hi > lo else 0
         end = int(self.offs[hi - 1] + self.lens[hi - 1]) if hi > lo else 0
+        tc = to = None
+        if self.truth_cigar is not None:
+            c0, c1 = int(self.truth_cig_off[lo]), int(self.truth_cig_off[hi])
+            tc, to = self.truth_cigar[c0:c1].copy(), (self.truth_cig_off[lo:hi + 1] - c0).astype(np.uint64)
         return Reads(self.seq[base:end].copy(), (self.offs[lo:hi] - base).astype(np.uint64), self.lens[lo:hi].copy(),
-                     self.truth_pos[lo:hi].copy(), self.truth_rev[lo:hi].copy(), self.first_index + lo)
+                     self.truth_pos[lo:hi].copy(), self.truth_rev[lo:hi].copy(), self.first_index + lo, tc, to)
 
     def read(self, i: int) -> bytes:
         o = int(self.offs[i])
@@ -84,7 +90,8 @@ def make_genome(kind: int, total_len: int, n_contigs: int, seed: int, repeat_fra
 
 def make_reads(g: Genome, n: int, len_mode: int, len_a: int, len_b: int = 0, sub: float = 0.01,
                indel: float = 0.001, indel_max: int = 3, rev_prob: float = 0.5, seed: int = 11,
-               n_rate: float = 0.0, end_prob: float = 0.0, random_prob: float = 0.0) -> Reads:
+               n_rate: float = 0.0, end_prob: float = 0.0, random_prob: float = 0.0, truth: bool = False) -> Reads:
+    """Reads drawn from g; truth=True also records each read's true CIGAR (the reads are the same)."""
     lib = _lib.host()
     p = _ReadParams(n, len_mode, len_a, len_b or len_a, sub, indel, indel_max, rev_prob, n_rate, end_prob,
                     random_prob, seed)
@@ -94,12 +101,26 @@ def make_reads(g: Genome, n: int, len_mode: int, len_a: int, len_b: int = 0, sub
     lens = np.empty(n, dtype=np.int32)
     tpos = np.empty(n, dtype=np.int64)
     trev = np.empty(n, dtype=np.uint8)
-    total = lib.synth_reads(g.codes.ctypes.data, g.contig_off.ctypes.data, g.contig_len.ctypes.data, g.n_contigs,
-                            C.byref(p), seq.ctypes.data, offs.ctypes.data, lens.ctypes.data, tpos.ctypes.data,
-                            trev.ctypes.data)
+    if not truth:
+        total = lib.synth_reads(g.codes.ctypes.data, g.contig_off.ctypes.data, g.contig_len.ctypes.data, g.n_contigs,
+                                C.byref(p), seq.ctypes.data, offs.ctypes.data, lens.ctypes.data, tpos.ctypes.data,
+                                trev.ctypes.data)
+        if total < 0:
+            raise RuntimeError("synth_reads failed")
+        return Reads(seq[:total].copy(), offs, lens, tpos, trev)
+    ccap = 8 * n + 1024
+    while True:
+        cig = np.empty(ccap, dtype=np.uint32)
+        coff = np.empty(n + 1, dtype=np.uint64)
+        total = lib.synth_reads_truth(g.codes.ctypes.data, g.contig_off.ctypes.data, g.contig_len.ctypes.data,
+                                      g.n_contigs, C.byref(p), seq.ctypes.data, offs.ctypes.data, lens.ctypes.data,
+                                      tpos.ctypes.data, trev.ctypes.data, cig.ctypes.data, coff.ctypes.data, ccap)
+        if total != -2:
+            break
+        ccap = 2 * cap + 2 * n + 2
     if total < 0:
-        raise RuntimeError("synth_reads failed")
-    return Reads(seq[:total].copy(), offs, lens, tpos, trev)
+        raise RuntimeError("synth_reads_truth failed")
+    return Reads(seq[:total].copy(), offs, lens, tpos, trev, 0, cig[:int(coff[n])].copy(), coff)
 
 
 def write_fasta(g: Genome, path: str) -> None:
@@ -189,7 +210,7 @@ def workload_genome(w: Workload) -> Genome:
     return _GENOME_CACHE[w.genome]
 
 
-def workload_reads(w: Workload, n: int | None = None) -> Reads:
+def workload_reads(w: Workload, n: int | None = None, truth: bool = False) -> Reads:
     g = workload_genome(w)
     return make_reads(g, w.n_reads if n is None else n, w.len_mode, w.len_a, w.len_b, w.sub, w.indel,
-                      w.indel_max, 0.5, w.read_seed)
+                      w.indel_max, 0.5, w.read_seed, truth=truth)

@@ -6,6 +6,8 @@
  *   dnaln_index_build_gpu, dnaln_index_save / dnaln_index_load
  *                       the same index built on the GPU, and written to / read from a file.
  *   dnaln_map_file      reads file -> SAM records with input, device and output overlapped.
+ *   dnaln_evaluate, dnaln_truth_check
+ *                       accuracy of records and alignments against simulated truth.
  *   dnaln_sam_format    Mapper::append_sam (src/mapper.cpp:231-270) for one record.
  *   dnaln_fnv1a         fnv1a (src/common.hpp:136-140), the equal-best tie-break hash.
  *   dnaln_encode        encode (src/common.hpp:36-38).
@@ -87,6 +89,31 @@ int dnaln_map_file(struct dnaln_hip_ctx* ctx, const dnaln_batch_config* cfg, con
 int dnaln_map_file_on(struct dnaln_hip_batch* b0, struct dnaln_hip_batch* b1, const char* reads_path, int out_fd,
                       const char* const* contig_names, int32_t n_contigs, uint64_t batch_bytes, int32_t threads,
                       dnaln_pipeline_stats* stats);
+/* Accuracy against simulated truth (no part of the mapping path; validation of its results).
+ * dnaln_evaluate is Evaluator::add (src/evaluate.cpp:40-84) over n records, record i against
+ * truth i: truth_start[i] is the true 0-based start within contig truth_contig[i] (-1: not in
+ * the truth set) and truth_cigar[truth_cigar_off[i] .. [i + 1]) its true CIGAR in reference
+ * orientation (truth_cigar NULL: no base concordance). A mapped record is correct on the true
+ * contig and strand with its unclipped start within tol bp of the true start. */
+typedef struct {
+  uint64_t n_truth, records, mapped, correct, bases, bases_ok;
+  uint64_t mq_mapped[61], mq_wrong[61]; /* by MAPQ */
+} dnaln_eval_stats;
+int dnaln_evaluate(const dnaln_record* recs, const uint32_t* cigar_pool, uint64_t n, const int32_t* truth_contig,
+                   const int64_t* truth_start, const uint8_t* truth_rev, const uint32_t* truth_cigar,
+                   const uint64_t* truth_cigar_off, int32_t tol, dnaln_eval_stats* out);
+/* The true-locus checks of src/bench.cpp:272-314 on aligner results (dnaln_hip_align with
+ * traceback): task i is read i in reference orientation against the window from its true start
+ * minus bw (src/groundtruth.cpp:11-44), truth_cigar its true CIGAR. Counts results whose CIGAR
+ * re-scores to their score, tasks whose true path stays in the band and, of those, results
+ * scoring at least the true path, and query bases placed on their true target column. */
+typedef struct {
+  uint64_t n, rescored_ok, in_band, optimal_ok, majority_ok, bases, bases_ok;
+} dnaln_truth_stats;
+int dnaln_truth_check(const dnaln_score_params* sp, const uint8_t* pool, const dnaln_task* tasks, uint64_t n,
+                      const dnaln_result* res, const uint32_t* cigar_pool, const uint32_t* truth_cigar,
+                      const uint64_t* truth_cigar_off, dnaln_truth_stats* out);
+
 /* Message of the calling thread's last failed dnaln_map_file or dnaln_map_file_on. */
 const char* dnaln_host_last_error(void);
 
