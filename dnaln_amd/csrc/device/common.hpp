@@ -122,6 +122,16 @@ DN_HD uint32_t pk_signmask(pk16 a) { return perm_b32(0, pk_bits(a), 0x09090808u)
 // Bitfield insert: m ? a : b, bitwise.
 DN_HD uint32_t bfi(uint32_t m, uint32_t a, uint32_t b) { return (a & m) | (b & ~m); }
 
+// (a & b) | c as one v_bitop3_b32 on gfx950 (truth table 0xEA over a = 0xF0, b = 0xCC, c = 0xAA),
+// where the compiler would otherwise regroup a chain of such updates into ands and or3s.
+DN_HD uint32_t and_or(uint32_t a, uint32_t b, uint32_t c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_bitop3_b32(a, b, c, 0xEA);
+#else
+  return (a & b) | c;
+#endif
+}
+
 // Byte masks over a word's four bytes: bytes k..3, and bytes 0..k-1 (k clamped to 0..4).
 DN_HD uint32_t bytes_from(int k) { return k <= 0 ? ~0u : k >= 4 ? 0u : ~0u << (8 * k); }
 DN_HD uint32_t bytes_below(int k) { return ~bytes_from(k); }

@@ -324,8 +324,13 @@ __global__ void k_cigar_gather(ReadView rv, const int32_t* chosen, const uint64_
   gather_cigar(rv.lens[r], tb_res[t], slots + t * (uint64_t)cap, pool + cig_off[r]);
 }
 
+// Waves per SIMD the fill must keep: two for the traceback fills that fit in 256 registers
+// (through W = 49), so that scheduling changes cannot silently halve their occupancy.
+constexpr int sw_min_waves(int W, bool TB) { return TB && W <= 49 ? 2 : 1; }
+
 template <int W, bool TB>
-__global__ __launch_bounds__(kSwBlock) void k_sw_packed(SwArgs a) {
+__global__ __launch_bounds__(kSwBlock) __attribute__((amdgpu_waves_per_eu(sw_min_waves(W, TB)))) void k_sw_packed(
+    SwArgs a) {
   const int lane = threadIdx.x & 63;
   const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint64_t base = (uint64_t)wave * 128;

@@ -473,6 +473,13 @@ DN_HD void sw_pair(const SwArgs& a, uint32_t ida, uint32_t idb, bool va, bool vb
     }
   }
   uint64_t* slot_lane = slot + lane;
+  // Flag bit of band column d within its group of eight: 0x01010101 << (d & 7). Shifted by a
+  // value that is 0 at run time but unknown to the compiler, the eight masks live in scalar
+  // registers, so each flag update is one three-operand and_or (which takes no literal).
+  const uint32_t zs = (uint32_t)a.sp.bw >> 16;  // 0 (bw <= 1000)
+  uint32_t bits[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) bits[k] = 0x01010101u << (k + zs);
 
   for (int i0 = 0; i0 < qmax; i0 += U) {
     // Target positions i0 + W .. i0 + W + U - 2, which rows i0 + 1 .. of this step reach.
@@ -500,9 +507,9 @@ DN_HD void sw_pair(const SwArgs& a, uint32_t ida, uint32_t idb, bool va, bool vb
         if (TB) {
           const uint32_t p12 = perm_b32(pk_bits(pk_sub(eo, ee)), pk_bits(pk_sub(fo, fe)), 0x0B0A0908u);
           const uint32_t p34 = perm_b32(pk_bits(pk_sub(hd, h)), pk_bits(pk_sub(e, h)), 0x0B0A0908u);
-          const uint32_t bit = 0x01010101u << (d & 7);
-          acc12 |= p12 & bit;
-          acc34 |= p34 & bit;
+          const uint32_t bit = bits[d & 7];
+          acc12 = and_or(p12, bit, acc12);
+          acc34 = and_or(p34, bit, acc34);
           if ((d & 7) == 7 || d == W - 1) {
             store_stream(&slot_lane[(uint64_t)(i * G + (d >> 3)) * 64], (uint64_t)acc12 | (uint64_t)acc34 << 32);
             acc12 = acc34 = 0;
