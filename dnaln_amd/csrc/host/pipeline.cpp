@@ -75,6 +75,36 @@ struct Slot {
   uint64_t mapped = 0;
 };
 
+// Slots outlive the call that used them: their page-locked buffers (hipHostMalloc pins pages,
+// which takes milliseconds per batch-sized buffer) are reused by later calls, so a warm call
+// allocates nothing. Concurrent calls take different slots. The memory stays with the process.
+class SlotPool {
+ public:
+  std::vector<std::unique_ptr<Slot>> take(size_t n) {
+    std::lock_guard<std::mutex> g(m_);
+    std::vector<std::unique_ptr<Slot>> out;
+    while (out.size() < n && !free_.empty()) {
+      out.push_back(std::move(free_.back()));
+      free_.pop_back();
+    }
+    while (out.size() < n) out.emplace_back(new Slot);
+    return out;
+  }
+  void give(std::vector<std::unique_ptr<Slot>>& slots) {
+    std::lock_guard<std::mutex> g(m_);
+    for (auto& s : slots) free_.push_back(std::move(s));
+    slots.clear();
+  }
+
+ private:
+  std::mutex m_;
+  std::vector<std::unique_ptr<Slot>> free_;
+};
+SlotPool& slot_pool() {
+  static SlotPool* p = new SlotPool;  // never destroyed: pinned memory is freed with the process
+  return *p;
+}
+
 template <class T>
 class Queue {
  public:
@@ -197,12 +227,13 @@ struct Pipe {
   void run() {
     const double t0 = now();
     constexpr int kSlots = 4;
-    std::vector<std::unique_ptr<Slot>> slots;
+    std::vector<std::unique_ptr<Slot>> slots = slot_pool().take(kSlots);
+    struct GiveBack {
+      std::vector<std::unique_ptr<Slot>>& s;
+      ~GiveBack() { slot_pool().give(s); }
+    } give_back{slots};
     Queue<Slot*> free_q, ready_q, done_q;
-    for (int i = 0; i < kSlots; ++i) {
-      slots.emplace_back(new Slot);
-      free_q.push(slots.back().get());
-    }
+    for (auto& sl : slots) free_q.push(sl.get());
     std::string err;
     std::atomic<int> err_code{DNALN_OK};
     std::mutex err_m;
