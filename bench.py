@@ -136,9 +136,16 @@ def summarize_pmc(fetch_csv: str, write_csv: str, calib: int) -> int:
     out = {}
     for counter, path in (("FETCH_SIZE", fetch_csv), ("WRITE_SIZE", write_csv)):
         rows = [r for r in _pmc_rows(path) if r.get("Counter_Name") == counter]
-        per = {}
+        # per kernel, only the launches of its largest grid: the batch-wide launch of the step (the
+        # fused kernel's second pass over the few reads its first shape declined is left out)
+        per, top = {}, {}
         for r in rows:
-            per.setdefault(r["Kernel_Name"], []).append(float(r["Counter_Value"]))
+            g = int(r.get("Grid_Size") or 0)
+            k = r["Kernel_Name"]
+            if g > top.get(k, -1):
+                top[k], per[k] = g, []
+            if g == top[k]:
+                per[k].append(float(r["Counter_Value"]))
         copy = [v for k, vs in per.items() if "copyBuffer" in k for v in vs]
         fill = [v for k, vs in per.items() if k.startswith("void (anonymous namespace)::k_sw_packed<") and ", true>" in k
                 for v in vs]
