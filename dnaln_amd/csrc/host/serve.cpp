@@ -38,6 +38,39 @@ struct Request {
   bool view = false;
 };
 
+// Grow-only host array whose new elements are left uninitialized: a std::vector's resize
+// zero-fills every element it exposes, hundreds of MB per batch for the CIGAR bound, and that
+// stalled the worker whenever batches first grew (at each rise of the offered load).
+template <class T>
+struct HostBuf {
+  std::unique_ptr<T[]> p;
+  size_t n = 0, cap = 0;
+  void resize(size_t k) {
+    if (k > cap) {
+      const size_t c = std::max(k, cap + cap / 2);
+      std::unique_ptr<T[]> q(new T[c]);
+      if (n) std::memcpy(q.get(), p.get(), n * sizeof(T));
+      p.swap(q);
+      cap = c;
+    }
+    n = k;
+  }
+  T* data() { return p.get(); }
+  const T* data() const { return p.get(); }
+  size_t size() const { return n; }
+  // Allocates room for k elements and writes it once, so that the pages are mapped before the
+  // first batch instead of during it.
+  void prefault(size_t k) {
+    resize(k);
+    if (k) std::memset(static_cast<void*>(p.get()), 0, k * sizeof(T));
+    n = 0;
+  }
+  T& operator[](size_t i) { return p[i]; }
+  const T& operator[](size_t i) const { return p[i]; }
+  const T* begin() const { return p.get(); }
+  const T* end() const { return p.get() + n; }
+};
+
 struct Done {
   uint64_t id = 0;
   double t_done = 0;
@@ -61,12 +94,12 @@ struct Slot {
   uint64_t est = 0;
   std::vector<Request> reqs;  // the batch
   std::vector<Done> out;      // its requests, finished
-  std::vector<uint8_t> codes;
-  std::vector<uint64_t> offs;
-  std::vector<int32_t> lens;
-  std::vector<uint32_t> fnv;
-  std::vector<dnaln_record> rec;
-  std::vector<uint32_t> cig;
+  HostBuf<uint8_t> codes;
+  HostBuf<uint64_t> offs;
+  HostBuf<int32_t> lens;
+  HostBuf<uint32_t> fnv;
+  HostBuf<dnaln_record> rec;
+  HostBuf<uint32_t> cig;
 
   void gather() {
     uint64_t n = 0, nb = 0;
@@ -124,11 +157,16 @@ struct Slot {
       if (r == DNALN_OK) r = dnaln_hip_batch_run(b);
       if (r == DNALN_OK) r = dnaln_hip_batch_poll(b, 1);
       if (r == DNALN_OK) {
-        uint64_t cap = 1;
-        for (const int32_t l : lens) cap += 2 * (uint64_t)l + 4;
+        // A few CIGAR words per read; the bound (two per base) only when the device holds more.
         rec.resize(lens.size());
-        cig.resize(cap);
+        cig.resize(std::max<size_t>(cig.cap, 8 * lens.size() + 64));
         r = dnaln_hip_batch_download(b, rec.data(), cig.data(), cig.size(), nullptr);
+        if (r == DNALN_E_CAPACITY) {
+          uint64_t cap = 1;
+          for (const int32_t l : lens) cap += 2 * (uint64_t)l + 4;
+          cig.resize(cap);
+          r = dnaln_hip_batch_download(b, rec.data(), cig.data(), cig.size(), nullptr);
+        }
       }
       const std::string e = r == DNALN_OK ? std::string() : std::string(dnaln_hip_last_error());
       if (r == DNALN_OK) split();
@@ -313,6 +351,14 @@ int dnaln_serve_create(dnaln_hip_ctx* ctx, const dnaln_serve_config* cfg, dnaln_
       dnaln_hip_batch_destroy(slot->b);
       return rc;
     }
+    // host staging for the largest batch admitted (gather, download), mapped up front
+    const uint64_t nr = s->cfg.max_batch_reads, nb = s->cfg.max_batch_bases;
+    slot->codes.prefault(nb);
+    slot->offs.prefault(nr);
+    slot->lens.prefault(nr);
+    slot->fnv.prefault(nr);
+    slot->rec.prefault(nr);
+    slot->cig.prefault(8 * nr + 64);
     Slot* raw = slot.get();
     slot->worker = std::thread([raw] { raw->work(); });
     s->slots.push_back(std::move(slot));
