@@ -324,9 +324,11 @@ __global__ void k_cigar_gather(ReadView rv, const int32_t* chosen, const uint64_
   gather_cigar(rv.lens[r], tb_res[t], slots + t * (uint64_t)cap, pool + cig_off[r]);
 }
 
-// Waves per SIMD the fill must keep: two for the traceback fills that fit in 256 registers
-// (through W = 49), so that scheduling changes cannot silently halve their occupancy.
-constexpr int sw_min_waves(int W, bool TB) { return TB && W <= 49 ? 2 : 1; }
+// Waves per SIMD the fill must keep: two for the traceback fills through W = 65. Up to W = 49
+// they fit in 256 registers, and the bound keeps scheduling changes from silently halving their
+// occupancy. W = 65 spills 44 registers to scratch in the once-per-four-rows block section and
+// still gains 4% (25.6 against 26.7 ms per launch on the 4M x 250 bp repeat workload).
+constexpr int sw_min_waves(int W, bool TB) { return TB && W <= 65 ? 2 : 1; }
 
 template <int W, bool TB>
 __global__ __launch_bounds__(kSwBlock) __attribute__((amdgpu_waves_per_eu(sw_min_waves(W, TB)))) void k_sw_packed(
