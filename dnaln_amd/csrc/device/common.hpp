@@ -93,6 +93,13 @@ DN_HD void store_stream(uint64_t* p, uint64_t v) {
   *p = v;
 #endif
 }
+DN_HD void store_stream(uint32_t* p, uint32_t v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  __builtin_nontemporal_store(v, p);
+#else
+  *p = v;
+#endif
+}
 
 // Packed pairs of int16 in one 32-bit word: lane pair (A = bits 0..15, B = bits 16..31).
 #if defined(__HIP_DEVICE_COMPILE__)

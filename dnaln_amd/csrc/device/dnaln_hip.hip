@@ -417,7 +417,7 @@ __global__ __launch_bounds__(128) void k_walk_win(SwArgs a, int rows) {
     }
     __syncthreads();
     while (active && !w.done && (w.i >= lo || w.i < 0))
-      walk_step<W>(w, half, tk, win + lane, a.sp, out, a.cigar_cap, seq, lo);
+      walk_step<W>(w, tk, reinterpret_cast<const uint32_t*>(win) + half * 64 + lane, a.sp, out, a.cigar_cap, seq, lo);
     __syncthreads();
   }
   if (active) walk_finish(w, i_end, d_end, score, a.cigar_cap, a.res[id]);

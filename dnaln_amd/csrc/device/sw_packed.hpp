@@ -310,11 +310,13 @@ DN_HD void walk_flush(Walk& w, uint32_t* out, int cap) {
   w.pend_len = 0;
 }
 
-// One traceback step. slot_lane holds the lane's flag words from row row0 on (word of row i,
-// group g at ((i - row0) * G + g) * 64); a step reads only row w.i, and none when w.i < 0.
+// One traceback step. plane holds the flag words of the walk's own half (its lane and task half
+// of the fill wave) from row row0 on: the word of row i, group g at ((i - row0) * G + g) * 128,
+// bytes extF, extE, E and not-diagonal, bit d & 7 in each. A step reads only row w.i, and none
+// when w.i < 0.
 template <int W>
-DN_HD void walk_step(Walk& w, int half, const SwTask& tk, const uint64_t* slot_lane, const ScoreP& sp,
-                     uint32_t* out, int cap, WalkSeq& seq, int row0 = 0) {
+DN_HD void walk_step(Walk& w, const SwTask& tk, const uint32_t* plane, const ScoreP& sp, uint32_t* out, int cap,
+                     WalkSeq& seq, int row0 = 0) {
   constexpr int G = (W + 7) / 8;
   const int oe = sp.gap_open + sp.gap_ext, ge = sp.gap_ext;
   if (w.state == 0 && (w.i < 0 || w.v <= 0)) {
@@ -330,11 +332,11 @@ DN_HD void walk_step(Walk& w, int half, const SwTask& tk, const uint64_t* slot_l
     return;
   }
   // The flag word and, speculatively, the two bases of the cell (mostly from registers).
-  const uint64_t word = slot_lane[(uint64_t)((w.i - row0) * G + (w.d >> 3)) * 64];
+  const uint32_t word = plane[(uint64_t)((w.i - row0) * G + (w.d >> 3)) * 128];
   const uint8_t a = seq.qb(tk, w.i), c = seq.tb(tk, w.i + w.d);
   const int b = w.d & 7;
   if (w.state == 0) {
-    if (!((word >> (48 + 8 * half + b)) & 1)) {  // diagonal
+    if (!((word >> (24 + b)) & 1)) {  // diagonal
       w.v -= sub_score(sp, a, c);
       w.nm += (a != c || a > 3);
       ++w.run;
@@ -343,11 +345,11 @@ DN_HD void walk_step(Walk& w, int half, const SwTask& tk, const uint64_t* slot_l
     }
     if (w.run) walk_push(w, 0, w.run, out, cap);
     w.run = 0;
-    w.state = ((word >> (32 + 8 * half + b)) & 1) ? 2 : 1;  // notE set: deletion
+    w.state = ((word >> (16 + b)) & 1) ? 2 : 1;  // notE set: deletion
     return;
   }
   if (w.state == 1) {
-    const bool ext = (word >> (16 + 8 * half + b)) & 1;
+    const bool ext = (word >> (8 + b)) & 1;
     w.v += ext ? ge : oe;
     ++w.run;
     --w.i;
@@ -360,7 +362,7 @@ DN_HD void walk_step(Walk& w, int half, const SwTask& tk, const uint64_t* slot_l
     }
     return;
   }
-  const bool ext = (word >> (8 * half + b)) & 1;
+  const bool ext = (word >> b) & 1;
   w.v += ext ? ge : oe;
   ++w.run;
   --w.d;
@@ -472,7 +474,7 @@ DN_HD void sw_pair(const SwArgs& a, uint32_t ida, uint32_t idb, bool va, bool vb
       else BH[kBhLds ? 0 : d] = zero;
     }
   }
-  uint64_t* slot_lane = slot + lane;
+  uint32_t* slot_lane = reinterpret_cast<uint32_t*>(slot) + lane;
   // Flag bit of band column d within its group of eight: 0x01010101 << (d & 7). Shifted by a
   // value that is 0 at run time but unknown to the compiler, the eight masks live in scalar
   // registers, so each flag update is one three-operand and_or (which takes no literal).
@@ -511,7 +513,11 @@ DN_HD void sw_pair(const SwArgs& a, uint32_t ida, uint32_t idb, bool va, bool vb
           acc12 = and_or(p12, bit, acc12);
           acc34 = and_or(p34, bit, acc34);
           if ((d & 7) == 7 || d == W - 1) {
-            store_stream(&slot_lane[(uint64_t)(i * G + (d >> 3)) * 64], (uint64_t)acc12 | (uint64_t)acc34 << 32);
+            // One 32-bit plane per half (bytes: extF, extE, E, not-diagonal), so that a walk,
+            // which follows one half, reads only its own plane.
+            uint32_t* row = slot_lane + (uint64_t)(i * G + (d >> 3)) * 128;
+            store_stream(row, perm_b32(acc34, acc12, 0x06040200u));
+            store_stream(row + 64, perm_b32(acc34, acc12, 0x07050301u));
             acc12 = acc34 = 0;
           }
           rowmax = pk_max(rowmax, h);
@@ -608,11 +614,11 @@ DN_HD void walk_task(const SwArgs& a, uint32_t k, WalkSeq& seq) {
   }
   const SwTask tk = a.tasks[id];
   const uint32_t wave = k >> 7, lane = k & 63, half = (k >> 6) & 1;
-  const uint64_t* slot_lane = dir_slot(a, wave) + lane;
+  const uint32_t* plane = reinterpret_cast<const uint32_t*>(dir_slot(a, wave)) + half * 64 + lane;
   uint32_t* out = a.cigar + (uint64_t)id * (uint64_t)a.cigar_cap;
   Walk w = walk_init(i_end, d_end, score);
   seq.q.at = seq.t.at = ~0ull;
-  while (!w.done) walk_step<W>(w, (int)half, tk, slot_lane, a.sp, out, a.cigar_cap, seq);
+  while (!w.done) walk_step<W>(w, tk, plane, a.sp, out, a.cigar_cap, seq);
   walk_finish(w, i_end, d_end, score, a.cigar_cap, r);
 }
 

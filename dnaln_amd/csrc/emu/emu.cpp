@@ -71,7 +71,8 @@ void walk_wave_window(const SwArgs& a, uint32_t wave, int rows) {
     for (int t = 0; t < 128; ++t) {
       Th& x = th[t];
       while (x.active && !x.w.done && (x.w.i >= lo || x.w.i < 0))
-        walk_step<W>(x.w, t >> 6, x.tk, win.data() + (t & 63), a.sp, x.out, a.cigar_cap, x.seq, lo);
+        walk_step<W>(x.w, x.tk, reinterpret_cast<const uint32_t*>(win.data()) + (t >> 6) * 64 + (t & 63), a.sp, x.out,
+                     a.cigar_cap, x.seq, lo);
     }
   }
   for (Th& x : th)
