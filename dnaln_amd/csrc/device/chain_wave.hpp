@@ -97,10 +97,18 @@ __device__ __forceinline__ void chain_wave_read(const ChainArgs& a, uint32_t r, 
   constexpr int kBias = 1 << 20;
   uint64_t kw = 0;  // key of anchor i-1-lane
   int fw = 0;       // f of anchor i-1-lane
-  uint64_t knext = s.keys[0];
-  for (int i = 0; i < n; ++i) {
-    const uint64_t ki = knext;
-    if (i + 1 < n) knext = s.keys[i + 1];
+  // Keys of anchors i0 .. i0 + 63, one per lane, loaded a block ahead: a step takes anchor i's
+  // key from lane i % 64 (readlane), so no step waits for memory. The block's successor is
+  // loaded at the block's start and read only at its end. This matters for reads whose arrays
+  // are in global memory (more than 256 anchors); in LDS a step is bound by its ~40 VALU
+  // instructions (wave max, window shift, f / prev), not by the key load.
+  uint64_t kblk = lane < n ? s.keys[lane] : 0;
+  for (int i0 = 0; i0 < n; i0 += 64) {
+  const uint64_t knxt = i0 + 64 + lane < n ? s.keys[i0 + 64 + lane] : 0;
+  const int iend = n < i0 + 64 ? n : i0 + 64;
+  for (int i = i0; i < iend; ++i) {
+    const uint64_t ki = (uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(kblk >> 32), i - i0) << 32 |
+                        (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)kblk, i - i0);
     const uint32_t hi_i = (uint32_t)(ki >> 32);
     const int qi = key_qpos(ki);
     const int j = i - 1 - lane;
@@ -135,6 +143,8 @@ __device__ __forceinline__ void chain_wave_read(const ChainArgs& a, uint32_t r, 
     const uint32_t fs = wave_shr1((uint32_t)fw);
     kw = lane == 0 ? ki : ((uint64_t)khi << 32 | klo);
     fw = lane == 0 ? fi : (int)fs;
+  }
+  kblk = knxt;
   }
   wave_sync();
 
