@@ -4,17 +4,20 @@
 // Same results as chain_read (src/chain.cpp:12-71, src/mapper.cpp:44-62), organised for a wave:
 //   * the read's anchors are sorted with a wave-wide bitonic network whose comparators all put
 //     the smaller key first, so a virtual +inf pad beyond n never moves into range;
-//   * DP: lane l holds anchor i-1-l (its key and f) in registers and scores it as predecessor
-//     of anchor i. The reference's scan stops at the first j that changes strand or lies more
-//     than max_gap behind; both conditions are monotone in j on sorted anchors, so each lane
-//     tests its own j. A DPP max-reduction over (score << 6 | 63 - l) picks the best score and,
-//     on ties, the nearest j (smallest l), as the reference's strict > scan from j = i-1 down
-//     does. The window then shifts one lane (DPP wave_shr) and lane 0 takes anchor i;
+//   * DP: the last 64 anchors sit in a circular register window, anchor j in lane j % 64 (its
+//     key, f and prev), and each lane scores its anchor as predecessor of anchor i. The
+//     reference's scan stops at the first j that changes strand or lies more than max_gap
+//     behind; both conditions are monotone in j on sorted anchors, so each lane tests its own
+//     j. A DPP max-reduction over (score << 6 | 63 - (i-1-j)) picks the best score and, on ties,
+//     the nearest j, as the reference's strict > scan from j = i-1 down does. Lane i % 64 then
+//     takes anchor i's f and prev, which reach memory once per 64 anchors;
 //   * extraction walks ends in (f desc, index asc) order, a second bitonic sort of packed
-//     keys; chains go to a list that is sorted (score desc, extraction order asc) the same way;
+//     keys; chains go to a list that is sorted (score desc, extraction order asc) the same way.
+//     prev is stored as prev + 1 with the used flag in bit 31, so a walk step is one dependent
+//     load, and ends already used when a block of 64 ends starts are skipped by a ballot;
 //   * candidates: lane x keeps accepted candidate x's diagonal, so the dedupe test is a ballot.
-// Every lane writes the same value wherever all lanes later read (f, prev, used), so each lane
-// reads its own writes. Lookback <= 64 and max_cand <= 64 (checked by the host).
+// Every lane writes the same value wherever all lanes later read (prev during extraction), so
+// each lane reads its own writes. Lookback <= 64 and max_cand <= 64 (checked by the host).
 #pragma once
 
 #include "seed_chain.hpp"
@@ -37,11 +40,6 @@ __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
   v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false)); // row_bcast:15
   v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false)); // row_bcast:31
   return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
-}
-
-// Lane l receives lane l-1's value; lane 0 receives 0.
-__device__ __forceinline__ uint32_t wave_shr1(uint32_t v) {
-  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x138, 0xf, 0xf, false);
 }
 
 // Ascending sort of a[0..n) by the wave (the 64 lanes take the comparators of each step).
@@ -71,9 +69,8 @@ struct WaveChainMem {
   uint64_t* keys;     // n anchors
   uint64_t* order;    // n sort keys
   int32_t* f;         // n
-  int32_t* prev;      // n
+  uint32_t* prev;     // n: prev + 1 in bits 0..30, bit 31 set once extraction has used the anchor
   int32_t* chains;    // chain_cap x {score, rpos | strand << 31, dmin, dmax}
-  uint8_t* used;      // n
 };
 
 __device__ __forceinline__ void chain_wave_read(const ChainArgs& a, uint32_t r, int n, const WaveChainMem& s,
@@ -95,92 +92,110 @@ __device__ __forceinline__ void chain_wave_read(const ChainArgs& a, uint32_t r, 
   const int band = a.band;
   const bool nib = a.cost_nib_ok;
   constexpr int kBias = 1 << 20;
-  uint64_t kw = 0;  // key of anchor i-1-lane
-  int fw = 0;       // f of anchor i-1-lane
-  // Keys of anchors i0 .. i0 + 63, one per lane, loaded a block ahead: a step takes anchor i's
-  // key from lane i % 64 (readlane), so no step waits for memory. The block's successor is
-  // loaded at the block's start and read only at its end. This matters for reads whose arrays
-  // are in global memory (more than 256 anchors); in LDS a step is bound by its ~40 VALU
-  // instructions (wave max, window shift, f / prev), not by the key load.
-  uint64_t kblk = lane < n ? s.keys[lane] : 0;
+  // Keys of anchors i0 .. i0 + 63 (kblk) and of the block before (kprv), one per lane, with
+  // the next block loaded at a block's start and read only at its end, so no step waits for
+  // memory: a step reads anchor i's key from lane i % 64 (readlane) and each lane's window
+  // anchor from kblk or kprv. fw and pw hold the f and prev + 1 of the lane's window anchor;
+  // after the block's last step they hold the whole block, which goes to memory in one store.
+  uint64_t kprv = 0, kblk = lane < n ? s.keys[lane] : 0;
+  int fw = 0;
+  uint32_t pw = 0;
   for (int i0 = 0; i0 < n; i0 += 64) {
-  const uint64_t knxt = i0 + 64 + lane < n ? s.keys[i0 + 64 + lane] : 0;
-  const int iend = n < i0 + 64 ? n : i0 + 64;
-  for (int i = i0; i < iend; ++i) {
-    const uint64_t ki = (uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(kblk >> 32), i - i0) << 32 |
-                        (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)kblk, i - i0);
-    const uint32_t hi_i = (uint32_t)(ki >> 32);
-    const int qi = key_qpos(ki);
-    const int j = i - 1 - lane;
-    uint32_t cand = 0;
-    if (lane < a.lookback && j >= 0) {
-      const uint32_t hj = (uint32_t)(kw >> 32);
-      const uint32_t dr = hi_i - hj;  // same strand: the strand bits cancel
-      const int dq = qi - key_qpos(kw);
-      const uint32_t t = dr - (uint32_t)dq;
-      const uint32_t dd = (int32_t)t < 0 ? 0u - t : t;
-      const bool alive = ((hi_i ^ hj) >> 31) == 0 && dr <= max_gap;
-      if (alive && dr > 0 && dq > 0 && dd <= (uint32_t)band) {
-        const int m = dq < (int)dr ? dq : (int)dr;
-        const int c = nib ? (dd == (uint32_t)band ? a.cost_at_band : (int)((a.cost_nib >> (4 * (dd & 15))) & 15u))
-                          : cost[dd];
-        const int sc = fw + (m < k ? m : k) - c;
-        cand = (uint32_t)(sc + kBias) << 6 | (uint32_t)(63 - lane);
+    const uint64_t knxt = i0 + 64 + lane < n ? s.keys[i0 + 64 + lane] : 0;
+    const int iend = n < i0 + 64 ? n : i0 + 64;
+    for (int i = i0; i < iend; ++i) {
+      const int ri = i - i0;  // lane of anchor i
+      const uint64_t ki = (uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(kblk >> 32), ri) << 32 |
+                          (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)kblk, ri);
+      const uint32_t hi_i = (uint32_t)(ki >> 32);
+      const int qi = key_qpos(ki);
+      const uint64_t kw = lane < ri ? kblk : kprv;  // anchor j = i - 1 - dist
+      const int dist = (ri - 1 - lane) & 63;
+      const int reach = a.lookback < i ? a.lookback : i;  // j >= 0 and within the lookback
+      uint32_t cand = 0;
+      if (dist < reach) {
+        const uint32_t hj = (uint32_t)(kw >> 32);
+        const uint32_t dr = hi_i - hj;  // same strand: the strand bits cancel
+        const int dq = qi - key_qpos(kw);
+        const uint32_t t = dr - (uint32_t)dq;
+        const uint32_t dd = (int32_t)t < 0 ? 0u - t : t;
+        const bool alive = ((hi_i ^ hj) >> 31) == 0 && dr <= max_gap;
+        if (alive && dr > 0 && dq > 0 && dd <= (uint32_t)band) {
+          const int m = dq < (int)dr ? dq : (int)dr;
+          const int c = nib ? (dd == (uint32_t)band ? a.cost_at_band : (int)((a.cost_nib >> (4 * (dd & 15))) & 15u))
+                            : cost[dd];
+          const int sc = fw + (m < k ? m : k) - c;
+          cand = (uint32_t)(sc + kBias) << 6 | (uint32_t)(63 - dist);
+        }
+      }
+      const uint32_t best = wave_max_u32(cand);
+      int fi = k, pi = -1;
+      if (best) {
+        const int sc = (int)(best >> 6) - kBias;
+        if (sc > k) {
+          fi = sc;
+          pi = i - 1 - (63 - (int)(best & 63));
+        }
+      }
+      if (lane == ri) {
+        fw = fi;
+        pw = (uint32_t)(pi + 1);
       }
     }
-    const uint32_t best = wave_max_u32(cand);
-    int fi = k, pi = -1;
-    if (best) {
-      const int sc = (int)(best >> 6) - kBias;
-      if (sc > k) {
-        fi = sc;
-        pi = i - 1 - (63 - (int)(best & 63));
-      }
+    if (lane < iend - i0) {
+      s.f[i0 + lane] = fw;
+      s.prev[i0 + lane] = pw;
     }
-    s.f[i] = fi;
-    s.prev[i] = pi;
-    const uint32_t klo = wave_shr1((uint32_t)kw), khi = wave_shr1((uint32_t)(kw >> 32));
-    const uint32_t fs = wave_shr1((uint32_t)fw);
-    kw = lane == 0 ? ki : ((uint64_t)khi << 32 | klo);
-    fw = lane == 0 ? fi : (int)fs;
-  }
-  kblk = knxt;
+    kprv = kblk;
+    kblk = knxt;
   }
   wave_sync();
 
   // 3. extraction order: f desc, index asc (src/chain.cpp:47-52)
-  for (int x = lane; x < n; x += 64) {
-    s.order[x] = (uint64_t)(uint32_t)(0x7fffffff - s.f[x]) << 32 | (uint32_t)x;
-    s.used[x] = 0;
-  }
+  for (int x = lane; x < n; x += 64) s.order[x] = (uint64_t)(uint32_t)(0x7fffffff - s.f[x]) << 32 | (uint32_t)x;
   wave_sync();
   wave_sort(s.order, n, lane);
 
-  // 4. chains (src/chain.cpp:53-68), lane-uniform walks
+  // 4. chains (src/chain.cpp:53-68), lane-uniform walks. Ends are taken 64 at a time: those
+  // unused at the block's start are visited in order and checked again before their walk,
+  // since an earlier walk of the block may have used them.
   int m = 0;
-  for (int o = 0; o < n; ++o) {
-    const int end = (int)(uint32_t)s.order[o];
-    if (s.used[end]) continue;
-    int i = end, cnt = 0;
-    int64_t dmin = INT64_MAX, dmax = INT64_MIN;
-    while (i >= 0 && !s.used[i]) {
-      s.used[i] = 1;
-      ++cnt;
-      const uint64_t kk = s.keys[i];
-      const int64_t d = (int64_t)key_rpos(kk) - key_qpos(kk);
-      dmin = d < dmin ? d : dmin;
-      dmax = d > dmax ? d : dmax;
-      i = s.prev[i];
+  for (int o0 = 0; o0 < n; o0 += 64) {
+    const uint32_t my_end = o0 + lane < n ? (uint32_t)s.order[o0 + lane] : 0u;
+    uint64_t open = __builtin_amdgcn_ballot_w64(o0 + lane < n && !(s.prev[my_end] >> 31));
+    while (open) {
+      const int end = __builtin_amdgcn_readlane((int)my_end, (int)__builtin_ctzll(open));
+      open &= open - 1;
+      uint32_t pu = s.prev[end];
+      if (pu >> 31) continue;
+      int i = end, cnt = 0, stop = -1;
+      int64_t dmin = INT64_MAX, dmax = INT64_MIN;
+      for (;;) {
+        const uint64_t kk = s.keys[i];
+        s.prev[i] = pu | 0x80000000u;
+        ++cnt;
+        const int64_t d = (int64_t)key_rpos(kk) - key_qpos(kk);
+        dmin = d < dmin ? d : dmin;
+        dmax = d > dmax ? d : dmax;
+        const int nx = (int)(pu & 0x7fffffffu) - 1;
+        if (nx < 0) break;
+        pu = s.prev[nx];
+        if (pu >> 31) {
+          stop = nx;
+          break;
+        }
+        i = nx;
+      }
+      const int score = s.f[end] - (stop >= 0 ? s.f[stop] : 0);
+      if (score < a.min_score || cnt < a.min_anchors) continue;
+      const uint64_t ke = s.keys[end];
+      int32_t* c = s.chains + 4 * m;
+      c[0] = score;
+      c[1] = (int32_t)(key_rpos(ke) | key_strand(ke) << 31);
+      c[2] = (int32_t)dmin;
+      c[3] = (int32_t)dmax;
+      ++m;
     }
-    const int score = s.f[end] - (i >= 0 ? s.f[i] : 0);
-    if (score < a.min_score || cnt < a.min_anchors) continue;
-    const uint64_t ke = s.keys[end];
-    int32_t* c = s.chains + 4 * m;
-    c[0] = score;
-    c[1] = (int32_t)(key_rpos(ke) | key_strand(ke) << 31);
-    c[2] = (int32_t)dmin;
-    c[3] = (int32_t)dmax;
-    ++m;
   }
   wave_sync();
   if (m == 0) {

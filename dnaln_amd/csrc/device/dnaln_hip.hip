@@ -178,11 +178,12 @@ __global__ __launch_bounds__(kSeedBlock) void k_chain(ChainArgs a) {
 }
 
 // One wave per read (chain_wave.hpp), four waves per block. LDS: the gap-cost table, then per
-// wave kWaveChainLds anchors' keys, sort keys, f, prev, chains and used flags. Reads with more
+// wave kWaveChainLds anchors' keys, sort keys, f, prev and chains. Reads with more
 // anchors, or min_anchors < 2 (more chains than the LDS list holds), work in the global
-// scratch arrays of the chain stage instead.
-constexpr int kWaveChainLds = 256;
-constexpr size_t kWaveChainBytes = (size_t)33 * kWaveChainLds;
+// scratch arrays of the chain stage instead. 248 anchors keep a block at the default band under
+// 32 KiB, so five blocks (20 waves) share a CU's 160 KiB.
+constexpr int kWaveChainLds = 248;
+constexpr size_t kWaveChainBytes = (size_t)32 * kWaveChainLds;
 
 size_t wave_chain_lds_bytes(int band) { return (size_t)((band + 4) & ~3) * 4 + 4 * kWaveChainBytes; }
 
@@ -191,7 +192,8 @@ __global__ __launch_bounds__(256) void k_chain_wave(ChainArgs a) {
   int32_t* cost = reinterpret_cast<int32_t*>(lds);
   for (int x = threadIdx.x; x <= a.band; x += blockDim.x) cost[x] = a.cost[x];
   __syncthreads();
-  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  // The wave index as a scalar, so the read's anchor count and the loops over it are uniform.
+  const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)), lane = threadIdx.x & 63;
   const uint32_t li = blockIdx.x * 4 + wv;
   if (li >= a.n_reads) return;
   const uint32_t r = read_id(a.list, li);
@@ -203,9 +205,8 @@ __global__ __launch_bounds__(256) void k_chain_wave(ChainArgs a) {
     s.keys = reinterpret_cast<uint64_t*>(base);
     s.order = s.keys + kWaveChainLds;
     s.f = reinterpret_cast<int32_t*>(s.order + kWaveChainLds);
-    s.prev = s.f + kWaveChainLds;
-    s.chains = s.prev + kWaveChainLds;
-    s.used = reinterpret_cast<uint8_t*>(s.chains + 2 * kWaveChainLds);
+    s.prev = reinterpret_cast<uint32_t*>(s.f + kWaveChainLds);
+    s.chains = s.f + 2 * kWaveChainLds;
     for (int x = lane; x < n; x += 64) s.keys[x] = a.keys[off + x];
     wave_sync();
     chain_wave_read(a, r, n, s, cost, lane);
@@ -213,9 +214,8 @@ __global__ __launch_bounds__(256) void k_chain_wave(ChainArgs a) {
     s.keys = a.keys + off;
     s.order = a.order + off;
     s.f = a.f + off;
-    s.prev = a.prev + off;
+    s.prev = reinterpret_cast<uint32_t*>(a.prev + off);
     s.chains = reinterpret_cast<int32_t*>(a.chains + off);
-    s.used = reinterpret_cast<uint8_t*>(a.chains + off) + (size_t)16 * n;
     chain_wave_read(a, r, n, s, cost, lane);
   }
 }
