@@ -7,6 +7,7 @@
 
 #include "device/seed_chain.hpp"
 #include "dnaln_host.h"
+#include "host/c_guard.hpp"
 #include "host/sam_text.hpp"
 
 namespace {
@@ -32,50 +33,52 @@ extern "C" {
 
 int dnaln_index_build(const uint8_t* ref, const dnaln_contig* contigs, int32_t n_contigs, int32_t k, int32_t w,
                       dnaln_index* out) {
-  if (!out || !contigs || n_contigs < 1 || k < 4 || k > 28 || w < 1 || w > 255) return DNALN_E_INVALID;
-  std::memset(out, 0, sizeof *out);
-  uint64_t total = 0;
-  for (int c = 0; c < n_contigs; ++c) total = std::max<uint64_t>(total, contigs[c].offset + contigs[c].len);
-  if (total >= (1ull << 31)) return DNALN_E_INVALID;
-  std::vector<Occ> occ;
-  occ.reserve(total * 2 / (w + 1) + 1024);
-  std::vector<uint64_t> rh(w);
-  std::vector<uint32_t> rps(w);
-  dn::Ring ring{rh.data(), rps.data(), 1};
-  for (int c = 0; c < n_contigs; ++c) {
-    const uint32_t base = (uint32_t)contigs[c].offset;
-    dn::sketch_stream(dn::PtrReader{ref + contigs[c].offset}, (int)contigs[c].len, k, w, ring,
-                      [&](uint64_t h, uint32_t p, uint32_t s) { occ.push_back({h, (p + base) << 1 | s}); });
-  }
-  std::sort(occ.begin(), occ.end(),
-            [](const Occ& a, const Occ& b) { return a.hash != b.hash ? a.hash < b.hash : a.pos < b.pos; });
-  uint64_t n_keys = 0;
-  for (size_t i = 0; i < occ.size(); ++i) n_keys += i == 0 || occ[i].hash != occ[i - 1].hash;
-  uint64_t cap = 16;
-  while (cap < n_keys * 2) cap <<= 1;
-  dnaln_bucket* table = new dnaln_bucket[cap];
-  for (uint64_t s = 0; s < cap; ++s) table[s] = {~0ULL, 0, 0};
-  uint32_t* pos = new uint32_t[occ.size() + 1];
-  const uint64_t mask = cap - 1;
-  uint32_t max_count = 0;
-  for (size_t i = 0; i < occ.size();) {
-    size_t j = i;
-    for (; j < occ.size() && occ[j].hash == occ[i].hash; ++j) pos[j] = occ[j].pos;
-    uint64_t s = occ[i].hash & mask;
-    while (table[s].key != ~0ULL) s = (s + 1) & mask;
-    table[s] = {occ[i].hash, (uint32_t)i, (uint32_t)(j - i)};
-    max_count = std::max(max_count, (uint32_t)(j - i));
-    i = j;
-  }
-  out->view.table = table;
-  out->view.n_buckets = cap;
-  out->view.pos = pos;
-  out->view.n_pos = occ.size();
-  out->view.k = k;
-  out->view.w = w;
-  out->n_keys = n_keys;
-  out->max_count = max_count;
-  return DNALN_OK;
+  return dnaln_amd::c_guard([&]() -> int {
+    if (!out || !contigs || n_contigs < 1 || k < 4 || k > 28 || w < 1 || w > 255) return DNALN_E_INVALID;
+    std::memset(out, 0, sizeof *out);
+    uint64_t total = 0;
+    for (int c = 0; c < n_contigs; ++c) total = std::max<uint64_t>(total, contigs[c].offset + contigs[c].len);
+    if (total >= (1ull << 31)) return DNALN_E_INVALID;
+    std::vector<Occ> occ;
+    occ.reserve(total * 2 / (w + 1) + 1024);
+    std::vector<uint64_t> rh(w);
+    std::vector<uint32_t> rps(w);
+    dn::Ring ring{rh.data(), rps.data(), 1};
+    for (int c = 0; c < n_contigs; ++c) {
+      const uint32_t base = (uint32_t)contigs[c].offset;
+      dn::sketch_stream(dn::PtrReader{ref + contigs[c].offset}, (int)contigs[c].len, k, w, ring,
+                        [&](uint64_t h, uint32_t p, uint32_t s) { occ.push_back({h, (p + base) << 1 | s}); });
+    }
+    std::sort(occ.begin(), occ.end(),
+              [](const Occ& a, const Occ& b) { return a.hash != b.hash ? a.hash < b.hash : a.pos < b.pos; });
+    uint64_t n_keys = 0;
+    for (size_t i = 0; i < occ.size(); ++i) n_keys += i == 0 || occ[i].hash != occ[i - 1].hash;
+    uint64_t cap = 16;
+    while (cap < n_keys * 2) cap <<= 1;
+    std::unique_ptr<dnaln_bucket[]> table(new dnaln_bucket[cap]);
+    for (uint64_t s = 0; s < cap; ++s) table[s] = {~0ULL, 0, 0};
+    std::unique_ptr<uint32_t[]> pos(new uint32_t[occ.size() + 1]);
+    const uint64_t mask = cap - 1;
+    uint32_t max_count = 0;
+    for (size_t i = 0; i < occ.size();) {
+      size_t j = i;
+      for (; j < occ.size() && occ[j].hash == occ[i].hash; ++j) pos[j] = occ[j].pos;
+      uint64_t s = occ[i].hash & mask;
+      while (table[s].key != ~0ULL) s = (s + 1) & mask;
+      table[s] = {occ[i].hash, (uint32_t)i, (uint32_t)(j - i)};
+      max_count = std::max(max_count, (uint32_t)(j - i));
+      i = j;
+    }
+    out->view.table = table.release();
+    out->view.n_buckets = cap;
+    out->view.pos = pos.release();
+    out->view.n_pos = occ.size();
+    out->view.k = k;
+    out->view.w = w;
+    out->n_keys = n_keys;
+    out->max_count = max_count;
+    return DNALN_OK;
+  });
 }
 
 void dnaln_index_free(dnaln_index* idx) {
@@ -87,36 +90,41 @@ void dnaln_index_free(dnaln_index* idx) {
 
 int dnaln_index_build_gpu(const uint8_t* ref, const dnaln_contig* contigs, int32_t n_contigs, int32_t k, int32_t w,
                           int32_t device, dnaln_index* out, double* build_ms) {
-  if (!out || !contigs || n_contigs < 1) return DNALN_E_INVALID;
-  std::memset(out, 0, sizeof *out);
-  uint64_t total = 0;
-  for (int c = 0; c < n_contigs; ++c) total = std::max<uint64_t>(total, contigs[c].offset + contigs[c].len);
-  dnaln_hip_index* ix = nullptr;
-  int rc = dnaln_hip_index_build(ref, total, contigs, n_contigs, k, w, device, &ix);
-  if (rc != DNALN_OK) return rc;
-  uint64_t nb = 0, np = 0, nk = 0;
-  uint32_t mc = 0;
-  double ms = 0;
-  dnaln_hip_index_info(ix, &nb, &np, &nk, &mc, &ms);
-  dnaln_bucket* table = new dnaln_bucket[nb];
-  uint32_t* pos = new uint32_t[np + 1];
-  rc = dnaln_hip_index_download(ix, table, pos);
-  dnaln_hip_index_free(ix);
-  if (rc != DNALN_OK) {
-    delete[] table;
-    delete[] pos;
-    return rc;
-  }
-  out->view.table = table;
-  out->view.n_buckets = nb;
-  out->view.pos = pos;
-  out->view.n_pos = np;
-  out->view.k = k;
-  out->view.w = w;
-  out->n_keys = nk;
-  out->max_count = mc;
-  if (build_ms) *build_ms = ms;
-  return DNALN_OK;
+  return dnaln_amd::c_guard([&]() -> int {
+    if (!out || !contigs || n_contigs < 1) return DNALN_E_INVALID;
+    std::memset(out, 0, sizeof *out);
+    uint64_t total = 0;
+    for (int c = 0; c < n_contigs; ++c) total = std::max<uint64_t>(total, contigs[c].offset + contigs[c].len);
+    dnaln_hip_index* ix = nullptr;
+    int rc = dnaln_hip_index_build(ref, total, contigs, n_contigs, k, w, device, &ix);
+    if (rc != DNALN_OK) return rc;
+    uint64_t nb = 0, np = 0, nk = 0;
+    uint32_t mc = 0;
+    double ms = 0;
+    dnaln_hip_index_info(ix, &nb, &np, &nk, &mc, &ms);
+    std::unique_ptr<dnaln_bucket[]> table;
+    std::unique_ptr<uint32_t[]> pos;
+    try {
+      table.reset(new dnaln_bucket[nb]);
+      pos.reset(new uint32_t[np + 1]);
+    } catch (...) {
+      dnaln_hip_index_free(ix);
+      throw;
+    }
+    rc = dnaln_hip_index_download(ix, table.get(), pos.get());
+    dnaln_hip_index_free(ix);
+    if (rc != DNALN_OK) return rc;
+    out->view.table = table.release();
+    out->view.n_buckets = nb;
+    out->view.pos = pos.release();
+    out->view.n_pos = np;
+    out->view.k = k;
+    out->view.w = w;
+    out->n_keys = nk;
+    out->max_count = mc;
+    if (build_ms) *build_ms = ms;
+    return DNALN_OK;
+  });
 }
 
 // A 64-bit multiply-xorshift digest over 8-byte words (not cryptographic: it catches an index
@@ -193,52 +201,54 @@ int dnaln_index_save(const dnaln_index* idx, const uint8_t* ref, const dnaln_con
 
 int dnaln_index_load(const char* path, const uint8_t* ref, const dnaln_contig* contigs, int32_t n_contigs,
                      dnaln_index* out) {
-  if (!out || !path || !contigs || n_contigs < 1) return DNALN_E_INVALID;
-  std::memset(out, 0, sizeof *out);
-  File f{std::fopen(path, "rb")};
-  if (!f.f) return DNALN_E_IO;
-  IndexFileHeader h{};
-  if (std::fread(&h, sizeof h, 1, f.f) != 1 || std::memcmp(h.magic, kIndexMagic, 8) != 0 || h.version != 1)
-    return DNALN_E_IO;
-  if (h.k < 4 || h.k > 28 || h.w < 1 || h.w > 255 || h.n_buckets < 16 || (h.n_buckets & (h.n_buckets - 1)) ||
-      h.n_buckets > (1ull << 40) || h.n_keys * 2 > h.n_buckets || h.n_pos >= (1ull << 32) || h.n_keys > h.n_pos)
-    return DNALN_E_IO;
-  if (std::fseek(f.f, 0, SEEK_END) != 0) return DNALN_E_IO;
-  const long long size = std::ftell(f.f);
-  if (size < 0 || (uint64_t)size != sizeof h + h.n_buckets * sizeof(dnaln_bucket) + h.n_pos * 4) return DNALN_E_IO;
-  if (h.digest != dnaln_reference_digest(ref, contigs, n_contigs)) return DNALN_E_IO;
-  if (std::fseek(f.f, (long)sizeof h, SEEK_SET) != 0) return DNALN_E_IO;
-  std::unique_ptr<dnaln_bucket[]> table(new dnaln_bucket[h.n_buckets]);
-  std::unique_ptr<uint32_t[]> pos(new uint32_t[h.n_pos + 1]);
-  pos[h.n_pos] = 0;
-  if (std::fread(table.get(), sizeof(dnaln_bucket), h.n_buckets, f.f) != h.n_buckets) return DNALN_E_IO;
-  if (h.n_pos && std::fread(pos.get(), 4, h.n_pos, f.f) != h.n_pos) return DNALN_E_IO;
-  // What the device kernels read stays in bounds: every slice inside pos[], every position
-  // inside the reference, at least half the buckets empty (so every probe ends); the stated key
-  // count and largest count match the table.
-  uint64_t total = 0;
-  for (int c = 0; c < n_contigs; ++c) total = std::max<uint64_t>(total, contigs[c].offset + contigs[c].len);
-  uint64_t keys = 0;
-  uint32_t max_count = 0;
-  for (uint64_t i = 0; i < h.n_buckets; ++i) {
-    const dnaln_bucket& b = table[i];
-    if (b.key == ~0ULL) continue;
-    if (b.count == 0 || (uint64_t)b.offset + b.count > h.n_pos) return DNALN_E_IO;
-    ++keys;
-    max_count = std::max(max_count, b.count);
-  }
-  if (keys != h.n_keys || max_count != h.max_count) return DNALN_E_IO;
-  for (uint64_t i = 0; i < h.n_pos; ++i)
-    if ((pos[i] >> 1) >= total) return DNALN_E_IO;
-  out->view.table = table.release();
-  out->view.pos = pos.release();
-  out->view.n_buckets = h.n_buckets;
-  out->view.n_pos = h.n_pos;
-  out->view.k = h.k;
-  out->view.w = h.w;
-  out->n_keys = h.n_keys;
-  out->max_count = h.max_count;
-  return DNALN_OK;
+  return dnaln_amd::c_guard([&]() -> int {
+    if (!out || !path || !contigs || n_contigs < 1) return DNALN_E_INVALID;
+    std::memset(out, 0, sizeof *out);
+    File f{std::fopen(path, "rb")};
+    if (!f.f) return DNALN_E_IO;
+    IndexFileHeader h{};
+    if (std::fread(&h, sizeof h, 1, f.f) != 1 || std::memcmp(h.magic, kIndexMagic, 8) != 0 || h.version != 1)
+      return DNALN_E_IO;
+    if (h.k < 4 || h.k > 28 || h.w < 1 || h.w > 255 || h.n_buckets < 16 || (h.n_buckets & (h.n_buckets - 1)) ||
+        h.n_buckets > (1ull << 40) || h.n_keys * 2 > h.n_buckets || h.n_pos >= (1ull << 32) || h.n_keys > h.n_pos)
+      return DNALN_E_IO;
+    if (std::fseek(f.f, 0, SEEK_END) != 0) return DNALN_E_IO;
+    const long long size = std::ftell(f.f);
+    if (size < 0 || (uint64_t)size != sizeof h + h.n_buckets * sizeof(dnaln_bucket) + h.n_pos * 4) return DNALN_E_IO;
+    if (h.digest != dnaln_reference_digest(ref, contigs, n_contigs)) return DNALN_E_IO;
+    if (std::fseek(f.f, (long)sizeof h, SEEK_SET) != 0) return DNALN_E_IO;
+    std::unique_ptr<dnaln_bucket[]> table(new dnaln_bucket[h.n_buckets]);
+    std::unique_ptr<uint32_t[]> pos(new uint32_t[h.n_pos + 1]);
+    pos[h.n_pos] = 0;
+    if (std::fread(table.get(), sizeof(dnaln_bucket), h.n_buckets, f.f) != h.n_buckets) return DNALN_E_IO;
+    if (h.n_pos && std::fread(pos.get(), 4, h.n_pos, f.f) != h.n_pos) return DNALN_E_IO;
+    // What the device kernels read stays in bounds: every slice inside pos[], every position
+    // inside the reference, at least half the buckets empty (so every probe ends); the stated key
+    // count and largest count match the table.
+    uint64_t total = 0;
+    for (int c = 0; c < n_contigs; ++c) total = std::max<uint64_t>(total, contigs[c].offset + contigs[c].len);
+    uint64_t keys = 0;
+    uint32_t max_count = 0;
+    for (uint64_t i = 0; i < h.n_buckets; ++i) {
+      const dnaln_bucket& b = table[i];
+      if (b.key == ~0ULL) continue;
+      if (b.count == 0 || (uint64_t)b.offset + b.count > h.n_pos) return DNALN_E_IO;
+      ++keys;
+      max_count = std::max(max_count, b.count);
+    }
+    if (keys != h.n_keys || max_count != h.max_count) return DNALN_E_IO;
+    for (uint64_t i = 0; i < h.n_pos; ++i)
+      if ((pos[i] >> 1) >= total) return DNALN_E_IO;
+    out->view.table = table.release();
+    out->view.pos = pos.release();
+    out->view.n_buckets = h.n_buckets;
+    out->view.n_pos = h.n_pos;
+    out->view.k = h.k;
+    out->view.w = h.w;
+    out->n_keys = h.n_keys;
+    out->max_count = h.max_count;
+    return DNALN_OK;
+  });
 }
 
 uint32_t dnaln_fnv1a(const char* s, size_t n) {

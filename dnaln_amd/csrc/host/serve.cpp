@@ -1,5 +1,6 @@
 // serve.cpp — continuous batching of mapping requests on the resident index (dnaln_serve.h).
 #include "dnaln_serve.h"
+#include "host/c_guard.hpp"
 
 #include <algorithm>
 #include <atomic>
@@ -320,51 +321,51 @@ void dnaln_serve_default_config(dnaln_serve_config* cfg) {
 }
 
 int dnaln_serve_create(dnaln_hip_ctx* ctx, const dnaln_serve_config* cfg, dnaln_server** out) {
-  if (!ctx || !out) return DNALN_E_INVALID;
-  std::unique_ptr<dnaln_server> s(new dnaln_server);
-  s->ctx = ctx;
-  if (cfg) s->cfg = *cfg;
-  else dnaln_serve_default_config(&s->cfg);
-  if (s->cfg.slots < 1 || s->cfg.slots > 64 || s->cfg.max_batch_reads == 0 || s->cfg.max_batch_bases == 0 ||
-      s->cfg.max_read_len < 1)
-    return DNALN_E_INVALID;
-  dnaln_map_params p;
-  int rc = dnaln_hip_context_params(ctx, &p);
-  if (rc != DNALN_OK) return rc;
-  s->bw = p.sc.bw;
-  s->max_cand = std::max(p.max_cand, 1);
-  uint64_t free_b = 0, total_b = 0;
-  rc = dnaln_hip_device_memory(ctx, &free_b, &total_b);
-  if (rc != DNALN_OK) return rc;
-  s->st.budget_bytes = s->cfg.hbm_budget ? s->cfg.hbm_budget : free_b / 2;
-  if (!s->cfg.dir_budget) s->cfg.dir_budget = std::max<uint64_t>(s->st.budget_bytes / s->cfg.slots / 4, 1 << 24);
-  for (int i = 0; i < s->cfg.slots; ++i) {
-    auto slot = std::make_unique<Slot>();
-    dnaln_batch_config bc;
-    dnaln_hip_default_batch_config(&bc);
-    bc.mode = s->cfg.mode;
-    bc.dir_budget = s->cfg.dir_budget;
-    rc = dnaln_hip_batch_create(ctx, &bc, &slot->b);
+  return dnaln_amd::c_guard([&]() -> int {
+    if (!ctx || !out) return DNALN_E_INVALID;
+    std::unique_ptr<dnaln_server> s(new dnaln_server);
+    s->ctx = ctx;
+    if (cfg) s->cfg = *cfg;
+    else dnaln_serve_default_config(&s->cfg);
+    if (s->cfg.slots < 1 || s->cfg.slots > 64 || s->cfg.max_batch_reads == 0 || s->cfg.max_batch_bases == 0 ||
+        s->cfg.max_read_len < 1)
+      return DNALN_E_INVALID;
+    dnaln_map_params p;
+    int rc = dnaln_hip_context_params(ctx, &p);
     if (rc != DNALN_OK) return rc;
-    rc = dnaln_hip_batch_reserve(slot->b, s->cfg.max_batch_reads, s->cfg.max_batch_bases, s->cfg.max_read_len, 64);
-    if (rc != DNALN_OK) {
-      dnaln_hip_batch_destroy(slot->b);
-      return rc;
+    s->bw = p.sc.bw;
+    s->max_cand = std::max(p.max_cand, 1);
+    uint64_t free_b = 0, total_b = 0;
+    rc = dnaln_hip_device_memory(ctx, &free_b, &total_b);
+    if (rc != DNALN_OK) return rc;
+    s->st.budget_bytes = s->cfg.hbm_budget ? s->cfg.hbm_budget : free_b / 2;
+    if (!s->cfg.dir_budget) s->cfg.dir_budget = std::max<uint64_t>(s->st.budget_bytes / s->cfg.slots / 4, 1 << 24);
+    for (int i = 0; i < s->cfg.slots; ++i) {
+      // The server owns the slot from the start, so its destructor releases the batch and joins
+      // the worker whichever step below fails.
+      s->slots.push_back(std::make_unique<Slot>());
+      Slot* slot = s->slots.back().get();
+      dnaln_batch_config bc;
+      dnaln_hip_default_batch_config(&bc);
+      bc.mode = s->cfg.mode;
+      bc.dir_budget = s->cfg.dir_budget;
+      rc = dnaln_hip_batch_create(ctx, &bc, &slot->b);
+      if (rc != DNALN_OK) return rc;
+      rc = dnaln_hip_batch_reserve(slot->b, s->cfg.max_batch_reads, s->cfg.max_batch_bases, s->cfg.max_read_len, 64);
+      if (rc != DNALN_OK) return rc;
+      // host staging for the largest batch admitted (gather, download), mapped up front
+      const uint64_t nr = s->cfg.max_batch_reads, nb = s->cfg.max_batch_bases;
+      slot->codes.prefault(nb);
+      slot->offs.prefault(nr);
+      slot->lens.prefault(nr);
+      slot->fnv.prefault(nr);
+      slot->rec.prefault(nr);
+      slot->cig.prefault(8 * nr + 64);
+      slot->worker = std::thread([slot] { slot->work(); });
     }
-    // host staging for the largest batch admitted (gather, download), mapped up front
-    const uint64_t nr = s->cfg.max_batch_reads, nb = s->cfg.max_batch_bases;
-    slot->codes.prefault(nb);
-    slot->offs.prefault(nr);
-    slot->lens.prefault(nr);
-    slot->fnv.prefault(nr);
-    slot->rec.prefault(nr);
-    slot->cig.prefault(8 * nr + 64);
-    Slot* raw = slot.get();
-    slot->worker = std::thread([raw] { raw->work(); });
-    s->slots.push_back(std::move(slot));
-  }
-  *out = s.release();
-  return DNALN_OK;
+    *out = s.release();
+    return DNALN_OK;
+  });
 }
 
 void dnaln_serve_destroy(dnaln_server* s) { delete s; }
@@ -418,16 +419,20 @@ int submit_request(dnaln_server* s, uint64_t id, const uint8_t* codes, const uin
 
 int dnaln_serve_submit(dnaln_server* s, uint64_t id, const uint8_t* codes, const uint64_t* offs, const int32_t* lens,
                        const uint32_t* fnv, uint64_t n) {
-  return submit_request(s, id, codes, offs, lens, fnv, n, true);
+  return dnaln_amd::c_guard([&]() -> int {
+    return submit_request(s, id, codes, offs, lens, fnv, n, true);
+  });
 }
 
 int dnaln_serve_progress(dnaln_server* s) {
-  if (!s) return DNALN_E_INVALID;
-  const int fin = s->collect();
-  if (fin < 0) return fin;
-  const int rc = s->admit();
-  if (rc != DNALN_OK) return rc;
-  return fin;
+  return dnaln_amd::c_guard([&]() -> int {
+    if (!s) return DNALN_E_INVALID;
+    const int fin = s->collect();
+    if (fin < 0) return fin;
+    const int rc = s->admit();
+    if (rc != DNALN_OK) return rc;
+    return fin;
+  });
 }
 
 int dnaln_serve_take(dnaln_server* s, uint64_t* id, dnaln_record* out, uint64_t out_cap, uint32_t* cigar_pool,
@@ -459,74 +464,76 @@ int dnaln_serve_run_trace(dnaln_server* s, const uint8_t* codes, const uint64_t*
                           const uint32_t* fnv, uint64_t n_reads, const uint64_t* req_first, const uint64_t* req_count,
                           const double* arrival_s, uint64_t n_req, double* latency_s, dnaln_record* records,
                           uint32_t* cigar_pool, uint64_t cigar_cap, double* wall_s) {
-  if (!s || !req_first || !req_count || !arrival_s || !latency_s ||
-      (n_reads && (!codes || !offs || !lens || !fnv)))
-    return DNALN_E_INVALID;
-  for (uint64_t r = 0; r < n_req; ++r)
-    if (req_first[r] + req_count[r] > n_reads || (r && arrival_s[r] < arrival_s[r - 1])) return DNALN_E_INVALID;
-  // Request offsets relative to each request's first base, for all requests up front: the
-  // requests are queued as views of these arrays and of the caller's, which outlive the replay.
-  std::vector<uint64_t> local_offs;
-  std::vector<uint64_t> local_at(n_req + 1, 0);
-  for (uint64_t r = 0; r < n_req; ++r) local_at[r + 1] = local_at[r] + req_count[r];
-  local_offs.resize(local_at[n_req]);
-  for (uint64_t r = 0; r < n_req; ++r) {
-    const uint64_t a = req_first[r], n = req_count[r], base = n ? offs[a] : 0;
-    for (uint64_t i = 0; i < n; ++i) local_offs[local_at[r] + i] = offs[a + i] - base;
-  }
-  uint64_t next = 0, finished = 0, cig_used = 0;
-  const double t0 = now_s();
-  double t_last = t0;
-  while (finished < n_req) {
-    const double t = now_s() - t0;
-    while (next < n_req && arrival_s[next] <= t) {
-      const uint64_t a = req_first[next], n = req_count[next];
-      const uint64_t base = n ? offs[a] : 0;
-      const int rc = submit_request(s, next, codes + base, local_offs.data() + local_at[next], lens + a, fnv + a, n,
-                                    false);
-      if (rc != DNALN_OK) {
+  return dnaln_amd::c_guard([&]() -> int {
+    if (!s || !req_first || !req_count || !arrival_s || !latency_s ||
+        (n_reads && (!codes || !offs || !lens || !fnv)))
+      return DNALN_E_INVALID;
+    for (uint64_t r = 0; r < n_req; ++r)
+      if (req_first[r] + req_count[r] > n_reads || (r && arrival_s[r] < arrival_s[r - 1])) return DNALN_E_INVALID;
+    // Request offsets relative to each request's first base, for all requests up front: the
+    // requests are queued as views of these arrays and of the caller's, which outlive the replay.
+    std::vector<uint64_t> local_offs;
+    std::vector<uint64_t> local_at(n_req + 1, 0);
+    for (uint64_t r = 0; r < n_req; ++r) local_at[r + 1] = local_at[r] + req_count[r];
+    local_offs.resize(local_at[n_req]);
+    for (uint64_t r = 0; r < n_req; ++r) {
+      const uint64_t a = req_first[r], n = req_count[r], base = n ? offs[a] : 0;
+      for (uint64_t i = 0; i < n; ++i) local_offs[local_at[r] + i] = offs[a + i] - base;
+    }
+    uint64_t next = 0, finished = 0, cig_used = 0;
+    const double t0 = now_s();
+    double t_last = t0;
+    while (finished < n_req) {
+      const double t = now_s() - t0;
+      while (next < n_req && arrival_s[next] <= t) {
+        const uint64_t a = req_first[next], n = req_count[next];
+        const uint64_t base = n ? offs[a] : 0;
+        const int rc = submit_request(s, next, codes + base, local_offs.data() + local_at[next], lens + a, fnv + a, n,
+                                      false);
+        if (rc != DNALN_OK) {
+          s->drop_views();
+          return rc;
+        }
+        ++next;
+      }
+      const int rc = dnaln_serve_progress(s);
+      if (rc < 0) {
         s->drop_views();
         return rc;
       }
-      ++next;
-    }
-    const int rc = dnaln_serve_progress(s);
-    if (rc < 0) {
-      s->drop_views();
-      return rc;
-    }
-    while (!s->done.empty()) {
-      Done& d = s->done.front();
-      const uint64_t r = d.id;
-      latency_s[r] = (d.t_done - t0) - arrival_s[r];
-      t_last = std::max(t_last, d.t_done);
-      if (records) {
-        const uint64_t a = req_first[r];
-        for (size_t i = 0; i < d.rec.size(); ++i) {
-          dnaln_record m = d.rec[i];
-          if (cig_used + m.cigar_len > cigar_cap) {
-            s->drop_views();
-            return DNALN_E_CAPACITY;
+      while (!s->done.empty()) {
+        Done& d = s->done.front();
+        const uint64_t r = d.id;
+        latency_s[r] = (d.t_done - t0) - arrival_s[r];
+        t_last = std::max(t_last, d.t_done);
+        if (records) {
+          const uint64_t a = req_first[r];
+          for (size_t i = 0; i < d.rec.size(); ++i) {
+            dnaln_record m = d.rec[i];
+            if (cig_used + m.cigar_len > cigar_cap) {
+              s->drop_views();
+              return DNALN_E_CAPACITY;
+            }
+            std::copy(d.cig.begin() + m.cigar_off, d.cig.begin() + m.cigar_off + m.cigar_len, cigar_pool + cig_used);
+            m.cigar_off = cig_used;
+            cig_used += m.cigar_len;
+            records[a + i] = m;
           }
-          std::copy(d.cig.begin() + m.cigar_off, d.cig.begin() + m.cigar_off + m.cigar_len, cigar_pool + cig_used);
-          m.cigar_off = cig_used;
-          cig_used += m.cigar_len;
-          records[a + i] = m;
         }
+        s->done.pop_front();
+        ++finished;
       }
-      s->done.pop_front();
-      ++finished;
+      // Nothing running and the next arrival is later: wait for it without spinning hard.
+      if (s->st.inflight_batches == 0 && s->queue.empty() && next < n_req) {
+        const double wait = arrival_s[next] - (now_s() - t0);
+        if (wait > 2e-4) std::this_thread::sleep_for(std::chrono::duration<double>(wait - 1e-4));
+      } else {
+        std::this_thread::yield();
+      }
     }
-    // Nothing running and the next arrival is later: wait for it without spinning hard.
-    if (s->st.inflight_batches == 0 && s->queue.empty() && next < n_req) {
-      const double wait = arrival_s[next] - (now_s() - t0);
-      if (wait > 2e-4) std::this_thread::sleep_for(std::chrono::duration<double>(wait - 1e-4));
-    } else {
-      std::this_thread::yield();
-    }
-  }
-  if (wall_s) *wall_s = t_last - (t0 + (n_req ? arrival_s[0] : 0.0));
-  return DNALN_OK;
+    if (wall_s) *wall_s = t_last - (t0 + (n_req ? arrival_s[0] : 0.0));
+    return DNALN_OK;
+  });
 }
 
 }  // extern "C"

@@ -34,7 +34,7 @@ enum {
   DNALN_E_INVALID = -1,   /* bad argument or parameter outside the validated range */
   DNALN_E_CAPACITY = -2,  /* a caller-provided output buffer is too small */
   DNALN_E_DEVICE = -3,    /* HIP runtime error */
-  DNALN_E_NOMEM = -4,     /* device allocation failed */
+  DNALN_E_NOMEM = -4,     /* device or host allocation failed */
   DNALN_E_INTERNAL = -5,  /* an internal consistency check failed */
   DNALN_E_PENDING = -6,   /* dnaln_hip_batch_poll: still running */
   DNALN_E_IO = -7         /* index file: unreadable, truncated, corrupt, or built from other codes */
