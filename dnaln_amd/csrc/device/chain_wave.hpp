@@ -14,7 +14,7 @@
 //   * extraction walks ends in (f desc, index asc) order, a second bitonic sort of packed
 //     keys; chains go to a list that is sorted (score desc, extraction order asc) the same way.
 //     prev is stored as prev + 1 with the used flag in bit 31, so a walk step is one dependent
-//     load, and ends already used when a block of 64 ends starts are skipped by a ballot;
+//     load, and ends already used when their block of 64 starts are skipped by a ballot;
 //   * candidates: lane x keeps accepted candidate x's diagonal, so the dedupe test is a ballot.
 // Every lane writes the same value wherever all lanes later read (prev during extraction), so
 // each lane reads its own writes. Lookback <= 64 and max_cand <= 64 (checked by the host).
