@@ -112,22 +112,20 @@ __device__ __forceinline__ void chain_wave_read(const ChainArgs& a, uint32_t r, 
       const uint64_t kw = lane < ri ? kblk : kprv;  // anchor j = i - 1 - dist
       const int dist = (ri - 1 - lane) & 63;
       const int reach = a.lookback < i ? a.lookback : i;  // j >= 0 and within the lookback
-      uint32_t cand = 0;
-      if (dist < reach) {
-        const uint32_t hj = (uint32_t)(kw >> 32);
-        const uint32_t dr = hi_i - hj;  // same strand: the strand bits cancel
-        const int dq = qi - key_qpos(kw);
-        const uint32_t t = dr - (uint32_t)dq;
-        const uint32_t dd = (int32_t)t < 0 ? 0u - t : t;
-        const bool alive = ((hi_i ^ hj) >> 31) == 0 && dr <= max_gap;
-        if (alive && dr > 0 && dq > 0 && dd <= (uint32_t)band) {
-          const int m = dq < (int)dr ? dq : (int)dr;
-          const int c = nib ? (dd == (uint32_t)band ? a.cost_at_band : (int)((a.cost_nib >> (4 * (dd & 15))) & 15u))
-                            : cost[dd];
-          const int sc = fw + (m < k ? m : k) - c;
-          cand = (uint32_t)(sc + kBias) << 6 | (uint32_t)(63 - dist);
-        }
-      }
+      // Every lane computes its score and one select keeps it, so a step has no divergent
+      // branch; the byte table is read at min(dd, band), inside the table for every lane.
+      const uint32_t hj = (uint32_t)(kw >> 32);
+      const uint32_t dr = hi_i - hj;  // same strand: the strand bits cancel
+      const int dq = qi - key_qpos(kw);
+      const uint32_t t = dr - (uint32_t)dq;
+      const uint32_t dd = (int32_t)t < 0 ? 0u - t : t;
+      const bool ok = dist < reach && ((hi_i ^ hj) >> 31) == 0 && dr <= max_gap && dr > 0 && dq > 0 &&
+                      dd <= (uint32_t)band;
+      const int m = dq < (int)dr ? dq : (int)dr;
+      const int c = nib ? (dd == (uint32_t)band ? a.cost_at_band : (int)((a.cost_nib >> (4 * (dd & 15))) & 15u))
+                        : cost[dd < (uint32_t)band ? dd : (uint32_t)band];
+      const int sc = fw + (m < k ? m : k) - c;
+      const uint32_t cand = ok ? (uint32_t)(sc + kBias) << 6 | (uint32_t)(63 - dist) : 0u;
       const uint32_t best = wave_max_u32(cand);
       int fi = k, pi = -1;
       if (best) {
