@@ -64,6 +64,63 @@ __device__ __forceinline__ void wave_sort(T* a, int n, int lane) {
   }
 }
 
+// Lane lane ^ mask's value (all lanes active).
+__device__ __forceinline__ uint64_t shfl_xor64(uint64_t x, int mask, int lane) {
+  const int addr = (lane ^ mask) << 2;
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_ds_bpermute(addr, (int)(uint32_t)x);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_ds_bpermute(addr, (int)(uint32_t)(x >> 32));
+  return (uint64_t)hi << 32 | lo;
+}
+
+// The network of wave_sort over 64 * R keys held in registers: key e in v[e / 64] of lane
+// e % 64. A comparator's partner e ^ m sits in slot (e / 64) ^ (m / 64) of lane
+// (e % 64) ^ (m % 64), fetched with ds_bpermute; the lower index keeps the smaller key.
+template <int R>
+__device__ __forceinline__ void reg_sort(uint64_t (&v)[R], int lane) {
+  for (int k = 2; k <= 64 * R; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      const int m = j == (k >> 1) ? k - 1 : j;
+      const int ms = m >> 6, ml = m & 63;
+      uint64_t pv[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        uint64_t src = v[0];
+#pragma unroll
+        for (int q = 1; q < R; ++q) src = (r ^ ms) == q ? v[q] : src;
+        pv[r] = ml ? shfl_xor64(src, ml, lane) : src;
+      }
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const bool lower = j >= 64 ? (r & (j >> 6)) == 0 : (lane & j) == 0;
+        const bool lt = v[r] < pv[r];
+        v[r] = lower == lt ? v[r] : pv[r];
+      }
+    }
+  }
+}
+
+// Ascending sort of a[0..n) through registers (n <= 64 * R), padded with ~0 keys.
+template <int R>
+__device__ __forceinline__ void reg_sort_array(uint64_t* a, int n, int lane) {
+  uint64_t v[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) v[r] = 64 * r + lane < n ? a[64 * r + lane] : ~0ull;
+  reg_sort<R>(v, lane);
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+    if (64 * r + lane < n) a[64 * r + lane] = v[r];
+  wave_sync();
+}
+
+// Sorts a[0..n) ascending: in registers up to 256 keys, else the memory network. No key of
+// the callers is ~0, so the padding stays beyond n.
+__device__ __forceinline__ void sort_keys(uint64_t* a, int n, int lane) {
+  if (n <= 64) reg_sort_array<1>(a, n, lane);
+  else if (n <= 128) reg_sort_array<2>(a, n, lane);
+  else if (n <= 256) reg_sort_array<4>(a, n, lane);
+  else wave_sort(a, n, lane);
+}
+
 // Per-read working arrays (LDS, or global scratch for reads too large for LDS).
 struct WaveChainMem {
   uint64_t* keys;     // n anchors
@@ -81,7 +138,7 @@ __device__ __forceinline__ void chain_wave_read(const ChainArgs& a, uint32_t r, 
     return;
   }
   // 1. anchors in (strand, rpos, qpos) order
-  wave_sort(s.keys, n, lane);
+  sort_keys(s.keys, n, lane);
 
   // 2. DP (src/chain.cpp:21-45). 32-bit arithmetic: rpos < 2^31 and qpos < 2^31, and
   // dr, dq and their difference are only used where 0 < dr <= max_gap and 0 < dq. With
@@ -152,7 +209,7 @@ __device__ __forceinline__ void chain_wave_read(const ChainArgs& a, uint32_t r, 
   // 3. extraction order: f desc, index asc (src/chain.cpp:47-52)
   for (int x = lane; x < n; x += 64) s.order[x] = (uint64_t)(uint32_t)(0x7fffffff - s.f[x]) << 32 | (uint32_t)x;
   wave_sync();
-  wave_sort(s.order, n, lane);
+  sort_keys(s.order, n, lane);
 
   // 4. chains (src/chain.cpp:53-68), lane-uniform walks. Ends are taken 64 at a time: those
   // unused at the block's start are visited in order and checked again before their walk,
@@ -205,7 +262,7 @@ __device__ __forceinline__ void chain_wave_read(const ChainArgs& a, uint32_t r, 
   for (int x = lane; x < m; x += 64)
     s.order[x] = (uint64_t)(uint32_t)(0x7fffffff - s.chains[4 * x]) << 32 | (uint32_t)x;
   wave_sync();
-  wave_sort(s.order, m, lane);
+  sort_keys(s.order, m, lane);
 
   // 6. candidates (src/mapper.cpp:44-62)
   const double threshold = s.chains[4 * (int)(uint32_t)s.order[0]] * a.pri_ratio;
