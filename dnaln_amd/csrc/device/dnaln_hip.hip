@@ -181,9 +181,9 @@ __global__ __launch_bounds__(kSeedBlock) void k_chain(ChainArgs a) {
 // wave kWaveChainLds anchors' keys, sort keys, f, prev and chains. Reads with more
 // anchors, or min_anchors < 2 (more chains than the LDS list holds), work in the global
 // scratch arrays of the chain stage instead. The stage is bound by VALU issue and LDS latency,
-// so occupancy pays: 192 anchors keep a block at the default band under 24 KiB, and six blocks
-// (24 waves) share a CU's 160 KiB.
-constexpr int kWaveChainLds = 192;
+// so occupancy pays: 128 anchors keep a block at the default band near 16 KiB, so a CU's
+// 160 KiB holds the LDS of nine blocks; reads with more anchors use the global arrays.
+constexpr int kWaveChainLds = 128;
 constexpr size_t kWaveChainBytes = (size_t)32 * kWaveChainLds;
 
 size_t wave_chain_lds_bytes(int band) { return (size_t)((band + 4) & ~3) * 4 + 4 * kWaveChainBytes; }
