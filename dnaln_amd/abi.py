@@ -73,13 +73,14 @@ class TuneReport(C.Structure):
                 ("cand_walk", i32 * 16)]
 
 
+CAND_DTYPE = np.dtype([("rev", "<u4"), ("contig", "<i4"), ("diag", "<i8"), ("chain_score", "<i4"), ("reserved", "<i4")])
 TASK_DTYPE = np.dtype([("q_off", "<u8"), ("t_off", "<u8"), ("qlen", "<i4"), ("tlen", "<i4")])
 RESULT_DTYPE = np.dtype([("score", "<i4"), ("qb", "<i4"), ("qe", "<i4"), ("tb", "<i4"), ("te", "<i4"), ("nm", "<i4"),
                          ("cigar_off", "<u8"), ("cigar_len", "<u4"), ("reserved", "<u4")])
 RECORD_DTYPE = np.dtype([("contig", "<i4"), ("rev", "u1"), ("mapq", "u1"), ("reserved", "<u2"), ("pos", "<i8"),
                          ("score", "<i4"), ("nm", "<i4"), ("n_cand", "<i4"), ("cigar_len", "<u4"),
                          ("cigar_off", "<u8")])
-assert TASK_DTYPE.itemsize == 24 and RESULT_DTYPE.itemsize == 40 and RECORD_DTYPE.itemsize == 40
+assert CAND_DTYPE.itemsize == 24 and TASK_DTYPE.itemsize == 24 and RESULT_DTYPE.itemsize == 40 and RECORD_DTYPE.itemsize == 40
 assert C.sizeof(MapParams) == 80 and C.sizeof(BatchConfig) == 56 and C.sizeof(MapStats) == 112
 
 # Every symbol include/dnaln_hip.h declares, with (restype, argtypes).
@@ -111,6 +112,7 @@ HIP_SYMBOLS = {
     "dnaln_hip_context_index": (i32, [vp, vp, vp, vp]),
     "dnaln_hip_host_alloc": (i32, [u64, vp]),
     "dnaln_hip_host_free": (None, [vp]),
+    "dnaln_hip_batch_candidates": (i32, [vp, vp, vp, u64]),
 }
 
 class ServeConfig(C.Structure):

@@ -314,6 +314,15 @@ class Batch:
                "dnaln_hip_batch_download")
         return out, cig, stats
 
+    def candidates(self):
+        """The last run's candidates after seeding and chaining: (n_cand[n], cand[n, max_cand] of
+        abi.CAND_DTYPE), in the reference's order (dnaln_hip_batch_candidates)."""
+        mc = max(int(self.ctx.params.max_cand), 1)
+        ncand = np.zeros(max(self.n, 1), np.int32)
+        cand = np.zeros((max(self.n, 1), mc), abi.CAND_DTYPE)
+        _check(_lib.hip().dnaln_hip_batch_candidates(self.b, _p(ncand), _p(cand), mc), "dnaln_hip_batch_candidates")
+        return ncand[:self.n], cand[:self.n]
+
     FAMILIES = ("seed", "chain", "rank_fill", "traceback_fill", "traceback_walk", "run")
 
     def timings(self) -> tuple[list[float], list[int]]:

@@ -285,7 +285,8 @@ __device__ __forceinline__ void chain_wave_read(const ChainArgs& a, uint32_t r, 
       cd.rev = strand;
       cd.contig = contig_of(a.contig_off, a.n_contigs, (uint32_t)c[1] & 0x7fffffffu);
       cd.diag = diag;
-      cd.wstart = diag - bw;
+      cd.score = c[0];
+      cd.reserved = 0;
       cands[nc] = cd;
     }
     ++nc;

@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -45,6 +46,27 @@ int fail(int code, const std::string& msg) {
     if (e_ != hipSuccess)                                                                     \
       return fail(DNALN_E_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_));         \
   } while (0)
+
+// Entry points run their bodies under guarded(): a C++ exception (a failed host allocation in a
+// std::vector, say) becomes a status and a message instead of crossing the C ABI.
+template <class F>
+int guarded(F&& body) noexcept {
+  const char* what = nullptr;
+  int code = DNALN_E_INTERNAL;
+  try {
+    return body();
+  } catch (const std::bad_alloc&) {
+    code = DNALN_E_NOMEM;
+    what = "host allocation failed";
+  } catch (...) {
+    what = "unexpected C++ exception";
+  }
+  try {
+    g_err = what;
+  } catch (...) {
+  }
+  return code;
+}
 
 #define DN_TRY(expr)          \
   do {                        \
@@ -1016,6 +1038,21 @@ int launch_fused(const FusedShape& fs, const SeedArgs& sa, const ChainArgs& ca, 
   return DNALN_OK;
 }
 
+// The context's chain and candidate parameters (the chain band already clamped to bw).
+ChainSetup chain_setup(const dnaln_hip_ctx& c) {
+  ChainSetup s;
+  s.k = c.k;
+  s.max_gap = c.p.ch.max_gap;
+  s.band = c.p.ch.band;
+  s.lookback = c.p.ch.lookback;
+  s.min_score = c.p.ch.min_score;
+  s.min_anchors = c.p.ch.min_anchors;
+  s.max_cand = std::max(c.p.max_cand, 0);
+  s.bw = c.p.sc.bw;
+  s.pri_ratio = c.p.pri_ratio;
+  return s;
+}
+
 // One chunk of reads [r0, r0 + n) through the whole pipeline (plain form).
 int run_chunk(dnaln_hip_batch* b, uint64_t r0, uint32_t n, bool first_chunk) {
   dnaln_hip_ctx* c = b->ctx;
@@ -1054,19 +1091,7 @@ int run_chunk(dnaln_hip_batch* b, uint64_t r0, uint32_t n, bool first_chunk) {
   sa.w = c->w;
   sa.max_occ = p.max_occ;
   ChainArgs ca{};
-  ca.cost = c->cost.as<int32_t>();
-  pack_gap_cost(ca, c->h_cost.data(), ca.band);
-  ca.k = c->k;
-  ca.max_gap = p.ch.max_gap;
-  ca.band = p.ch.band;
-  ca.lookback = p.ch.lookback;
-  ca.min_score = p.ch.min_score;
-  ca.min_anchors = p.ch.min_anchors;
-  ca.max_cand = max_cand;
-  ca.bw = p.sc.bw;
-  ca.pri_ratio = p.pri_ratio;
-  ca.contig_off = c->coff.as<uint64_t>();
-  ca.n_contigs = c->nc;
+  init_chain_args(ca, chain_setup(*c), c->cost.as<int32_t>(), c->h_cost.data(), c->coff.as<uint64_t>(), c->nc);
   ca.cands = b->cands.as<Cand>();
   ca.n_cand = b->ncand.as<int32_t>();
   const unsigned sgrid = (n + kSeedBlock - 1) / kSeedBlock;
@@ -1352,75 +1377,77 @@ const char* dnaln_hip_last_error(void) { return g_err.c_str(); }
 int dnaln_hip_create(const dnaln_map_params* params, const uint8_t* ref_codes, uint64_t ref_len,
                      const dnaln_contig* contigs, int32_t n_contigs, const dnaln_index_view* index, int32_t device,
                      dnaln_hip_ctx** out) {
-  if (!params || !out || (!ref_codes && ref_len) || n_contigs < 1 || !contigs)
-    return fail(DNALN_E_INVALID, "null argument");
-  DN_TRY(validate(*params));
-  if (index && (index->k != params->k || index->w != params->w))
-    return fail(DNALN_E_INVALID, "index k/w differ from params");
-  if (index && (index->n_buckets == 0 || (index->n_buckets & (index->n_buckets - 1))))
-    return fail(DNALN_E_INVALID, "index bucket count must be a power of two");
-  if (ref_len >= (1ull << 31)) return fail(DNALN_E_INVALID, "reference is too large (limit: 2^31 bp)");
-  std::unique_ptr<dnaln_hip_ctx> c(new dnaln_hip_ctx);
-  c->p = *params;
-  c->p.ch.band = std::min(c->p.ch.band, c->p.sc.bw);  // src/mapper.cpp:9
-  c->device = device;
-  DN_HIP(hipSetDevice(device));
-  DN_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-  c->ref_len = ref_len;
-  c->nc = n_contigs;
-  c->k = params->k;
-  c->w = params->w;
-  // The packed kernel streams aligned dwords and may touch bytes just outside a window: the
-  // reference sits between two guard regions of kSeqGuard bytes (never used as sequence).
-  DN_TRY(upload_reference(c->ref, ref_codes, ref_len, c->stream, &c->ref_base));
-  c->h_coff.resize(n_contigs);
-  c->h_clen.resize(n_contigs);
-  for (int i = 0; i < n_contigs; ++i) {
-    c->h_coff[i] = contigs[i].offset;
-    c->h_clen[i] = contigs[i].len;
-    if (contigs[i].offset + contigs[i].len > ref_len) return fail(DNALN_E_INVALID, "contig outside the reference");
-  }
-  DN_TRY(c->coff.reserve(n_contigs * 8));
-  DN_TRY(c->clen.reserve(n_contigs * 4));
-  DN_HIP(hipMemcpy(c->coff.p, c->h_coff.data(), n_contigs * 8, hipMemcpyHostToDevice));
-  DN_HIP(hipMemcpy(c->clen.p, c->h_clen.data(), n_contigs * 4, hipMemcpyHostToDevice));
-  if (index) {
-    c->ix.n_buckets = index->n_buckets;
-    c->ix.n_pos = index->n_pos;
-    DN_TRY(c->table.reserve(index->n_buckets * sizeof(dnaln_bucket)));
-    DN_HIP(hipMemcpy(c->table.p, index->table, index->n_buckets * sizeof(dnaln_bucket), hipMemcpyHostToDevice));
-    DN_TRY(c->pos.reserve(index->n_pos * 4 + 4));
-    if (index->n_pos) DN_HIP(hipMemcpy(c->pos.p, index->pos, index->n_pos * 4, hipMemcpyHostToDevice));
-  } else {
-    DN_TRY(build_index_dev(c->ref_base, c->h_coff.data(), c->h_clen.data(), n_contigs, c->k, c->w, c->stream, c->table,
-                           c->pos, c->ix));
-  }
-  c->mask = c->ix.n_buckets - 1;
-  c->n_pos = c->ix.n_pos;
-  hipLaunchKernelGGL(k_inline_unique, dim3((unsigned)((c->ix.n_buckets + 255) / 256)), dim3(256), 0, c->stream,
-                     c->table.as<Bucket>(), c->ix.n_buckets, c->pos.as<uint32_t>());
-  DN_HIP(hipGetLastError());
-  DN_HIP(hipStreamSynchronize(c->stream));
-  // Chain gap cost by drift dd (src/chain.cpp:36), evaluated in double exactly as the reference.
-  std::vector<int32_t> cost(c->p.ch.band + 1, 0);
-  for (int dd = 1; dd <= c->p.ch.band; ++dd)
-    cost[dd] = (int)(0.01 * c->k * (double)dd + 0.5 * (63 - __builtin_clzll((uint64_t)dd)));
-  c->h_cost = cost;
-  DN_TRY(c->cost.reserve(cost.size() * 4));
-  DN_HIP(hipMemcpy(c->cost.p, cost.data(), cost.size() * 4, hipMemcpyHostToDevice));
-  // MAPQ by score difference (src/mapper.cpp:212-214).
-  std::vector<uint8_t> lut;
-  for (int d = 0;; ++d) {
-    const double q = 6.02 * d / c->p.sc.match + 0.499;
-    const uint8_t v = (uint8_t)std::min(60.0, q);
-    if (v >= 60) break;
-    lut.push_back(v);
-  }
-  c->lut_len = (int)lut.size();
-  DN_TRY(c->lut.reserve(lut.size() + 1));
-  if (!lut.empty()) DN_HIP(hipMemcpy(c->lut.p, lut.data(), lut.size(), hipMemcpyHostToDevice));
-  *out = c.release();
-  return DNALN_OK;
+  return guarded([&]() -> int {
+    if (!params || !out || (!ref_codes && ref_len) || n_contigs < 1 || !contigs)
+      return fail(DNALN_E_INVALID, "null argument");
+    DN_TRY(validate(*params));
+    if (index && (index->k != params->k || index->w != params->w))
+      return fail(DNALN_E_INVALID, "index k/w differ from params");
+    if (index && (index->n_buckets == 0 || (index->n_buckets & (index->n_buckets - 1))))
+      return fail(DNALN_E_INVALID, "index bucket count must be a power of two");
+    if (ref_len >= (1ull << 31)) return fail(DNALN_E_INVALID, "reference is too large (limit: 2^31 bp)");
+    std::unique_ptr<dnaln_hip_ctx> c(new dnaln_hip_ctx);
+    c->p = *params;
+    c->p.ch.band = std::min(c->p.ch.band, c->p.sc.bw);  // src/mapper.cpp:9
+    c->device = device;
+    DN_HIP(hipSetDevice(device));
+    DN_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    c->ref_len = ref_len;
+    c->nc = n_contigs;
+    c->k = params->k;
+    c->w = params->w;
+    // The packed kernel streams aligned dwords and may touch bytes just outside a window: the
+    // reference sits between two guard regions of kSeqGuard bytes (never used as sequence).
+    DN_TRY(upload_reference(c->ref, ref_codes, ref_len, c->stream, &c->ref_base));
+    c->h_coff.resize(n_contigs);
+    c->h_clen.resize(n_contigs);
+    for (int i = 0; i < n_contigs; ++i) {
+      c->h_coff[i] = contigs[i].offset;
+      c->h_clen[i] = contigs[i].len;
+      if (contigs[i].offset + contigs[i].len > ref_len) return fail(DNALN_E_INVALID, "contig outside the reference");
+    }
+    DN_TRY(c->coff.reserve(n_contigs * 8));
+    DN_TRY(c->clen.reserve(n_contigs * 4));
+    DN_HIP(hipMemcpy(c->coff.p, c->h_coff.data(), n_contigs * 8, hipMemcpyHostToDevice));
+    DN_HIP(hipMemcpy(c->clen.p, c->h_clen.data(), n_contigs * 4, hipMemcpyHostToDevice));
+    if (index) {
+      c->ix.n_buckets = index->n_buckets;
+      c->ix.n_pos = index->n_pos;
+      DN_TRY(c->table.reserve(index->n_buckets * sizeof(dnaln_bucket)));
+      DN_HIP(hipMemcpy(c->table.p, index->table, index->n_buckets * sizeof(dnaln_bucket), hipMemcpyHostToDevice));
+      DN_TRY(c->pos.reserve(index->n_pos * 4 + 4));
+      if (index->n_pos) DN_HIP(hipMemcpy(c->pos.p, index->pos, index->n_pos * 4, hipMemcpyHostToDevice));
+    } else {
+      DN_TRY(build_index_dev(c->ref_base, c->h_coff.data(), c->h_clen.data(), n_contigs, c->k, c->w, c->stream, c->table,
+                             c->pos, c->ix));
+    }
+    c->mask = c->ix.n_buckets - 1;
+    c->n_pos = c->ix.n_pos;
+    hipLaunchKernelGGL(k_inline_unique, dim3((unsigned)((c->ix.n_buckets + 255) / 256)), dim3(256), 0, c->stream,
+                       c->table.as<Bucket>(), c->ix.n_buckets, c->pos.as<uint32_t>());
+    DN_HIP(hipGetLastError());
+    DN_HIP(hipStreamSynchronize(c->stream));
+    // Chain gap cost by drift dd (src/chain.cpp:36), evaluated in double exactly as the reference.
+    std::vector<int32_t> cost(c->p.ch.band + 1, 0);
+    for (int dd = 1; dd <= c->p.ch.band; ++dd)
+      cost[dd] = (int)(0.01 * c->k * (double)dd + 0.5 * (63 - __builtin_clzll((uint64_t)dd)));
+    c->h_cost = cost;
+    DN_TRY(c->cost.reserve(cost.size() * 4));
+    DN_HIP(hipMemcpy(c->cost.p, cost.data(), cost.size() * 4, hipMemcpyHostToDevice));
+    // MAPQ by score difference (src/mapper.cpp:212-214).
+    std::vector<uint8_t> lut;
+    for (int d = 0;; ++d) {
+      const double q = 6.02 * d / c->p.sc.match + 0.499;
+      const uint8_t v = (uint8_t)std::min(60.0, q);
+      if (v >= 60) break;
+      lut.push_back(v);
+    }
+    c->lut_len = (int)lut.size();
+    DN_TRY(c->lut.reserve(lut.size() + 1));
+    if (!lut.empty()) DN_HIP(hipMemcpy(c->lut.p, lut.data(), lut.size(), hipMemcpyHostToDevice));
+    *out = c.release();
+    return DNALN_OK;
+  });
 }
 
 void dnaln_hip_destroy(dnaln_hip_ctx* c) {
@@ -1434,67 +1461,69 @@ void dnaln_hip_destroy(dnaln_hip_ctx* c) {
 int dnaln_hip_align(dnaln_hip_ctx* c, const dnaln_score_params* spp, const dnaln_task* tasks, uint64_t n,
                     const uint8_t* seq_pool, uint64_t pool_len, int traceback, dnaln_result* out,
                     uint32_t* cigar_pool, uint64_t cigar_cap) {
-  if (!c || (n && (!tasks || !out || !seq_pool))) return fail(DNALN_E_INVALID, "null argument");
-  if (n >= (1ull << 31)) return fail(DNALN_E_INVALID, "too many tasks");
-  const dnaln_score_params s = spp ? *spp : c->p.sc;
-  if (s.bw < 1 || s.bw > 1000 || s.match < 1 || s.mismatch < 0 || s.gap_open < 0 || s.gap_ext < 1 || s.ambig < 0)
-    return fail(DNALN_E_INVALID, "score parameters outside the validated range");
-  const ScoreP sp{s.match, s.mismatch, s.gap_open, s.gap_ext, s.ambig, s.bw};
-  DN_HIP(hipSetDevice(c->device));
-  int qmax = 0, qmin = n ? tasks[0].qlen : 0;
-  for (uint64_t i = 0; i < n; ++i) {
-    if (tasks[i].qlen < 0 || tasks[i].tlen < 0 || tasks[i].q_off + (uint64_t)tasks[i].qlen > pool_len ||
-        tasks[i].t_off + (uint64_t)tasks[i].tlen > pool_len)
-      return fail(DNALN_E_INVALID, "task outside the sequence pool");
-    qmax = std::max(qmax, tasks[i].qlen);
-    qmin = std::min(qmin, tasks[i].qlen);
-  }
-  if (n == 0) return DNALN_OK;
-  const int cap = 2 * qmax + 4;
-  hipStream_t st = c->stream;
-  DN_TRY(c->a_pool.reserve(pool_len + 2 * kReadGuard));
-  DN_TRY(c->a_in.reserve(n * sizeof(dnaln_task)));
-  DN_TRY(c->a_tasks.reserve(n * sizeof(SwTask)));
-  DN_TRY(c->a_res.reserve(n * sizeof(SwResult)));
-  if (traceback) DN_TRY(c->a_cig.reserve(n * (uint64_t)cap * 4));
-  DN_HIP(hipMemcpyAsync(c->a_pool.as<uint8_t>() + kReadGuard, seq_pool, pool_len, hipMemcpyHostToDevice, st));
-  DN_HIP(hipMemcpyAsync(c->a_in.p, tasks, n * sizeof(dnaln_task), hipMemcpyHostToDevice, st));
-  hipLaunchKernelGGL(k_pool_tasks, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, c->a_in.as<dnaln_task>(),
-                     (uint64_t)(uintptr_t)(c->a_pool.as<uint8_t>() + kReadGuard), sp.bw, c->a_tasks.as<SwTask>(),
-                     (uint32_t)n);
-  DN_HIP(hipGetLastError());
-  DN_TRY(run_sw(sp, c->a_tasks.as<SwTask>(), (uint32_t)n, qmin, qmax, traceback != 0, c->a_res.as<SwResult>(),
-                c->a_cig.as<uint32_t>(), cap, c->a_sw, st, nullptr));
-  std::vector<SwResult> res(n);
-  DN_HIP(hipMemcpyAsync(res.data(), c->a_res.p, n * sizeof(SwResult), hipMemcpyDeviceToHost, st));
-  std::vector<uint32_t> cig;
-  if (traceback) {
-    cig.resize(n * (uint64_t)cap);
-    DN_HIP(hipMemcpyAsync(cig.data(), c->a_cig.p, n * (uint64_t)cap * 4, hipMemcpyDeviceToHost, st));
-  }
-  DN_HIP(hipStreamSynchronize(st));
-  uint64_t used = 0;
-  for (uint64_t i = 0; i < n; ++i) {
-    const SwResult& r = res[i];
-    if (r.status != 0 || r.n_cigar < 0) return fail(DNALN_E_INTERNAL, "traceback failed on task " + std::to_string(i));
-    dnaln_result& o = out[i];
-    o.score = r.score;
-    o.qb = r.qb;
-    o.qe = r.qe;
-    o.tb = r.tb;
-    o.te = r.te;
-    o.nm = r.nm;
-    o.cigar_off = used;
-    o.cigar_len = (uint32_t)r.n_cigar;
-    o.reserved = 0;
-    if (r.n_cigar > 0) {
-      if (!cigar_pool || used + (uint64_t)r.n_cigar > cigar_cap)
-        return fail(DNALN_E_CAPACITY, "cigar_pool too small");
-      for (int k = 0; k < r.n_cigar; ++k) cigar_pool[used + k] = cig[i * (uint64_t)cap + (r.n_cigar - 1 - k)];
-      used += (uint64_t)r.n_cigar;
+  return guarded([&]() -> int {
+    if (!c || (n && (!tasks || !out || !seq_pool))) return fail(DNALN_E_INVALID, "null argument");
+    if (n >= (1ull << 31)) return fail(DNALN_E_INVALID, "too many tasks");
+    const dnaln_score_params s = spp ? *spp : c->p.sc;
+    if (s.bw < 1 || s.bw > 1000 || s.match < 1 || s.mismatch < 0 || s.gap_open < 0 || s.gap_ext < 1 || s.ambig < 0)
+      return fail(DNALN_E_INVALID, "score parameters outside the validated range");
+    const ScoreP sp{s.match, s.mismatch, s.gap_open, s.gap_ext, s.ambig, s.bw};
+    DN_HIP(hipSetDevice(c->device));
+    int qmax = 0, qmin = n ? tasks[0].qlen : 0;
+    for (uint64_t i = 0; i < n; ++i) {
+      if (tasks[i].qlen < 0 || tasks[i].tlen < 0 || tasks[i].q_off + (uint64_t)tasks[i].qlen > pool_len ||
+          tasks[i].t_off + (uint64_t)tasks[i].tlen > pool_len)
+        return fail(DNALN_E_INVALID, "task outside the sequence pool");
+      qmax = std::max(qmax, tasks[i].qlen);
+      qmin = std::min(qmin, tasks[i].qlen);
     }
-  }
-  return DNALN_OK;
+    if (n == 0) return DNALN_OK;
+    const int cap = 2 * qmax + 4;
+    hipStream_t st = c->stream;
+    DN_TRY(c->a_pool.reserve(pool_len + 2 * kReadGuard));
+    DN_TRY(c->a_in.reserve(n * sizeof(dnaln_task)));
+    DN_TRY(c->a_tasks.reserve(n * sizeof(SwTask)));
+    DN_TRY(c->a_res.reserve(n * sizeof(SwResult)));
+    if (traceback) DN_TRY(c->a_cig.reserve(n * (uint64_t)cap * 4));
+    DN_HIP(hipMemcpyAsync(c->a_pool.as<uint8_t>() + kReadGuard, seq_pool, pool_len, hipMemcpyHostToDevice, st));
+    DN_HIP(hipMemcpyAsync(c->a_in.p, tasks, n * sizeof(dnaln_task), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_pool_tasks, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, c->a_in.as<dnaln_task>(),
+                       (uint64_t)(uintptr_t)(c->a_pool.as<uint8_t>() + kReadGuard), sp.bw, c->a_tasks.as<SwTask>(),
+                       (uint32_t)n);
+    DN_HIP(hipGetLastError());
+    DN_TRY(run_sw(sp, c->a_tasks.as<SwTask>(), (uint32_t)n, qmin, qmax, traceback != 0, c->a_res.as<SwResult>(),
+                  c->a_cig.as<uint32_t>(), cap, c->a_sw, st, nullptr));
+    std::vector<SwResult> res(n);
+    DN_HIP(hipMemcpyAsync(res.data(), c->a_res.p, n * sizeof(SwResult), hipMemcpyDeviceToHost, st));
+    std::vector<uint32_t> cig;
+    if (traceback) {
+      cig.resize(n * (uint64_t)cap);
+      DN_HIP(hipMemcpyAsync(cig.data(), c->a_cig.p, n * (uint64_t)cap * 4, hipMemcpyDeviceToHost, st));
+    }
+    DN_HIP(hipStreamSynchronize(st));
+    uint64_t used = 0;
+    for (uint64_t i = 0; i < n; ++i) {
+      const SwResult& r = res[i];
+      if (r.status != 0 || r.n_cigar < 0) return fail(DNALN_E_INTERNAL, "traceback failed on task " + std::to_string(i));
+      dnaln_result& o = out[i];
+      o.score = r.score;
+      o.qb = r.qb;
+      o.qe = r.qe;
+      o.tb = r.tb;
+      o.te = r.te;
+      o.nm = r.nm;
+      o.cigar_off = used;
+      o.cigar_len = (uint32_t)r.n_cigar;
+      o.reserved = 0;
+      if (r.n_cigar > 0) {
+        if (!cigar_pool || used + (uint64_t)r.n_cigar > cigar_cap)
+          return fail(DNALN_E_CAPACITY, "cigar_pool too small");
+        for (int k = 0; k < r.n_cigar; ++k) cigar_pool[used + k] = cig[i * (uint64_t)cap + (r.n_cigar - 1 - k)];
+        used += (uint64_t)r.n_cigar;
+      }
+    }
+    return DNALN_OK;
+  });
 }
 
 void dnaln_hip_default_batch_config(dnaln_batch_config* cfg) {
@@ -1510,28 +1539,30 @@ void dnaln_hip_default_batch_config(dnaln_batch_config* cfg) {
 }
 
 int dnaln_hip_batch_create(dnaln_hip_ctx* c, const dnaln_batch_config* cfg, dnaln_hip_batch** out) {
-  if (!c || !out) return fail(DNALN_E_INVALID, "null argument");
-  std::unique_ptr<dnaln_hip_batch> b(new dnaln_hip_batch);
-  b->ctx = c;
-  if (cfg) b->cfg = *cfg;
-  else dnaln_hip_default_batch_config(&b->cfg);
-  if (b->cfg.chunk_reads == 0) b->cfg.chunk_reads = 65536;
-  DN_HIP(hipSetDevice(c->device));
-  b->sw.dir_budget = b->cfg.dir_budget ? std::max<uint64_t>(b->cfg.dir_budget, 1 << 20) : default_dir_budget();
-  if (b->cfg.walk_kind < 0 || b->cfg.walk_kind > 1) return fail(DNALN_E_INVALID, "walk_kind must be 0 or 1");
-  b->sw.walk_kind = b->cfg.walk_kind;
-  DN_HIP(hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking));
-  DN_HIP(hipEventCreate(&b->done));
-  DN_TRY(b->err.reserve(64));
-  if (b->cfg.max_reads) {
-    DN_TRY(b->offs.reserve(b->cfg.max_reads * 8));
-    DN_TRY(b->lens.reserve(b->cfg.max_reads * 4));
-    DN_TRY(b->fnv.reserve(b->cfg.max_reads * 4));
-    DN_TRY(b->rec.reserve(b->cfg.max_reads * sizeof(dnaln_record)));
-  }
-  if (b->cfg.max_bases) DN_TRY(b->codes.reserve(b->cfg.max_bases));
-  *out = b.release();
-  return DNALN_OK;
+  return guarded([&]() -> int {
+    if (!c || !out) return fail(DNALN_E_INVALID, "null argument");
+    std::unique_ptr<dnaln_hip_batch> b(new dnaln_hip_batch);
+    b->ctx = c;
+    if (cfg) b->cfg = *cfg;
+    else dnaln_hip_default_batch_config(&b->cfg);
+    if (b->cfg.chunk_reads == 0) b->cfg.chunk_reads = 65536;
+    DN_HIP(hipSetDevice(c->device));
+    b->sw.dir_budget = b->cfg.dir_budget ? std::max<uint64_t>(b->cfg.dir_budget, 1 << 20) : default_dir_budget();
+    if (b->cfg.walk_kind < 0 || b->cfg.walk_kind > 1) return fail(DNALN_E_INVALID, "walk_kind must be 0 or 1");
+    b->sw.walk_kind = b->cfg.walk_kind;
+    DN_HIP(hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking));
+    DN_HIP(hipEventCreate(&b->done));
+    DN_TRY(b->err.reserve(64));
+    if (b->cfg.max_reads) {
+      DN_TRY(b->offs.reserve(b->cfg.max_reads * 8));
+      DN_TRY(b->lens.reserve(b->cfg.max_reads * 4));
+      DN_TRY(b->fnv.reserve(b->cfg.max_reads * 4));
+      DN_TRY(b->rec.reserve(b->cfg.max_reads * sizeof(dnaln_record)));
+    }
+    if (b->cfg.max_bases) DN_TRY(b->codes.reserve(b->cfg.max_bases));
+    *out = b.release();
+    return DNALN_OK;
+  });
 }
 
 void dnaln_hip_batch_destroy(dnaln_hip_batch* b) {
@@ -1565,195 +1596,229 @@ static void set_splits(dnaln_hip_batch* b, int want0, int want1) {
 
 int dnaln_hip_batch_upload(dnaln_hip_batch* b, const uint8_t* codes, const uint64_t* offs, const int32_t* lens,
                            const uint32_t* fnv, uint64_t n) {
-  if (!b || (n && (!codes || !offs || !lens || !fnv))) return fail(DNALN_E_INVALID, "null argument");
-  if (n >= (1ull << 31)) return fail(DNALN_E_INVALID, "too many reads in one batch");
-  DN_HIP(hipSetDevice(b->ctx->device));
-  b->last_rc = DNALN_OK;
-  uint64_t bases = 0;
-  int minlen = n ? lens[0] : 0, maxlen = 0;
-  for (uint64_t i = 0; i < n; ++i) {
-    if (lens[i] < 0) return fail(DNALN_E_INVALID, "negative read length");
-    bases = std::max(bases, offs[i] + (uint64_t)lens[i]);
-    minlen = std::min(minlen, lens[i]);
-    maxlen = std::max(maxlen, lens[i]);
-  }
-  DN_HIP(hipStreamSynchronize(b->st));
-  DN_TRY(b->codes.reserve(bases + 2 * kReadGuard));
-  DN_TRY(b->offs.reserve(n * 8 + 8));
-  DN_TRY(b->lens.reserve(n * 4 + 4));
-  DN_TRY(b->fnv.reserve(n * 4 + 4));
-  DN_TRY(b->rec.reserve(n * sizeof(dnaln_record) + sizeof(dnaln_record)));
-  if (bases) DN_HIP(hipMemcpyAsync(b->codes.as<uint8_t>() + kReadGuard, codes, bases, hipMemcpyHostToDevice, b->st));
-  if (n) {
-    DN_HIP(hipMemcpyAsync(b->offs.p, offs, n * 8, hipMemcpyHostToDevice, b->st));
-    DN_HIP(hipMemcpyAsync(b->lens.p, lens, n * 4, hipMemcpyHostToDevice, b->st));
-    DN_HIP(hipMemcpyAsync(b->fnv.p, fnv, n * 4, hipMemcpyHostToDevice, b->st));
-  }
-  b->n = n;
-  b->bases = bases;
-  b->minlen = minlen;
-  b->maxlen = maxlen;
-  b->lens_h.assign(lens, lens + n);
-  set_splits(b, b->cfg.len_split[0], b->cfg.len_split[1]);
-  return DNALN_OK;
+  return guarded([&]() -> int {
+    if (!b || (n && (!codes || !offs || !lens || !fnv))) return fail(DNALN_E_INVALID, "null argument");
+    if (n >= (1ull << 31)) return fail(DNALN_E_INVALID, "too many reads in one batch");
+    DN_HIP(hipSetDevice(b->ctx->device));
+    b->last_rc = DNALN_OK;
+    uint64_t bases = 0;
+    int minlen = n ? lens[0] : 0, maxlen = 0;
+    for (uint64_t i = 0; i < n; ++i) {
+      if (lens[i] < 0) return fail(DNALN_E_INVALID, "negative read length");
+      bases = std::max(bases, offs[i] + (uint64_t)lens[i]);
+      minlen = std::min(minlen, lens[i]);
+      maxlen = std::max(maxlen, lens[i]);
+    }
+    DN_HIP(hipStreamSynchronize(b->st));
+    DN_TRY(b->codes.reserve(bases + 2 * kReadGuard));
+    DN_TRY(b->offs.reserve(n * 8 + 8));
+    DN_TRY(b->lens.reserve(n * 4 + 4));
+    DN_TRY(b->fnv.reserve(n * 4 + 4));
+    DN_TRY(b->rec.reserve(n * sizeof(dnaln_record) + sizeof(dnaln_record)));
+    if (bases) DN_HIP(hipMemcpyAsync(b->codes.as<uint8_t>() + kReadGuard, codes, bases, hipMemcpyHostToDevice, b->st));
+    if (n) {
+      DN_HIP(hipMemcpyAsync(b->offs.p, offs, n * 8, hipMemcpyHostToDevice, b->st));
+      DN_HIP(hipMemcpyAsync(b->lens.p, lens, n * 4, hipMemcpyHostToDevice, b->st));
+      DN_HIP(hipMemcpyAsync(b->fnv.p, fnv, n * 4, hipMemcpyHostToDevice, b->st));
+    }
+    b->n = n;
+    b->bases = bases;
+    b->minlen = minlen;
+    b->maxlen = maxlen;
+    b->lens_h.assign(lens, lens + n);
+    set_splits(b, b->cfg.len_split[0], b->cfg.len_split[1]);
+    return DNALN_OK;
+  });
 }
 
 int dnaln_hip_batch_run(dnaln_hip_batch* b) {
-  if (!b) return fail(DNALN_E_INVALID, "null batch");
-  if (b->cfg.tune && !b->tuned && b->cfg.mode == DNALN_MODE_BATCHED && b->n) DN_TRY(dnaln_hip_batch_tune(b, nullptr));
-  dnaln_hip_ctx* c = b->ctx;
-  DN_HIP(hipSetDevice(c->device));
-  std::memset(&b->stats, 0, sizeof b->stats);
-  b->pool_used = 0;
-  b->n_spans = 0;
-  for (auto& l : b->launches) l = 0;
-  DN_HIP(hipMemsetAsync(b->err.p, 0, 4, b->st));
-  size_t run_span = 0;
-  DN_TRY(b->span_begin(5, run_span));
-  const uint64_t chunk = b->cfg.mode == DNALN_MODE_PLAIN ? b->cfg.chunk_reads : std::max<uint64_t>(b->n, 1);
-  b->last_rc = DNALN_OK;
-  for (uint64_t r0 = 0; r0 < b->n || (r0 == 0 && b->n == 0); r0 += chunk) {
-    const uint32_t m = (uint32_t)std::min<uint64_t>(chunk, b->n - r0);
-    const int rc = run_chunk(b, r0, m, r0 == 0);
-    if (rc != DNALN_OK) {
-      (void)hipStreamSynchronize(b->st);
-      b->last_rc = rc;
-      b->pending = false;
-      return rc;
+  return guarded([&]() -> int {
+    if (!b) return fail(DNALN_E_INVALID, "null batch");
+    if (b->cfg.tune && !b->tuned && b->cfg.mode == DNALN_MODE_BATCHED && b->n) DN_TRY(dnaln_hip_batch_tune(b, nullptr));
+    dnaln_hip_ctx* c = b->ctx;
+    DN_HIP(hipSetDevice(c->device));
+    std::memset(&b->stats, 0, sizeof b->stats);
+    b->pool_used = 0;
+    b->n_spans = 0;
+    for (auto& l : b->launches) l = 0;
+    DN_HIP(hipMemsetAsync(b->err.p, 0, 4, b->st));
+    size_t run_span = 0;
+    DN_TRY(b->span_begin(5, run_span));
+    const uint64_t chunk = b->cfg.mode == DNALN_MODE_PLAIN ? b->cfg.chunk_reads : std::max<uint64_t>(b->n, 1);
+    b->last_rc = DNALN_OK;
+    for (uint64_t r0 = 0; r0 < b->n || (r0 == 0 && b->n == 0); r0 += chunk) {
+      const uint32_t m = (uint32_t)std::min<uint64_t>(chunk, b->n - r0);
+      const int rc = run_chunk(b, r0, m, r0 == 0);
+      if (rc != DNALN_OK) {
+        (void)hipStreamSynchronize(b->st);
+        b->last_rc = rc;
+        b->pending = false;
+        return rc;
+      }
+      if (b->n == 0) break;
     }
-    if (b->n == 0) break;
-  }
-  DN_TRY(b->span_end(run_span));
-  DN_HIP(hipEventRecord(b->done, b->st));
-  b->stats.reads = b->n;
-  b->pending = true;
-  return DNALN_OK;
+    DN_TRY(b->span_end(run_span));
+    DN_HIP(hipEventRecord(b->done, b->st));
+    b->stats.reads = b->n;
+    b->pending = true;
+    return DNALN_OK;
+  });
 }
 
 int dnaln_hip_batch_tune(dnaln_hip_batch* b, dnaln_tune_report* rep) {
-  if (!b) return fail(DNALN_E_INVALID, "null batch");
-  if (b->cfg.mode != DNALN_MODE_BATCHED) return fail(DNALN_E_INVALID, "the autotuner applies to the batched form");
-  b->tuned = true;
-  dnaln_tune_report r{};
-  // One candidate: apply, run twice (the first run grows the workspace), keep the second time.
-  auto trial = [&](int s0, int s1, uint64_t budget, int walk, bool& skipped) -> int {
-    set_splits(b, s0, s1);
-    b->sw.dir_budget = budget;
-    b->sw.walk_kind = walk;
-    const int e0 = b->n_short > 0 ? b->split[0] : 0, e1 = b->n_mid > b->n_short ? b->split[1] : 0;
-    skipped = false;
-    for (int x = 0; x < r.n_tried; ++x)
-      if (r.cand_split[x][0] == e0 && r.cand_split[x][1] == e1 && r.cand_budget[x] == budget && r.cand_walk[x] == walk)
+  return guarded([&]() -> int {
+    if (!b) return fail(DNALN_E_INVALID, "null batch");
+    if (b->cfg.mode != DNALN_MODE_BATCHED) return fail(DNALN_E_INVALID, "the autotuner applies to the batched form");
+    b->tuned = true;
+    dnaln_tune_report r{};
+    // One candidate: apply, run twice (the first run grows the workspace), keep the second time.
+    auto trial = [&](int s0, int s1, uint64_t budget, int walk, bool& skipped) -> int {
+      set_splits(b, s0, s1);
+      b->sw.dir_budget = budget;
+      b->sw.walk_kind = walk;
+      const int e0 = b->n_short > 0 ? b->split[0] : 0, e1 = b->n_mid > b->n_short ? b->split[1] : 0;
+      skipped = false;
+      for (int x = 0; x < r.n_tried; ++x)
+        if (r.cand_split[x][0] == e0 && r.cand_split[x][1] == e1 && r.cand_budget[x] == budget && r.cand_walk[x] == walk)
+          skipped = true;
+      if (skipped || r.n_tried >= 16) {
         skipped = true;
-    if (skipped || r.n_tried >= 16) {
-      skipped = true;
+        return DNALN_OK;
+      }
+      double ms = 0;
+      for (int rep_i = 0; rep_i < 2; ++rep_i) {
+        DN_TRY(dnaln_hip_batch_run(b));
+        DN_TRY(dnaln_hip_batch_poll(b, 1));
+        ms = b->ms[5];
+      }
+      const int t = r.n_tried++;
+      r.ms[t] = ms;
+      r.cand_split[t][0] = e0;
+      r.cand_split[t][1] = e1;
+      r.cand_budget[t] = budget;
+      r.cand_walk[t] = walk;
       return DNALN_OK;
-    }
-    double ms = 0;
-    for (int rep_i = 0; rep_i < 2; ++rep_i) {
-      DN_TRY(dnaln_hip_batch_run(b));
-      DN_TRY(dnaln_hip_batch_poll(b, 1));
-      ms = b->ms[5];
-    }
-    const int t = r.n_tried++;
-    r.ms[t] = ms;
-    r.cand_split[t][0] = e0;
-    r.cand_split[t][1] = e1;
-    r.cand_budget[t] = budget;
-    r.cand_walk[t] = walk;
+    };
+    const uint64_t big = default_dir_budget();
+    std::vector<uint64_t> budgets{big};
+    if (big != (1ull << 30)) budgets.push_back(1ull << 30);
+    bool skipped = false;
+    // Stage 1: read-length bucketing x direction budget, with the per-thread walk.
+    for (const auto& sp : {std::make_pair(0, 0), std::make_pair(0, -1), std::make_pair(-1, 0), std::make_pair(-1, -1)})
+      for (const uint64_t bud : budgets) DN_TRY(trial(sp.first, sp.second, bud, 0, skipped));
+    auto best_of = [&]() {
+      int bi = 0;
+      for (int x = 1; x < r.n_tried; ++x)
+        if (r.ms[x] < r.ms[bi]) bi = x;
+      return bi;
+    };
+    // Stage 2: the other walk on the best setting.
+    int bi = best_of();
+    DN_TRY(trial(r.cand_split[bi][0] > 0 ? r.cand_split[bi][0] : -1, r.cand_split[bi][1] > 0 ? r.cand_split[bi][1] : -1,
+                 r.cand_budget[bi], 1, skipped));
+    bi = best_of();
+    r.chosen = bi;
+    r.len_split[0] = r.cand_split[bi][0];
+    r.len_split[1] = r.cand_split[bi][1];
+    r.dir_budget = r.cand_budget[bi];
+    r.walk_kind = r.cand_walk[bi];
+    b->cfg.len_split[0] = r.len_split[0] > 0 ? r.len_split[0] : -1;
+    b->cfg.len_split[1] = r.len_split[1] > 0 ? r.len_split[1] : -1;
+    b->cfg.dir_budget = r.dir_budget;
+    b->cfg.walk_kind = r.walk_kind;
+    set_splits(b, b->cfg.len_split[0], b->cfg.len_split[1]);
+    b->sw.dir_budget = r.dir_budget;
+    b->sw.walk_kind = r.walk_kind;
+    if (rep) *rep = r;
     return DNALN_OK;
-  };
-  const uint64_t big = default_dir_budget();
-  std::vector<uint64_t> budgets{big};
-  if (big != (1ull << 30)) budgets.push_back(1ull << 30);
-  bool skipped = false;
-  // Stage 1: read-length bucketing x direction budget, with the per-thread walk.
-  for (const auto& sp : {std::make_pair(0, 0), std::make_pair(0, -1), std::make_pair(-1, 0), std::make_pair(-1, -1)})
-    for (const uint64_t bud : budgets) DN_TRY(trial(sp.first, sp.second, bud, 0, skipped));
-  auto best_of = [&]() {
-    int bi = 0;
-    for (int x = 1; x < r.n_tried; ++x)
-      if (r.ms[x] < r.ms[bi]) bi = x;
-    return bi;
-  };
-  // Stage 2: the other walk on the best setting.
-  int bi = best_of();
-  DN_TRY(trial(r.cand_split[bi][0] > 0 ? r.cand_split[bi][0] : -1, r.cand_split[bi][1] > 0 ? r.cand_split[bi][1] : -1,
-               r.cand_budget[bi], 1, skipped));
-  bi = best_of();
-  r.chosen = bi;
-  r.len_split[0] = r.cand_split[bi][0];
-  r.len_split[1] = r.cand_split[bi][1];
-  r.dir_budget = r.cand_budget[bi];
-  r.walk_kind = r.cand_walk[bi];
-  b->cfg.len_split[0] = r.len_split[0] > 0 ? r.len_split[0] : -1;
-  b->cfg.len_split[1] = r.len_split[1] > 0 ? r.len_split[1] : -1;
-  b->cfg.dir_budget = r.dir_budget;
-  b->cfg.walk_kind = r.walk_kind;
-  set_splits(b, b->cfg.len_split[0], b->cfg.len_split[1]);
-  b->sw.dir_budget = r.dir_budget;
-  b->sw.walk_kind = r.walk_kind;
-  if (rep) *rep = r;
-  return DNALN_OK;
+  });
 }
 
 int dnaln_hip_batch_poll(dnaln_hip_batch* b, int wait) {
-  if (!b) return fail(DNALN_E_INVALID, "null batch");
-  if (b->last_rc != DNALN_OK) return b->last_rc;
-  if (!b->pending) return DNALN_OK;
-  if (!wait) {
-    const hipError_t q = hipEventQuery(b->done);
-    if (q == hipErrorNotReady) return DNALN_E_PENDING;
-    if (q != hipSuccess) return fail(DNALN_E_DEVICE, hipGetErrorString(q));
-  }
-  DN_HIP(hipEventSynchronize(b->done));
-  int err = 0;
-  DN_HIP(hipMemcpy(&err, b->err.p, 4, hipMemcpyDeviceToHost));
-  b->pending = false;
-  for (auto& m : b->ms) m = 0;
-  for (size_t i = 0; i < b->n_spans; ++i) {
-    float t = 0;
-    DN_HIP(hipEventElapsedTime(&t, b->spans[i].a, b->spans[i].b));
-    b->ms[b->spans[i].family] += t;
-  }
-  if (err) {
-    b->last_rc = DNALN_E_INTERNAL;
-    return fail(DNALN_E_INTERNAL, err & 1 ? "CIGAR slot overflow" : "inconsistent traceback");
-  }
-  return DNALN_OK;
+  return guarded([&]() -> int {
+    if (!b) return fail(DNALN_E_INVALID, "null batch");
+    if (b->last_rc != DNALN_OK) return b->last_rc;
+    if (!b->pending) return DNALN_OK;
+    if (!wait) {
+      const hipError_t q = hipEventQuery(b->done);
+      if (q == hipErrorNotReady) return DNALN_E_PENDING;
+      if (q != hipSuccess) return fail(DNALN_E_DEVICE, hipGetErrorString(q));
+    }
+    DN_HIP(hipEventSynchronize(b->done));
+    int err = 0;
+    DN_HIP(hipMemcpy(&err, b->err.p, 4, hipMemcpyDeviceToHost));
+    b->pending = false;
+    for (auto& m : b->ms) m = 0;
+    for (size_t i = 0; i < b->n_spans; ++i) {
+      float t = 0;
+      DN_HIP(hipEventElapsedTime(&t, b->spans[i].a, b->spans[i].b));
+      b->ms[b->spans[i].family] += t;
+    }
+    if (err) {
+      b->last_rc = DNALN_E_INTERNAL;
+      return fail(DNALN_E_INTERNAL, err & 1 ? "CIGAR slot overflow" : "inconsistent traceback");
+    }
+    return DNALN_OK;
+  });
 }
 
 int dnaln_hip_batch_download(dnaln_hip_batch* b, dnaln_record* out, uint32_t* cigar_pool, uint64_t cigar_cap,
                              dnaln_map_stats* stats) {
-  if (!b) return fail(DNALN_E_INVALID, "null batch");
-  DN_TRY(dnaln_hip_batch_poll(b, 1));
-  if (b->pool_used > cigar_cap) return fail(DNALN_E_CAPACITY, "cigar_pool too small");
-  if (b->n && !out) return fail(DNALN_E_INVALID, "null output");
-  if (b->n) DN_HIP(hipMemcpy(out, b->rec.p, b->n * sizeof(dnaln_record), hipMemcpyDeviceToHost));
-  if (b->pool_used) DN_HIP(hipMemcpy(cigar_pool, b->pool.p, b->pool_used * 4, hipMemcpyDeviceToHost));
-  if (stats) {
-    // Band cells as the reference counts them (src/mapper.cpp:139, 186): every candidate of a
-    // read with two or more in the rank pass, and the chosen one in the traceback pass.
-    dnaln_map_stats s = b->stats;
-    const uint64_t W = 2 * (uint64_t)b->ctx->p.sc.bw + 1;
-    uint64_t mapped = 0, cand = 0, cells = 0;
-    for (uint64_t i = 0; i < b->n; ++i) {
-      const uint64_t nc = (uint64_t)std::max(out[i].n_cand, 0), len = (uint64_t)b->lens_h[i];
-      mapped += out[i].contig >= 0;
-      cand += nc;
-      cells += ((nc >= 2 ? nc : 0) + (nc >= 1 ? 1 : 0)) * len * W;
+  return guarded([&]() -> int {
+    if (!b) return fail(DNALN_E_INVALID, "null batch");
+    DN_TRY(dnaln_hip_batch_poll(b, 1));
+    if (b->pool_used > cigar_cap) return fail(DNALN_E_CAPACITY, "cigar_pool too small");
+    if (b->n && !out) return fail(DNALN_E_INVALID, "null output");
+    if (b->n) DN_HIP(hipMemcpy(out, b->rec.p, b->n * sizeof(dnaln_record), hipMemcpyDeviceToHost));
+    if (b->pool_used) DN_HIP(hipMemcpy(cigar_pool, b->pool.p, b->pool_used * 4, hipMemcpyDeviceToHost));
+    if (stats) {
+      // Band cells as the reference counts them (src/mapper.cpp:139, 186): every candidate of a
+      // read with two or more in the rank pass, and the chosen one in the traceback pass.
+      dnaln_map_stats s = b->stats;
+      const uint64_t W = 2 * (uint64_t)b->ctx->p.sc.bw + 1;
+      uint64_t mapped = 0, cand = 0, cells = 0;
+      for (uint64_t i = 0; i < b->n; ++i) {
+        const uint64_t nc = (uint64_t)std::max(out[i].n_cand, 0), len = (uint64_t)b->lens_h[i];
+        mapped += out[i].contig >= 0;
+        cand += nc;
+        cells += ((nc >= 2 ? nc : 0) + (nc >= 1 ? 1 : 0)) * len * W;
+      }
+      s.candidates = cand;
+      s.mapped = mapped;
+      s.cells = cells;
+      s.ms_seed = b->ms[0];
+      s.ms_chain = b->ms[1];
+      s.ms_rank = b->ms[2];
+      s.ms_traceback = b->ms[3] + b->ms[4];
+      s.ms_total = b->ms[5];
+      *stats = s;
     }
-    s.candidates = cand;
-    s.mapped = mapped;
-    s.cells = cells;
-    s.ms_seed = b->ms[0];
-    s.ms_chain = b->ms[1];
-    s.ms_rank = b->ms[2];
-    s.ms_traceback = b->ms[3] + b->ms[4];
-    s.ms_total = b->ms[5];
-    *stats = s;
-  }
-  return DNALN_OK;
+    return DNALN_OK;
+  });
+}
+
+int dnaln_hip_batch_candidates(dnaln_hip_batch* b, int32_t* n_cand, dnaln_candidate* cands, uint64_t per_read) {
+  return guarded([&]() -> int {
+    static_assert(sizeof(Cand) == sizeof(dnaln_candidate), "candidate layout");
+    if (!b || (b->n && (!n_cand || !cands))) return fail(DNALN_E_INVALID, "null argument");
+    const uint64_t mc = (uint64_t)std::max(b->ctx->p.max_cand, 0);
+    if (per_read < mc) return fail(DNALN_E_CAPACITY, "per_read is below max_cand");
+    if (b->cfg.mode == DNALN_MODE_PLAIN && b->n > b->cfg.chunk_reads)
+      return fail(DNALN_E_INVALID, "the plain form keeps the candidates of one chunk only");
+    DN_TRY(dnaln_hip_batch_poll(b, 1));
+    if (b->n == 0) return DNALN_OK;
+    DN_HIP(hipSetDevice(b->ctx->device));
+    DN_HIP(hipMemcpy(n_cand, b->ncand.p, b->n * 4, hipMemcpyDeviceToHost));
+    std::vector<Cand> h(b->n * std::max<uint64_t>(mc, 1));
+    if (mc) DN_HIP(hipMemcpy(h.data(), b->cands.p, b->n * mc * sizeof(Cand), hipMemcpyDeviceToHost));
+    for (uint64_t i = 0; i < b->n; ++i)
+      for (uint64_t c = 0; c < per_read; ++c) {
+        dnaln_candidate& o = cands[i * per_read + c];
+        if (c < mc && (int64_t)c < n_cand[i]) std::memcpy(&o, &h[i * mc + c], sizeof o);
+        else std::memset(&o, 0, sizeof o);
+      }
+    return DNALN_OK;
+  });
 }
 
 uint64_t dnaln_hip_batch_device_bytes(dnaln_hip_ctx* c, const dnaln_batch_config* cfg) {
@@ -1765,110 +1830,118 @@ uint64_t dnaln_hip_batch_device_bytes(dnaln_hip_ctx* c, const dnaln_batch_config
 }
 
 int dnaln_hip_batch_timings(dnaln_hip_batch* b, double* ms, uint64_t* launches) {
-  static_assert(kFamilies == DNALN_TIMING_FAMILIES, "timing families");
-  if (!b || !ms) return fail(DNALN_E_INVALID, "null argument");
-  for (int i = 0; i < kFamilies; ++i) ms[i] = b->ms[i];
-  if (launches)
-    for (int i = 0; i < kFamilies; ++i) launches[i] = b->launches[i];
-  return DNALN_OK;
+  return guarded([&]() -> int {
+    static_assert(kFamilies == DNALN_TIMING_FAMILIES, "timing families");
+    if (!b || !ms) return fail(DNALN_E_INVALID, "null argument");
+    for (int i = 0; i < kFamilies; ++i) ms[i] = b->ms[i];
+    if (launches)
+      for (int i = 0; i < kFamilies; ++i) launches[i] = b->launches[i];
+    return DNALN_OK;
+  });
 }
 
 int dnaln_hip_batch_reserve(dnaln_hip_batch* b, uint64_t n, uint64_t bases, int32_t maxlen,
                             uint32_t anchors_per_read) {
-  if (!b || maxlen < 0) return fail(DNALN_E_INVALID, "bad argument");
-  dnaln_hip_ctx* c = b->ctx;
-  DN_HIP(hipSetDevice(c->device));
-  const uint64_t mc = (uint64_t)std::max(c->p.max_cand, 1);
-  const uint64_t cap = 2 * (uint64_t)maxlen + 4;
-  const uint64_t na = n * anchors_per_read;
-  const uint64_t tasks = n * mc;
-  // inputs and outputs
-  DN_TRY(b->codes.reserve(bases + 2 * kReadGuard));
-  DN_TRY(b->offs.reserve(n * 8 + 8));
-  DN_TRY(b->lens.reserve(n * 4 + 4));
-  DN_TRY(b->fnv.reserve(n * 4 + 4));
-  DN_TRY(b->rec.reserve(n * sizeof(dnaln_record) + sizeof(dnaln_record)));
-  DN_TRY(b->pool.reserve(n * (cap + 2) * 4 + 4));
-  // seeds and chains
-  DN_TRY(b->cands.reserve(n * mc * sizeof(Cand)));
-  DN_TRY(b->ncand.reserve(n * 4 + 4));
-  DN_TRY(b->lids.reserve(n * 4));
-  DN_TRY(b->lkeys.reserve(n * 4));
-  DN_TRY(b->lorder.reserve(n * 4));
-  DN_TRY(b->ovf.reserve(n * 4 + 8));
-  DN_TRY(b->ovf2.reserve(n * 4 + 8));
-  {
-    FusedShape fs;
-    fs.maxmin = 64;
-    DN_TRY(b->fmins.reserve(fused_mins_words(fs, n) * 8));
-  }
-  DN_TRY(b->acnt.reserve(n * 4 + 4));
-  DN_TRY(b->aoff.reserve(n * 8 + 8));
-  DN_TRY(b->keys.reserve(na * 8 + 8));
-  DN_TRY(b->f.reserve(na * 4 + 4));
-  DN_TRY(b->prev.reserve(na * 4 + 4));
-  DN_TRY(b->order.reserve(na * 8 + 8));
-  DN_TRY(b->chains.reserve(na * sizeof(ChainRec) + sizeof(ChainRec)));
-  // rank and traceback passes
-  DN_TRY(b->rcnt.reserve(n * 4 + 4));
-  DN_TRY(b->roff.reserve(n * 8 + 8));
-  DN_TRY(b->rtasks.reserve(tasks * sizeof(SwTask) + sizeof(SwTask)));
-  DN_TRY(b->rres.reserve(tasks * sizeof(SwResult) + sizeof(SwResult)));
-  DN_TRY(b->chosen.reserve(n * 4 + 4));
-  DN_TRY(b->second.reserve(n * 4 + 4));
-  DN_TRY(b->tbflag.reserve(n * 4 + 4));
-  DN_TRY(b->tboff.reserve(n * 8 + 8));
-  DN_TRY(b->ttasks.reserve(n * sizeof(SwTask) + sizeof(SwTask)));
-  DN_TRY(b->tres.reserve(n * sizeof(SwResult) + sizeof(SwResult)));
-  DN_TRY(b->tcig.reserve(n * cap * 4 + 4));
-  DN_TRY(b->ciglen.reserve(n * 4 + 4));
-  DN_TRY(b->cigoff.reserve(n * 8 + 8));
-  // SW workspace: length sort, per-wave slot offsets, directions of one launch
-  DN_TRY(b->sw.keys.reserve(tasks * 4));
-  DN_TRY(b->sw.keys2.reserve(tasks * 4));
-  DN_TRY(b->sw.ids.reserve(tasks * 4));
-  DN_TRY(b->sw.ids2.reserve(tasks * 4));
-  const uint64_t waves = (tasks + 127) / 128;
-  DN_TRY(b->sw.woff.reserve((waves + 2) * 16));
-  const int W = 2 * c->p.sc.bw + 1;
-  const uint64_t dir = std::min<uint64_t>(b->sw.dir_budget, ((n + 127) / 128 + 1) * (uint64_t)std::max(maxlen, 1) *
-                                                                ((W + 7) / 8) * 512);
-  DN_TRY(b->sw.dir.reserve(dir));
-  // scan and sort temporaries for the largest counts
-  size_t t1 = 0, t2 = 0, t3 = 0;
-  DN_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, t1, (const uint32_t*)nullptr, (uint64_t*)nullptr, (uint32_t)tasks,
-                                          b->st));
-  DN_HIP(hipcub::DeviceRadixSort::SortPairsDescending(nullptr, t2, (const uint32_t*)nullptr, (uint32_t*)nullptr,
-                                                      (const uint32_t*)nullptr, (uint32_t*)nullptr, (uint32_t)tasks, 0,
-                                                      16, b->st));
-  DN_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, t3, (const uint64_t*)nullptr, (uint64_t*)nullptr,
-                                          (uint32_t)(waves + 1), b->st));
-  DN_TRY(b->scan_tmp.reserve(t1));
-  DN_TRY(b->sw.tmp.reserve(std::max(t2, t3)));
-  DN_TRY(b->ltmp.reserve(t2));
-  DN_HIP(hipStreamSynchronize(b->st));
-  return DNALN_OK;
+  return guarded([&]() -> int {
+    if (!b || maxlen < 0) return fail(DNALN_E_INVALID, "bad argument");
+    dnaln_hip_ctx* c = b->ctx;
+    DN_HIP(hipSetDevice(c->device));
+    const uint64_t mc = (uint64_t)std::max(c->p.max_cand, 1);
+    const uint64_t cap = 2 * (uint64_t)maxlen + 4;
+    const uint64_t na = n * anchors_per_read;
+    const uint64_t tasks = n * mc;
+    // inputs and outputs
+    DN_TRY(b->codes.reserve(bases + 2 * kReadGuard));
+    DN_TRY(b->offs.reserve(n * 8 + 8));
+    DN_TRY(b->lens.reserve(n * 4 + 4));
+    DN_TRY(b->fnv.reserve(n * 4 + 4));
+    DN_TRY(b->rec.reserve(n * sizeof(dnaln_record) + sizeof(dnaln_record)));
+    DN_TRY(b->pool.reserve(n * (cap + 2) * 4 + 4));
+    // seeds and chains
+    DN_TRY(b->cands.reserve(n * mc * sizeof(Cand)));
+    DN_TRY(b->ncand.reserve(n * 4 + 4));
+    DN_TRY(b->lids.reserve(n * 4));
+    DN_TRY(b->lkeys.reserve(n * 4));
+    DN_TRY(b->lorder.reserve(n * 4));
+    DN_TRY(b->ovf.reserve(n * 4 + 8));
+    DN_TRY(b->ovf2.reserve(n * 4 + 8));
+    {
+      FusedShape fs;
+      fs.maxmin = 64;
+      DN_TRY(b->fmins.reserve(fused_mins_words(fs, n) * 8));
+    }
+    DN_TRY(b->acnt.reserve(n * 4 + 4));
+    DN_TRY(b->aoff.reserve(n * 8 + 8));
+    DN_TRY(b->keys.reserve(na * 8 + 8));
+    DN_TRY(b->f.reserve(na * 4 + 4));
+    DN_TRY(b->prev.reserve(na * 4 + 4));
+    DN_TRY(b->order.reserve(na * 8 + 8));
+    DN_TRY(b->chains.reserve(na * sizeof(ChainRec) + sizeof(ChainRec)));
+    // rank and traceback passes
+    DN_TRY(b->rcnt.reserve(n * 4 + 4));
+    DN_TRY(b->roff.reserve(n * 8 + 8));
+    DN_TRY(b->rtasks.reserve(tasks * sizeof(SwTask) + sizeof(SwTask)));
+    DN_TRY(b->rres.reserve(tasks * sizeof(SwResult) + sizeof(SwResult)));
+    DN_TRY(b->chosen.reserve(n * 4 + 4));
+    DN_TRY(b->second.reserve(n * 4 + 4));
+    DN_TRY(b->tbflag.reserve(n * 4 + 4));
+    DN_TRY(b->tboff.reserve(n * 8 + 8));
+    DN_TRY(b->ttasks.reserve(n * sizeof(SwTask) + sizeof(SwTask)));
+    DN_TRY(b->tres.reserve(n * sizeof(SwResult) + sizeof(SwResult)));
+    DN_TRY(b->tcig.reserve(n * cap * 4 + 4));
+    DN_TRY(b->ciglen.reserve(n * 4 + 4));
+    DN_TRY(b->cigoff.reserve(n * 8 + 8));
+    // SW workspace: length sort, per-wave slot offsets, directions of one launch
+    DN_TRY(b->sw.keys.reserve(tasks * 4));
+    DN_TRY(b->sw.keys2.reserve(tasks * 4));
+    DN_TRY(b->sw.ids.reserve(tasks * 4));
+    DN_TRY(b->sw.ids2.reserve(tasks * 4));
+    const uint64_t waves = (tasks + 127) / 128;
+    DN_TRY(b->sw.woff.reserve((waves + 2) * 16));
+    const int W = 2 * c->p.sc.bw + 1;
+    const uint64_t dir = std::min<uint64_t>(b->sw.dir_budget, ((n + 127) / 128 + 1) * (uint64_t)std::max(maxlen, 1) *
+                                                                  ((W + 7) / 8) * 512);
+    DN_TRY(b->sw.dir.reserve(dir));
+    // scan and sort temporaries for the largest counts
+    size_t t1 = 0, t2 = 0, t3 = 0;
+    DN_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, t1, (const uint32_t*)nullptr, (uint64_t*)nullptr, (uint32_t)tasks,
+                                            b->st));
+    DN_HIP(hipcub::DeviceRadixSort::SortPairsDescending(nullptr, t2, (const uint32_t*)nullptr, (uint32_t*)nullptr,
+                                                        (const uint32_t*)nullptr, (uint32_t*)nullptr, (uint32_t)tasks, 0,
+                                                        16, b->st));
+    DN_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, t3, (const uint64_t*)nullptr, (uint64_t*)nullptr,
+                                            (uint32_t)(waves + 1), b->st));
+    DN_TRY(b->scan_tmp.reserve(t1));
+    DN_TRY(b->sw.tmp.reserve(std::max(t2, t3)));
+    DN_TRY(b->ltmp.reserve(t2));
+    DN_HIP(hipStreamSynchronize(b->st));
+    return DNALN_OK;
+  });
 }
 
 int dnaln_hip_device_memory(dnaln_hip_ctx* c, uint64_t* free_bytes, uint64_t* total_bytes) {
-  if (!c || !free_bytes || !total_bytes) return fail(DNALN_E_INVALID, "null argument");
-  DN_HIP(hipSetDevice(c->device));
-  size_t f = 0, t = 0;
-  DN_HIP(hipMemGetInfo(&f, &t));
-  *free_bytes = f;
-  *total_bytes = t;
-  return DNALN_OK;
+  return guarded([&]() -> int {
+    if (!c || !free_bytes || !total_bytes) return fail(DNALN_E_INVALID, "null argument");
+    DN_HIP(hipSetDevice(c->device));
+    size_t f = 0, t = 0;
+    DN_HIP(hipMemGetInfo(&f, &t));
+    *free_bytes = f;
+    *total_bytes = t;
+    return DNALN_OK;
+  });
 }
 
 int dnaln_hip_host_alloc(uint64_t bytes, void** out) {
-  if (!out) return fail(DNALN_E_INVALID, "null argument");
-  *out = nullptr;
-  if (hipHostMalloc(out, std::max<uint64_t>(bytes, 1), hipHostMallocDefault) != hipSuccess) {
-    (void)hipGetLastError();
+  return guarded([&]() -> int {
+    if (!out) return fail(DNALN_E_INVALID, "null argument");
     *out = nullptr;
-    return fail(DNALN_E_NOMEM, "hipHostMalloc of " + std::to_string(bytes) + " bytes failed");
-  }
-  return DNALN_OK;
+    if (hipHostMalloc(out, std::max<uint64_t>(bytes, 1), hipHostMallocDefault) != hipSuccess) {
+      (void)hipGetLastError();
+      *out = nullptr;
+      return fail(DNALN_E_NOMEM, "hipHostMalloc of " + std::to_string(bytes) + " bytes failed");
+    }
+    return DNALN_OK;
+  });
 }
 
 void dnaln_hip_host_free(void* p) {
@@ -1876,62 +1949,70 @@ void dnaln_hip_host_free(void* p) {
 }
 
 int dnaln_hip_context_index(dnaln_hip_ctx* c, uint64_t* n_buckets, uint64_t* n_pos, double* build_ms) {
-  if (!c) return fail(DNALN_E_INVALID, "null argument");
-  if (n_buckets) *n_buckets = c->ix.n_buckets;
-  if (n_pos) *n_pos = c->ix.n_pos;
-  if (build_ms) *build_ms = c->ix.ms;
-  return DNALN_OK;
+  return guarded([&]() -> int {
+    if (!c) return fail(DNALN_E_INVALID, "null argument");
+    if (n_buckets) *n_buckets = c->ix.n_buckets;
+    if (n_pos) *n_pos = c->ix.n_pos;
+    if (build_ms) *build_ms = c->ix.ms;
+    return DNALN_OK;
+  });
 }
 
 int dnaln_hip_index_build(const uint8_t* ref_codes, uint64_t ref_len, const dnaln_contig* contigs, int32_t n_contigs,
                           int32_t k, int32_t w, int32_t device, dnaln_hip_index** out) {
-  if (!out || !contigs || n_contigs < 1 || (!ref_codes && ref_len)) return fail(DNALN_E_INVALID, "null argument");
-  *out = nullptr;
-  if (ref_len >= (1ull << 31)) return fail(DNALN_E_INVALID, "reference is too large (limit: 2^31 bp)");
-  std::vector<uint64_t> coff(n_contigs);
-  std::vector<uint32_t> clen(n_contigs);
-  for (int i = 0; i < n_contigs; ++i) {
-    if (contigs[i].offset + contigs[i].len > ref_len) return fail(DNALN_E_INVALID, "contig outside the reference");
-    coff[i] = contigs[i].offset;
-    clen[i] = (uint32_t)contigs[i].len;
-  }
-  std::unique_ptr<dnaln_hip_index> ix(new dnaln_hip_index);
-  ix->device = device;
-  DN_HIP(hipSetDevice(device));
-  hipStream_t st = nullptr;
-  DN_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-  int rc;
-  {
-    DevBuf ref;
-    uint8_t* base = nullptr;
-    rc = upload_reference(ref, ref_codes, ref_len, st, &base);
-    if (rc == DNALN_OK)
-      rc = build_index_dev(base, coff.data(), clen.data(), n_contigs, k, w, st, ix->table, ix->pos, ix->info);
-    (void)hipStreamSynchronize(st);
-  }
-  (void)hipStreamDestroy(st);
-  if (rc != DNALN_OK) return rc;
-  *out = ix.release();
-  return DNALN_OK;
+  return guarded([&]() -> int {
+    if (!out || !contigs || n_contigs < 1 || (!ref_codes && ref_len)) return fail(DNALN_E_INVALID, "null argument");
+    *out = nullptr;
+    if (ref_len >= (1ull << 31)) return fail(DNALN_E_INVALID, "reference is too large (limit: 2^31 bp)");
+    std::vector<uint64_t> coff(n_contigs);
+    std::vector<uint32_t> clen(n_contigs);
+    for (int i = 0; i < n_contigs; ++i) {
+      if (contigs[i].offset + contigs[i].len > ref_len) return fail(DNALN_E_INVALID, "contig outside the reference");
+      coff[i] = contigs[i].offset;
+      clen[i] = (uint32_t)contigs[i].len;
+    }
+    std::unique_ptr<dnaln_hip_index> ix(new dnaln_hip_index);
+    ix->device = device;
+    DN_HIP(hipSetDevice(device));
+    hipStream_t st = nullptr;
+    DN_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    int rc;
+    {
+      DevBuf ref;
+      uint8_t* base = nullptr;
+      rc = upload_reference(ref, ref_codes, ref_len, st, &base);
+      if (rc == DNALN_OK)
+        rc = build_index_dev(base, coff.data(), clen.data(), n_contigs, k, w, st, ix->table, ix->pos, ix->info);
+      (void)hipStreamSynchronize(st);
+    }
+    (void)hipStreamDestroy(st);
+    if (rc != DNALN_OK) return rc;
+    *out = ix.release();
+    return DNALN_OK;
+  });
 }
 
 int dnaln_hip_index_info(const dnaln_hip_index* ix, uint64_t* n_buckets, uint64_t* n_pos, uint64_t* n_keys,
                          uint32_t* max_count, double* build_ms) {
-  if (!ix) return fail(DNALN_E_INVALID, "null argument");
-  if (n_buckets) *n_buckets = ix->info.n_buckets;
-  if (n_pos) *n_pos = ix->info.n_pos;
-  if (n_keys) *n_keys = ix->info.n_keys;
-  if (max_count) *max_count = ix->info.max_count;
-  if (build_ms) *build_ms = ix->info.ms;
-  return DNALN_OK;
+  return guarded([&]() -> int {
+    if (!ix) return fail(DNALN_E_INVALID, "null argument");
+    if (n_buckets) *n_buckets = ix->info.n_buckets;
+    if (n_pos) *n_pos = ix->info.n_pos;
+    if (n_keys) *n_keys = ix->info.n_keys;
+    if (max_count) *max_count = ix->info.max_count;
+    if (build_ms) *build_ms = ix->info.ms;
+    return DNALN_OK;
+  });
 }
 
 int dnaln_hip_index_download(const dnaln_hip_index* ix, dnaln_bucket* table, uint32_t* pos) {
-  if (!ix || !table || (!pos && ix->info.n_pos)) return fail(DNALN_E_INVALID, "null argument");
-  DN_HIP(hipSetDevice(ix->device));
-  DN_HIP(hipMemcpy(table, ix->table.p, ix->info.n_buckets * sizeof(dnaln_bucket), hipMemcpyDeviceToHost));
-  if (ix->info.n_pos) DN_HIP(hipMemcpy(pos, ix->pos.p, ix->info.n_pos * 4, hipMemcpyDeviceToHost));
-  return DNALN_OK;
+  return guarded([&]() -> int {
+    if (!ix || !table || (!pos && ix->info.n_pos)) return fail(DNALN_E_INVALID, "null argument");
+    DN_HIP(hipSetDevice(ix->device));
+    DN_HIP(hipMemcpy(table, ix->table.p, ix->info.n_buckets * sizeof(dnaln_bucket), hipMemcpyDeviceToHost));
+    if (ix->info.n_pos) DN_HIP(hipMemcpy(pos, ix->pos.p, ix->info.n_pos * 4, hipMemcpyDeviceToHost));
+    return DNALN_OK;
+  });
 }
 
 void dnaln_hip_index_free(dnaln_hip_index* ix) {
@@ -1941,9 +2022,11 @@ void dnaln_hip_index_free(dnaln_hip_index* ix) {
 }
 
 int dnaln_hip_context_params(dnaln_hip_ctx* c, dnaln_map_params* out) {
-  if (!c || !out) return fail(DNALN_E_INVALID, "null argument");
-  *out = c->p;
-  return DNALN_OK;
+  return guarded([&]() -> int {
+    if (!c || !out) return fail(DNALN_E_INVALID, "null argument");
+    *out = c->p;
+    return DNALN_OK;
+  });
 }
 
 void dnaln_hip_alloc_counters(uint64_t* events, uint64_t* bytes) {
@@ -1954,15 +2037,17 @@ void dnaln_hip_alloc_counters(uint64_t* events, uint64_t* bytes) {
 int dnaln_hip_map_batch(dnaln_hip_ctx* c, const uint8_t* read_codes, const uint64_t* offs, const int32_t* lens,
                         const uint32_t* name_fnv1a, uint64_t n, dnaln_record* out, uint32_t* cigar_pool,
                         uint64_t cigar_cap, dnaln_map_stats* stats) {
-  if (!c) return fail(DNALN_E_INVALID, "null context");
-  if (!c->batch) {
-    dnaln_batch_config cfg;
-    dnaln_hip_default_batch_config(&cfg);
-    DN_TRY(dnaln_hip_batch_create(c, &cfg, &c->batch));
-  }
-  DN_TRY(dnaln_hip_batch_upload(c->batch, read_codes, offs, lens, name_fnv1a, n));
-  DN_TRY(dnaln_hip_batch_run(c->batch));
-  return dnaln_hip_batch_download(c->batch, out, cigar_pool, cigar_cap, stats);
+  return guarded([&]() -> int {
+    if (!c) return fail(DNALN_E_INVALID, "null context");
+    if (!c->batch) {
+      dnaln_batch_config cfg;
+      dnaln_hip_default_batch_config(&cfg);
+      DN_TRY(dnaln_hip_batch_create(c, &cfg, &c->batch));
+    }
+    DN_TRY(dnaln_hip_batch_upload(c->batch, read_codes, offs, lens, name_fnv1a, n));
+    DN_TRY(dnaln_hip_batch_run(c->batch));
+    return dnaln_hip_batch_download(c->batch, out, cigar_pool, cigar_cap, stats);
+  });
 }
 
 }  // extern "C"

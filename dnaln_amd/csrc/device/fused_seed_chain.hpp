@@ -260,7 +260,8 @@ DN_HD int chain_extract_lds(const ChainArgs& a, const LdsView& v, int n, uint32_
     cd.rev = strand;
     cd.contig = contig_of(a.contig_off, a.n_contigs, top[x].rpos_strand & 0x7fffffffu);
     cd.diag = diag;
-    cd.wstart = diag - bw;
+    cd.score = top[x].score;
+    cd.reserved = 0;
     cands[nc++] = cd;
     chosen |= 1u << x;
   }

@@ -21,10 +21,11 @@ DN_HD SwTask make_task(const MapGeom& g, uint64_t q, int qlen, const Cand& c) {
   t.q = q;
   t.qlen = qlen;
   t.flags = c.rev;
-  t.t = g.ref + (uint64_t)c.wstart;
+  const int64_t ws = cand_wstart(c, g.bw);
+  t.t = g.ref + (uint64_t)ws;
   const int64_t wl = (int64_t)qlen + 2 * (int64_t)g.bw;
   const int64_t cb = (int64_t)g.coff[c.contig], ce = cb + (int64_t)g.clen[c.contig];
-  int64_t lo = cb - c.wstart, hi = ce - c.wstart;
+  int64_t lo = cb - ws, hi = ce - ws;
   lo = lo < 0 ? 0 : (lo > wl ? wl : lo);
   hi = hi < lo ? lo : (hi > wl ? wl : hi);
   t.tlo = (int32_t)lo;
@@ -83,7 +84,7 @@ DN_HD void record_read(const MapGeom& g, int qlen, int nc, const Cand* c, int ch
   const Cand& cd = c[chosen];
   m.contig = cd.contig;
   m.rev = (uint8_t)cd.rev;
-  m.pos = cd.wstart + res->tb - (int64_t)g.coff[cd.contig];
+  m.pos = cand_wstart(cd, g.bw) + res->tb - (int64_t)g.coff[cd.contig];
   m.score = res->score;
   m.nm = res->nm;
   m.cigar_len = (uint32_t)res->n_cigar + (res->qb > 0) + (qlen > res->qe);

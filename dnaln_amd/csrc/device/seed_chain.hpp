@@ -311,11 +311,17 @@ struct ChainRec {
   int64_t dmin, dmax;
 };
 
+// A candidate locus (src/mapper.hpp:61-67). The reference's window start is always diag - bw
+// (src/mapper.cpp:60), so it is not stored; `score` is the score of the chain it came from.
 struct Cand {
   uint32_t rev;
   int32_t contig;
-  int64_t diag, wstart;
+  int64_t diag;
+  int32_t score;
+  int32_t reserved;
 };
+
+DN_HD int64_t cand_wstart(const Cand& c, int bw) { return c.diag - bw; }
 
 struct ChainArgs {
   uint32_t n_reads;        // entries to process (of `list` when set)
@@ -406,6 +412,32 @@ inline void pack_gap_cost(ChainArgs& a, const int32_t* host_cost, int band) {
     a.cost_pk[d >> 3] |= (uint64_t)host_cost[d] << (8 * (d & 7));
   }
   a.cost_packed = 1;
+}
+
+// Chain and candidate parameters of one run (src/chain.hpp:30-36 after the band clamp of
+// src/mapper.cpp:9, and src/mapper.cpp:44-62).
+struct ChainSetup {
+  int k, max_gap, band, lookback, min_score, min_anchors, max_cand, bw;
+  double pri_ratio;
+};
+
+// Fills the parameter part of a ChainArgs. The gap-cost table cost[0..band] (host copy) is
+// packed for the kernels only after the band is set, from the band the DP uses.
+inline void init_chain_args(ChainArgs& a, const ChainSetup& s, const int32_t* cost_dev, const int32_t* cost_host,
+                            const uint64_t* contig_off, int n_contigs) {
+  a.k = s.k;
+  a.max_gap = s.max_gap;
+  a.band = s.band;
+  a.lookback = s.lookback;
+  a.min_score = s.min_score;
+  a.min_anchors = s.min_anchors;
+  a.max_cand = s.max_cand;
+  a.bw = s.bw;
+  a.pri_ratio = s.pri_ratio;
+  a.cost = cost_dev;
+  pack_gap_cost(a, cost_host, a.band);
+  a.contig_off = contig_off;
+  a.n_contigs = n_contigs;
 }
 
 DN_HD int contig_of(const uint64_t* off, int nc, uint64_t gpos) {  // src/reference.cpp:25-32
@@ -513,7 +545,8 @@ DN_HD void chain_read(const ChainArgs& a, uint32_t li) {
       cd.rev = ch[c].strand;
       cd.contig = contig_of(a.contig_off, a.n_contigs, ch[c].rpos);
       cd.diag = diag;
-      cd.wstart = diag - bw;
+      cd.score = ch[c].score;
+      cd.reserved = 0;
       cands[nc++] = cd;
     }
   }

@@ -234,10 +234,12 @@ int emu_align_ex(const int* sp6, const uint8_t* pool, const int64_t* q_off, cons
 // mp = k, w, max_occ, bw, max_cand, min_chain, lookback, max_gap, band, min_anchors.
 // Returns the number of reads whose candidates differ (-1 on bad input); stats[0] = reads the
 // LDS program sent to the global path (overflow), stats[1] = total anchors, stats[2] = max.
+// out_n / out_c (may be null): the global-memory program's candidates, n_cand per read and
+// max_cand entries of (rev, contig, diag, chain score) per read.
 long emu_seed_chain(const uint8_t* codes, const uint64_t* offs, const int32_t* lens, uint64_t n,
                     const void* table, uint64_t n_buckets, const uint32_t* pos, const uint64_t* contig_off,
                     int n_contigs, const int* mp, double pri_ratio, const int32_t* cost, int cap,
-                    int maxmin, uint64_t* stats) {
+                    int maxmin, uint64_t* stats, int32_t* out_n, int64_t* out_c) {
   uint64_t total = 0;
   for (uint64_t r = 0; r < n; ++r) total = std::max<uint64_t>(total, offs[r] + (uint64_t)lens[r]);
   std::vector<uint8_t> guarded(total + 256, 0xEE);
@@ -259,19 +261,17 @@ long emu_seed_chain(const uint8_t* codes, const uint64_t* offs, const int32_t* l
   sa.w = mp[1];
   sa.max_occ = (uint32_t)mp[2];
   ChainArgs ca{};
-  ca.cost = cost;
-  pack_gap_cost(ca, cost, mp[8]);
-  ca.k = mp[0];
-  ca.bw = mp[3];
-  ca.max_cand = mp[4];
-  ca.min_score = mp[5];
-  ca.lookback = mp[6];
-  ca.max_gap = mp[7];
-  ca.band = mp[8];
-  ca.min_anchors = mp[9];
-  ca.pri_ratio = pri_ratio;
-  ca.contig_off = contig_off;
-  ca.n_contigs = n_contigs;
+  ChainSetup cs;
+  cs.k = mp[0];
+  cs.bw = mp[3];
+  cs.max_cand = mp[4];
+  cs.min_score = mp[5];
+  cs.lookback = mp[6];
+  cs.max_gap = mp[7];
+  cs.band = mp[8];
+  cs.min_anchors = mp[9];
+  cs.pri_ratio = pri_ratio;
+  init_chain_args(ca, cs, cost, cost, contig_off, n_contigs);
   const int mc = std::max(mp[4], 1);
   std::vector<Cand> c1((size_t)n * mc), c2((size_t)n * mc);
   std::vector<int32_t> n1(n), n2(n);
@@ -316,6 +316,17 @@ long emu_seed_chain(const uint8_t* codes, const uint64_t* offs, const int32_t* l
     g.cands = c1.data();
     g.n_cand = n1.data();
     chain_read(g, 0);
+    if (out_n) {
+      out_n[r] = n1[r];
+      for (int c = 0; c < n1[r]; ++c) {
+        const Cand& x = c1[r * mc + c];
+        int64_t* o = out_c + 4 * (r * mc + c);
+        o[0] = x.rev;
+        o[1] = x.contig;
+        o[2] = x.diag;
+        o[3] = x.score;
+      }
+    }
     // LDS path
     ChainArgs l = ca;
     l.cands = c2.data();
@@ -330,7 +341,7 @@ long emu_seed_chain(const uint8_t* codes, const uint64_t* offs, const int32_t* l
     for (int c = 0; same && c < nc; ++c) {
       const Cand& x = c1[r * mc + c];
       const Cand& y = c2[r * mc + c];
-      same = x.rev == y.rev && x.contig == y.contig && x.diag == y.diag && x.wstart == y.wstart;
+      same = x.rev == y.rev && x.contig == y.contig && x.diag == y.diag && x.score == y.score;
     }
     differ += !same;
   }

@@ -10,6 +10,7 @@
 #include <deque>
 #include <memory>
 #include <mutex>
+#include <new>
 #include <string>
 #include <thread>
 #include <vector>
@@ -153,24 +154,40 @@ struct Slot {
       cv.wait(lk, [&] { return state == 1 || quit; });
       if (state != 1) return;
       lk.unlock();
-      gather();
-      int r = dnaln_hip_batch_upload(b, codes.data(), offs.data(), lens.data(), fnv.data(), lens.size());
-      if (r == DNALN_OK) r = dnaln_hip_batch_run(b);
-      if (r == DNALN_OK) r = dnaln_hip_batch_poll(b, 1);
-      if (r == DNALN_OK) {
-        // A few CIGAR words per read; the bound (two per base) only when the device holds more.
-        rec.resize(lens.size());
-        cig.resize(std::max<size_t>(cig.cap, 8 * lens.size() + 64));
-        r = dnaln_hip_batch_download(b, rec.data(), cig.data(), cig.size(), nullptr);
-        if (r == DNALN_E_CAPACITY) {
-          uint64_t cap = 1;
-          for (const int32_t l : lens) cap += 2 * (uint64_t)l + 4;
-          cig.resize(cap);
+      // The batch's whole turn runs under one try: an exception on this thread (a host buffer
+      // that cannot grow) comes back through collect() as the slot's status, like a device error.
+      int r = DNALN_OK;
+      std::string e;
+      try {
+        gather();
+        r = dnaln_hip_batch_upload(b, codes.data(), offs.data(), lens.data(), fnv.data(), lens.size());
+        if (r == DNALN_OK) r = dnaln_hip_batch_run(b);
+        if (r == DNALN_OK) r = dnaln_hip_batch_poll(b, 1);
+        if (r == DNALN_OK) {
+          // A few CIGAR words per read; the bound (two per base) only when the device holds more.
+          rec.resize(lens.size());
+          cig.resize(std::max<size_t>(cig.cap, 8 * lens.size() + 64));
           r = dnaln_hip_batch_download(b, rec.data(), cig.data(), cig.size(), nullptr);
+          if (r == DNALN_E_CAPACITY) {
+            uint64_t cap = 1;
+            for (const int32_t l : lens) cap += 2 * (uint64_t)l + 4;
+            cig.resize(cap);
+            r = dnaln_hip_batch_download(b, rec.data(), cig.data(), cig.size(), nullptr);
+          }
         }
+        if (r != DNALN_OK) e = dnaln_hip_last_error();
+        if (r == DNALN_OK) split();
+      } catch (const std::bad_alloc&) {
+        r = DNALN_E_NOMEM;
+        e = "serving slot: host allocation failed";
+      } catch (...) {
+        r = DNALN_E_INTERNAL;
+        e = "serving slot: unexpected C++ exception";
       }
-      const std::string e = r == DNALN_OK ? std::string() : std::string(dnaln_hip_last_error());
-      if (r == DNALN_OK) split();
+      if (r != DNALN_OK) {
+        out.clear();
+        reqs.clear();
+      }
       lk.lock();
       rc = r;
       err = e;

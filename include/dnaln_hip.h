@@ -225,6 +225,23 @@ int dnaln_hip_batch_poll(struct dnaln_hip_batch* b, int wait);
 /* Copies records and CIGARs to the host (after poll returned DNALN_OK). */
 int dnaln_hip_batch_download(struct dnaln_hip_batch* b, dnaln_record* out, uint32_t* cigar_pool, uint64_t cigar_cap,
                              dnaln_map_stats* stats);
+/* A candidate locus of one read after seeding and chaining (Mapper::Cand, src/mapper.hpp:61-67):
+ * strand, contig, diagonal (the window starts at diag - bw, src/mapper.cpp:60) and the score of
+ * the chain it came from (Chain::score, src/chain.hpp:22-28). */
+typedef struct {
+  uint32_t rev;
+  int32_t contig;
+  int64_t diag;
+  int32_t chain_score;
+  int32_t reserved;
+} dnaln_candidate;
+
+/* The candidates the last run's seed and chain stages left for the rank pass (the state of
+ * Mapper::cands_ after seed_and_chain, src/mapper.cpp:14-63): n_cand[i], and candidate c of read
+ * i at cands[i * per_read + c] in the reference's order; per_read must be at least max_cand.
+ * The plain form keeps one chunk's candidates, so there the batch must fit one chunk. For
+ * checks of the seed and chain stages; the mapping path never needs it. */
+int dnaln_hip_batch_candidates(struct dnaln_hip_batch* b, int32_t* n_cand, dnaln_candidate* cands, uint64_t per_read);
 /* Device bytes a batch of this configuration holds (for admission by HBM budget). */
 uint64_t dnaln_hip_batch_device_bytes(struct dnaln_hip_ctx* ctx, const dnaln_batch_config* cfg);
 /* Time of the last run measured with HIP events on the batch stream, per family (ms):

@@ -85,6 +85,8 @@ def lib() -> C.CDLL:
         L.or_score_cigar.argtypes = [C.POINTER(ScoreT), vp, vp, C.c_int, C.c_int, vp, C.c_int]
         L.or_map.argtypes = [C.POINTER(MapParamsT), vp, vp, vp, C.c_int, C.POINTER(IndexT), vp, vp, vp, vp, vp, vp,
                              C.c_size_t, vp, vp, C.c_size_t, C.POINTER(C.c_uint64)]
+        L.or_candidates.argtypes = [C.POINTER(MapParamsT), vp, C.c_int, C.POINTER(IndexT), vp, vp, vp, C.c_size_t,
+                                    vp, vp]
         L.or_format_sam.argtypes = [C.c_char_p, C.c_int, C.c_char_p, C.c_char_p, C.c_int, C.POINTER(RecordT), vp,
                                     C.c_char_p, C.c_char_p, C.c_size_t]
         L.or_format_sam.restype = C.c_long
@@ -207,6 +209,19 @@ def map_reads(params: MapParamsT, genome, index: Index, reads):
         raise RuntimeError("oracle map failed (CIGAR pool too small)")
     arr = np.ctypeslib.as_array(recs, shape=(max(n, 1),))[:n].copy()
     return arr, pool, cells.value
+
+
+def candidates(params: MapParamsT, genome, index: Index, reads):
+    """Candidates after seeding and chaining (src/mapper.cpp:14-63): (n_cand[n], cand[n, max_cand, 4])
+    with columns rev, contig, diag and the score of the candidate's chain."""
+    n = len(reads)
+    mc = max(int(params.max_cand), 1)
+    ncand = np.zeros(max(n, 1), np.int32)
+    cand = np.zeros((max(n, 1), mc, 4), np.int64)
+    lib().or_candidates(C.byref(params), genome.contig_off.ctypes.data, genome.n_contigs, C.byref(index.idx),
+                        reads.seq.ctypes.data, reads.offs.ctypes.data, reads.lens.ctypes.data, n,
+                        ncand.ctypes.data, cand.ctypes.data)
+    return ncand[:n], cand[:n]
 
 
 def record_fields(rec, pool, contig_names) -> tuple:

@@ -467,6 +467,7 @@ typedef struct {
   int32_t contig;
   int64_t diag, wstart;
   int32_t sw;
+  int32_t chain_score; /* score of the chain the candidate came from */
 } cand_t;
 
 static int contig_of(const uint64_t* off, int nc, uint64_t gpos) { /* reference.cpp:25-32 */
@@ -489,13 +490,72 @@ static void fetch_window(const uint8_t* ref, const uint64_t* off, const uint32_t
   }
 }
 
+/* seed_and_chain (mapper.cpp:14-63) for one read (forward codes): sketch, lookups, anchors,
+ * Chainer::run, candidates. Writes at most p->max_cand candidates; returns their number. The
+ * anchor and chain arrays grow as needed (capacity *acap). */
+static int seed_and_chain(const or_map_params_t* pp, const or_index_t* idx, const uint64_t* contig_off, int n_contigs,
+                          const uint8_t* fwd, int len, or_mini_t* mins, or_anchor_t** anchors_p,
+                          or_chain_t** chains_p, size_t* acap_p, cand_t* cands) {
+  const or_map_params_t p = *pp;
+  const int k = idx->k, bw = p.sc.bw;
+  size_t acap = *acap_p;
+  or_anchor_t* anchors = *anchors_p;
+  or_chain_t* chains = *chains_p;
+  const size_t nm = or_sketch(fwd, (uint32_t)len, k, idx->w, 0, mins);
+  size_t na = 0;
+  for (size_t i = 0; i < nm; ++i) {
+    uint32_t cnt;
+    const uint32_t* pos = or_index_lookup(idx, mins[i].hash, &cnt);
+    if (cnt == 0 || cnt > p.max_occ) continue;
+    if (na + cnt > acap) {
+      while (na + cnt > acap) acap *= 2;
+      anchors = (or_anchor_t*)realloc(anchors, sizeof(or_anchor_t) * acap);
+      chains = (or_chain_t*)realloc(chains, sizeof(or_chain_t) * acap);
+    }
+    for (uint32_t j = 0; j < cnt; ++j) {
+      const uint32_t v = pos[j];
+      const uint32_t rev = (v & 1) ^ mins[i].strand;
+      anchors[na].rpos = v >> 1;
+      anchors[na].qpos = rev ? len - (int32_t)mins[i].pos - k : (int32_t)mins[i].pos;
+      anchors[na].strand = rev;
+      ++na;
+    }
+  }
+  int nc = 0;
+  const size_t nch = or_chain(anchors, na, k, &p.ch, chains);
+  if (nch > 0) {
+    const double min_score = chains[0].score * p.pri_ratio;
+    for (size_t c = 0; c < nch; ++c) {
+      if (chains[c].score < min_score || nc >= p.max_cand) break;
+      const int64_t diag = chains[c].dmin + (chains[c].dmax - chains[c].dmin) / 2;
+      int dup = 0;
+      for (int i = 0; i < nc && !dup; ++i) {
+        const int64_t dd = cands[i].diag - diag;
+        dup = cands[i].rev == chains[c].strand && (dd < 0 ? -dd : dd) <= bw;
+      }
+      if (dup) continue;
+      cands[nc].rev = chains[c].strand;
+      cands[nc].contig = contig_of(contig_off, n_contigs, chains[c].rpos);
+      cands[nc].diag = diag;
+      cands[nc].wstart = diag - bw;
+      cands[nc].sw = 0;
+      cands[nc].chain_score = chains[c].score;
+      ++nc;
+    }
+  }
+  *acap_p = acap;
+  *anchors_p = anchors;
+  *chains_p = chains;
+  return nc;
+}
+
 int or_map(const or_map_params_t* pin, const uint8_t* ref, const uint64_t* contig_off, const uint32_t* contig_len,
            int n_contigs, const or_index_t* idx, const char* seq, const uint64_t* seq_off, const int32_t* seq_len,
            const char* names, const uint64_t* name_off, const int32_t* name_len, size_t n, or_record_t* out,
            uint32_t* cigar_pool, size_t cap, uint64_t* cells) {
   or_map_params_t p = *pin;
   if (p.ch.band > p.sc.bw) p.ch.band = p.sc.bw; /* mapper.cpp:9 */
-  const int k = idx->k, bw = p.sc.bw, W = 2 * bw + 1;
+  const int bw = p.sc.bw, W = 2 * bw + 1;
   int maxlen = 0;
   for (size_t r = 0; r < n; ++r) maxlen = seq_len[r] > maxlen ? seq_len[r] : maxlen;
   uint8_t* codes = (uint8_t*)malloc(2 * (size_t)maxlen + 1);
@@ -515,48 +575,7 @@ int or_map(const or_map_params_t* pin, const uint8_t* ref, const uint64_t* conti
     uint8_t* rvc = codes + len;
     or_encode(seq + seq_off[r], (size_t)len, fwd);
     or_revcomp(fwd, (size_t)len, rvc);
-    /* seed_and_chain (mapper.cpp:14-63) */
-    const size_t nm = or_sketch(fwd, (uint32_t)len, k, idx->w, 0, mins);
-    size_t na = 0;
-    for (size_t i = 0; i < nm; ++i) {
-      uint32_t cnt;
-      const uint32_t* pos = or_index_lookup(idx, mins[i].hash, &cnt);
-      if (cnt == 0 || cnt > p.max_occ) continue;
-      if (na + cnt > acap) {
-        while (na + cnt > acap) acap *= 2;
-        anchors = (or_anchor_t*)realloc(anchors, sizeof(or_anchor_t) * acap);
-        chains = (or_chain_t*)realloc(chains, sizeof(or_chain_t) * acap);
-      }
-      for (uint32_t j = 0; j < cnt; ++j) {
-        const uint32_t v = pos[j];
-        const uint32_t rev = (v & 1) ^ mins[i].strand;
-        anchors[na].rpos = v >> 1;
-        anchors[na].qpos = rev ? len - (int32_t)mins[i].pos - k : (int32_t)mins[i].pos;
-        anchors[na].strand = rev;
-        ++na;
-      }
-    }
-    int nc = 0;
-    const size_t nch = or_chain(anchors, na, k, &p.ch, chains);
-    if (nch > 0) {
-      const double min_score = chains[0].score * p.pri_ratio;
-      for (size_t c = 0; c < nch; ++c) {
-        if (chains[c].score < min_score || nc >= p.max_cand) break;
-        const int64_t diag = chains[c].dmin + (chains[c].dmax - chains[c].dmin) / 2;
-        int dup = 0;
-        for (int i = 0; i < nc && !dup; ++i) {
-          const int64_t dd = cands[i].diag - diag;
-          dup = cands[i].rev == chains[c].strand && (dd < 0 ? -dd : dd) <= bw;
-        }
-        if (dup) continue;
-        cands[nc].rev = chains[c].strand;
-        cands[nc].contig = contig_of(contig_off, n_contigs, chains[c].rpos);
-        cands[nc].diag = diag;
-        cands[nc].wstart = diag - bw;
-        cands[nc].sw = 0;
-        ++nc;
-      }
-    }
+    const int nc = seed_and_chain(&p, idx, contig_off, n_contigs, fwd, len, mins, &anchors, &chains, &acap, cands);
     /* windows (mapper.cpp:103-121): the N-padded fetch covers both cases */
     const int wl = len + 2 * bw;
     for (int c = 0; c < nc; ++c)
@@ -645,6 +664,43 @@ int or_map(const or_map_params_t* pin, const uint8_t* ref, const uint64_t* conti
   free(cig);
   free(cands);
   return rc;
+}
+
+/* The candidates of n reads (ASCII) as seed_and_chain leaves them (src/mapper.cpp:14-63):
+ * n_cand[r], and for candidate c of read r the four values rev, contig, diag and the score of
+ * its chain at cand[4 * (r * max_cand + c)]. */
+int or_candidates(const or_map_params_t* pin, const uint64_t* contig_off, int n_contigs, const or_index_t* idx,
+                  const char* seq, const uint64_t* seq_off, const int32_t* seq_len, size_t n, int32_t* n_cand,
+                  int64_t* cand) {
+  or_map_params_t p = *pin;
+  if (p.ch.band > p.sc.bw) p.ch.band = p.sc.bw; /* mapper.cpp:9 */
+  const int mc = p.max_cand > 0 ? p.max_cand : 1;
+  int maxlen = 0;
+  for (size_t r = 0; r < n; ++r) maxlen = seq_len[r] > maxlen ? seq_len[r] : maxlen;
+  uint8_t* fwd = (uint8_t*)malloc((size_t)maxlen + 1);
+  or_mini_t* mins = (or_mini_t*)malloc(sizeof(or_mini_t) * ((size_t)maxlen + 1));
+  size_t acap = 1024;
+  or_anchor_t* anchors = (or_anchor_t*)malloc(sizeof(or_anchor_t) * acap);
+  or_chain_t* chains = (or_chain_t*)malloc(sizeof(or_chain_t) * acap);
+  cand_t* cands = (cand_t*)malloc(sizeof(cand_t) * (size_t)mc);
+  for (size_t r = 0; r < n; ++r) {
+    or_encode(seq + seq_off[r], (size_t)seq_len[r], fwd);
+    const int nc = seed_and_chain(&p, idx, contig_off, n_contigs, fwd, seq_len[r], mins, &anchors, &chains, &acap, cands);
+    n_cand[r] = nc;
+    for (int c = 0; c < nc; ++c) {
+      int64_t* o = cand + 4 * ((size_t)r * mc + c);
+      o[0] = cands[c].rev;
+      o[1] = cands[c].contig;
+      o[2] = cands[c].diag;
+      o[3] = cands[c].chain_score;
+    }
+  }
+  free(fwd);
+  free(mins);
+  free(anchors);
+  free(chains);
+  free(cands);
+  return 0;
 }
 
 /* ------------------------------------------------------------------ SAM (mapper.cpp:231-270) */

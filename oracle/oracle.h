@@ -13,6 +13,7 @@
  *   or_score_cigar               src/align.cpp:145-162
  *   or_map                       src/mapper.cpp:8-223 (seed, chain, windows, rank, select,
  *                                traceback pass, records, MAPQ)
+ *   or_candidates                src/mapper.cpp:14-63 (seed_and_chain alone)
  *   or_format_sam                src/mapper.cpp:231-270, src/common.cpp:6-22
  *   or_fnv1a                     src/common.hpp:136-140
  *
@@ -128,6 +129,12 @@ int or_map(const or_map_params_t* p, const uint8_t* ref, const uint64_t* contig_
            int n_contigs, const or_index_t* idx, const char* seq, const uint64_t* seq_off, const int32_t* seq_len,
            const char* names, const uint64_t* name_off, const int32_t* name_len, size_t n, or_record_t* out,
            uint32_t* cigar_pool, size_t cap, uint64_t* cells);
+
+/* Candidates of n reads (ASCII) after seeding and chaining: n_cand[r], and 4 values per candidate
+ * (rev, contig, diag, chain score) at cand[4 * (r * max_cand + c)]. */
+int or_candidates(const or_map_params_t* p, const uint64_t* contig_off, int n_contigs, const or_index_t* idx,
+                  const char* seq, const uint64_t* seq_off, const int32_t* seq_len, size_t n, int32_t* n_cand,
+                  int64_t* cand);
 
 /* Appends one SAM line (with '\n') to buf; returns the length written, or -1 if cap is short. */
 long or_format_sam(const char* name, int name_len, const char* seq, const char* qual, int len,

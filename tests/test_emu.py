@@ -108,7 +108,12 @@ def test_emulated_packed_kernel_reverse_queries_and_clipped_windows(emu):
             assert cig[offs[i]:offs[i + 1]].tolist() == exp[6], (bw, i)
 
 
-def _emu_seed_chain(emu, spec, g, r, cap, maxmin, band=None):
+def _gap_costs(k, band):
+    """cost[dd] = int(0.01 k dd + 0.5 ilog2 dd) for dd = 1..band, 0 at dd = 0 (src/chain.cpp:36)."""
+    return np.array([0] + [int(0.01 * k * dd + 0.5 * (dd.bit_length() - 1)) for dd in range(1, band + 1)], np.int32)
+
+
+def _emu_seed_chain(emu, spec, g, r, cap, maxmin, band=None, cost=None, want_cands=False):
     from dnaln_amd import api
     mp = sets.map_params(spec)
     k, w, bw = mp["k"], mp["w"], mp["bw"]
@@ -117,16 +122,54 @@ def _emu_seed_chain(emu, spec, g, r, cap, maxmin, band=None):
     rb = api.prepare(r)
     if band is None:
         band = min(16, bw)  # ChainParams::band, clamped to the SW band (src/mapper.cpp:9)
-    cost = np.array([0] + [int(0.01 * k * dd + 0.5 * (dd.bit_length() - 1)) for dd in range(1, band + 1)], np.int32)
+    if cost is None:
+        cost = _gap_costs(k, band)
     m = np.array([k, w, mp["max_occ"], bw, mp["max_cand"], mp["min_chain"], 50, 5000, band, 2], np.int32)
     st = np.zeros(3, np.uint64)
+    mc = max(mp["max_cand"], 1)
+    out_n = np.zeros(max(len(r), 1), np.int32)
+    out_c = np.zeros((max(len(r), 1), mc, 4), np.int64)
     v = C.c_void_p
     emu.emu_seed_chain.restype = C.c_long
     d = emu.emu_seed_chain(v(rb.codes.ctypes.data), v(rb.offs.ctypes.data), v(rb.lens.ctypes.data),
                            C.c_uint64(len(r)), v(tab.ctypes.data), C.c_uint64(len(tab)), v(pos.ctypes.data),
                            v(g.contig_off.ctypes.data), g.n_contigs, v(m.ctypes.data), C.c_double(0.5),
-                           v(cost.ctypes.data), cap, maxmin, v(st.ctypes.data))
+                           v(cost.ctypes.data), cap, maxmin, v(st.ctypes.data),
+                           v(out_n.ctypes.data) if want_cands else None, v(out_c.ctypes.data) if want_cands else None)
+    if want_cands:
+        return d, int(st[0]), out_n[:len(r)], out_c[:len(r)]
     return d, int(st[0])
+
+
+def _candidate_mismatches(n_a, c_a, n_b, c_b):
+    """Reads whose candidate lists (count, then every column of every candidate) differ."""
+    bad = []
+    for i in range(len(n_a)):
+        if n_a[i] != n_b[i] or not np.array_equal(c_a[i, :n_a[i]], c_b[i, :n_b[i]]):
+            bad.append(i)
+    return bad
+
+
+@pytest.mark.parametrize("name", ["edge_uniform_bw64", "edge_uniform_bw16", "edge_repeats_bwa_bw8",
+                                  "edge_repeats_params"])
+def test_emulated_candidates_and_chain_scores_equal_the_oracle(emu, name):
+    """Every read's candidates — strand, contig, diagonal and the score of the chain each came
+    from — as the device's seed and chain programs compute them with the parameter set-up the
+    device path uses (init_chain_args), equal the oracle's (src/mapper.cpp:14-63). The reads
+    carry indels of up to 12 bp, so chains cross drifts with a non-zero gap cost: the same
+    check with the gap-cost table zeroed finds differing reads, so a lost gap cost cannot pass."""
+    spec = sets.SETS[name]
+    g = sets.genome(spec)
+    r = sets.reads(spec, g)
+    mp = sets.map_params(spec)
+    oidx = oracle.Index(g.codes, g.contig_off, g.contig_len, mp["k"], mp["w"])
+    on, oc = oracle.candidates(oracle.default_params(**mp), g, oidx, r)
+    assert (on > 0).sum() > len(r) // 2
+    _, _, en, ec = _emu_seed_chain(emu, spec, g, r, 64, 56, want_cands=True)
+    assert _candidate_mismatches(en, ec, on, oc) == []
+    band = min(16, mp["bw"])
+    _, _, zn, zc = _emu_seed_chain(emu, spec, g, r, 64, 56, cost=np.zeros(band + 1, np.int32), want_cands=True)
+    assert len(_candidate_mismatches(zn, zc, on, oc)) > 0
 
 
 @pytest.mark.parametrize("name", ["uniform4m_150bp_10k", "edge_twins_bw16", "edge_repeats_params",

@@ -144,3 +144,38 @@ def test_both_traceback_walks_give_reference_records(name, walk_kind, mode):
     recs, pool, _ = b.download()
     b.close()
     _compare(name, recs, pool, g)
+
+
+def _oracle_candidates(spec, g, r):
+    import oracle
+    mp = sets.map_params(spec)
+    idx = oracle.Index(g.codes, g.contig_off, g.contig_len, mp["k"], mp["w"])
+    return oracle.candidates(oracle.default_params(**mp), g, idx, r)
+
+
+@pytest.mark.parametrize("name", ["edge_uniform_bw64", "edge_uniform_bw16", "edge_repeats_bw32", "edge_repeats_bwa_bw8",
+                                  "edge_repeats_params"])
+@pytest.mark.parametrize("mode,split", [(abi.MODE_PLAIN, (0, 0)), (abi.MODE_BATCHED, (0, 0)),
+                                        (abi.MODE_BATCHED, (-1, -1))])
+def test_device_candidates_and_chain_scores_equal_the_oracle(name, mode, split):
+    """The candidates the device's seed and chain stages leave for the rank pass — strand, contig,
+    diagonal and the score of the chain each came from (src/mapper.cpp:14-63, src/chain.cpp:12-71)
+    — equal the oracle's for every read: one thread per read over global memory (plain form), the
+    fused LDS kernel in its shapes, and one wave per read (batched form, every bucket off). Chain
+    scores depend on every gap cost the reads' indels (up to 12 bp) cross, which the records
+    alone rarely reveal."""
+    spec = sets.SETS[name]
+    g = sets.genome(spec)
+    r = sets.reads(spec, g)
+    print(f"set {name} mode {mode} split {split}", flush=True)
+    b = _context(spec).batch(mode=mode, chunk_reads=len(r), len_split=split)
+    b.upload(api.prepare(r))
+    b.run()
+    dn, dc = b.candidates()
+    b.close()
+    on, oc = _oracle_candidates(spec, g, r)
+    assert np.array_equal(dn, on), np.flatnonzero(dn != on)[:5]
+    cols = np.stack([dc["rev"].astype(np.int64), dc["contig"].astype(np.int64), dc["diag"],
+                     dc["chain_score"].astype(np.int64)], axis=-1)
+    bad = [i for i in range(len(r)) if not np.array_equal(cols[i, :on[i]], oc[i, :on[i]])]
+    assert not bad, (len(bad), [(i, cols[i, :on[i]].tolist(), oc[i, :on[i]].tolist()) for i in bad[:3]])
