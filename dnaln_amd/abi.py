@@ -90,6 +90,7 @@ HIP_SYMBOLS = {
     "dnaln_hip_destroy": (None, [vp]),
     "dnaln_hip_last_error": (C.c_char_p, []),
     "dnaln_hip_align": (i32, [vp, vp, vp, u64, vp, u64, i32, vp, vp, u64]),
+    "dnaln_hip_align_static": (i32, [vp, vp, vp, u64, vp, u64, i32, vp, vp, u64]),
     "dnaln_hip_map_batch": (i32, [vp, vp, vp, vp, vp, u64, vp, vp, u64, vp]),
     "dnaln_hip_default_batch_config": (None, [vp]),
     "dnaln_hip_batch_create": (i32, [vp, vp, vp]),

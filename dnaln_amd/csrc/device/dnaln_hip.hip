@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -626,6 +627,7 @@ struct dnaln_hip_ctx {
   DevBuf a_pool, a_in, a_tasks, a_res, a_cig;
   SwWork a_sw;
   dnaln_hip_batch* batch = nullptr;  // used by dnaln_hip_map_batch
+  uint8_t* scratch = nullptr;         // device address of this GPU's static scratch array
   ~dnaln_hip_ctx() {
     if (stream) (void)hipStreamDestroy(stream);  // also when create fails part-way
   }
@@ -1350,6 +1352,140 @@ int upload_reference(DevBuf& buf, const uint8_t* codes, uint64_t len, hipStream_
 
 }  // namespace
 
+// ------------------------------------------------------------------ alignment without allocation
+// dnaln_hip_align_static works entirely inside this array: one copy per GPU, held for the life of
+// the process. In host code the name `scratch` is a host-side stand-in, not a device address, so
+// host code never hands it to a kernel or a copy: it uses the address hipGetSymbolAddress gave
+// for the device (scratch_base, looked up once per device with that device current).
+static __device__ __align__(256) uint8_t scratch[DNALN_STATIC_SCRATCH_BYTES];
+
+namespace {
+
+std::mutex g_scratch_mu;
+uint8_t* g_scratch_at[256] = {};  // by device ordinal
+
+int scratch_base(int device, uint8_t** out) {
+  if (device < 0 || device >= 256) return fail(DNALN_E_INVALID, "device ordinal out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_at[device]) {
+    void* p = nullptr;
+    DN_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_at[device] = static_cast<uint8_t*>(p);
+  }
+  *out = g_scratch_at[device];
+  return DNALN_OK;
+}
+
+// Byte offsets of one dnaln_hip_align_static call's buffers inside the array (256-aligned).
+struct StaticLayout {
+  uint64_t ctl, pool, tin, tasks, res, cig, dir, rows, off, out, outcig, end;
+};
+
+// Status words of a static call: the CIGAR total and error bits (1: a CIGAR overflowed its slot
+// or the walk met an inconsistent cell).
+struct StaticCtl {
+  uint64_t total;
+  uint32_t err;
+  uint32_t reserved;
+};
+
+StaticLayout static_layout(uint64_t n, uint64_t pool_len, int qmax, int W, bool packed, bool tb) {
+  auto up = [](uint64_t x) { return (x + 255) & ~(uint64_t)255; };
+  const uint64_t cap = 2 * (uint64_t)qmax + 4;
+  StaticLayout L{};
+  uint64_t o = 0;
+  L.ctl = o;
+  o += 256;
+  L.pool = o;
+  o = up(o + pool_len + 2 * kReadGuard);
+  L.tin = o;
+  o = up(o + n * sizeof(dnaln_task));
+  L.tasks = o;
+  o = up(o + n * sizeof(SwTask));
+  L.res = o;
+  o = up(o + n * sizeof(SwResult));
+  L.cig = o;
+  o = up(o + (tb ? n * cap * 4 : 0));
+  L.dir = o;
+  if (packed) {
+    const uint64_t waves = (n + 127) / 128, row_words = (uint64_t)((W + 7) / 8) * 64;
+    o = up(o + (tb ? waves * fill_rows(qmax) * row_words * 8 : 0));
+    L.rows = o;
+  } else {
+    o = up(o + (tb ? n * (uint64_t)std::max(qmax, 1) * W : 0));
+    L.rows = o;
+    o = up(o + n * 2 * (uint64_t)(W + 1) * 4);
+  }
+  L.off = o;
+  o = up(o + n * 8);
+  L.out = o;
+  o = up(o + n * sizeof(dnaln_result));
+  L.outcig = o;
+  o = up(o + (tb ? n * cap * 4 : 0));
+  L.end = o;
+  return L;
+}
+
+// Exclusive scan of the CIGAR lengths of n results, by one block: off[i], and the total with
+// the error bit of any failed walk in ctl.
+__global__ __launch_bounds__(1024) void k_static_scan(const SwResult* res, uint32_t n, uint64_t* off, StaticCtl* ctl) {
+  __shared__ uint64_t part[1024];
+  __shared__ uint64_t carry;
+  __shared__ uint32_t err;
+  const uint32_t t = threadIdx.x;
+  if (t == 0) carry = 0, err = 0;
+  __syncthreads();
+  for (uint32_t base = 0; base < n; base += 1024) {
+    const uint32_t i = base + t;
+    uint64_t v = 0;
+    if (i < n) {
+      const SwResult r = res[i];
+      if (r.n_cigar < 0 || r.status != 0) atomicOr(&err, 1u);
+      else v = (uint64_t)r.n_cigar;
+    }
+    part[t] = v;
+    __syncthreads();
+    for (uint32_t d = 1; d < 1024; d <<= 1) {  // inclusive Hillis-Steele scan
+      const uint64_t x = t >= d ? part[t - d] : 0;
+      __syncthreads();
+      part[t] += x;
+      __syncthreads();
+    }
+    if (i < n) off[i] = carry + part[t] - v;
+    __syncthreads();
+    if (t == 1023) carry += part[1023];
+    __syncthreads();
+  }
+  if (t == 0) {
+    ctl->total = carry;
+    ctl->err = err;
+  }
+}
+
+// AlignResult of task i, and its CIGAR in forward order at outcig[off[i]] (the walk stored it
+// reversed in the task's slot of cap words).
+__global__ void k_static_out(const SwResult* res, const uint32_t* slots, int cap, const uint64_t* off, uint32_t n,
+                             dnaln_result* out, uint32_t* outcig) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const SwResult r = res[i];
+  dnaln_result o;
+  o.score = r.score;
+  o.qb = r.qb;
+  o.qe = r.qe;
+  o.tb = r.tb;
+  o.te = r.te;
+  o.nm = r.nm;
+  o.cigar_off = off[i];
+  o.cigar_len = r.n_cigar > 0 && r.status == 0 ? (uint32_t)r.n_cigar : 0u;
+  o.reserved = 0;
+  out[i] = o;
+  const uint32_t* s = slots + (uint64_t)i * (uint64_t)cap;
+  for (uint32_t k = 0; k < o.cigar_len; ++k) outcig[o.cigar_off + k] = s[o.cigar_len - 1 - k];
+}
+
+}  // namespace
+
 // A GPU-built index held on its device until downloaded (dnaln_hip_index_build).
 struct dnaln_hip_index {
   int device = 0;
@@ -1392,6 +1528,7 @@ int dnaln_hip_create(const dnaln_map_params* params, const uint8_t* ref_codes, u
     c->device = device;
     DN_HIP(hipSetDevice(device));
     DN_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    DN_TRY(scratch_base(device, &c->scratch));
     c->ref_len = ref_len;
     c->nc = n_contigs;
     c->k = params->k;
@@ -1522,6 +1659,106 @@ int dnaln_hip_align(dnaln_hip_ctx* c, const dnaln_score_params* spp, const dnaln
         used += (uint64_t)r.n_cigar;
       }
     }
+    return DNALN_OK;
+  });
+}
+
+int dnaln_hip_align_static(dnaln_hip_ctx* c, const dnaln_score_params* spp, const dnaln_task* tasks, uint64_t n,
+                           const uint8_t* seq_pool, uint64_t pool_len, int traceback, dnaln_result* out,
+                           uint32_t* cigar_pool, uint64_t cigar_cap) {
+  return guarded([&]() -> int {
+    if (!c || (n && (!tasks || !out || !seq_pool))) return fail(DNALN_E_INVALID, "null argument");
+    if (n >= (1ull << 31)) return fail(DNALN_E_INVALID, "too many tasks");
+    const dnaln_score_params s = spp ? *spp : c->p.sc;
+    if (s.bw < 1 || s.bw > 1000 || s.match < 1 || s.mismatch < 0 || s.gap_open < 0 || s.gap_ext < 1 || s.ambig < 0)
+      return fail(DNALN_E_INVALID, "score parameters outside the validated range");
+    const ScoreP sp{s.match, s.mismatch, s.gap_open, s.gap_ext, s.ambig, s.bw};
+    int qmax = 0, qmin = n ? tasks[0].qlen : 0;
+    for (uint64_t i = 0; i < n; ++i) {
+      if (tasks[i].qlen < 0 || tasks[i].tlen < 0 || tasks[i].q_off + (uint64_t)tasks[i].qlen > pool_len ||
+          tasks[i].t_off + (uint64_t)tasks[i].tlen > pool_len)
+        return fail(DNALN_E_INVALID, "task outside the sequence pool");
+      qmax = std::max(qmax, tasks[i].qlen);
+      qmin = std::min(qmin, tasks[i].qlen);
+    }
+    if (n == 0) return DNALN_OK;
+    const int W = 2 * sp.bw + 1, cap = 2 * qmax + 4;
+    const bool tb = traceback != 0;
+    const bool packed = packed_bw_available(sp.bw) && packed_ok(sp, qmax);
+    const StaticLayout L = static_layout(n, pool_len, qmax, W, packed, tb);
+    if (L.end > DNALN_STATIC_SCRATCH_BYTES)
+      return fail(DNALN_E_CAPACITY, "the call needs " + std::to_string(L.end) + " bytes of device scratch, more than " +
+                                        std::to_string((unsigned long long)DNALN_STATIC_SCRATCH_BYTES));
+    DN_HIP(hipSetDevice(c->device));
+    hipStream_t st = c->stream;
+    uint8_t* base = c->scratch;
+    auto at = [base](uint64_t o) { return base + o; };
+    uint8_t* pool = at(L.pool) + kReadGuard;
+    DN_HIP(hipMemcpyAsync(pool, seq_pool, pool_len, hipMemcpyHostToDevice, st));
+    DN_HIP(hipMemcpyAsync(at(L.tin), tasks, n * sizeof(dnaln_task), hipMemcpyHostToDevice, st));
+    SwTask* tk = reinterpret_cast<SwTask*>(at(L.tasks));
+    SwResult* res = reinterpret_cast<SwResult*>(at(L.res));
+    uint32_t* cig = reinterpret_cast<uint32_t*>(at(L.cig));
+    hipLaunchKernelGGL(k_pool_tasks, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+                       reinterpret_cast<const dnaln_task*>(at(L.tin)), (uint64_t)(uintptr_t)pool, sp.bw, tk, (uint32_t)n);
+    DN_HIP(hipGetLastError());
+    if (packed) {
+      // The fill of dnaln_hip_align in one launch, tasks in call order, fixed-size slots.
+      SwArgs a{};
+      a.sp = sp;
+      score_table(sp, a.tab0, a.tab1);
+      a.cigar_cap = cap;
+      a.tasks = tk;
+      a.order = nullptr;
+      a.res = res;
+      a.cigar = cig;
+      a.n = (uint32_t)n;
+      a.dir = tb ? reinterpret_cast<uint64_t*>(at(L.dir)) : nullptr;
+      a.dir_slot_words = fill_rows(qmax) * (uint64_t)((W + 7) / 8) * 64;
+      a.woff = nullptr;
+      a.woff_base = 0;
+      const unsigned lw = (unsigned)((n + 127) / 128);
+      const unsigned grid = (lw + kSwBlock / 64 - 1) / (kSwBlock / 64);
+      const bool ok = tb ? launch_packed_bw<true>(sp.bw, a, grid, st) : launch_packed_bw<false>(sp.bw, a, grid, st);
+      if (!ok) return fail(DNALN_E_INTERNAL, "no packed kernel for this band width");
+      DN_HIP(hipGetLastError());
+      if (tb) {
+        if (!launch_walk_bw(sp.bw, a, 0, st)) return fail(DNALN_E_INTERNAL, "no walk kernel for this band width");
+        DN_HIP(hipGetLastError());
+      }
+    } else {
+      SwGenericArgs g{};
+      g.sp = sp;
+      g.cigar_cap = cap;
+      g.traceback = tb;
+      g.tasks = tk;
+      g.order = nullptr;
+      g.n = (uint32_t)n;
+      g.res = res;
+      g.cigar = cig;
+      g.rows = reinterpret_cast<int32_t*>(at(L.rows));
+      g.dir = tb ? at(L.dir) : nullptr;
+      g.dir_stride = tb ? (uint64_t)std::max(qmax, 1) * W : 0;
+      hipLaunchKernelGGL(k_sw_generic, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, st, g);
+      DN_HIP(hipGetLastError());
+    }
+    StaticCtl* ctl = reinterpret_cast<StaticCtl*>(at(L.ctl));
+    uint64_t* off = reinterpret_cast<uint64_t*>(at(L.off));
+    dnaln_result* dout = reinterpret_cast<dnaln_result*>(at(L.out));
+    uint32_t* outcig = reinterpret_cast<uint32_t*>(at(L.outcig));
+    hipLaunchKernelGGL(k_static_scan, dim3(1), dim3(1024), 0, st, res, (uint32_t)n, off, ctl);
+    DN_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_static_out, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, res, cig, tb ? cap : 0, off,
+                       (uint32_t)n, dout, outcig);
+    DN_HIP(hipGetLastError());
+    StaticCtl h{};
+    DN_HIP(hipMemcpyAsync(&h, ctl, sizeof h, hipMemcpyDeviceToHost, st));
+    DN_HIP(hipStreamSynchronize(st));
+    if (h.err) return fail(DNALN_E_INTERNAL, "traceback failed on a task");
+    if (h.total && (!cigar_pool || h.total > cigar_cap)) return fail(DNALN_E_CAPACITY, "cigar_pool too small");
+    DN_HIP(hipMemcpyAsync(out, dout, n * sizeof(dnaln_result), hipMemcpyDeviceToHost, st));
+    if (h.total) DN_HIP(hipMemcpyAsync(cigar_pool, outcig, h.total * 4, hipMemcpyDeviceToHost, st));
+    DN_HIP(hipStreamSynchronize(st));
     return DNALN_OK;
   });
 }

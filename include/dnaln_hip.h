@@ -176,6 +176,22 @@ int dnaln_hip_align(struct dnaln_hip_ctx* ctx, const dnaln_score_params* sp, con
                     const uint8_t* seq_pool, uint64_t pool_len, int traceback, dnaln_result* out,
                     uint32_t* cigar_pool, uint64_t cigar_cap);
 
+/* Device scratch of dnaln_hip_align_static: one array of this size per GPU the library runs on,
+ * held until the process ends. */
+#define DNALN_STATIC_SCRATCH_BYTES (64ull << 20)
+
+/* Aligner::align exactly as dnaln_hip_align (same arguments, results and errors), for callers
+ * that cannot allocate device memory: code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc. No counterpart in the reference. It makes no device
+ * allocation: every buffer of the call (the copied tasks and sequence pool, results, traceback
+ * directions, CIGARs) lies in one static device array of DNALN_STATIC_SCRATCH_BYTES per GPU,
+ * and a call whose buffers do not fit returns DNALN_E_CAPACITY. Every call on a GPU shares that
+ * GPU's array, so calls that overlap on one GPU (from several threads or contexts) race; the
+ * caller serialises them. Nothing of a previous call's contents is read. */
+int dnaln_hip_align_static(struct dnaln_hip_ctx* ctx, const dnaln_score_params* sp, const dnaln_task* tasks,
+                           uint64_t n, const uint8_t* seq_pool, uint64_t pool_len, int traceback, dnaln_result* out,
+                           uint32_t* cigar_pool, uint64_t cigar_cap);
+
 /* Mapper::map on n reads (forward-strand codes 0..4 at read_codes[offs[i] .. offs[i]+lens[i]),
  * name_fnv1a[i] = FNV-1a of the read name for the equal-best tie-break). Records are identical
  * to the reference's. stats may be NULL. */
