@@ -64,7 +64,7 @@ class MapStats(C.Structure):
 class BatchConfig(C.Structure):
     _fields_ = [("mode", i32), ("chunk_reads", u32), ("max_reads", u64), ("max_bases", u64),
                 ("anchors_per_read", u32), ("tune", i32), ("len_split", i32 * 2), ("dir_budget", u64),
-                ("walk_kind", i32), ("reserved", i32)]
+                ("walk_kind", i32), ("graph", i32)]
 
 
 class TuneReport(C.Structure):

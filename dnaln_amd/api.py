@@ -265,15 +265,21 @@ class Context:
 
     def batch(self, mode: int = abi.MODE_PLAIN, chunk_reads: int = 65536, max_reads: int = 0, max_bases: int = 0,
               tune: bool = False, len_split: tuple[int, int] = (0, 0), dir_budget: int = 0,
-              walk_kind: int = 0) -> "Batch":
-        return Batch(self, mode, chunk_reads, max_reads, max_bases, tune, len_split, dir_budget, walk_kind)
+              walk_kind: int = 0, graph: int = 0, anchors_per_read: int = 0) -> "Batch":
+        return Batch(self, mode, chunk_reads, max_reads, max_bases, tune, len_split, dir_budget, walk_kind, graph,
+                     anchors_per_read)
 
 
 class Batch:
     """A device-resident batch (dnaln_hip_batch_*): upload once, run many times, download."""
 
     def __init__(self, ctx: Context, mode: int, chunk_reads: int, max_reads: int, max_bases: int, tune: bool,
-                 len_split: tuple[int, int] = (0, 0), dir_budget: int = 0, walk_kind: int = 0):
+                 len_split: tuple[int, int] = (0, 0), dir_budget: int = 0, walk_kind: int = 0, graph: int = 0,
+                 anchors_per_read: int = 0):
+        """graph (batched form): 0 one captured HIP graph per plan with timing events inside, 1 the
+        graph without them, 2 the stages launched one by one. anchors_per_read (batched form): first
+        anchor capacity of the global path per read (0: estimated; a run that needs more grows it
+        and runs again)."""
         self.ctx = ctx
         cfg = abi.BatchConfig()
         _lib.hip().dnaln_hip_default_batch_config(C.byref(cfg))
@@ -281,6 +287,9 @@ class Batch:
         cfg.len_split[0], cfg.len_split[1] = len_split
         cfg.dir_budget = dir_budget
         cfg.walk_kind = walk_kind
+        cfg.graph = graph
+        if anchors_per_read:
+            cfg.anchors_per_read = anchors_per_read
         self.cfg = cfg
         self.b = C.c_void_p()
         _check(_lib.hip().dnaln_hip_batch_create(ctx.ctx, C.byref(cfg), C.byref(self.b)), "dnaln_hip_batch_create")

@@ -85,10 +85,12 @@ struct DevBuf {
   ~DevBuf() {
     if (p) (void)hipFree(p);
   }
-  int reserve(size_t bytes) {
+  // Grows the buffer, discarding its contents. Work still queued on the buffer must finish
+  // first: `user` names the one stream that uses it (waited for), else the whole device is.
+  int reserve(size_t bytes, hipStream_t user = nullptr) {
     if (bytes <= cap) return DNALN_OK;
     if (p) {
-      (void)hipDeviceSynchronize();
+      (void)(user ? hipStreamSynchronize(user) : hipDeviceSynchronize());
       (void)hipFree(p);
       p = nullptr;
       cap = 0;
@@ -603,6 +605,8 @@ bool packed_bw_available(int bw) {
 
 }  // namespace
 
+#include "batched_graph.hpp"
+
 // ------------------------------------------------------------------------------------ context
 // Sizes of a device index, and the wall time of its GPU build (0 when uploaded from the host).
 struct IndexInfo {
@@ -650,10 +654,6 @@ int validate(const dnaln_map_params& p) {
 // Runs a score-only or traceback fill over tasks[0, n) (device arrays), results in res[n],
 // reversed CIGARs in cig[n * cap]. Chunks the traceback launches so the direction scratch stays
 // under `dir_budget` bytes.
-__global__ void k_iota(uint32_t* out, uint64_t n) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) out[i] = (uint32_t)i;
-}
 
 __global__ void k_len_keys(const SwTask* tasks, uint32_t n, uint32_t* keys, uint32_t* ids) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -827,21 +827,46 @@ int exclusive_scan(DevBuf& tmp, const T* in, uint64_t* out, uint32_t n, hipStrea
 }  // namespace
 
 // ------------------------------------------------------------------------------------ batches
+// What a captured graph of the batched form bakes in: the stages it runs, their shapes and
+// grids, the traceback launch cuts, and its buffers (`gen` changes whenever one is reallocated).
+// A run whose plan differs from the captured one captures a new graph.
+struct GraphPlan {
+  int32_t sorted, bucket, split0, split1;
+  int32_t fused_on[2], fs_cap[2], fs_maxmin[2];
+  int32_t retry_on, retry_cap, retry_maxmin;
+  uint32_t bins;
+  int32_t maxlen;          // CIGAR slot words per task: 2 * maxlen + 4
+  int32_t packed, walk_kind, spans, wave_chain, seed_lds;
+  uint32_t n_cuts, cut_waves, generic_threads;
+  uint64_t slot_words;     // direction words per fill wave when slots are of one size (packed),
+                           // direction bytes per thread (int32 fill)
+  uint64_t dir_words;      // direction buffer of one cut
+  uint64_t anchor_cap;
+  uint64_t n_grid;         // reads the grids cover (a power of two >= the batch size)
+  uint64_t gen;
+};
+
+// A captured graph and the plan it was captured under.
+struct GraphEntry {
+  GraphPlan plan;
+  hipGraphExec_t exec;
+  size_t spans;  // timing spans it records (after the whole-run span)
+  uint64_t launches[6];  // SW launches it holds, per timing family
+};
+
 struct dnaln_hip_batch : SpanSink {
   dnaln_hip_ctx* ctx = nullptr;
   dnaln_batch_config cfg{};
   hipStream_t st = nullptr;
   // inputs
   DevBuf codes, offs, lens, fnv;
-  uint64_t n = 0, bases = 0;
+  uint64_t n = 0, bases = 0, sum_len = 0;
   int minlen = 0, maxlen = 0;
   int split[2] = {0, 0};             // resolved len_split
   uint64_t n_short = 0, n_mid = 0;   // reads with length <= split[0], <= split[1]
   std::vector<int32_t> lens_h;       // host copy of the read lengths (bucket counts)
   bool tuned = false;
-  DevBuf lids, lkeys, lorder, ltmp;  // length-sorted read order
   DevBuf fmins;                      // minimizer lists of the fused kernel
-  DevBuf ovf2;                       // reads the fused kernel's second pass declined
   // workspace
   DevBuf acnt, aoff, keys, f, prev, order, chains, ring_h, ring_ps;
   DevBuf cands, ncand, rcnt, roff, rtasks, rres, chosen, second, tbflag, tboff, ttasks, tres, tcig;
@@ -851,6 +876,18 @@ struct dnaln_hip_batch : SpanSink {
   DevBuf rec, pool;
   uint64_t pool_used = 0;
   dnaln_map_stats stats{};
+  // The batched form as a graph (batched_graph.hpp): control block, length-sort histogram and
+  // order, global list, scan partials, per-position and per-read task offsets, slot offsets.
+  DevBuf gctl, ghist, gorder, glist, gpart, groffp, grank_off, gtoffp, gtb_idx, gwoff;
+  std::vector<GraphEntry> graphs;  // captured graphs, most recently used first (a few plans)
+  GraphPlan gplan{};        // plan of the current run
+  hipGraphExec_t gexec = nullptr;  // graph of the current run
+  size_t gspans = 0;        // timing spans it records (after the whole-run span)
+  uint64_t gen = 0;         // bumped when a buffer of the graph is reallocated
+  uint64_t anchor_cap = 0;  // anchors the global path may hold (grown when a run overflows)
+  bool graph_events_ok = true;  // event records can be captured (else the graph holds none)
+  BatchCtl ctl_h{};         // the control block of the last finished run
+  std::vector<uint32_t> len_hist;  // host histogram of read lengths (direction-slot bound)
   // Timing: event pairs recorded on the batch stream around each launch group, summed per
   // family (kFamilies) at poll time.
   struct Span {
@@ -1010,10 +1047,6 @@ FusedShape fused_retry_shape(int k, int w, int maxlen) {
   return f;
 }
 
-// Minimizer scratch of a fused launch over n reads (u64 entries).
-uint64_t fused_mins_words(const FusedShape& fs, uint64_t n) {
-  return (n + kFusedBlock - 1) / kFusedBlock * kFusedBlock * (uint64_t)fs.maxmin;
-}
 
 // Default length buckets: the longest reads whose expected minimizer list fits the 32-entry
 // shape (200 bp at most) and the 64-entry shape; 0 where the shape cannot be used.
@@ -1031,14 +1064,6 @@ void default_splits(int k, int w, int out[2]) {
     }
 }
 
-int launch_fused(const FusedShape& fs, const SeedArgs& sa, const ChainArgs& ca, uint64_t* mins, uint32_t* ovf,
-                 uint32_t* novf, hipStream_t st) {
-  const unsigned grid = (sa.n_reads + kFusedBlock - 1) / kFusedBlock;
-  hipLaunchKernelGGL((k_seed_chain_fused<10>), dim3(grid), dim3(kFusedBlock), fs.lds, st, sa, ca, fs.cap, fs.maxmin,
-                     mins, ovf, novf);
-  DN_HIP(hipGetLastError());
-  return DNALN_OK;
-}
 
 // The context's chain and candidate parameters (the chain band already clamped to bw).
 ChainSetup chain_setup(const dnaln_hip_ctx& c) {
@@ -1055,7 +1080,510 @@ ChainSetup chain_setup(const dnaln_hip_ctx& c) {
   return s;
 }
 
-// One chunk of reads [r0, r0 + n) through the whole pipeline (plain form).
+// ------------------------------------------------------------------ batched form as a graph
+// Compute units of a device (persistent grids are sized from it).
+int device_cus(int device) {
+  static std::mutex mu;
+  static int cache[256] = {};
+  std::lock_guard<std::mutex> lk(mu);
+  if (device < 0 || device >= 256) return 256;
+  if (!cache[device]) {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || v <= 0) {
+      (void)hipGetLastError();
+      v = 256;
+    }
+    cache[device] = v;
+  }
+  return cache[device];
+}
+
+// Threads of the per-read kernels of the graph (grid-stride loops), from the CU count.
+unsigned read_blocks(int device) { return (unsigned)device_cus(device) * 8; }
+
+// Grows buffer d of batch b for the graph path; a reallocation changes the buffer generation,
+// so the next run captures a new graph. Only the batch's own stream can be using d.
+int greserve(dnaln_hip_batch* b, DevBuf& d, uint64_t bytes) {
+  if (bytes <= d.cap) return DNALN_OK;
+  DN_TRY(d.reserve(bytes, b->st));
+  ++b->gen;
+  return DNALN_OK;
+}
+
+uint64_t next_pow2(uint64_t x) {
+  uint64_t p = 1;
+  while (p < x) p <<= 1;
+  return p;
+}
+
+// Bound on the traceback direction words of a run, from the batch's read lengths. The device
+// sorts reads by length bin, bin = min(len, bins - 1), longest first; a fill wave's slot holds
+// the rows of its longest task. Count a read of the top bin as maxlen (its bin is not sorted
+// inside): then the traceback tasks, a subset of the reads in the same order, put no task longer
+// than the (128 w)-th read of this order into wave w. Walks the histogram from the top.
+uint64_t dir_bound_words(const std::vector<uint32_t>& hist, uint64_t n, uint64_t row_words, uint32_t bins) {
+  uint64_t total = 0, seen = 0, next = 0;  // next: position of the next wave's first task
+  const int64_t maxlen = (int64_t)hist.size() - 1;
+  for (int64_t L = maxlen; L >= 0 && next < n; --L) {
+    const uint64_t c = hist[(size_t)L];
+    const int eff = bins && L >= (int64_t)bins - 1 ? (int)maxlen : (int)L;
+    while (next < seen + c && next < n) {
+      total += fill_rows(eff) * row_words;
+      next += 128;
+    }
+    seen += c;
+  }
+  return total;
+}
+
+// Sizes and reserves every buffer of the batched graph for the resident batch, and fills the
+// plan. Reallocations bump b->gen before pl.gen is taken.
+int plan_batched(dnaln_hip_batch* b, GraphPlan& pl) {
+  dnaln_hip_ctx* c = b->ctx;
+  const dnaln_map_params& p = c->p;
+  const ScoreP sp{p.sc.match, p.sc.mismatch, p.sc.gap_open, p.sc.gap_ext, p.sc.ambig, p.sc.bw};
+  std::memset(&pl, 0, sizeof pl);
+  const uint64_t n = b->n, mc = (uint64_t)std::max(p.max_cand, 1);
+  const int W = 2 * p.sc.bw + 1;
+  const uint64_t row_words = (uint64_t)((W + 7) / 8) * 64;
+  pl.n_grid = std::max<uint64_t>(4096, next_pow2(n));
+  pl.sorted = b->minlen < b->maxlen;
+  // Buckets: reads up to split[0] / split[1] bases take the fused kernel in the shape for that
+  // length; a bucket whose shape cannot hold the sketch ring (cap 0) joins the next one, so its
+  // reads take the global path.
+  int cut[2] = {b->split[0], b->split[1]};
+  for (int x = 0; x < 2; ++x) {
+    const FusedShape fs = fused_shape(c->k, c->w, std::min(b->split[x], b->maxlen));
+    pl.fs_cap[x] = fs.cap;
+    pl.fs_maxmin[x] = fs.maxmin;
+    if (!fs.cap) cut[x] = x == 0 ? -1 : cut[0];
+  }
+  pl.split0 = cut[0];
+  pl.split1 = std::max(cut[1], cut[0]);
+  pl.bucket = pl.sorted ? -1 : (b->maxlen <= pl.split0 ? 0 : b->maxlen <= pl.split1 ? 1 : 2);
+  bool small = false;
+  for (int x = 0; x < 2; ++x) {
+    pl.fused_on[x] = pl.fs_cap[x] && (pl.sorted ? (x == 0 ? pl.split0 >= 0 : pl.split1 > pl.split0) : pl.bucket == x);
+    if (pl.fused_on[x] && pl.fs_cap[x] < 64) small = true;
+  }
+  const FusedShape retry = fused_retry_shape(c->k, c->w, b->maxlen);
+  pl.retry_on = small && retry.cap;
+  pl.retry_cap = retry.cap;
+  pl.retry_maxmin = retry.maxmin;
+  pl.bins = pl.sorted ? (uint32_t)std::min<uint64_t>(kMaxLenBins, std::max<uint64_t>(256, next_pow2(b->maxlen + 1))) : 0;
+  pl.maxlen = (int)std::max<uint64_t>(64, next_pow2((uint64_t)std::max(b->maxlen, 1)));
+  pl.packed = packed_bw_available(p.sc.bw) && packed_ok(sp, b->maxlen);
+  pl.walk_kind = b->sw.walk_kind;
+  pl.spans = b->cfg.graph != 1 && b->graph_events_ok;
+  pl.wave_chain = p.ch.lookback <= 64 && p.max_cand <= 64;
+  pl.seed_lds = (size_t)c->w * kSeedBlock * 12 <= 48 * 1024;
+  // Traceback directions: one cut when the bound fits the budget, else cuts of cut_waves waves,
+  // each no larger than the budget at the longest read (at most kMaxCuts cuts).
+  const uint64_t waves = (n + 127) / 128, budget_words = std::max<uint64_t>(b->sw.dir_budget / 8, 1);
+  uint64_t dir_bytes = 0, rows_bytes = 0;
+  if (pl.packed) {
+    const uint64_t at_max = fill_rows(b->maxlen) * row_words;
+    const uint64_t bound = pl.sorted ? dir_bound_words(b->len_hist, n, row_words, pl.bins) : waves * at_max;
+    pl.slot_words = pl.sorted ? 0 : at_max;
+    if (bound <= budget_words) {
+      pl.n_cuts = 1;
+      pl.cut_waves = 0;  // all
+      dir_bytes = bound * 8;
+    } else {
+      uint64_t per = std::max<uint64_t>(1, budget_words / std::max<uint64_t>(at_max, 1));
+      if ((waves + per - 1) / per > (uint64_t)kMaxCuts) per = (waves + kMaxCuts - 1) / kMaxCuts;
+      pl.cut_waves = (uint32_t)per;
+      pl.n_cuts = (uint32_t)((waves + per - 1) / per);
+      dir_bytes = per * at_max * 8;
+    }
+  } else {
+    // Per thread of a fixed grid: the direction bytes of the longest read the plan covers.
+    const uint64_t per_task = (uint64_t)pl.maxlen * W;
+    uint64_t t = std::min<uint64_t>(std::max<uint64_t>(n, 1), std::max<uint64_t>(256, b->sw.dir_budget / per_task));
+    t = (t + 127) / 128 * 128;
+    pl.generic_threads = (uint32_t)t;
+    pl.slot_words = per_task;  // bytes of directions per thread
+    dir_bytes = t * per_task;
+    rows_bytes = t * 2 * (uint64_t)(W + 1) * 4;
+    pl.n_cuts = 1;
+  }
+  // Anchors of the global path: anchors_per_read per read when configured, else an estimate
+  // (8 per read for declined reads, one per two bases of the long reads); a run that needs more
+  // grows the buffer and runs again.
+  {
+    uint64_t want = 0;
+    if (b->cfg.anchors_per_read) {
+      want = n * b->cfg.anchors_per_read + 1;
+    } else {
+      uint64_t long_bases = 0;
+      for (size_t L = (size_t)std::max(b->split[1], 0) + 1; L < b->len_hist.size(); ++L)
+        long_bases += L * b->len_hist[L];
+      want = n * 8 + long_bases / 2 + 65536;
+    }
+    b->anchor_cap = std::max(b->anchor_cap, want);
+  }
+  const uint64_t na = b->anchor_cap, cap = 2 * (uint64_t)pl.maxlen + 4;
+  // Reserve (grow-only) every buffer the graph touches.
+  DN_TRY(greserve(b, b->gctl, sizeof(BatchCtl)));
+  DN_TRY(greserve(b, b->ghist, std::max<uint64_t>(pl.bins, 1) * 4));
+  DN_TRY(greserve(b, b->gorder, n * 4 + 4));
+  DN_TRY(greserve(b, b->cands, n * mc * sizeof(Cand) + sizeof(Cand)));
+  DN_TRY(greserve(b, b->ncand, n * 4 + 4));
+  DN_TRY(greserve(b, b->ovf, n * 4 + 4));
+  DN_TRY(greserve(b, b->glist, n * 4 + 4));
+  DN_TRY(greserve(b, b->fmins, (n + kFusedBlock) / kFusedBlock * kFusedBlock * 64 * 8));
+  DN_TRY(greserve(b, b->acnt, n * 4 + 4));
+  DN_TRY(greserve(b, b->aoff, n * 8 + 8));
+  DN_TRY(greserve(b, b->keys, na * 8 + 8));
+  DN_TRY(greserve(b, b->f, na * 4 + 4));
+  DN_TRY(greserve(b, b->prev, na * 4 + 4));
+  DN_TRY(greserve(b, b->order, na * 8 + 8));
+  DN_TRY(greserve(b, b->chains, na * sizeof(ChainRec) + sizeof(ChainRec)));
+  if (!pl.seed_lds) {
+    const uint64_t threads = (pl.n_grid + kSeedBlock - 1) / kSeedBlock * kSeedBlock;
+    DN_TRY(greserve(b, b->ring_h, threads * c->w * 8));
+    DN_TRY(greserve(b, b->ring_ps, threads * c->w * 4));
+  }
+  DN_TRY(greserve(b, b->gpart, kScanBlocks * 8));
+  DN_TRY(greserve(b, b->groffp, n * 8 + 8));
+  DN_TRY(greserve(b, b->grank_off, n * 8 + 8));
+  DN_TRY(greserve(b, b->rtasks, n * mc * sizeof(SwTask) + sizeof(SwTask)));
+  DN_TRY(greserve(b, b->rres, n * mc * sizeof(SwResult) + sizeof(SwResult)));
+  DN_TRY(greserve(b, b->chosen, n * 4 + 4));
+  DN_TRY(greserve(b, b->second, n * 4 + 4));
+  DN_TRY(greserve(b, b->gtoffp, n * 8 + 8));
+  DN_TRY(greserve(b, b->gtb_idx, n * 8 + 8));
+  DN_TRY(greserve(b, b->ttasks, n * sizeof(SwTask) + sizeof(SwTask)));
+  DN_TRY(greserve(b, b->tres, n * sizeof(SwResult) + sizeof(SwResult)));
+  DN_TRY(greserve(b, b->tcig, n * cap * 4 + 4));
+  DN_TRY(greserve(b, b->gwoff, (waves + 2) * 8));
+  DN_TRY(greserve(b, b->sw.dir, std::max<uint64_t>(dir_bytes, 8)));
+  if (rows_bytes) DN_TRY(greserve(b, b->sw.rows, rows_bytes));
+  DN_TRY(greserve(b, b->ciglen, n * 4 + 4));
+  DN_TRY(greserve(b, b->cigoff, n * 8 + 8));
+  DN_TRY(greserve(b, b->pool, (2 * b->sum_len + 4 * n + 4) * 4));
+  DN_TRY(greserve(b, b->rec, n * sizeof(dnaln_record) + sizeof(dnaln_record)));
+  pl.anchor_cap = na;
+  pl.dir_words = b->sw.dir.cap / 8;
+  pl.gen = b->gen;
+  return DNALN_OK;
+}
+
+// Every stage of the batched form on the batch stream (captured into a graph, or launched
+// directly). Nothing here reads a count on the host or touches memory outside a kernel.
+int enqueue_batched(dnaln_hip_batch* b, const GraphPlan& pl, bool spans) {
+  dnaln_hip_ctx* c = b->ctx;
+  const dnaln_map_params& p = c->p;
+  hipStream_t st = b->st;
+  const int max_cand = std::max(p.max_cand, 0);
+  const ScoreP sp{p.sc.match, p.sc.mismatch, p.sc.gap_open, p.sc.gap_ext, p.sc.ambig, p.sc.bw};
+  const int W = 2 * sp.bw + 1, cap = 2 * pl.maxlen + 4;
+  const uint64_t row_words = (uint64_t)((W + 7) / 8) * 64;
+  const uint64_t ng = pl.n_grid, mc = (uint64_t)std::max(max_cand, 1);
+  const unsigned rb = read_blocks(c->device);
+  BatchCtl* ctl = b->gctl.as<BatchCtl>();
+  uint32_t* order = pl.sorted ? b->gorder.as<uint32_t>() : nullptr;
+  size_t si = 0;
+  auto open = [&](int family) -> int { return spans ? b->span_begin(family, si) : DNALN_OK; };
+  auto close = [&]() -> int { return spans ? b->span_end(si) : DNALN_OK; };
+
+  ReadView rv;
+  rv.codes = b->codes.as<uint8_t>() + kReadGuard;
+  rv.offs = b->offs.as<uint64_t>();
+  rv.lens = b->lens.as<int32_t>();
+  rv.fnv = b->fnv.as<uint32_t>();
+  MapGeom g;
+  g.ref = (uint64_t)(uintptr_t)c->ref_base;
+  g.coff = c->coff.as<uint64_t>();
+  g.clen = c->clen.as<uint32_t>();
+  g.bw = p.sc.bw;
+  SeedArgs sa{};
+  sa.codes = rv.codes;
+  sa.offs = rv.offs;
+  sa.lens = rv.lens;
+  sa.table = c->table.as<Bucket>();
+  sa.mask = c->mask;
+  sa.pos = c->pos.as<uint32_t>();
+  sa.pos_inline = 1;
+  sa.k = c->k;
+  sa.w = c->w;
+  sa.max_occ = p.max_occ;
+  ChainArgs ca{};
+  init_chain_args(ca, chain_setup(*c), c->cost.as<int32_t>(), c->h_cost.data(), c->coff.as<uint64_t>(), c->nc);
+  ca.cands = b->cands.as<Cand>();
+  ca.n_cand = b->ncand.as<int32_t>();
+
+  // 1. reset, length order, fused seed + chain
+  DN_TRY(open(0));
+  hipLaunchKernelGGL(k_reset, dim3(1), dim3(256), 0, st, ctl, pl.sorted ? -1 : pl.bucket, b->ghist.as<uint32_t>(),
+                     pl.bins);
+  DN_HIP(hipGetLastError());
+  if (pl.sorted) {
+    hipLaunchKernelGGL(k_len_hist, dim3(rb / 4), dim3(kHistThreads), (size_t)pl.bins * 4, st, rv.lens, ctl,
+                       pl.bins, b->ghist.as<uint32_t>(), pl.split0, pl.split1);
+    hipLaunchKernelGGL(k_len_starts, dim3(1), dim3(kHistThreads), 0, st, ctl, pl.bins, b->ghist.as<uint32_t>());
+    hipLaunchKernelGGL(k_len_scatter, dim3(rb), dim3(256), 0, st, rv.lens, ctl, pl.bins,
+                       b->ghist.as<uint32_t>(), order);
+    DN_HIP(hipGetLastError());
+  }
+  uint32_t* ovf = b->ovf.as<uint32_t>();
+  uint32_t* glist = b->glist.as<uint32_t>();
+  uint64_t* fmins = b->fmins.as<uint64_t>();
+  for (int x = 0; x < 2; ++x) {
+    if (!pl.fused_on[x]) continue;
+    hipLaunchKernelGGL((k_fused_g<10>), dim3((unsigned)(ng / kFusedBlock)), dim3(kFusedBlock),
+                       (size_t)kFusedBlock * 8 * pl.fs_cap[x], st,
+                       sa, ca, pl.fs_cap[x], pl.fs_maxmin[x], fmins, ctl, x, order, (const uint32_t*)nullptr,
+                       pl.retry_on ? ovf : glist, !pl.retry_on);
+    DN_HIP(hipGetLastError());
+  }
+  if (pl.retry_on) {
+    hipLaunchKernelGGL((k_fused_g<10>), dim3((unsigned)(ng / kFusedBlock)), dim3(kFusedBlock),
+                       (size_t)kFusedBlock * 8 * pl.retry_cap, st,
+                       sa, ca, pl.retry_cap, pl.retry_maxmin, fmins, ctl, 2, order, (const uint32_t*)ovf, glist, true);
+    DN_HIP(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_long_to_global, dim3(1), dim3(1024), 0, st, ctl, order, glist, &ctl->nglobal);
+  DN_HIP(hipGetLastError());
+  DN_TRY(close());
+
+  // 2. global path: long reads and declined reads
+  DN_TRY(open(0));
+  sa.list = glist;
+  sa.n_reads = 0;
+  sa.anchor_count = b->acnt.as<uint32_t>();
+  sa.anchor_off = b->aoff.as<uint64_t>();
+  sa.keys = b->keys.as<uint64_t>();
+  const size_t seed_lds = pl.seed_lds ? (size_t)c->w * kSeedBlock * 12 : 0;
+  uint64_t* ring_h = pl.seed_lds ? nullptr : b->ring_h.as<uint64_t>();
+  uint32_t* ring_ps = pl.seed_lds ? nullptr : b->ring_ps.as<uint32_t>();
+  const unsigned sgrid = (unsigned)((ng + kSeedBlock - 1) / kSeedBlock);
+  hipLaunchKernelGGL(k_gseed_count, dim3(sgrid), dim3(kSeedBlock), seed_lds, st, sa, ctl, ring_h, ring_ps);
+  DN_HIP(hipGetLastError());
+  enqueue_scan(ScanAnchors{b->acnt.as<uint32_t>()}, &ctl->nglobal, b->gpart.as<uint64_t>(), b->aoff.as<uint64_t>(),
+               (uint32_t*)nullptr, &ctl->anchors, st);
+  hipLaunchKernelGGL(k_gseed_emit, dim3(sgrid), dim3(kSeedBlock), seed_lds, st, sa, ctl, pl.anchor_cap, ring_h,
+                     ring_ps);
+  DN_HIP(hipGetLastError());
+  DN_TRY(close());
+  DN_TRY(open(1));
+  ChainArgs cg = ca;
+  cg.list = glist;
+  cg.n_reads = 0;
+  cg.anchor_count = b->acnt.as<uint32_t>();
+  cg.anchor_off = b->aoff.as<uint64_t>();
+  cg.keys = b->keys.as<uint64_t>();
+  cg.f = b->f.as<int32_t>();
+  cg.prev = b->prev.as<int32_t>();
+  cg.order = b->order.as<uint64_t>();
+  cg.chains = b->chains.as<ChainRec>();
+  if (pl.wave_chain)
+    hipLaunchKernelGGL(k_gchain, dim3((unsigned)(ng / 4)), dim3(256), wave_chain_lds_bytes(cg.band), st, cg, ctl,
+                       pl.anchor_cap);
+  else
+    hipLaunchKernelGGL(k_gchain_thread, dim3(sgrid), dim3(kSeedBlock), 0, st, cg, ctl, pl.anchor_cap);
+  DN_HIP(hipGetLastError());
+  DN_TRY(close());
+
+  // 3. rank pass
+  enqueue_scan(ScanRank{order, b->ncand.as<int32_t>()}, &ctl->n, b->gpart.as<uint64_t>(), b->groffp.as<uint64_t>(),
+               &ctl->nr, (uint64_t*)nullptr, st);
+  hipLaunchKernelGGL(k_grank_tasks, dim3(rb), dim3(256), 0, st, rv, g, b->cands.as<Cand>(),
+                     b->ncand.as<int32_t>(), max_cand, order, b->groffp.as<uint64_t>(), b->grank_off.as<uint64_t>(),
+                     b->rtasks.as<SwTask>(), ctl);
+  DN_HIP(hipGetLastError());
+  DN_TRY(open(2));
+  SwArgs a{};
+  a.sp = sp;
+  score_table(sp, a.tab0, a.tab1);
+  SwGenericArgs ga{};
+  ga.sp = sp;
+  ga.cigar_cap = cap;
+  ga.order = nullptr;
+  ga.rows = b->sw.rows.as<int32_t>();
+  ga.dir = b->sw.dir.as<uint8_t>();
+  ga.dir_stride = pl.packed ? 0 : pl.slot_words;
+  const unsigned gen_grid = (pl.generic_threads + 127) / 128;
+  if (pl.packed) {
+    a.tasks = b->rtasks.as<SwTask>();
+    a.order = nullptr;
+    a.res = b->rres.as<SwResult>();
+    a.n = 0xffffffffu;
+    if (!launch_packed_g<false>(sp.bw, a, (ng * mc + 127) / 128, &ctl->nr, 0, st))
+      return fail(DNALN_E_INTERNAL, "no packed kernel for this band width");
+  } else {
+    ga.tasks = b->rtasks.as<SwTask>();
+    ga.res = b->rres.as<SwResult>();
+    ga.cigar = nullptr;
+    ga.traceback = 0;
+    hipLaunchKernelGGL(k_sw_generic_g, dim3(gen_grid), dim3(128), 0, st, ga, &ctl->nr);
+  }
+  DN_HIP(hipGetLastError());
+  ++b->launches[2];
+  DN_TRY(close());
+  hipLaunchKernelGGL(k_gselect, dim3(rb), dim3(256), 0, st, rv, b->ncand.as<int32_t>(),
+                     b->grank_off.as<uint64_t>(), b->rres.as<SwResult>(), b->chosen.as<int32_t>(),
+                     b->second.as<int32_t>(), ctl);
+  DN_HIP(hipGetLastError());
+
+  // 4. traceback pass
+  enqueue_scan(ScanChosen{order, b->chosen.as<int32_t>()}, &ctl->n, b->gpart.as<uint64_t>(), b->gtoffp.as<uint64_t>(),
+               &ctl->nt, (uint64_t*)nullptr, st);
+  hipLaunchKernelGGL(k_gtb_tasks, dim3(rb), dim3(256), 0, st, rv, g, b->cands.as<Cand>(), max_cand,
+                     b->chosen.as<int32_t>(), order, b->gtoffp.as<uint64_t>(), b->gtb_idx.as<int64_t>(),
+                     b->ttasks.as<SwTask>(), ctl);
+  DN_HIP(hipGetLastError());
+  SwTask* tt = b->ttasks.as<SwTask>();
+  SwResult* tr = b->tres.as<SwResult>();
+  uint32_t* tc = b->tcig.as<uint32_t>();
+  if (pl.packed) {
+    if (pl.sorted) {
+      enqueue_scan(ScanWaveWords{tt, &ctl->nt, row_words}, &ctl->tb_waves, b->gpart.as<uint64_t>(), b->gwoff.as<uint64_t>(),
+                   (uint32_t*)nullptr, (uint64_t*)nullptr, st);
+    }
+    a.cigar_cap = cap;
+    a.dir = b->sw.dir.as<uint64_t>();
+    a.dir_slot_words = pl.slot_words;
+    for (uint32_t j = 0; j < pl.n_cuts; ++j) {
+      const uint64_t w0 = (uint64_t)j * pl.cut_waves, first = w0 * 128;
+      a.tasks = tt + first;
+      a.res = tr + first;
+      a.cigar = tc + first * cap;
+      a.n = pl.cut_waves ? pl.cut_waves * 128u : 0xffffffffu;
+      a.woff = pl.sorted ? b->gwoff.as<uint64_t>() + w0 : nullptr;
+      a.woff_base = 0;
+      const uint64_t cut_waves = pl.cut_waves ? pl.cut_waves : (ng + 127) / 128;
+      DN_TRY(open(3));
+      if (!launch_packed_g<true>(sp.bw, a, cut_waves, &ctl->nt, (uint32_t)first, st))
+        return fail(DNALN_E_INTERNAL, "no packed kernel for this band width");
+      DN_HIP(hipGetLastError());
+      DN_TRY(close());
+      ++b->launches[3];
+      DN_TRY(open(4));
+      if (!launch_walk_g(sp.bw, a, pl.walk_kind, cut_waves * 128, &ctl->nt, (uint32_t)first, st))
+        return fail(DNALN_E_INTERNAL, "no walk kernel for this band width");
+      DN_HIP(hipGetLastError());
+      DN_TRY(close());
+      ++b->launches[4];
+    }
+  } else {
+    ga.tasks = tt;
+    ga.res = tr;
+    ga.cigar = tc;
+    ga.traceback = 1;
+    DN_TRY(open(3));
+    hipLaunchKernelGGL(k_sw_generic_g, dim3(gen_grid), dim3(128), 0, st, ga, &ctl->nt);
+    DN_HIP(hipGetLastError());
+    DN_TRY(close());
+    ++b->launches[3];
+  }
+
+  // 5. records and CIGARs
+  hipLaunchKernelGGL(k_grecords, dim3(rb), dim3(256), 0, st, rv, g, b->cands.as<Cand>(), b->ncand.as<int32_t>(),
+                     max_cand, b->chosen.as<int32_t>(), b->second.as<int32_t>(), b->gtb_idx.as<int64_t>(), tr,
+                     p.min_aln_score, c->lut.as<uint8_t>(), c->lut_len, b->rec.as<dnaln_record>(),
+                     b->ciglen.as<uint32_t>(), ctl);
+  DN_HIP(hipGetLastError());
+  enqueue_scan(ScanCigar{b->ciglen.as<uint32_t>()}, &ctl->n, b->gpart.as<uint64_t>(), b->cigoff.as<uint64_t>(),
+               (uint32_t*)nullptr, &ctl->ncig, st);
+  hipLaunchKernelGGL(k_gcigar, dim3(rb), dim3(256), 0, st, rv, b->chosen.as<int32_t>(), b->gtb_idx.as<int64_t>(),
+                     tr, tc, cap, b->cigoff.as<uint64_t>(), b->rec.as<dnaln_record>(), b->pool.as<uint32_t>(), ctl);
+  DN_HIP(hipGetLastError());
+  return DNALN_OK;
+}
+
+// Captures the batched form under plan pl into a new graph at the front of b->graphs.
+constexpr size_t kGraphCache = 6;  // graphs kept per batch (plans differing in grid size or shape)
+
+int capture_batched(dnaln_hip_batch* b, const GraphPlan& pl) {
+  // Index 0 is the whole-run span, recorded outside the graph; the graph's spans follow it.
+  if (b->spans.empty()) {
+    dnaln_hip_batch::Span sp0;
+    DN_HIP(hipEventCreate(&sp0.a));
+    DN_HIP(hipEventCreate(&sp0.b));
+    b->spans.push_back(sp0);
+  }
+  b->n_spans = 1;
+  for (auto& l : b->launches) l = 0;
+  DN_HIP(hipStreamBeginCapture(b->st, hipStreamCaptureModeRelaxed));
+  const int rc = enqueue_batched(b, pl, pl.spans != 0);
+  hipGraph_t graph = nullptr;
+  const hipError_t e = hipStreamEndCapture(b->st, &graph);
+  const size_t spans = b->n_spans - 1;
+  b->n_spans = 0;
+  if (rc != DNALN_OK) {
+    if (graph) (void)hipGraphDestroy(graph);
+    (void)hipGetLastError();
+    return rc;
+  }
+  if (e != hipSuccess) return fail(DNALN_E_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  hipGraphExec_t exec = nullptr;
+  const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (ei != hipSuccess) return fail(DNALN_E_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+  if (b->graphs.size() >= kGraphCache) {
+    (void)hipGraphExecDestroy(b->graphs.back().exec);
+    b->graphs.pop_back();
+  }
+  GraphEntry ge{pl, exec, spans, {}};
+  std::memcpy(ge.launches, b->launches, sizeof ge.launches);
+  b->graphs.insert(b->graphs.begin(), ge);
+  return DNALN_OK;
+}
+
+// Plans the next run of the batched form and reserves its buffers; in graph mode finds the
+// graph captured under the same plan, or captures one.
+int prepare_batched(dnaln_hip_batch* b) {
+  GraphPlan pl;
+  DN_TRY(plan_batched(b, pl));
+  b->gplan = pl;
+  if (b->cfg.graph == 2) return DNALN_OK;
+  // Graphs from before a buffer was reallocated point at freed memory.
+  for (size_t i = 0; i < b->graphs.size();) {
+    if (b->graphs[i].plan.gen != pl.gen) {
+      DN_HIP(hipGraphExecDestroy(b->graphs[i].exec));
+      b->graphs.erase(b->graphs.begin() + (long)i);
+    } else {
+      ++i;
+    }
+  }
+  for (size_t i = 0; i < b->graphs.size(); ++i) {
+    if (std::memcmp(&b->graphs[i].plan, &pl, sizeof pl) == 0) {
+      std::rotate(b->graphs.begin(), b->graphs.begin() + (long)i, b->graphs.begin() + (long)i + 1);
+      b->gexec = b->graphs[0].exec;
+      b->gspans = b->graphs[0].spans;
+      std::memcpy(b->launches, b->graphs[0].launches, sizeof b->launches);
+      return DNALN_OK;
+    }
+  }
+  int rc = capture_batched(b, pl);
+  if (rc != DNALN_OK && pl.spans) {
+    // Event records that cannot be captured: the graph goes without per-stage timing.
+    b->graph_events_ok = false;
+    pl.spans = 0;
+    b->gplan = pl;
+    rc = capture_batched(b, pl);
+  }
+  DN_TRY(rc);
+  b->gexec = b->graphs[0].exec;
+  b->gspans = b->graphs[0].spans;
+  std::memcpy(b->launches, b->graphs[0].launches, sizeof b->launches);
+  return DNALN_OK;
+}
+
+// Enqueues the prepared run: the batch size, then the graph (or, with cfg.graph == 2, the stages
+// launched directly). Returns without waiting for anything.
+int launch_batched(dnaln_hip_batch* b) {
+  hipLaunchKernelGGL(k_begin, dim3(1), dim3(1), 0, b->st, b->gctl.as<BatchCtl>(), (uint32_t)b->n);
+  DN_HIP(hipGetLastError());
+  if (b->cfg.graph == 2) return enqueue_batched(b, b->gplan, true);
+  DN_HIP(hipGraphLaunch(b->gexec, b->st));
+  b->n_spans += b->gspans;
+  return DNALN_OK;
+}
+
+// One chunk of reads [r0, r0 + n) through the whole pipeline, launch by launch, with host reads
+// of the task counts (the plain form).
 int run_chunk(dnaln_hip_batch* b, uint64_t r0, uint32_t n, bool first_chunk) {
   dnaln_hip_ctx* c = b->ctx;
   const dnaln_map_params& p = c->p;
@@ -1063,8 +1591,8 @@ int run_chunk(dnaln_hip_batch* b, uint64_t r0, uint32_t n, bool first_chunk) {
   const int max_cand = std::max(p.max_cand, 0);
   const ScoreP sp{p.sc.match, p.sc.mismatch, p.sc.gap_open, p.sc.gap_ext, p.sc.ambig, p.sc.bw};
   const int cap = 2 * b->maxlen + 4;
-  // The batched form sorts SW tasks by length; the plain form runs them in read order.
-  const int qmin = b->cfg.mode == DNALN_MODE_BATCHED ? b->minlen : b->maxlen;
+  // The plain form runs its SW tasks in read order, with slots for the longest read.
+  const int qmin = b->maxlen;
   size_t sp_i = 0;
 
   ReadView rv;
@@ -1097,89 +1625,7 @@ int run_chunk(dnaln_hip_batch* b, uint64_t r0, uint32_t n, bool first_chunk) {
   ca.cands = b->cands.as<Cand>();
   ca.n_cand = b->ncand.as<int32_t>();
   const unsigned sgrid = (n + kSeedBlock - 1) / kSeedBlock;
-  if (b->cfg.mode == DNALN_MODE_BATCHED && n) {
-    // Batched form (one chunk: the whole batch). Length buckets: reads of up to split[0] and
-    // split[1] bases run the fused LDS kernel in its two shapes, longer ones the global-memory
-    // path, and reads the fused kernel declines are listed for the global path too. With more
-    // than one bucket in use the reads are sorted by length on the device first.
-    if (r0 != 0 || n != b->n) return fail(DNALN_E_INTERNAL, "batched form runs the whole batch as one chunk");
-    const uint64_t na = b->n_short, nb = b->n_mid;
-    const int used = (na > 0) + (nb > na) + (n > nb);
-    const uint32_t* lorder = nullptr;
-    DN_TRY(b->span_begin(0, sp_i));
-    if (used > 1) {
-      DN_TRY(b->lids.reserve((uint64_t)n * 4));
-      DN_TRY(b->lkeys.reserve((uint64_t)n * 4));
-      DN_TRY(b->lorder.reserve((uint64_t)n * 4));
-      hipLaunchKernelGGL(k_iota, dim3((n + 255) / 256), dim3(256), 0, st, b->lids.as<uint32_t>(), n);
-      DN_HIP(hipGetLastError());
-      const int bits = 32 - __builtin_clz((unsigned)std::max(b->maxlen, 1));
-      const uint32_t* lens_u = reinterpret_cast<const uint32_t*>(b->lens.as<int32_t>());
-      size_t bytes = 0;
-      DN_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, lens_u, b->lkeys.as<uint32_t>(), b->lids.as<uint32_t>(),
-                                                b->lorder.as<uint32_t>(), (uint32_t)n, 0, bits, st));
-      DN_TRY(b->ltmp.reserve(bytes));
-      DN_HIP(hipcub::DeviceRadixSort::SortPairs(b->ltmp.p, bytes, lens_u, b->lkeys.as<uint32_t>(), b->lids.as<uint32_t>(),
-                                                b->lorder.as<uint32_t>(), (uint32_t)n, 0, bits, st));
-      lorder = b->lorder.as<uint32_t>();
-    }
-    DN_TRY(b->ovf.reserve((uint64_t)n * 4 + 8));
-    uint32_t* ovf = b->ovf.as<uint32_t>();
-    uint32_t* novf_d = ovf + n;
-    DN_HIP(hipMemsetAsync(novf_d, 0, 4, st));
-    const uint64_t lo[2] = {0, na}, hi[2] = {na, nb};
-    for (int x = 0; x < 2; ++x) {
-      if (hi[x] <= lo[x]) continue;
-      const FusedShape fs = fused_shape(c->k, c->w, std::min(b->split[x], b->maxlen));
-      DN_TRY(b->fmins.reserve(fused_mins_words(fs, hi[x] - lo[x]) * 8));
-    }
-    for (int x = 0; x < 2; ++x) {
-      if (hi[x] <= lo[x]) continue;
-      const FusedShape fs = fused_shape(c->k, c->w, std::min(b->split[x], b->maxlen));
-      sa.list = lorder ? lorder + lo[x] : nullptr;
-      sa.n_reads = (uint32_t)(hi[x] - lo[x]);
-      DN_TRY(launch_fused(fs, sa, ca, b->fmins.as<uint64_t>(), ovf, novf_d, st));
-    }
-    uint32_t novf = 0;
-    DN_HIP(hipMemcpyAsync(&novf, novf_d, 4, hipMemcpyDeviceToHost, st));
-    DN_HIP(hipStreamSynchronize(st));
-    // Reads a 32-anchor shape declined (more minimizers or anchors than its slots) get a second
-    // fused pass in the 64-anchor shape; only what that declines too takes the global path.
-    bool small = false;
-    for (int x = 0; x < 2; ++x)
-      if (hi[x] > lo[x]) small = small || fused_shape(c->k, c->w, std::min(b->split[x], b->maxlen)).cap < 64;
-    const FusedShape retry = fused_retry_shape(c->k, c->w, b->maxlen);
-    if (novf && small && retry.cap) {
-      DN_TRY(b->ovf2.reserve((uint64_t)n * 4 + 8));
-      uint32_t* ovf2 = b->ovf2.as<uint32_t>();
-      uint32_t* novf2_d = ovf2 + n;
-      DN_HIP(hipMemsetAsync(novf2_d, 0, 4, st));
-      DN_TRY(b->fmins.reserve(fused_mins_words(retry, novf) * 8));
-      sa.list = ovf;
-      sa.n_reads = novf;
-      DN_TRY(launch_fused(retry, sa, ca, b->fmins.as<uint64_t>(), ovf2, novf2_d, st));
-      b->stats.fused_retry_reads += novf;
-      DN_HIP(hipMemcpyAsync(&novf, novf2_d, 4, hipMemcpyDeviceToHost, st));
-      DN_HIP(hipStreamSynchronize(st));
-      ovf = ovf2;
-    }
-    DN_TRY(b->span_end(sp_i));
-    const uint32_t* list = ovf;
-    uint32_t nl = novf;
-    if (n > nb) {
-      if (lorder) {
-        DN_HIP(hipMemcpyAsync(ovf + novf, lorder + nb, (n - nb) * 4, hipMemcpyDeviceToDevice, st));
-        nl += (uint32_t)(n - nb);
-      } else {  // every read is long: the global path in read order
-        list = nullptr;
-        nl = n;
-      }
-    }
-    b->stats.global_path_reads += nl;
-    if (nl) DN_TRY(seed_chain_global(b, sa, ca, list, nl, sp_i, true));
-  } else {
-    DN_TRY(seed_chain_global(b, sa, ca, nullptr, n, sp_i));
-  }
+  DN_TRY(seed_chain_global(b, sa, ca, nullptr, n, sp_i));
 
   // 3. rank pass
   DN_TRY(b->rcnt.reserve((uint64_t)n * 4 + 4));
@@ -1769,7 +2215,7 @@ void dnaln_hip_default_batch_config(dnaln_batch_config* cfg) {
   cfg->chunk_reads = 65536;
   cfg->max_reads = 0;
   cfg->max_bases = 0;
-  cfg->anchors_per_read = 64;
+  cfg->anchors_per_read = 0;
   cfg->tune = 0;
   cfg->len_split[0] = cfg->len_split[1] = 0;
   cfg->dir_budget = 0;
@@ -1811,6 +2257,7 @@ void dnaln_hip_batch_destroy(dnaln_hip_batch* b) {
     (void)hipEventDestroy(sp.b);
   }
   if (b->done) (void)hipEventDestroy(b->done);
+  for (auto& ge : b->graphs) (void)hipGraphExecDestroy(ge.exec);
   if (b->st) (void)hipStreamDestroy(b->st);
   delete b;
 }
@@ -1847,11 +2294,12 @@ int dnaln_hip_batch_upload(dnaln_hip_batch* b, const uint8_t* codes, const uint6
       maxlen = std::max(maxlen, lens[i]);
     }
     DN_HIP(hipStreamSynchronize(b->st));
-    DN_TRY(b->codes.reserve(bases + 2 * kReadGuard));
-    DN_TRY(b->offs.reserve(n * 8 + 8));
-    DN_TRY(b->lens.reserve(n * 4 + 4));
-    DN_TRY(b->fnv.reserve(n * 4 + 4));
-    DN_TRY(b->rec.reserve(n * sizeof(dnaln_record) + sizeof(dnaln_record)));
+    // (greserve: a reallocated input buffer also retires the graphs that point at it)
+    DN_TRY(greserve(b, b->codes, bases + 2 * kReadGuard));
+    DN_TRY(greserve(b, b->offs, n * 8 + 8));
+    DN_TRY(greserve(b, b->lens, n * 4 + 4));
+    DN_TRY(greserve(b, b->fnv, n * 4 + 4));
+    DN_TRY(greserve(b, b->rec, n * sizeof(dnaln_record) + sizeof(dnaln_record)));
     if (bases) DN_HIP(hipMemcpyAsync(b->codes.as<uint8_t>() + kReadGuard, codes, bases, hipMemcpyHostToDevice, b->st));
     if (n) {
       DN_HIP(hipMemcpyAsync(b->offs.p, offs, n * 8, hipMemcpyHostToDevice, b->st));
@@ -1863,6 +2311,12 @@ int dnaln_hip_batch_upload(dnaln_hip_batch* b, const uint8_t* codes, const uint6
     b->minlen = minlen;
     b->maxlen = maxlen;
     b->lens_h.assign(lens, lens + n);
+    b->sum_len = 0;
+    for (uint64_t i = 0; i < n; ++i) b->sum_len += (uint64_t)lens[i];
+    if (b->cfg.mode == DNALN_MODE_BATCHED) {
+      b->len_hist.assign((size_t)maxlen + 1, 0);
+      for (uint64_t i = 0; i < n; ++i) ++b->len_hist[(size_t)lens[i]];
+    }
     set_splits(b, b->cfg.len_split[0], b->cfg.len_split[1]);
     return DNALN_OK;
   });
@@ -1878,21 +2332,29 @@ int dnaln_hip_batch_run(dnaln_hip_batch* b) {
     b->pool_used = 0;
     b->n_spans = 0;
     for (auto& l : b->launches) l = 0;
-    DN_HIP(hipMemsetAsync(b->err.p, 0, 4, b->st));
-    size_t run_span = 0;
-    DN_TRY(b->span_begin(5, run_span));
-    const uint64_t chunk = b->cfg.mode == DNALN_MODE_PLAIN ? b->cfg.chunk_reads : std::max<uint64_t>(b->n, 1);
     b->last_rc = DNALN_OK;
-    for (uint64_t r0 = 0; r0 < b->n || (r0 == 0 && b->n == 0); r0 += chunk) {
-      const uint32_t m = (uint32_t)std::min<uint64_t>(chunk, b->n - r0);
-      const int rc = run_chunk(b, r0, m, r0 == 0);
-      if (rc != DNALN_OK) {
-        (void)hipStreamSynchronize(b->st);
-        b->last_rc = rc;
-        b->pending = false;
-        return rc;
+    size_t run_span = 0;
+    int rc = DNALN_OK;
+    if (b->cfg.mode == DNALN_MODE_BATCHED) {
+      // The whole batch as one graph launch (batched_graph.hpp): nothing waits on the device.
+      rc = prepare_batched(b);
+      if (rc == DNALN_OK) rc = b->span_begin(5, run_span);
+      if (rc == DNALN_OK) rc = launch_batched(b);
+    } else {
+      DN_HIP(hipMemsetAsync(b->err.p, 0, 4, b->st));
+      rc = b->span_begin(5, run_span);
+      const uint64_t chunk = b->cfg.chunk_reads;
+      for (uint64_t r0 = 0; rc == DNALN_OK && (r0 < b->n || (r0 == 0 && b->n == 0)); r0 += chunk) {
+        const uint32_t m = (uint32_t)std::min<uint64_t>(chunk, b->n - r0);
+        rc = run_chunk(b, r0, m, r0 == 0);
+        if (b->n == 0) break;
       }
-      if (b->n == 0) break;
+    }
+    if (rc != DNALN_OK) {
+      (void)hipStreamSynchronize(b->st);
+      b->last_rc = rc;
+      b->pending = false;
+      return rc;
     }
     DN_TRY(b->span_end(run_span));
     DN_HIP(hipEventRecord(b->done, b->st));
@@ -1976,20 +2438,45 @@ int dnaln_hip_batch_poll(dnaln_hip_batch* b, int wait) {
     if (!b) return fail(DNALN_E_INVALID, "null batch");
     if (b->last_rc != DNALN_OK) return b->last_rc;
     if (!b->pending) return DNALN_OK;
-    if (!wait) {
-      const hipError_t q = hipEventQuery(b->done);
-      if (q == hipErrorNotReady) return DNALN_E_PENDING;
-      if (q != hipSuccess) return fail(DNALN_E_DEVICE, hipGetErrorString(q));
-    }
-    DN_HIP(hipEventSynchronize(b->done));
-    int err = 0;
-    DN_HIP(hipMemcpy(&err, b->err.p, 4, hipMemcpyDeviceToHost));
-    b->pending = false;
-    for (auto& m : b->ms) m = 0;
-    for (size_t i = 0; i < b->n_spans; ++i) {
-      float t = 0;
-      DN_HIP(hipEventElapsedTime(&t, b->spans[i].a, b->spans[i].b));
-      b->ms[b->spans[i].family] += t;
+    uint32_t err = 0;
+    for (;;) {
+      if (!wait) {
+        const hipError_t q = hipEventQuery(b->done);
+        if (q == hipErrorNotReady) return DNALN_E_PENDING;
+        if (q != hipSuccess) return fail(DNALN_E_DEVICE, hipGetErrorString(q));
+      }
+      DN_HIP(hipEventSynchronize(b->done));
+      b->pending = false;
+      for (auto& m : b->ms) m = 0;
+      for (size_t i = 0; i < b->n_spans; ++i) {
+        float t = 0;
+        DN_HIP(hipEventElapsedTime(&t, b->spans[i].a, b->spans[i].b));
+        b->ms[b->spans[i].family] += t;
+      }
+      if (b->cfg.mode != DNALN_MODE_BATCHED) {
+        int e = 0;
+        DN_HIP(hipMemcpy(&e, b->err.p, 4, hipMemcpyDeviceToHost));
+        err = (uint32_t)e;
+        break;
+      }
+      DN_HIP(hipMemcpy(&b->ctl_h, b->gctl.p, sizeof(BatchCtl), hipMemcpyDeviceToHost));
+      const BatchCtl& k = b->ctl_h;
+      if (k.anchor_overflow) {
+        // The global path needed more anchors than planned: grow the buffer and run again.
+        b->anchor_cap = k.anchors + k.anchors / 4 + 65536;
+        const int rc = dnaln_hip_batch_run(b);
+        if (rc != DNALN_OK) return rc;
+        wait = 1;
+        continue;
+      }
+      err = k.err;
+      b->stats.anchors = k.anchors;
+      b->stats.global_path_reads = k.nglobal;
+      b->stats.fused_retry_reads = b->gplan.retry_on ? k.novf : 0;
+      b->stats.rank_tasks = k.nr;
+      b->stats.tb_tasks = k.nt;
+      b->pool_used = k.ncig;
+      break;
     }
     if (err) {
       b->last_rc = DNALN_E_INTERNAL;
@@ -2083,30 +2570,48 @@ int dnaln_hip_batch_reserve(dnaln_hip_batch* b, uint64_t n, uint64_t bases, int3
     if (!b || maxlen < 0) return fail(DNALN_E_INVALID, "bad argument");
     dnaln_hip_ctx* c = b->ctx;
     DN_HIP(hipSetDevice(c->device));
+    DN_HIP(hipStreamSynchronize(b->st));
     const uint64_t mc = (uint64_t)std::max(c->p.max_cand, 1);
     const uint64_t cap = 2 * (uint64_t)maxlen + 4;
     const uint64_t na = n * anchors_per_read;
     const uint64_t tasks = n * mc;
-    // inputs and outputs
-    DN_TRY(b->codes.reserve(bases + 2 * kReadGuard));
-    DN_TRY(b->offs.reserve(n * 8 + 8));
-    DN_TRY(b->lens.reserve(n * 4 + 4));
-    DN_TRY(b->fnv.reserve(n * 4 + 4));
+    // inputs
+    DN_TRY(b->codes.reserve(bases + 2 * kReadGuard, b->st));
+    DN_TRY(b->offs.reserve(n * 8 + 8, b->st));
+    DN_TRY(b->lens.reserve(n * 4 + 4, b->st));
+    DN_TRY(b->fnv.reserve(n * 4 + 4, b->st));
+    if (b->cfg.mode == DNALN_MODE_BATCHED) {
+      // Plan the graph for the largest such batch: n reads of every length up to maxlen, all
+      // (for the direction bound) of length maxlen.
+      const uint64_t n0 = b->n, bases0 = b->bases, sum0 = b->sum_len;
+      const int min0 = b->minlen, max0 = b->maxlen;
+      std::vector<uint32_t> hist0;
+      hist0.swap(b->len_hist);
+      b->n = n;
+      b->bases = bases;
+      b->sum_len = std::max<uint64_t>(bases, n * (uint64_t)maxlen);
+      b->minlen = 0;
+      b->maxlen = maxlen;
+      b->len_hist.assign((size_t)maxlen + 1, 0);
+      b->len_hist[(size_t)maxlen] = (uint32_t)n;
+      b->anchor_cap = std::max<uint64_t>(b->anchor_cap, na + 65536);
+      GraphPlan pl;
+      const int rc = plan_batched(b, pl);
+      b->n = n0;
+      b->bases = bases0;
+      b->sum_len = sum0;
+      b->minlen = min0;
+      b->maxlen = max0;
+      b->len_hist.swap(hist0);
+      DN_TRY(rc);
+      DN_HIP(hipStreamSynchronize(b->st));
+      return DNALN_OK;
+    }
     DN_TRY(b->rec.reserve(n * sizeof(dnaln_record) + sizeof(dnaln_record)));
     DN_TRY(b->pool.reserve(n * (cap + 2) * 4 + 4));
     // seeds and chains
     DN_TRY(b->cands.reserve(n * mc * sizeof(Cand)));
     DN_TRY(b->ncand.reserve(n * 4 + 4));
-    DN_TRY(b->lids.reserve(n * 4));
-    DN_TRY(b->lkeys.reserve(n * 4));
-    DN_TRY(b->lorder.reserve(n * 4));
-    DN_TRY(b->ovf.reserve(n * 4 + 8));
-    DN_TRY(b->ovf2.reserve(n * 4 + 8));
-    {
-      FusedShape fs;
-      fs.maxmin = 64;
-      DN_TRY(b->fmins.reserve(fused_mins_words(fs, n) * 8));
-    }
     DN_TRY(b->acnt.reserve(n * 4 + 4));
     DN_TRY(b->aoff.reserve(n * 8 + 8));
     DN_TRY(b->keys.reserve(na * 8 + 8));
@@ -2128,29 +2633,16 @@ int dnaln_hip_batch_reserve(dnaln_hip_batch* b, uint64_t n, uint64_t bases, int3
     DN_TRY(b->tcig.reserve(n * cap * 4 + 4));
     DN_TRY(b->ciglen.reserve(n * 4 + 4));
     DN_TRY(b->cigoff.reserve(n * 8 + 8));
-    // SW workspace: length sort, per-wave slot offsets, directions of one launch
-    DN_TRY(b->sw.keys.reserve(tasks * 4));
-    DN_TRY(b->sw.keys2.reserve(tasks * 4));
-    DN_TRY(b->sw.ids.reserve(tasks * 4));
-    DN_TRY(b->sw.ids2.reserve(tasks * 4));
-    const uint64_t waves = (tasks + 127) / 128;
-    DN_TRY(b->sw.woff.reserve((waves + 2) * 16));
+    // directions of one launch (the plain form runs its tasks in read order, fixed slots)
     const int W = 2 * c->p.sc.bw + 1;
     const uint64_t dir = std::min<uint64_t>(b->sw.dir_budget, ((n + 127) / 128 + 1) * (uint64_t)std::max(maxlen, 1) *
                                                                   ((W + 7) / 8) * 512);
     DN_TRY(b->sw.dir.reserve(dir));
-    // scan and sort temporaries for the largest counts
-    size_t t1 = 0, t2 = 0, t3 = 0;
+    // scan temporaries for the largest counts
+    size_t t1 = 0;
     DN_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, t1, (const uint32_t*)nullptr, (uint64_t*)nullptr, (uint32_t)tasks,
                                             b->st));
-    DN_HIP(hipcub::DeviceRadixSort::SortPairsDescending(nullptr, t2, (const uint32_t*)nullptr, (uint32_t*)nullptr,
-                                                        (const uint32_t*)nullptr, (uint32_t*)nullptr, (uint32_t)tasks, 0,
-                                                        16, b->st));
-    DN_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, t3, (const uint64_t*)nullptr, (uint64_t*)nullptr,
-                                            (uint32_t)(waves + 1), b->st));
     DN_TRY(b->scan_tmp.reserve(t1));
-    DN_TRY(b->sw.tmp.reserve(std::max(t2, t3)));
-    DN_TRY(b->ltmp.reserve(t2));
     DN_HIP(hipStreamSynchronize(b->st));
     return DNALN_OK;
   });

@@ -203,12 +203,17 @@ int dnaln_hip_map_batch(struct dnaln_hip_ctx* ctx, const uint8_t* read_codes, co
 enum { DNALN_MODE_PLAIN = 0, DNALN_MODE_BATCHED = 1 };
 
 typedef struct {
-  int32_t mode;             /* DNALN_MODE_PLAIN: one launch sequence per chunk of chunk_reads;
-                               DNALN_MODE_BATCHED: length-bucketed persistent kernels, graph-captured */
+  int32_t mode;             /* DNALN_MODE_PLAIN: one launch sequence per chunk of chunk_reads, with
+                               host reads of the task counts between launches;
+                               DNALN_MODE_BATCHED: the whole batch as one HIP graph of persistent
+                               kernels (length-sorted reads, fused seed + chain buckets, SW tasks
+                               in length order), with no host read until the batch is polled */
   uint32_t chunk_reads;     /* plain form: reads per chunk (65536 by default) */
   uint64_t max_reads;       /* capacity of the batch */
   uint64_t max_bases;
-  uint32_t anchors_per_read;/* batched form: anchor capacity per read before the overflow pass */
+  uint32_t anchors_per_read;/* batched form: first anchor capacity of the global path, per read
+                               (0: estimated from the read lengths); a run that needs more grows
+                               it and runs again */
   int32_t tune;             /* batched form: 1 = run the autotuner on the first batch */
   int32_t len_split[2];     /* batched form: read-length limits of the two LDS shapes of the fused
                                seed+chain kernel; longer reads take the global-memory path
@@ -217,7 +222,9 @@ typedef struct {
                                (0: min(16 GiB, free HBM / 8)) */
   int32_t walk_kind;        /* traceback walk: 0 one thread per task, 1 one block per fill wave
                                through LDS windows of flag words */
-  int32_t reserved;
+  int32_t graph;            /* batched form: 0 = replay one captured HIP graph per plan, with the
+                               per-stage timing events inside it; 1 = the same graph without
+                               timing events; 2 = launch the stages one by one (no graph) */
 } dnaln_batch_config;
 
 void dnaln_hip_default_batch_config(dnaln_batch_config* c);

@@ -1,0 +1,123 @@
+"""The batched form as one captured HIP graph (batched_graph.hpp): replays of a graph, graphs of
+other plans in one batch object, the graph with and without timing events and the same stages
+launched one by one, and the run again after the global path outgrew its anchor buffer. Every
+case must give the reference binary's records."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from dnaln_amd import abi, api
+from tests import golden_io
+from tests.golden import sets
+
+pytestmark = pytest.mark.gpu
+
+_CTX: dict = {}
+
+
+def _context(spec):
+    mp = sets.map_params(spec)
+    key = (id(sets.genome(spec)), tuple(sorted(mp.items())))
+    if key not in _CTX:
+        _CTX[key] = api.Context(sets.genome(spec), **mp)
+    return _CTX[key]
+
+
+def _bad(name, recs, pool, g, lo=0):
+    gold = golden_io.records(name)
+    return [i for i in range(len(recs))
+            if api.record_fields(recs[i], pool, g.names) != gold[lo + i][1:]]
+
+
+@pytest.mark.parametrize("name", ["uniform4m_150bp_10k", "edge_uniform_bw64", "edge_repeats_bw32"])
+def test_graph_replays_give_reference_records(name):
+    """Three replays of one graph, with other device work (a plain-form map) between them."""
+    spec = sets.SETS[name]
+    g = sets.genome(spec)
+    r = sets.reads(spec, g)
+    ctx = _context(spec)
+    b = ctx.batch(mode=abi.MODE_BATCHED)
+    b.upload(api.prepare(r))
+    for rep in range(3):
+        print(f"set {name} replay {rep}", flush=True)
+        b.run()
+        recs, pool, stats = b.download()
+        assert _bad(name, recs, pool, g) == []
+        assert stats.reads == len(r)
+        ctx.map(r.subset(0, 500))
+    b.close()
+
+
+@pytest.mark.parametrize("graph", [0, 1, 2])
+@pytest.mark.parametrize("name", ["edge_uniform_bw16", "edge_twins_bw16"])
+def test_graph_modes_give_reference_records(name, graph):
+    """The graph with timing events (0), without them (1), and the stages launched directly (2)."""
+    spec = sets.SETS[name]
+    g = sets.genome(spec)
+    r = sets.reads(spec, g)
+    print(f"set {name} graph mode {graph}", flush=True)
+    b = _context(spec).batch(mode=abi.MODE_BATCHED, graph=graph)
+    b.upload(api.prepare(r))
+    b.run()
+    recs, pool, _ = b.download()
+    ms, launches = b.timings()
+    b.close()
+    assert _bad(name, recs, pool, g) == []
+    assert ms[5] > 0
+    print("timings", [round(x, 3) for x in ms], launches, flush=True)
+
+
+def test_batches_of_many_sizes_through_one_batch_object():
+    """Batch sizes that need graphs of other grid sizes (and back), each the reference's records."""
+    name = "edge_uniform_bw16"
+    spec = sets.SETS[name]
+    g = sets.genome(spec)
+    r = sets.reads(spec, g)
+    b = _context(spec).batch(mode=abi.MODE_BATCHED)
+    for lo, hi in ((0, 100), (100, 4000), (7, 307), (0, 4000), (3999, 4000), (50, 60)):
+        print(f"reads [{lo}, {hi})", flush=True)
+        b.upload(api.prepare(r.subset(lo, hi)))
+        b.run()
+        recs, pool, _ = b.download()
+        assert _bad(name, recs, pool, g, lo) == []
+    b.close()
+
+
+@pytest.mark.parametrize("name", ["edge_repeats_bw32", "edge_uniform_bw64"])
+def test_anchor_buffer_overflow_runs_again(name):
+    """A first anchor capacity of one per read is too small for the global path; the run reports
+    it, the buffer grows and the batch runs again, with the reference's records."""
+    spec = sets.SETS[name]
+    g = sets.genome(spec)
+    r = sets.reads(spec, g)
+    b = _context(spec).batch(mode=abi.MODE_BATCHED, anchors_per_read=1, len_split=(-1, -1))
+    b.upload(api.prepare(r))
+    b.run()
+    recs, pool, stats = b.download()
+    b.close()
+    assert stats.anchors > len(r)
+    assert _bad(name, recs, pool, g) == []
+
+
+def test_band_without_packed_kernel_in_the_graph():
+    """bw 7 has no packed specialisation: the graph runs the int32 fill (one task per thread of a
+    fixed grid); its records equal the oracle's."""
+    import oracle
+    spec = dict(sets.SETS["edge_uniform_bw16"])
+    spec["flags"] = {"--bw": 7}
+    g = sets.genome(spec)
+    r = sets.reads(spec, g).subset(0, 1500)
+    mp = sets.map_params(spec)
+    ctx = api.Context(g, **mp)
+    b = ctx.batch(mode=abi.MODE_BATCHED)
+    b.upload(api.prepare(r))
+    b.run()
+    recs, pool, _ = b.download()
+    b.close()
+    idx = oracle.Index(g.codes, g.contig_off, g.contig_len, mp["k"], mp["w"])
+    orecs, opool, _ = oracle.map_reads(oracle.default_params(**mp), g, idx, r)
+    bad = [i for i in range(len(r))
+           if api.record_fields(recs[i], pool, g.names) != oracle.record_fields(orecs[i], opool, g.names)]
+    ctx.close()
+    assert bad == []
