@@ -276,10 +276,10 @@ class Batch:
     def __init__(self, ctx: Context, mode: int, chunk_reads: int, max_reads: int, max_bases: int, tune: bool,
                  len_split: tuple[int, int] = (0, 0), dir_budget: int = 0, walk_kind: int = 0, graph: int = 0,
                  anchors_per_read: int = 0):
-        """graph (batched form): 0 one captured HIP graph per plan with timing events inside, 1 the
-        graph without them, 2 the stages launched one by one. anchors_per_read (batched form): first
-        anchor capacity of the global path per read (0: estimated; a run that needs more grows it
-        and runs again)."""
+        """graph (batched form): 0 the kernels launched one by one (per-stage timing), 1 one captured
+        HIP graph of them per plan (refused where the build has no graph capture).
+        anchors_per_read (batched form): first anchor capacity of the global path per read (0:
+        estimated; a run that needs more grows it and runs again)."""
         self.ctx = ctx
         cfg = abi.BatchConfig()
         _lib.hip().dnaln_hip_default_batch_config(C.byref(cfg))

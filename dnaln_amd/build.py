@@ -7,6 +7,8 @@ Outputs (git-ignored, but they travel with the tree):
   dnaln_amd/lib/libdnaln_hip.so   HIP kernels + the C-ABI of include/dnaln_hip.h (gfx950)
   dnaln_amd/lib/libdnaln_host.so  host C/C++: index build, windows, SAM, generator, C++ mirror
   dnaln_amd/bin/dnaln-amd         `map` CLI with the reference's flags, on the HIP path
+  dnaln_amd/lib/libdnaln_emu.so   CPU build of the per-lane device programs (test infrastructure)
+  dnaln_amd/bin/graph_emu         the library over a CPU emulation of HIP, with ASan (test infrastructure)
   oracle/liboracle.so             C oracle (test infrastructure)
   oracle/_ref/                    the reference built from its sources (only where it is mounted)
 """
@@ -102,12 +104,25 @@ def build_host() -> Path:
 
 
 def build_emu() -> Path:
-    """CPU build of the device lane programs (test infrastructure: kernel logic checked without a GPU)."""
+    """CPU builds of the device code (test infrastructure: kernel logic checked without a GPU).
+
+    libdnaln_emu.so holds the per-lane programs; bin/graph_emu is the whole library (HIP runtime
+    calls and kernels) over the CPU emulation of HIP in csrc/emu/hip, with AddressSanitizer and
+    exact-size device buffers, so the batched form's kernel sequence (direct and captured) runs
+    with every access bounds-checked."""
     LIB.mkdir(parents=True, exist_ok=True)
     out = LIB / "libdnaln_emu.so"
     tmp = out.with_suffix(".so.tmp")
     _run([*HOST_CXX, "-Wno-unknown-pragmas", "-O2", "-shared", CSRC / "emu" / "emu.cpp", "-o", tmp])
     os.replace(tmp, out)
+    BIN.mkdir(parents=True, exist_ok=True)
+    exe = BIN / "graph_emu"
+    tmp = exe.with_suffix(".tmp")
+    _run(["g++", "-std=c++20", "-O1", "-g1", "-fsanitize=address", "-fno-omit-frame-pointer", "-DDN_EMU",
+          "-DDN_GRAPH_CAPTURE", "-Wno-attributes", "-Wno-unknown-pragmas", "-I", CSRC / "emu", "-I", INCLUDE,
+          "-I", CSRC, "-x", "c++", CSRC / "device" / "dnaln_hip.hip", "-x", "none", CSRC / "host" / "host_api.cpp",
+          CSRC / "emu" / "graph_emu_main.cpp", "-o", tmp, "-lz", "-pthread"])
+    os.replace(tmp, exe)
     return out
 
 

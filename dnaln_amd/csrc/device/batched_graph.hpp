@@ -73,7 +73,12 @@ __global__ void k_reset(BatchCtl* ctl, int bucket, uint32_t* hist, uint32_t bins
 // ------------------------------------------------------------------------------ device scans
 // Exclusive scan of f(0 .. n-1), n = *n_ptr, over kScanBlocks segments: k_scan_part sums each
 // segment, k_scan_out adds the sums of the segments before it and scans the segment in tiles.
+// (The CPU emulation, where each block costs fiber set-up, scans in fewer segments.)
+#if defined(DN_EMU)
+constexpr int kScanBlocks = 16, kScanThreads = 256;
+#else
 constexpr int kScanBlocks = 1024, kScanThreads = 256;
+#endif
 
 __device__ inline uint32_t scan_seg(uint32_t n) {
   const uint32_t s = (n + kScanBlocks - 1) / kScanBlocks;
@@ -197,7 +202,7 @@ constexpr uint32_t kMaxLenBins = 8192;  // 32 KiB of LDS histogram per block
 
 __global__ __launch_bounds__(kHistThreads) void k_len_hist(const int32_t* lens, BatchCtl* ctl, uint32_t bins,
                                                            uint32_t* hist, int split0, int split1) {
-  extern __shared__ uint32_t h[];
+  DN_DYN_LDS(uint32_t, h);
   __shared__ uint32_t cs, cm;
   const uint32_t n = ctl->n;
   for (uint32_t b = threadIdx.x; b < bins; b += blockDim.x) h[b] = 0;
@@ -271,7 +276,7 @@ template <int WT>
 __global__ __launch_bounds__(kFusedBlock) void k_fused_g(SeedArgs s, ChainArgs c, int cap, int maxmin, uint64_t* mins,
                                                          BatchCtl* ctl, int bucket, const uint32_t* order,
                                                          const uint32_t* in_list, uint32_t* decl, bool last) {
-  extern __shared__ uint64_t lds[];
+  DN_DYN_LDS(uint64_t, lds);
   const int t = threadIdx.x;
   uint32_t lo, hi;
   if (bucket == 0) lo = ctl->seg_short, hi = ctl->n;
@@ -331,7 +336,7 @@ __device__ inline Ring seed_ring(uint64_t* lds, int w, uint64_t* ring_h, uint32_
 
 __global__ __launch_bounds__(kSeedBlock) void k_gseed_count(SeedArgs a, const BatchCtl* ctl, uint64_t* ring_h,
                                                             uint32_t* ring_ps) {
-  extern __shared__ uint64_t lds[];
+  DN_DYN_LDS(uint64_t, lds);
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= ctl->nglobal) return;
   a.anchor_count[i] = seed_count_read(a, i, seed_ring(lds, a.w, ring_h, ring_ps));
@@ -339,7 +344,7 @@ __global__ __launch_bounds__(kSeedBlock) void k_gseed_count(SeedArgs a, const Ba
 
 __global__ __launch_bounds__(kSeedBlock) void k_gseed_emit(SeedArgs a, BatchCtl* ctl, uint64_t cap, uint64_t* ring_h,
                                                            uint32_t* ring_ps) {
-  extern __shared__ uint64_t lds[];
+  DN_DYN_LDS(uint64_t, lds);
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (ctl->anchors > cap) {
     if (i == 0) ctl->anchor_overflow = 1;
@@ -359,7 +364,7 @@ __global__ __launch_bounds__(kSeedBlock) void k_gchain_thread(ChainArgs a, const
 
 // One wave per read of the global list (chain_wave.hpp), four per block.
 __global__ __launch_bounds__(256) void k_gchain(ChainArgs a, BatchCtl* ctl, uint64_t cap) {
-  extern __shared__ uint64_t lds[];
+  DN_DYN_LDS(uint64_t, lds);
   if (ctl->anchors > cap) return;
   int32_t* cost = reinterpret_cast<int32_t*>(lds);
   for (int x = threadIdx.x; x <= a.band; x += blockDim.x) cost[x] = a.cost[x];
@@ -446,7 +451,7 @@ __global__ void k_gtb_tasks(ReadView rv, MapGeom g, const Cand* cands, int max_c
 // most such tasks; a.tasks, a.res, a.cigar and a.woff already point at the cut's first task and
 // wave, so the slot of wave w is dir + woff[w] - woff[0].
 template <int W, bool TB>
-__global__ __launch_bounds__(kSwBlock) __attribute__((amdgpu_waves_per_eu(sw_min_waves(W, TB)))) void k_sw_packed_g(
+__global__ __launch_bounds__(kSwBlock) DN_WAVES_PER_EU(sw_min_waves(W, TB)) void k_sw_packed_g(
     SwArgs a, const uint32_t* n_ptr, uint32_t first) {
   const int lane = threadIdx.x & 63;
   const uint32_t total = *n_ptr;
@@ -461,7 +466,7 @@ __global__ __launch_bounds__(kSwBlock) __attribute__((amdgpu_waves_per_eu(sw_min
   ql = wave_max(ql);
   uint64_t* slot = TB ? dir_slot(a, wave) : nullptr;
   if (TB && W > kBestRowInRegs) {
-    extern __shared__ uint16_t bh_rows[];
+    DN_DYN_LDS(uint16_t, bh_rows);
     uint16_t* bh = bh_rows + (size_t)(threadIdx.x >> 6) * best_row_lds_words(W) * 2 + 2 * lane;
     sw_pair<W, TB>(a, ka, kb, va, vb, ql, slot, lane, bh, 64);
   } else {
@@ -486,7 +491,7 @@ __global__ __launch_bounds__(256) void k_walk_g(SwArgs a, const uint32_t* n_ptr,
 template <int W>
 __global__ __launch_bounds__(128) void k_walk_win_g(SwArgs a, int rows, const uint32_t* n_ptr, uint32_t first) {
   constexpr int G = (W + 7) / 8;
-  extern __shared__ uint64_t win[];
+  DN_DYN_LDS(uint64_t, win);
   __shared__ int s_top;
   __shared__ uint32_t seqbuf[kWalkSeqDwords * 128];
   const int t = threadIdx.x, lane = t & 63, half = t >> 6;

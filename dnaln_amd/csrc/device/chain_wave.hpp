@@ -17,7 +17,9 @@
 //     load, and ends already used when their block of 64 starts are skipped by a ballot;
 //   * candidates: lane x keeps accepted candidate x's diagonal, so the dedupe test is a ballot.
 // Every lane writes the same value wherever all lanes later read (prev during extraction), so
-// each lane reads its own writes. Lookback <= 64 and max_cand <= 64 (checked by the host).
+// each lane reads its own writes; the walk takes prev through readfirstlane, which keeps the
+// walk's state in scalar registers and states that it is the same on every lane.
+// Lookback <= 64 and max_cand <= 64 (checked by the host).
 #pragma once
 
 #include "seed_chain.hpp"
@@ -221,7 +223,7 @@ __device__ __forceinline__ void chain_wave_read(const ChainArgs& a, uint32_t r, 
     while (open) {
       const int end = __builtin_amdgcn_readlane((int)my_end, (int)__builtin_ctzll(open));
       open &= open - 1;
-      uint32_t pu = s.prev[end];
+      uint32_t pu = (uint32_t)__builtin_amdgcn_readfirstlane((int)s.prev[end]);
       if (pu >> 31) continue;
       int i = end, cnt = 0, stop = -1;
       int64_t dmin = INT64_MAX, dmax = INT64_MIN;
@@ -234,7 +236,7 @@ __device__ __forceinline__ void chain_wave_read(const ChainArgs& a, uint32_t r, 
         dmax = d > dmax ? d : dmax;
         const int nx = (int)(pu & 0x7fffffffu) - 1;
         if (nx < 0) break;
-        pu = s.prev[nx];
+        pu = (uint32_t)__builtin_amdgcn_readfirstlane((int)s.prev[nx]);
         if (pu >> 31) {
           stop = nx;
           break;

@@ -16,6 +16,15 @@
 #define DN_DEV inline
 #endif
 
+// The dynamic LDS array of a kernel (sized at launch), and the waves per SIMD a kernel's
+// register allocation must allow.
+#if !defined(DN_DYN_LDS)
+#define DN_DYN_LDS(T, name) extern __shared__ T name[]
+#endif
+#if !defined(DN_WAVES_PER_EU)
+#define DN_WAVES_PER_EU(n) __attribute__((amdgpu_waves_per_eu(n)))
+#endif
+
 namespace dn {
 
 // ------------------------------------------------------------------------------ task descriptors

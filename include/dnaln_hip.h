@@ -14,8 +14,9 @@
  *   dnaln_hip_map_batch           Mapper::map (src/mapper.hpp:54, src/mapper.cpp:75-223), minus
  *                                 SAM text, which stays on the host (append_sam)
  *   dnaln_hip_batch_*             the same pipeline on device-resident batches (plain
- *                                 per-chunk launches or the batched persistent form), used by
- *                                 the serving loop and the benchmark
+ *                                 per-chunk launches, or the batched form: a fixed kernel
+ *                                 sequence that reads its task counts from device memory),
+ *                                 used by the serving loop and the benchmark
  */
 #ifndef DNALN_HIP_H
 #define DNALN_HIP_H
@@ -205,9 +206,12 @@ enum { DNALN_MODE_PLAIN = 0, DNALN_MODE_BATCHED = 1 };
 typedef struct {
   int32_t mode;             /* DNALN_MODE_PLAIN: one launch sequence per chunk of chunk_reads, with
                                host reads of the task counts between launches;
-                               DNALN_MODE_BATCHED: the whole batch as one HIP graph of persistent
+                               DNALN_MODE_BATCHED: the whole batch as one fixed sequence of
                                kernels (length-sorted reads, fused seed + chain buckets, SW tasks
-                               in length order), with no host read until the batch is polled */
+                               in length order); each kernel's grid is sized for the batch's
+                               worst case and its blocks read the live task counts from device
+                               memory, so no host read happens until the batch is polled, and
+                               the sequence can be captured as one HIP graph (`graph`) */
   uint32_t chunk_reads;     /* plain form: reads per chunk (65536 by default) */
   uint64_t max_reads;       /* capacity of the batch */
   uint64_t max_bases;
@@ -222,9 +226,10 @@ typedef struct {
                                (0: min(16 GiB, free HBM / 8)) */
   int32_t walk_kind;        /* traceback walk: 0 one thread per task, 1 one block per fill wave
                                through LDS windows of flag words */
-  int32_t graph;            /* batched form: 0 = replay one captured HIP graph per plan, with the
-                               per-stage timing events inside it; 1 = the same graph without
-                               timing events; 2 = launch the stages one by one (no graph) */
+  int32_t graph;            /* batched form: 0 = launch the kernels one by one, with timing events
+                               around the stages (per-stage timing); 1 = replay one captured HIP
+                               graph of the same kernels per plan (timing: the whole run only).
+                               1 is refused (DNALN_E_INVALID) by builds without graph capture */
 } dnaln_batch_config;
 
 void dnaln_hip_default_batch_config(dnaln_batch_config* c);
