@@ -354,18 +354,29 @@ __global__ __launch_bounds__(kSeedBlock) void k_gseed_emit(SeedArgs a, BatchCtl*
   seed_emit_read(a, i, seed_ring(lds, a.w, ring_h, ring_ps));
 }
 
+// A run whose anchors overflowed the buffer is run again by the host, but every later stage of
+// this run still reads n_cand of every read: the reads of the global list get none.
+__device__ inline bool anchors_overflowed(const ChainArgs& a, const BatchCtl* ctl, uint64_t cap, uint32_t li) {
+  if (ctl->anchors <= cap) return false;
+  if (li < ctl->nglobal) a.n_cand[read_id(a.list, li)] = 0;
+  return true;
+}
+
 // One thread per read of the global list (lookback or max_cand above 64, where the wave chain
 // does not apply).
 __global__ __launch_bounds__(kSeedBlock) void k_gchain_thread(ChainArgs a, const BatchCtl* ctl, uint64_t cap) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (ctl->anchors > cap || i >= ctl->nglobal) return;
+  if (anchors_overflowed(a, ctl, cap, i) || i >= ctl->nglobal) return;
   chain_read(a, i);
 }
 
 // One wave per read of the global list (chain_wave.hpp), four per block.
 __global__ __launch_bounds__(256) void k_gchain(ChainArgs a, BatchCtl* ctl, uint64_t cap) {
   DN_DYN_LDS(uint64_t, lds);
-  if (ctl->anchors > cap) return;
+  if (ctl->anchors > cap) {
+    if ((threadIdx.x & 63) == 0) (void)anchors_overflowed(a, ctl, cap, blockIdx.x * 4 + (threadIdx.x >> 6));
+    return;
+  }
   int32_t* cost = reinterpret_cast<int32_t*>(lds);
   for (int x = threadIdx.x; x <= a.band; x += blockDim.x) cost[x] = a.cost[x];
   __syncthreads();

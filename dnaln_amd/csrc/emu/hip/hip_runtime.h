@@ -414,9 +414,18 @@ inline T max(T a, T b) {
 inline hipError_t hipSetDevice(int) { return hipSuccess; }
 inline hipError_t hipGetLastError() { return hipSuccess; }
 inline const char* hipGetErrorString(hipError_t e) { return e == hipSuccess ? "no error" : "emulated error"; }
+// Device memory starts undefined: every byte of a new allocation is HIP_EMU_POISON (default 0xCD,
+// a negative int32; 0x01 makes stale counts large and positive, as reused device memory can be).
+inline int hip_emu_poison() {
+  static const int v = [] {
+    const char* e = std::getenv("HIP_EMU_POISON");
+    return e ? (int)(std::strtol(e, nullptr, 0) & 0xff) : 0xCD;
+  }();
+  return v;
+}
 inline hipError_t hipMalloc(void** p, size_t n) {
   *p = std::malloc(n ? n : 1);
-  if (*p) std::memset(*p, 0xCD, n);  // device memory starts undefined
+  if (*p) std::memset(*p, hip_emu_poison(), n);
   return *p ? hipSuccess : hipErrorInvalidValue;
 }
 inline hipError_t hipFree(void* p) {

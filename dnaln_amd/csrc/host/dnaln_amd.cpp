@@ -1,6 +1,7 @@
 // dnaln_amd.cpp — host classes of dnaln_amd.hpp over the C ABI.
 #include "host/dnaln_amd.hpp"
 
+#include <unistd.h>
 #include <zlib.h>
 
 #include <algorithm>
@@ -102,10 +103,10 @@ void HipAligner::align(const AlignTask* tasks, size_t n, bool traceback, AlignRe
 }
 
 // ------------------------------------------------------------------------------ SeqReader
-SeqReader::SeqReader(const std::string& path) : buf_(1 << 20) {
-  gzFile f = path == "-" ? gzdopen(fileno(stdin), "rb") : gzopen(path.c_str(), "rb");
+SeqReader::SeqReader(const std::string& path) : win_(4u << 20) {
+  gzFile f = path == "-" ? gzdopen(dup(fileno(stdin)), "rb") : gzopen(path.c_str(), "rb");
   if (!f) throw Error("cannot open '" + path + "'");
-  gzbuffer(f, 1 << 18);
+  gzbuffer(f, 1u << 20);
   fp_ = f;
 }
 
@@ -113,59 +114,37 @@ SeqReader::~SeqReader() {
   if (fp_) gzclose(static_cast<gzFile>(fp_));
 }
 
-int SeqReader::peek() {
-  if (pos_ >= len_) {
-    if (eof_) return -1;
-    const int n = gzread(static_cast<gzFile>(fp_), buf_.data(), (unsigned)buf_.size());
-    if (n < 0) throw Error("read error");
-    if (n == 0) {
-      eof_ = true;
-      return -1;
+bool SeqReader::next(std::string_view& ln) {
+  for (;;) {
+    const char* p = win_.data() + head_;
+    if (fastx::cut_line(p, win_.data() + tail_, eof_, ln)) {
+      last_ = head_;
+      head_ = (size_t)(p - win_.data());
+      return true;
     }
-    pos_ = 0;
-    len_ = (size_t)n;
+    if (eof_) return false;
+    // No whole line is left in the window: keep the partial line, read more behind it.
+    std::memmove(win_.data(), win_.data() + head_, tail_ - head_);
+    tail_ -= head_;
+    head_ = last_ = 0;
+    if (tail_ == win_.size()) win_.resize(win_.size() * 2);
+    const int got = gzread(static_cast<gzFile>(fp_), win_.data() + tail_, (unsigned)(win_.size() - tail_));
+    if (got < 0) throw Error("read error");
+    if (got == 0) eof_ = true;
+    tail_ += (size_t)got;
   }
-  return (unsigned char)buf_[pos_];
-}
-
-bool SeqReader::line(std::string& out) {
-  out.clear();
-  if (peek() < 0) return false;
-  while (peek() >= 0) {
-    const char* s = buf_.data() + pos_;
-    const char* nl = static_cast<const char*>(std::memchr(s, '\n', len_ - pos_));
-    const size_t take = nl ? (size_t)(nl - s) : len_ - pos_;
-    out.append(s, take);
-    pos_ += take;
-    if (nl) {
-      ++pos_;
-      break;
-    }
-  }
-  if (!out.empty() && out.back() == '\r') out.pop_back();
-  return true;
 }
 
 bool SeqReader::next(SeqRecord& r) {
-  int c;
-  while ((c = peek()) >= 0 && c != '>' && c != '@') line(line_);
-  if (c < 0) return false;
-  line(line_);
-  const bool fastq = line_[0] == '@';
-  const size_t end = line_.find_first_of(" \t", 1);
-  r.name.assign(line_, 1, end == std::string::npos ? std::string::npos : end - 1);
-  r.seq.clear();
-  r.qual.clear();
-  while ((c = peek()) >= 0 && c != '>' && c != '+' && c != '@') {
-    line(line_);
-    r.seq += line_;
+  fastx::RecordMachine& m = machine_;
+  try {
+    if (!fastx::read_record(*this, m)) return false;
+  } catch (const fastx::RecordError& e) {
+    throw Error(e.what());
   }
-  if (fastq) {
-    if (c != '+') throw Error("malformed FASTQ record '" + r.name + "'");
-    line(line_);
-    while (r.qual.size() < r.seq.size() && line(line_)) r.qual += line_;
-    if (r.qual.size() != r.seq.size()) throw Error("FASTQ record '" + r.name + "': quality length mismatch");
-  }
+  r.name.swap(m.name);
+  r.seq.swap(m.seq);
+  r.qual.swap(m.qual);
   return true;
 }
 

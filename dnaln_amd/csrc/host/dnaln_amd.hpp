@@ -18,10 +18,12 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <string_view>
 #include <vector>
 
 #include "dnaln_hip.h"
 #include "dnaln_host.h"
+#include "host/fastx_record.hpp"
 
 namespace dnaln_amd {
 
@@ -95,8 +97,10 @@ struct SeqRecord {
 };
 
 // FASTA/FASTQ reader over plain or gzipped files ("-" reads stdin), with the reference's
-// record rules (src/seqio.cpp:59-85): the name ends at the first space or tab, sequence lines
-// are joined, a FASTQ quality is read until it is as long as the sequence.
+// record rules (src/seqio.cpp:59-87; the rule list is in host/fastx_record.hpp): the name ends
+// at the first space or tab, sequence lines are joined, a FASTQ quality is read until it is as
+// long as the sequence. Whole lines are cut from a growing window of decompressed bytes and fed
+// to the shared record state machine.
 class SeqReader {
  public:
   explicit SeqReader(const std::string& path);
@@ -106,14 +110,17 @@ class SeqReader {
   bool next(SeqRecord& r);
   size_t next_batch(std::vector<SeqRecord>& out, size_t max);
 
+  // The line source of the record machine: next() cuts the next line out of the window,
+  // unread() puts the last one back.
+  bool next(std::string_view& ln);
+  void unread() { head_ = last_; }
+
  private:
-  int peek();
-  bool line(std::string& out);
   void* fp_ = nullptr;
-  std::vector<char> buf_;
-  size_t pos_ = 0, len_ = 0;
+  std::vector<char> win_;          // decompressed bytes; [head_, tail_) not yet cut into lines
+  size_t head_ = 0, tail_ = 0, last_ = 0;
   bool eof_ = false;
-  std::string line_;
+  fastx::RecordMachine machine_;
 };
 
 struct Contig {

@@ -1,6 +1,6 @@
 // fastx_input.hpp — the input stage of dnaln_map_file: FASTA/FASTQ (plain or gzip) cut into
 // batches and parsed by several threads when the file is plain 4-line FASTQ, by the reference's
-// sequential rules (src/seqio.cpp:59-87) otherwise.
+// sequential rules (src/seqio.cpp:59-87, as the record machine of fastx_record.hpp) otherwise.
 //
 // A 4-line FASTQ file can be cut at any record start, and record starts can be found from any
 // point: a line that begins with '@' and whose line two further on begins with '+' is a header
@@ -26,6 +26,7 @@
 #include <vector>
 
 #include "dnaln_hip.h"
+#include "host/fastx_record.hpp"
 
 namespace dnaln_amd {
 namespace fastx {
@@ -154,56 +155,32 @@ struct Input {
       buf.n += (size_t)got;
     }
   }
-  // The sequential reader's primitives (SeqReader of the C++ interface, on this buffer).
-  int peek() {
-    if (pos >= size()) {
-      if (eof) return -1;
-      top_up(1 << 20);
-      if (pos >= size()) return -1;
-    }
-    return (unsigned char)data()[pos];
-  }
-  bool line(std::string& out) {
-    out.clear();
-    if (peek() < 0) return false;
-    while (peek() >= 0) {
-      const char* s = data() + pos;
-      const char* nl = static_cast<const char*>(std::memchr(s, '\n', size() - pos));
-      const size_t take = nl ? (size_t)(nl - s) : size() - pos;
-      out.append(s, take);
-      pos += take;
-      if (nl) {
-        ++pos;
-        break;
+  // The line source of the record machine (host/fastx_record.hpp): the next whole line of the
+  // unparsed bytes, reading more as needed; unread() puts the last line back.
+  bool next(std::string_view& ln) {
+    for (;;) {
+      const char* p = data() + pos;
+      if (cut_line(p, data() + size(), eof, ln)) {
+        last = pos;
+        pos = (size_t)(p - data());
+        return true;
       }
+      if (eof) return false;
+      top_up(size() - pos + (1 << 20));
     }
-    if (!out.empty() && out.back() == '\r') out.pop_back();
-    return true;
   }
+  void unread() { pos = last; }
+  size_t last = 0;  // start of the line next() gave last
 };
 
-// One record with the reference's rules (src/seqio.cpp:59-87). False at the end of the input.
-inline bool next_general(Input& in, Part& part, std::string& ln, std::string& seq, std::string& qual) {
-  int c;
-  while ((c = in.peek()) >= 0 && c != '>' && c != '@') in.line(ln);
-  if (c < 0) return false;
-  in.line(ln);
-  const bool fastq = ln[0] == '@';
-  const size_t end = ln.find_first_of(" \t", 1);
-  const std::string name = ln.substr(1, end == std::string::npos ? std::string::npos : end - 1);
-  seq.clear();
-  qual.clear();
-  while ((c = in.peek()) >= 0 && c != '>' && c != '+' && c != '@') {
-    in.line(ln);
-    seq += ln;
+// One record by the sequential rules into `part`; false at the end of the input.
+inline bool next_general(Input& in, Part& part, RecordMachine& m) {
+  try {
+    if (!read_record(in, m)) return false;
+  } catch (const RecordError& e) {
+    throw Failure(DNALN_E_IO, e.what());
   }
-  if (fastq) {
-    if (c != '+') throw Failure(DNALN_E_IO, "malformed FASTQ record '" + name + "'");
-    in.line(ln);
-    while (qual.size() < seq.size() && in.line(ln)) qual += ln;
-    if (qual.size() != seq.size()) throw Failure(DNALN_E_IO, "FASTQ record '" + name + "': quality length mismatch");
-  }
-  part.add(name.data(), name.size(), seq.data(), seq.size(), qual.data(), fastq);
+  part.add(m.name.data(), m.name.size(), m.seq.data(), m.seq.size(), m.qual.data(), m.fastq);
   return true;
 }
 
@@ -336,9 +313,8 @@ class Batcher {
       }
     }
     if (general) {
-      std::string ln, seq, qual;
       Part& p = parts[0];
-      while (p.seq.n + p.names.n < batch_bytes / 2 && next_general(in, p, ln, seq, qual)) {
+      while (p.seq.n + p.names.n < batch_bytes / 2 && next_general(in, p, machine)) {
       }
       if (p.n() == 0) return false;
     }
@@ -350,6 +326,7 @@ class Batcher {
   uint64_t batch_bytes;
   int threads;
   Input in;
+  RecordMachine machine;  // the sequential rules, once a batch needs them
   bool general = false;
 };
 

@@ -1,6 +1,7 @@
 // fastx_check — the parallel input stage of dnaln_map_file against the sequential reader.
 //
 // Usage: fastx_check <reads.fa|fq[.gz]> <batch_bytes> <threads>
+//        fastx_check --dump <reads.fa|fq[.gz]>
 // Reads the file with fastx::Batcher (batches of batch_bytes, `threads` parts) and with
 // dnaln_amd::SeqReader (the reference's rules, src/seqio.cpp:59-87) and compares every
 // record's name, sequence, quality and FASTQ flag. Prints "ok <records> parallel=<0|1>" and exits
@@ -14,7 +15,24 @@
 #include "host/dnaln_amd.hpp"
 #include "host/fastx_input.hpp"
 
+// --dump <reads>: every record of dnaln_amd::SeqReader as "name TAB seq TAB qual" ('*' for an
+// empty field, as the reference's SAM writes an unmapped read), then "ERROR <message>" if the
+// reader failed.
+static int dump(const char* path) {
+  try {
+    dnaln_amd::SeqReader rd(path);
+    dnaln_amd::SeqRecord r;
+    while (rd.next(r))
+      std::printf("%s\t%s\t%s\n", r.name.c_str(), r.seq.empty() ? "*" : r.seq.c_str(),
+                  r.qual.empty() ? "*" : r.qual.c_str());
+  } catch (const std::exception& e) {
+    std::printf("ERROR %s\n", e.what());
+  }
+  return 0;
+}
+
 int main(int argc, char** argv) {
+  if (argc == 3 && std::string(argv[1]) == "--dump") return dump(argv[2]);
   if (argc != 4) {
     std::fprintf(stderr, "usage: fastx_check <reads> <batch_bytes> <threads>\n");
     return 2;

@@ -56,11 +56,14 @@ def tmp_root(tmp_path_factory):
 
 
 def _run(name: str, tmp_root: Path, ranges: list[tuple[int, int]], graph: int = 1, split=(0, 0), apr: int = 0,
-         dir_budget: int = 0, walk: int = 0) -> list[str]:
+         dir_budget: int = 0, walk: int = 0, poison: int = 0xCD, mode: int = abi.MODE_BATCHED) -> list[str]:
+    """poison: the byte every new device allocation starts with. 0xCD makes a stale int32 count
+    negative; 0x01 makes it large and positive, as reused device memory can hold."""
     d = _inputs(name, tmp_root)
-    cmd = [str(EXE), str(d), str(abi.MODE_BATCHED), str(graph), str(split[0]), str(split[1]), str(apr),
+    cmd = [str(EXE), str(d), str(mode), str(graph), str(split[0]), str(split[1]), str(apr),
            str(dir_budget), str(walk)] + [f"{lo}:{hi}" for lo, hi in ranges]
-    env = dict(os.environ, ASAN_OPTIONS="detect_stack_use_after_return=0:detect_leaks=0", HIP_EMU_CUS="8")
+    env = dict(os.environ, ASAN_OPTIONS="detect_stack_use_after_return=0:detect_leaks=0", HIP_EMU_CUS="8",
+               HIP_EMU_POISON=str(poison))
     p = subprocess.run(cmd, capture_output=True, text=True, env=env, timeout=900)
     assert p.returncode == 0, p.stderr[-4000:]
     spec = sets.SETS[name]
@@ -108,3 +111,22 @@ def test_cut_traceback_and_window_walk(tmp_root):
 def test_uneven_lengths_and_narrow_band(tmp_root):
     """Reads of uneven length (length sort, per-wave direction slots) at bw 5."""
     _run("edge_twins_sr_bw5", tmp_root, [(0, 300)])
+
+
+@pytest.mark.parametrize("poison", [0x01, 0x7F])
+def test_anchor_overflow_with_stale_device_memory(tmp_root, poison):
+    """The run that overflows the anchor buffer still runs every later stage before the host runs
+    it again; those stages must not read a count the overflowed chain stage left unwritten. With
+    device memory holding large positive stale values, a read's unwritten candidate count once sent
+    the rank-task stage far outside its buffers."""
+    out = _run("edge_repeats_bw32", tmp_root, [(0, 150)], graph=0, split=(-1, -1), apr=1, poison=poison)
+    f = out[0].split()
+    assert int(f[f.index("anchors") + 1]) > 150
+
+
+@pytest.mark.parametrize("mode", [abi.MODE_BATCHED, abi.MODE_PLAIN], ids=["batched", "plain"])
+@pytest.mark.parametrize("name", ["uniform4m_150bp_10k", "edge_twins_sr_bw5", "edge_uniform_bw64"])
+def test_stale_device_memory_both_forms(tmp_root, name, mode):
+    """Every stage of both forms on device memory that starts as large positive values: no stage
+    may consume a value it did not write first."""
+    _run(name, tmp_root, [(0, 200), (30, 230)], graph=0, poison=0x01, mode=mode)
