@@ -108,6 +108,63 @@ def cpu_baseline(w: synth.Workload, g: synth.Genome, reads: synth.Reads, per_pro
             "per_process_reads_per_s": total / procs / (sum(secs) / len(secs))}
 
 
+def measured_peak() -> dict:
+    """Packed-int16 VALU peak measured on this device (dnaln_hip_peak_int16: 16 independent chains
+    of v_pk_add_u16 + v_pk_max_i16 per lane, 8 waves per SIMD), beside the nominal figure."""
+    tops, ms = api.peak_int16(0)
+    print(f"[bench] packed int16 peak: measured {tops:.1f} T ops/s ({ms:.1f} ms), nominal {P16_PEAK / 1e12:.1f}",
+          file=sys.stderr, flush=True)
+    return {"measured_tops": tops, "nominal_tops": P16_PEAK / 1e12, "kernel_ms": ms,
+            "kernel": "k_peak_pk16: 16 chains x (v_pk_add_u16 + v_pk_max_i16) per lane, 8 waves/SIMD on every CU"}
+
+
+def config_block(name: str, peak_ops: float, steps: int, warmup: int, plain_steps: int, parity: bool) -> dict:
+    """One more workload at its full size, compactly: batched reads/s after the autotuner, the plain
+    form, the fill fractions of peak, the seed stage against its HBM byte model, chunk-digest
+    parity with the reference."""
+    w = synth.WORKLOADS[name]
+    t = time.time()
+    g = synth.workload_genome(w)
+    reads = synth.workload_reads(w)
+    rb = api.prepare(reads)
+    ix = api.Index(g, build="gpu")
+    ctx = api.Context(g, ix, bw=w.bw)
+    gen_s = time.time() - t
+    W = 2 * w.bw + 1
+    plain = time_form(ctx, rb, abi.MODE_PLAIN, plain_steps, 0) if plain_steps > 0 else None
+    b = time_form(ctx, rb, abi.MODE_BATCHED, steps, warmup, tune=True)
+    recs, pool = b["recs"], b["pool"]
+    rank_cells, tb_cells = cells_from_records(recs, reads.lens, W)
+    fam = b["family_ms"]
+    seed = ix.seed_traffic(rb, int(ctx.params.max_occ))
+    out = {"workload": name, "reads": len(reads), "read_len": f"{w.len_a}-{w.len_b}", "genome_bp": int(len(g.codes)),
+           "bw": w.bw, "reads_per_s": b["reads_per_s"], "ms_per_step": b["ms_per_step"],
+           "plain_reads_per_s": plain["reads_per_s"] if plain else None,
+           "batched_over_plain": b["reads_per_s"] / plain["reads_per_s"] if plain else None,
+           "stage_ms": dict(zip(api.Batch.FAMILIES, fam)),
+           "traceback_fill_frac": tb_cells * OPS_PER_CELL["traceback"] / (fam[3] * 1e-3) / peak_ops if fam[3] > 0 else None,
+           "score_fill_frac": rank_cells * OPS_PER_CELL["score"] / (fam[2] * 1e-3) / peak_ops if fam[2] > 0 else None,
+           "kernels": f"k_sw_packed_g<{W}, true> / <{W}, false>",
+           "seed_model": seed,
+           "seed_hbm_frac": seed["bytes"] / (fam[0] * 1e-3) / HBM_PEAK if fam[0] > 0 else None,
+           "index_bytes": int(ix.n_buckets) * 16 + int(ix.n_pos) * 4,
+           "autotune": {"len_split": b["tune"]["len_split"], "walk_kind": b["tune"]["walk_kind"],
+                        "tried": len(b["tune"]["tried"])} if b["tune"] else None}
+    if plain:
+        out["plain_equals_batched"] = bool(all(np.array_equal(plain["recs"][k], recs[k])
+                                               for k in ("contig", "pos", "rev", "mapq", "score", "nm")))
+    if parity:
+        dg = json.loads((ROOT / "tests" / "golden" / "chunk_digests.json").read_text()).get(name)
+        if dg and dg["n"] == len(recs):
+            got = chunk_digests(recs, pool, reads.names(), g.names, dg["chunk"])
+            out["reference_chunk_digests"] = f"{sum(x == y for x, y in zip(got, dg['sha256']))}/{len(dg['sha256'])}"
+    ctx.close()
+    print(f"[bench] {name}: {out['reads_per_s']:.0f} reads/s (plain {out['plain_reads_per_s'] or 0:.0f}), tb fill "
+          f"{out['traceback_fill_frac'] or 0:.3f}, seed HBM {out['seed_hbm_frac'] or 0:.3f}, digests "
+          f"{out.get('reference_chunk_digests')} (workload ready in {gen_s:.1f} s)", file=sys.stderr, flush=True)
+    return out
+
+
 def copy_calibration(nbytes: int) -> None:
     """One device-to-device hipMemcpy of nbytes: reads and writes exactly nbytes of HBM."""
     import ctypes as C
@@ -147,10 +204,9 @@ def summarize_pmc(fetch_csv: str, write_csv: str, calib: int) -> int:
             if g == top[k]:
                 per[k].append(float(r["Counter_Value"]))
         copy = [v for k, vs in per.items() if "copyBuffer" in k for v in vs]
-        fill = [v for k, vs in per.items() if k.startswith("void (anonymous namespace)::k_sw_packed<") and ", true>" in k
-                for v in vs]
-        walk = [v for k, vs in per.items() if "k_walk<" in k for v in vs]
-        fused = [v for k, vs in per.items() if "k_seed_chain_fused<" in k for v in vs]
+        fill = [v for k, vs in per.items() if re.search(r"::k_sw_packed(_g)?<", k) and ", true>" in k for v in vs]
+        walk = [v for k, vs in per.items() if re.search(r"k_walk(_g)?<", k) for v in vs]
+        fused = [v for k, vs in per.items() if re.search(r"k_seed_chain_fused<|k_fused_g<", k) for v in vs]
         out[counter] = {"copy_kib": max(copy) if copy else None, "fill_kib_per_launch": sum(fill) / len(fill) if fill else None,
                         "walk_kib_per_launch": sum(walk) / len(walk) if walk else None, "fill_launches": len(fill),
                         "fused_kib_per_launch": sum(fused) / len(fused) if fused else None}
@@ -322,6 +378,9 @@ def main() -> int:
     ap.add_argument("--serve-seconds", type=float, default=1.0, help="seconds of arrivals per load level")
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--no-e2e", action="store_true", help="skip the FASTQ-to-SAM end-to-end run")
+    ap.add_argument("--configs", default="uniform4m_uneven_1m_bw16,uniform4m_uneven_1m_bw64,repeats120m_250bp_4m",
+                    help="further workloads timed at full size in compact sub-blocks ('' for none)")
+    ap.add_argument("--config-steps", type=int, default=2)
     ap.add_argument("--no-accuracy", action="store_true", help="skip scoring the records against the truth")
     ap.add_argument("--e2e-threads", type=int, default=16, help="host threads of the end-to-end run")
     ap.add_argument("--e2e-batch-bytes", type=int, default=64 << 20, help="input bytes per batch of that run")
@@ -351,6 +410,8 @@ def main() -> int:
     host_index, ix_blk = index_block(g)
     ctx = api.Context(g, host_index, bw=w.bw)
     W = 2 * w.bw + 1
+    peak = measured_peak()
+    peak_ops = peak["measured_tops"] * 1e12
 
     plain = time_form(ctx, rb, abi.MODE_PLAIN, a.plain_steps, 1) if a.plain_steps > 0 else None
     if plain:
@@ -383,22 +444,28 @@ def main() -> int:
     # events around its launches on the batch stream.
     tb_ops = tb_cells * OPS_PER_CELL["traceback"]
     achieved = tb_ops / (tb_ms * 1e-3) if tb_ms > 0 else 0.0
-    roofline = {"bound": "valu-int16", "kernel": f"k_sw_packed<{W}, true>",
-                "achieved": achieved / 1e12, "peak": P16_PEAK / 1e12, "unit": "Tops/s (packed int16)",
-                "frac": achieved / P16_PEAK, "traffic": None,
+    roofline = {"bound": "valu-int16", "kernel": f"k_sw_packed_g<{W}, true>",
+                "achieved": achieved / 1e12, "peak": peak_ops / 1e12, "unit": "Tops/s (packed int16)",
+                "frac": achieved / peak_ops, "traffic": None, "peak_source": peak,
+                "frac_of_nominal_peak": achieved / P16_PEAK,
                 "ops_per_launch": tb_ops / tb_launches, "ms_per_launch": tb_ms / tb_launches,
                 "launches": int(tb_launches), "walk_ms_per_launch": walk_ms / max(batched["launches"][4], 1),
-                "fill_plus_walk_frac": (tb_ops / ((tb_ms + walk_ms) * 1e-3) / P16_PEAK) if tb_ms + walk_ms > 0 else None,
-                "score_fill_frac": (rank_cells * OPS_PER_CELL["score"] / (rank_ms * 1e-3) / P16_PEAK) if rank_ms > 0 else None}
+                "fill_plus_walk_frac": (tb_ops / ((tb_ms + walk_ms) * 1e-3) / peak_ops) if tb_ms + walk_ms > 0 else None,
+                "score_fill_frac": (rank_cells * OPS_PER_CELL["score"] / (rank_ms * 1e-3) / peak_ops) if rank_ms > 0 else None}
+    seed_model = host_index.seed_traffic(rb, int(ctx.params.max_occ))
+    seed_ms = fam[0]
+    hbm = {"peak_gbps": HBM_PEAK / 1e9,
+           "seed_stage_model": {"bytes": seed_model["bytes"], "ms": seed_ms, "counts": seed_model,
+                                "frac": seed_model["bytes"] / (seed_ms * 1e-3) / HBM_PEAK if seed_ms > 0 else None,
+                                "note": "read bases + 16 B per bucket visited + 4 B per occurrence read, over the "
+                                        "seed stage's event time (fused seed + chain kernels)"}}
     tpath = ROOT / "profiles" / "traffic.json"
-    hbm = None
     if tpath.exists():
         tr = json.loads(tpath.read_text())
         roofline["traffic"] = tr.get("k_sw_packed_tb_bytes_per_launch")
         # HBM-side view of the two memory-bound kernels: counted bytes per launch (from the
         # committed counter runs) over this run's measured time per launch.
-        seed_ms, n_seed = fam[0], max(batched["launches"][0], 1)
-        hbm = {"peak_gbps": HBM_PEAK / 1e9}
+        n_seed = max(batched["launches"][0], 1)
         for name, key, ms in (("k_seed_chain_fused", "k_seed_chain_fused_bytes_per_launch", seed_ms / n_seed),
                               ("k_walk", "k_walk_bytes_per_launch", walk_ms / max(batched["launches"][4], 1))):
             if tr.get(key) and ms > 0:
@@ -434,12 +501,18 @@ def main() -> int:
         out["accuracy"] = accuracy(ctx, w, g, len(reads), recs, pool, 20000)
     if not a.no_e2e:
         out["end_to_end"] = end_to_end(ctx, reads, a.e2e_threads, a.e2e_batch_bytes)
+        out["device_pipeline_reads_per_s"] = out["end_to_end"]["device_pipeline_reads_per_s"]
     if not a.no_cpu:
         cb = cpu_baseline(w, g, reads, a.cpu_reads_per_proc)
         out["cpu_baseline"] = cb
         if cb:
             out["gpu_over_cpu"] = batched["reads_per_s"] / cb["value"]
+            if "device_pipeline_reads_per_s" in out:
+                out["device_pipeline_over_cpu"] = out["device_pipeline_reads_per_s"] / cb["value"]
     ctx.close()
+    if a.configs:
+        out["configs"] = [config_block(n, peak_ops, a.config_steps, 1, 1, not a.no_parity)
+                          for n in a.configs.split(",") if n]
     if not a.no_serve:
         out["serving"] = serve_sweep(a.serve_seconds, [round(0.1 * x, 1) for x in range(1, 11)], a.serve_slots)
     if a.calibrate_copy:

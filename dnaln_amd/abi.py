@@ -51,6 +51,10 @@ class TruthStats(C.Structure):
     _fields_ = [(n, u64) for n in ("n", "rescored_ok", "in_band", "optimal_ok", "majority_ok", "bases", "bases_ok")]
 
 
+class SeedTraffic(C.Structure):
+    _fields_ = [(n, u64) for n in ("read_bases", "minimizers", "buckets", "occurrences", "bytes")]
+
+
 class Task(C.Structure):
     _fields_ = [("q_off", u64), ("t_off", u64), ("qlen", i32), ("tlen", i32)]
 
@@ -106,6 +110,7 @@ HIP_SYMBOLS = {
     "dnaln_hip_batch_reserve": (i32, [vp, u64, u64, i32, u32]),
     "dnaln_hip_context_params": (i32, [vp, vp]),
     "dnaln_hip_alloc_counters": (None, [vp, vp]),
+    "dnaln_hip_peak_int16": (i32, [i32, i32, vp, vp]),
     "dnaln_hip_index_build": (i32, [vp, u64, vp, i32, i32, i32, i32, vp]),
     "dnaln_hip_index_info": (i32, [vp, vp, vp, vp, vp, vp]),
     "dnaln_hip_index_download": (i32, [vp, vp, vp]),
@@ -147,6 +152,7 @@ HOST_SYMBOLS = {
     "dnaln_index_save": (i32, [vp, vp, vp, i32, C.c_char_p]),
     "dnaln_index_load": (i32, [C.c_char_p, vp, vp, i32, vp]),
     "dnaln_reference_digest": (u64, [vp, vp, i32]),
+    "dnaln_seed_traffic": (i32, [vp, vp, vp, vp, u64, u32, i32, vp]),
     "dnaln_map_file": (i32, [vp, vp, C.c_char_p, i32, vp, i32, u64, i32, vp]),
     "dnaln_map_file_on": (i32, [vp, vp, C.c_char_p, i32, vp, i32, u64, i32, vp]),
     "dnaln_host_last_error": (C.c_char_p, []),

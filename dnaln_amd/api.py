@@ -50,6 +50,13 @@ def device_allocs() -> tuple[int, int]:
     return int(ev.value), int(by.value)
 
 
+def peak_int16(device: int = 0, iters: int = 200_000) -> tuple[float, float]:
+    """Measured packed-int16 VALU peak of the device: (T int16 ops/s, ms of the timed launch)."""
+    tops, ms = C.c_double(), C.c_double()
+    _check(_lib.hip().dnaln_hip_peak_int16(device, iters, C.byref(tops), C.byref(ms)), "dnaln_hip_peak_int16")
+    return float(tops.value), float(ms.value)
+
+
 def map_params(**kw) -> abi.MapParams:
     p = abi.MapParams()
     _lib.hip().dnaln_hip_default_params(C.byref(p))
@@ -133,6 +140,15 @@ class Index:
         if n == 0:
             return np.zeros(0, np.uint32)
         return np.frombuffer((C.c_uint32 * n).from_address(self.idx.view.pos), dtype=np.uint32).copy()
+
+    def seed_traffic(self, rb: "ReadBatch", max_occ: int = 500, threads: int = 16) -> dict:
+        """Algorithmic bytes of the seed stage over a prepared batch (dnaln_seed_traffic): read
+        bases, minimizers, buckets visited, occurrences read, and bytes = bases + 16 B per bucket +
+        4 B per occurrence."""
+        t = abi.SeedTraffic()
+        _check(_lib.host().dnaln_seed_traffic(C.byref(self.idx.view), _p(rb.codes), _p(rb.offs), _p(rb.lens),
+                                              len(rb), max_occ, threads, C.byref(t)), "dnaln_seed_traffic")
+        return {k: int(getattr(t, k)) for k, _ in t._fields_}
 
     def __del__(self):
         if getattr(self, "idx", None) is not None and self.idx.view.table:
@@ -276,8 +292,9 @@ class Batch:
     def __init__(self, ctx: Context, mode: int, chunk_reads: int, max_reads: int, max_bases: int, tune: bool,
                  len_split: tuple[int, int] = (0, 0), dir_budget: int = 0, walk_kind: int = 0, graph: int = 0,
                  anchors_per_read: int = 0):
-        """graph (batched form): 0 the kernels launched one by one (per-stage timing), 1 one captured
-        HIP graph of them per plan (refused where the build has no graph capture).
+        """graph (batched form): 0 the host-synchronized form (default); 2 the device-count kernel
+        sequence launched one by one, 1 it captured as a HIP graph per plan (both refused by the
+        device build; the CPU emulation runs them).
         anchors_per_read (batched form): first anchor capacity of the global path per read (0:
         estimated; a run that needs more grows it and runs again)."""
         self.ctx = ctx

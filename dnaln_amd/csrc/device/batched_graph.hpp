@@ -31,6 +31,9 @@ namespace {
 
 // ------------------------------------------------------------------------------ control block
 constexpr int kMaxCuts = 256;  // traceback fill launches per run (direction budget)
+// dnaln_batch_config::graph of the batched form: 0 the host-synchronized form (run_chunk), 1 this
+// sequence captured as a graph, 2 this sequence launched one by one.
+constexpr int kFormHostSync = 0, kFormGraph = 1, kFormDirect = 2;
 
 // One run's sizes and counters in device memory. `n` is written before the graph is launched
 // (k_begin); every other field starts at zero (k_reset) or is written by an earlier stage.
@@ -74,7 +77,7 @@ __global__ void k_reset(BatchCtl* ctl, int bucket, uint32_t* hist, uint32_t bins
 // Exclusive scan of f(0 .. n-1), n = *n_ptr, over kScanBlocks segments: k_scan_part sums each
 // segment, k_scan_out adds the sums of the segments before it and scans the segment in tiles.
 // (The CPU emulation, where each block costs fiber set-up, scans in fewer segments.)
-#if defined(DN_EMU)
+#if defined(DN_EMU) && !defined(DN_EMU_GPU_SHAPES)
 constexpr int kScanBlocks = 16, kScanThreads = 256;
 #else
 constexpr int kScanBlocks = 1024, kScanThreads = 256;

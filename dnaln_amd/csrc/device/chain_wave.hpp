@@ -17,12 +17,25 @@
 //     load, and ends already used when their block of 64 starts are skipped by a ballot;
 //   * candidates: lane x keeps accepted candidate x's diagonal, so the dedupe test is a ballot.
 // Every lane writes the same value wherever all lanes later read (prev during extraction), so
-// each lane reads its own writes; the walk takes prev through readfirstlane, which keeps the
-// walk's state in scalar registers and states that it is the same on every lane.
+// each lane reads its own writes. In the CPU emulation, where the lanes of a wave run one after
+// another between collectives, the walk takes prev through readfirstlane (wave_same), so every
+// lane follows the first lane's value.
 // Lookback <= 64 and max_cand <= 64 (checked by the host).
 #pragma once
 
 #include "seed_chain.hpp"
+
+namespace dn {
+// A value every lane of the wave holds alike: as loaded on the device; through readfirstlane in
+// the CPU emulation.
+DN_HD uint32_t wave_same(uint32_t v) {
+#if defined(DN_EMU)
+  return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
+#else
+  return v;
+#endif
+}
+}  // namespace dn
 
 namespace dn {
 
@@ -223,7 +236,7 @@ __device__ __forceinline__ void chain_wave_read(const ChainArgs& a, uint32_t r, 
     while (open) {
       const int end = __builtin_amdgcn_readlane((int)my_end, (int)__builtin_ctzll(open));
       open &= open - 1;
-      uint32_t pu = (uint32_t)__builtin_amdgcn_readfirstlane((int)s.prev[end]);
+      uint32_t pu = wave_same(s.prev[end]);
       if (pu >> 31) continue;
       int i = end, cnt = 0, stop = -1;
       int64_t dmin = INT64_MAX, dmax = INT64_MIN;
@@ -236,7 +249,7 @@ __device__ __forceinline__ void chain_wave_read(const ChainArgs& a, uint32_t r, 
         dmax = d > dmax ? d : dmax;
         const int nx = (int)(pu & 0x7fffffffu) - 1;
         if (nx < 0) break;
-        pu = (uint32_t)__builtin_amdgcn_readfirstlane((int)s.prev[nx]);
+        pu = wave_same(s.prev[nx]);
         if (pu >> 31) {
           stop = nx;
           break;

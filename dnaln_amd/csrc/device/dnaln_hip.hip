@@ -659,6 +659,11 @@ int validate(const dnaln_map_params& p) {
 // reversed CIGARs in cig[n * cap]. Chunks the traceback launches so the direction scratch stays
 // under `dir_budget` bytes.
 
+__global__ void k_iota(uint32_t* out, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (uint32_t)i;
+}
+
 __global__ void k_len_keys(const SwTask* tasks, uint32_t n, uint32_t* keys, uint32_t* ids) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -870,6 +875,9 @@ struct dnaln_hip_batch : SpanSink {
   std::vector<int32_t> lens_h;       // host copy of the read lengths (bucket counts)
   bool tuned = false;
   DevBuf fmins;                      // minimizer lists of the fused kernel
+  // The host-synchronized batched form (run_chunk): length-sorted read order, second-pass list.
+  DevBuf lids, lkeys, lorder, ltmp, ovf2;
+  bool device_form = false;          // the last run took the device-count sequence
   // workspace
   DevBuf acnt, aoff, keys, f, prev, order, chains, ring_h, ring_ps;
   DevBuf cands, ncand, rcnt, roff, rtasks, rres, chosen, second, tbflag, tboff, ttasks, tres, tcig;
@@ -1050,6 +1058,22 @@ FusedShape fused_retry_shape(int k, int w, int maxlen) {
 }
 
 
+// Minimizer scratch of a fused launch over n reads (u64 entries).
+uint64_t fused_mins_words(const FusedShape& fs, uint64_t n) {
+  return (n + kFusedBlock - 1) / kFusedBlock * kFusedBlock * (uint64_t)fs.maxmin;
+}
+
+// The fused kernel over sa.list[0, sa.n_reads) (or reads 0 .. n_reads-1); reads it declines go
+// to ovf, counted in *novf.
+int launch_fused(const FusedShape& fs, const SeedArgs& sa, const ChainArgs& ca, uint64_t* mins, uint32_t* ovf,
+                 uint32_t* novf, hipStream_t st) {
+  const unsigned grid = (sa.n_reads + kFusedBlock - 1) / kFusedBlock;
+  hipLaunchKernelGGL((k_seed_chain_fused<10>), dim3(grid), dim3(kFusedBlock), fs.lds, st, sa, ca, fs.cap, fs.maxmin,
+                     mins, ovf, novf);
+  DN_HIP(hipGetLastError());
+  return DNALN_OK;
+}
+
 // Default length buckets: the longest reads whose expected minimizer list fits the 32-entry
 // shape (200 bp at most) and the 64-entry shape; 0 where the shape cannot be used.
 void default_splits(int k, int w, int out[2]) {
@@ -1115,7 +1139,7 @@ int greserve(dnaln_hip_batch* b, DevBuf& d, uint64_t bytes) {
 // Smallest planned batch size: grids of batches up to this many reads are the same, so their
 // plans (and graphs) are shared. The CPU emulation, where every block of a grid costs fiber
 // set-up, plans smaller grids.
-#if defined(DN_EMU)
+#if defined(DN_EMU) && !defined(DN_EMU_GPU_SHAPES)
 constexpr uint64_t kGridFloor = 128;
 #else
 constexpr uint64_t kGridFloor = 4096;
@@ -1598,7 +1622,7 @@ int prepare_batched(dnaln_hip_batch* b) {
   GraphPlan pl;
   DN_TRY(plan_batched(b, pl));
   b->gplan = pl;
-  if (b->cfg.graph != 1) return DNALN_OK;
+  if (b->cfg.graph != kFormGraph) return DNALN_OK;
   // Graphs from before a buffer was reallocated point at freed memory.
   for (size_t i = 0; i < b->graphs.size();) {
     if (b->graphs[i].plan.gen != pl.gen) {
@@ -1626,13 +1650,14 @@ int prepare_batched(dnaln_hip_batch* b) {
 int launch_batched(dnaln_hip_batch* b) {
   hipLaunchKernelGGL(k_begin, dim3(1), dim3(1), 0, b->st, b->gctl.as<BatchCtl>(), (uint32_t)b->n);
   DN_HIP(hipGetLastError());
-  if (b->cfg.graph != 1) return enqueue_batched(b, b->gplan, true);
+  if (b->cfg.graph != kFormGraph) return enqueue_batched(b, b->gplan, true);
   DN_HIP(hipGraphLaunch(b->gexec, b->st));
   return DNALN_OK;
 }
 
 // One chunk of reads [r0, r0 + n) through the whole pipeline, launch by launch, with host reads
-// of the task counts (the plain form).
+// of the task counts: the plain form (chunks of chunk_reads), and the host-synchronized batched
+// form (the whole batch as one chunk, length buckets, length-sorted SW tasks).
 int run_chunk(dnaln_hip_batch* b, uint64_t r0, uint32_t n, bool first_chunk) {
   dnaln_hip_ctx* c = b->ctx;
   const dnaln_map_params& p = c->p;
@@ -1640,8 +1665,8 @@ int run_chunk(dnaln_hip_batch* b, uint64_t r0, uint32_t n, bool first_chunk) {
   const int max_cand = std::max(p.max_cand, 0);
   const ScoreP sp{p.sc.match, p.sc.mismatch, p.sc.gap_open, p.sc.gap_ext, p.sc.ambig, p.sc.bw};
   const int cap = 2 * b->maxlen + 4;
-  // The plain form runs its SW tasks in read order, with slots for the longest read.
-  const int qmin = b->maxlen;
+  // The batched form sorts SW tasks by length; the plain form runs them in read order.
+  const int qmin = b->cfg.mode == DNALN_MODE_BATCHED ? b->minlen : b->maxlen;
   size_t sp_i = 0;
 
   ReadView rv;
@@ -1674,7 +1699,89 @@ int run_chunk(dnaln_hip_batch* b, uint64_t r0, uint32_t n, bool first_chunk) {
   ca.cands = b->cands.as<Cand>();
   ca.n_cand = b->ncand.as<int32_t>();
   const unsigned sgrid = (n + kSeedBlock - 1) / kSeedBlock;
-  DN_TRY(seed_chain_global(b, sa, ca, nullptr, n, sp_i));
+  if (b->cfg.mode == DNALN_MODE_BATCHED && n) {
+    // Batched form (one chunk: the whole batch). Length buckets: reads of up to split[0] and
+    // split[1] bases run the fused LDS kernel in its two shapes, longer ones the global-memory
+    // path, and reads the fused kernel declines are listed for the global path too. With more
+    // than one bucket in use the reads are sorted by length on the device first.
+    if (r0 != 0 || n != b->n) return fail(DNALN_E_INTERNAL, "batched form runs the whole batch as one chunk");
+    const uint64_t na = b->n_short, nb = b->n_mid;
+    const int used = (na > 0) + (nb > na) + (n > nb);
+    const uint32_t* lorder = nullptr;
+    DN_TRY(b->span_begin(0, sp_i));
+    if (used > 1) {
+      DN_TRY(b->lids.reserve((uint64_t)n * 4));
+      DN_TRY(b->lkeys.reserve((uint64_t)n * 4));
+      DN_TRY(b->lorder.reserve((uint64_t)n * 4));
+      hipLaunchKernelGGL(k_iota, dim3((n + 255) / 256), dim3(256), 0, st, b->lids.as<uint32_t>(), n);
+      DN_HIP(hipGetLastError());
+      const int bits = 32 - __builtin_clz((unsigned)std::max(b->maxlen, 1));
+      const uint32_t* lens_u = reinterpret_cast<const uint32_t*>(b->lens.as<int32_t>());
+      size_t bytes = 0;
+      DN_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, lens_u, b->lkeys.as<uint32_t>(), b->lids.as<uint32_t>(),
+                                                b->lorder.as<uint32_t>(), (uint32_t)n, 0, bits, st));
+      DN_TRY(b->ltmp.reserve(bytes));
+      DN_HIP(hipcub::DeviceRadixSort::SortPairs(b->ltmp.p, bytes, lens_u, b->lkeys.as<uint32_t>(), b->lids.as<uint32_t>(),
+                                                b->lorder.as<uint32_t>(), (uint32_t)n, 0, bits, st));
+      lorder = b->lorder.as<uint32_t>();
+    }
+    DN_TRY(b->ovf.reserve((uint64_t)n * 4 + 8));
+    uint32_t* ovf = b->ovf.as<uint32_t>();
+    uint32_t* novf_d = ovf + n;
+    DN_HIP(hipMemsetAsync(novf_d, 0, 4, st));
+    const uint64_t lo[2] = {0, na}, hi[2] = {na, nb};
+    for (int x = 0; x < 2; ++x) {
+      if (hi[x] <= lo[x]) continue;
+      const FusedShape fs = fused_shape(c->k, c->w, std::min(b->split[x], b->maxlen));
+      DN_TRY(b->fmins.reserve(fused_mins_words(fs, hi[x] - lo[x]) * 8));
+    }
+    for (int x = 0; x < 2; ++x) {
+      if (hi[x] <= lo[x]) continue;
+      const FusedShape fs = fused_shape(c->k, c->w, std::min(b->split[x], b->maxlen));
+      sa.list = lorder ? lorder + lo[x] : nullptr;
+      sa.n_reads = (uint32_t)(hi[x] - lo[x]);
+      DN_TRY(launch_fused(fs, sa, ca, b->fmins.as<uint64_t>(), ovf, novf_d, st));
+    }
+    uint32_t novf = 0;
+    DN_HIP(hipMemcpyAsync(&novf, novf_d, 4, hipMemcpyDeviceToHost, st));
+    DN_HIP(hipStreamSynchronize(st));
+    // Reads a 32-anchor shape declined (more minimizers or anchors than its slots) get a second
+    // fused pass in the 64-anchor shape; only what that declines too takes the global path.
+    bool small = false;
+    for (int x = 0; x < 2; ++x)
+      if (hi[x] > lo[x]) small = small || fused_shape(c->k, c->w, std::min(b->split[x], b->maxlen)).cap < 64;
+    const FusedShape retry = fused_retry_shape(c->k, c->w, b->maxlen);
+    if (novf && small && retry.cap) {
+      DN_TRY(b->ovf2.reserve((uint64_t)n * 4 + 8));
+      uint32_t* ovf2 = b->ovf2.as<uint32_t>();
+      uint32_t* novf2_d = ovf2 + n;
+      DN_HIP(hipMemsetAsync(novf2_d, 0, 4, st));
+      DN_TRY(b->fmins.reserve(fused_mins_words(retry, novf) * 8));
+      sa.list = ovf;
+      sa.n_reads = novf;
+      DN_TRY(launch_fused(retry, sa, ca, b->fmins.as<uint64_t>(), ovf2, novf2_d, st));
+      b->stats.fused_retry_reads += novf;
+      DN_HIP(hipMemcpyAsync(&novf, novf2_d, 4, hipMemcpyDeviceToHost, st));
+      DN_HIP(hipStreamSynchronize(st));
+      ovf = ovf2;
+    }
+    DN_TRY(b->span_end(sp_i));
+    const uint32_t* list = ovf;
+    uint32_t nl = novf;
+    if (n > nb) {
+      if (lorder) {
+        DN_HIP(hipMemcpyAsync(ovf + novf, lorder + nb, (n - nb) * 4, hipMemcpyDeviceToDevice, st));
+        nl += (uint32_t)(n - nb);
+      } else {  // every read is long: the global path in read order
+        list = nullptr;
+        nl = n;
+      }
+    }
+    b->stats.global_path_reads += nl;
+    if (nl) DN_TRY(seed_chain_global(b, sa, ca, list, nl, sp_i, true));
+  } else {
+    DN_TRY(seed_chain_global(b, sa, ca, nullptr, n, sp_i));
+  }
 
   // 3. rank pass
   DN_TRY(b->rcnt.reserve((uint64_t)n * 4 + 4));
@@ -2281,13 +2388,16 @@ int dnaln_hip_batch_create(dnaln_hip_ctx* c, const dnaln_batch_config* cfg, dnal
     DN_HIP(hipSetDevice(c->device));
     b->sw.dir_budget = b->cfg.dir_budget ? std::max<uint64_t>(b->cfg.dir_budget, 1 << 20) : default_dir_budget();
     if (b->cfg.walk_kind < 0 || b->cfg.walk_kind > 1) return fail(DNALN_E_INVALID, "walk_kind must be 0 or 1");
-    if (b->cfg.graph < 0 || b->cfg.graph > 1) return fail(DNALN_E_INVALID, "graph must be 0 or 1");
+    if (b->cfg.graph < 0 || b->cfg.graph > 2) return fail(DNALN_E_INVALID, "graph must be 0, 1 or 2");
 #if !defined(DN_GRAPH_CAPTURE)
-    // The captured form took a GPU memory fault on its first replay whose cause has not been
-    // found (the same kernels launched one by one are the default); it stays switched off until
-    // then. Builds that define DN_GRAPH_CAPTURE (the CPU emulation) keep it.
-    if (b->cfg.graph == 1)
-      return fail(DNALN_E_INVALID, "graph = 1: the captured form of the batched pipeline is switched off");
+    // The device-count sequence (graph = 2) and its captured form (graph = 1) faulted the GPU on
+    // their first run, and the cause has not been found: on the CPU emulation every case passes,
+    // bounds-checked, also on stale device memory and with the device's grid sizes. They stay
+    // switched off on the device until it is. Builds that define DN_GRAPH_CAPTURE (the CPU
+    // emulation) keep them.
+    if (b->cfg.graph != 0)
+      return fail(DNALN_E_INVALID, "graph = " + std::to_string(b->cfg.graph) +
+                                       ": the device-count forms of the batched pipeline are switched off");
 #endif
     b->sw.walk_kind = b->cfg.walk_kind;
     DN_HIP(hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking));
@@ -2392,16 +2502,18 @@ int dnaln_hip_batch_run(dnaln_hip_batch* b) {
     b->last_rc = DNALN_OK;
     size_t run_span = 0;
     int rc = DNALN_OK;
-    if (b->cfg.mode == DNALN_MODE_BATCHED) {
+    b->device_form = b->cfg.mode == DNALN_MODE_BATCHED && b->cfg.graph != 0;
+    if (b->device_form) {
       // The whole batch as launches whose sizes live on the device (batched_graph.hpp): nothing
       // waits on the device.
       rc = prepare_batched(b);
       if (rc == DNALN_OK) rc = b->span_begin(5, run_span);
       if (rc == DNALN_OK) rc = launch_batched(b);
     } else {
+      // Plain form: chunks of chunk_reads. Batched form: the whole batch as one chunk.
       DN_HIP(hipMemsetAsync(b->err.p, 0, 4, b->st));
       rc = b->span_begin(5, run_span);
-      const uint64_t chunk = b->cfg.chunk_reads;
+      const uint64_t chunk = b->cfg.mode == DNALN_MODE_PLAIN ? b->cfg.chunk_reads : std::max<uint64_t>(b->n, 1);
       for (uint64_t r0 = 0; rc == DNALN_OK && (r0 < b->n || (r0 == 0 && b->n == 0)); r0 += chunk) {
         const uint32_t m = (uint32_t)std::min<uint64_t>(chunk, b->n - r0);
         rc = run_chunk(b, r0, m, r0 == 0);
@@ -2511,7 +2623,7 @@ int dnaln_hip_batch_poll(dnaln_hip_batch* b, int wait) {
         DN_HIP(hipEventElapsedTime(&t, b->spans[i].a, b->spans[i].b));
         b->ms[b->spans[i].family] += t;
       }
-      if (b->cfg.mode != DNALN_MODE_BATCHED) {
+      if (!b->device_form) {
         int e = 0;
         DN_HIP(hipMemcpy(&e, b->err.p, 4, hipMemcpyDeviceToHost));
         err = (uint32_t)e;
@@ -2638,9 +2750,9 @@ int dnaln_hip_batch_reserve(dnaln_hip_batch* b, uint64_t n, uint64_t bases, int3
     DN_TRY(b->offs.reserve(n * 8 + 8, b->st));
     DN_TRY(b->lens.reserve(n * 4 + 4, b->st));
     DN_TRY(b->fnv.reserve(n * 4 + 4, b->st));
-    if (b->cfg.mode == DNALN_MODE_BATCHED) {
-      // Plan the graph for the largest such batch: n reads of every length up to maxlen, all
-      // (for the direction bound) of length maxlen.
+    if (b->cfg.mode == DNALN_MODE_BATCHED && b->cfg.graph != kFormHostSync) {
+      // Plan the device-count sequence for the largest such batch: n reads of every length up to
+      // maxlen, all (for the direction bound) of length maxlen.
       const uint64_t n0 = b->n, bases0 = b->bases, sum0 = b->sum_len;
       const int min0 = b->minlen, max0 = b->maxlen;
       std::vector<uint32_t> hist0;
@@ -2701,6 +2813,32 @@ int dnaln_hip_batch_reserve(dnaln_hip_batch* b, uint64_t n, uint64_t bases, int3
     DN_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, t1, (const uint32_t*)nullptr, (uint64_t*)nullptr, (uint32_t)tasks,
                                             b->st));
     DN_TRY(b->scan_tmp.reserve(t1));
+    if (b->cfg.mode == DNALN_MODE_BATCHED) {
+      // The host-synchronized batched form: read length order, the fused kernel's lists, and the
+      // SW tasks' length sort and per-wave slot offsets.
+      DN_TRY(b->lids.reserve(n * 4));
+      DN_TRY(b->lkeys.reserve(n * 4));
+      DN_TRY(b->lorder.reserve(n * 4));
+      DN_TRY(b->ovf.reserve(n * 4 + 8));
+      DN_TRY(b->ovf2.reserve(n * 4 + 8));
+      FusedShape fs;
+      fs.maxmin = 64;
+      DN_TRY(b->fmins.reserve(fused_mins_words(fs, n) * 8));
+      DN_TRY(b->sw.keys.reserve(tasks * 4));
+      DN_TRY(b->sw.keys2.reserve(tasks * 4));
+      DN_TRY(b->sw.ids.reserve(tasks * 4));
+      DN_TRY(b->sw.ids2.reserve(tasks * 4));
+      const uint64_t waves = (tasks + 127) / 128;
+      DN_TRY(b->sw.woff.reserve((waves + 2) * 16));
+      size_t t2 = 0, t3 = 0;
+      DN_HIP(hipcub::DeviceRadixSort::SortPairsDescending(nullptr, t2, (const uint32_t*)nullptr, (uint32_t*)nullptr,
+                                                          (const uint32_t*)nullptr, (uint32_t*)nullptr,
+                                                          (uint32_t)tasks, 0, 16, b->st));
+      DN_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, t3, (const uint64_t*)nullptr, (uint64_t*)nullptr,
+                                              (uint32_t)(waves + 1), b->st));
+      DN_TRY(b->sw.tmp.reserve(std::max(t2, t3)));
+      DN_TRY(b->ltmp.reserve(t2));
+    }
     DN_HIP(hipStreamSynchronize(b->st));
     return DNALN_OK;
   });
@@ -2819,6 +2957,68 @@ int dnaln_hip_context_params(dnaln_hip_ctx* c, dnaln_map_params* out) {
 void dnaln_hip_alloc_counters(uint64_t* events, uint64_t* bytes) {
   if (events) *events = g_alloc_events.load(std::memory_order_relaxed);
   if (bytes) *bytes = g_alloc_bytes.load(std::memory_order_relaxed);
+}
+
+// ------------------------------------------------------------------ packed int16 VALU peak
+// Sixteen independent chains of v_pk_add_u16 + v_pk_max_i16 per lane: 64 int16 operations per
+// lane per iteration and no memory access in the loop. The values depend on the thread and the
+// seed, and a thread stores only when its result equals a value the host chose, so every
+// operation stays live while the kernel (almost surely) writes nothing.
+namespace {
+constexpr int kPeakChains = 16, kPeakThreads = 256, kPeakWavesPerSimd = 8;
+
+__global__ __launch_bounds__(kPeakThreads) void k_peak_pk16(int iters, uint32_t seed, uint32_t magic,
+                                                             uint32_t* sink) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  pk16 a[kPeakChains];
+#pragma unroll
+  for (int j = 0; j < kPeakChains; ++j) a[j] = pk_from(t * 2654435761u + (uint32_t)j * 40503u + seed);
+  const pk16 inc = pk_from(seed | 0x00010001u), lim = pk_from(seed >> 3);
+  for (int it = 0; it < iters; ++it) {
+#pragma unroll
+    for (int j = 0; j < kPeakChains; ++j) a[j] = pk_max(pk_add(a[j], inc), lim);
+  }
+  uint32_t x = 0;
+#pragma unroll
+  for (int j = 0; j < kPeakChains; ++j) x ^= pk_bits(a[j]);
+  if (x == magic) sink[t & 255] = x;
+}
+}  // namespace
+
+int dnaln_hip_peak_int16(int32_t device, int32_t iters, double* tops, double* ms) {
+  return guarded([&]() -> int {
+    if (!tops || iters <= 0) return fail(DNALN_E_INVALID, "bad argument");
+    DN_HIP(hipSetDevice(device));
+    const unsigned blocks = (unsigned)device_cus(device) * 4 * kPeakWavesPerSimd / (kPeakThreads / 64);
+    DevBuf sink;
+    DN_TRY(sink.reserve(256 * 4));
+    hipStream_t st = nullptr;
+    DN_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = DNALN_OK;
+    float t = 0;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = fail(DNALN_E_DEVICE, "hipEventCreate");
+    if (rc == DNALN_OK) {
+      // One short launch to load the code object and raise the clocks, then the timed one.
+      hipLaunchKernelGGL(k_peak_pk16, dim3(blocks), dim3(kPeakThreads), 0, st, std::max(iters / 16, 1), 12345u,
+                         0xffffffffu, sink.as<uint32_t>());
+      (void)hipEventRecord(e0, st);
+      hipLaunchKernelGGL(k_peak_pk16, dim3(blocks), dim3(kPeakThreads), 0, st, iters, 0x2468ace1u, 0xffffffffu,
+                         sink.as<uint32_t>());
+      (void)hipEventRecord(e1, st);
+      const hipError_t e = hipEventSynchronize(e1);
+      if (e != hipSuccess) rc = fail(DNALN_E_DEVICE, std::string("k_peak_pk16: ") + hipGetErrorString(e));
+      else if (hipEventElapsedTime(&t, e0, e1) != hipSuccess || t <= 0) rc = fail(DNALN_E_DEVICE, "event timing");
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipStreamDestroy(st);
+    DN_TRY(rc);
+    const double ops = (double)blocks * kPeakThreads * (double)iters * kPeakChains * 2 * 2;
+    *tops = ops / (t * 1e-3) / 1e12;
+    if (ms) *ms = t;
+    return DNALN_OK;
+  });
 }
 
 int dnaln_hip_map_batch(dnaln_hip_ctx* c, const uint8_t* read_codes, const uint64_t* offs, const int32_t* lens,

@@ -423,12 +423,58 @@ inline int hip_emu_poison() {
   }();
   return v;
 }
+// HIP_EMU_STALE=1: a new allocation starts with the bytes of earlier freed allocations instead
+// (a device allocator hands out reused memory), so stale counts, offsets and addresses of freed
+// buffers turn up where code reads memory it has not written.
+namespace hip_emu {
+struct StalePool {
+  std::mutex mu;
+  std::vector<unsigned char> bytes;  // contents of freed allocations, oldest overwritten first
+  size_t head = 0, fill = 0;
+  std::vector<std::pair<void*, size_t>> live;
+  static bool on() {
+    static const bool v = std::getenv("HIP_EMU_STALE") != nullptr;
+    return v;
+  }
+};
+inline StalePool& stale_pool() {
+  static StalePool p;
+  return p;
+}
+}  // namespace hip_emu
 inline hipError_t hipMalloc(void** p, size_t n) {
   *p = std::malloc(n ? n : 1);
-  if (*p) std::memset(*p, hip_emu_poison(), n);
-  return *p ? hipSuccess : hipErrorInvalidValue;
+  if (!*p) return hipErrorInvalidValue;
+  std::memset(*p, hip_emu_poison(), n);
+  if (hip_emu::StalePool::on()) {
+    auto& sp = hip_emu::stale_pool();
+    std::lock_guard<std::mutex> lk(sp.mu);
+    unsigned char* d = static_cast<unsigned char*>(*p);
+    for (size_t i = 0; sp.fill && i < n;) {
+      const size_t off = (sp.head + i) % sp.fill, k = std::min(n - i, sp.fill - off);
+      std::memcpy(d + i, sp.bytes.data() + off, k);
+      i += k;
+    }
+    sp.head = sp.fill ? (sp.head + n) % sp.fill : 0;
+    sp.live.emplace_back(*p, n);
+  }
+  return hipSuccess;
 }
 inline hipError_t hipFree(void* p) {
+  if (p && hip_emu::StalePool::on()) {
+    auto& sp = hip_emu::stale_pool();
+    std::lock_guard<std::mutex> lk(sp.mu);
+    for (size_t i = 0; i < sp.live.size(); ++i)
+      if (sp.live[i].first == p) {
+        constexpr size_t kCap = 64u << 20;
+        if (sp.bytes.size() < kCap) sp.bytes.resize(kCap);
+        const unsigned char* s = static_cast<const unsigned char*>(p);
+        for (size_t j = 0; j < sp.live[i].second && j < kCap; ++j) sp.bytes[(sp.fill + j) % kCap] = s[j];
+        sp.fill = std::min(kCap, sp.fill + sp.live[i].second);
+        sp.live.erase(sp.live.begin() + (long)i);
+        break;
+      }
+  }
   std::free(p);
   return hipSuccess;
 }

@@ -3,6 +3,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <thread>
 #include <vector>
 
 #include "device/seed_chain.hpp"
@@ -279,3 +280,53 @@ void dnaln_fnv1a_many(const char* names, const uint64_t* offs, const int32_t* le
 }
 
 }  // extern "C"
+
+// The byte model of the seed stage: every read sketched (src/minimizer.cpp:5-68), every minimizer
+// looked up by linear probing (src/index.hpp:24-38), the occurrences of keys that pass max_occ
+// read as the reference's anchor loop reads them (src/mapper.cpp:24-40).
+int dnaln_seed_traffic(const dnaln_index_view* idx, const uint8_t* codes, const uint64_t* offs, const int32_t* lens,
+                       uint64_t n, uint32_t max_occ, int32_t threads, dnaln_seed_traffic_counts* out) {
+  return dnaln_amd::c_guard([&]() -> int {
+    if (!idx || !out || (n && (!codes || !offs || !lens)) || idx->n_buckets == 0 ||
+        (idx->n_buckets & (idx->n_buckets - 1)) || idx->w < 1 || idx->w > 255)
+      return DNALN_E_INVALID;
+    const int nt = std::max(1, std::min(threads > 0 ? threads : 8, 64));
+    std::vector<dnaln_seed_traffic_counts> part((size_t)nt);
+    std::vector<std::thread> th;
+    const uint64_t mask = idx->n_buckets - 1;
+    for (int t = 0; t < nt; ++t)
+      th.emplace_back([&, t] {
+        dnaln_seed_traffic_counts c{};
+        std::vector<uint64_t> rh((size_t)idx->w);
+        std::vector<uint32_t> rps((size_t)idx->w);
+        dn::Ring ring{rh.data(), rps.data(), 1};
+        for (uint64_t r = n * t / nt; r < n * (t + 1) / nt; ++r) {
+          c.read_bases += (uint64_t)std::max(lens[r], 0);
+          dn::sketch_stream(dn::PtrReader{codes + offs[r]}, lens[r], idx->k, idx->w, ring,
+                            [&](uint64_t h, uint32_t, uint32_t) {
+                              ++c.minimizers;
+                              for (uint64_t i = h & mask;; i = (i + 1) & mask) {
+                                ++c.buckets;
+                                const dnaln_bucket& b = idx->table[i];
+                                if (b.key == h) {
+                                  if (b.count >= 1 && b.count <= max_occ) c.occurrences += b.count;
+                                  break;
+                                }
+                                if (b.key == ~0ULL) break;
+                              }
+                            });
+        }
+        part[(size_t)t] = c;
+      });
+    for (auto& x : th) x.join();
+    std::memset(out, 0, sizeof *out);
+    for (const auto& c : part) {
+      out->read_bases += c.read_bases;
+      out->minimizers += c.minimizers;
+      out->buckets += c.buckets;
+      out->occurrences += c.occurrences;
+    }
+    out->bytes = out->read_bases + 16 * out->buckets + 4 * out->occurrences;
+    return DNALN_OK;
+  });
+}

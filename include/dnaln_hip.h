@@ -146,6 +146,11 @@ int dnaln_hip_context_params(struct dnaln_hip_ctx* ctx, dnaln_map_params* out);
 /* Device allocations the library has made in this process (count and bytes), for callers that
  * check that a warmed-up path allocates nothing (no reference counterpart). */
 void dnaln_hip_alloc_counters(uint64_t* events, uint64_t* bytes);
+/* Measured packed-int16 VALU peak of a device (no reference counterpart; the roofline the SW
+ * fill is judged against): sixteen independent chains of v_pk_add_u16 + v_pk_max_i16 per lane,
+ * eight waves per SIMD on every CU, `iters` iterations; *tops = int16 operations (two per packed
+ * instruction and lane) per second / 1e12, *ms = the timed launch. */
+int dnaln_hip_peak_int16(int32_t device, int32_t iters, double* tops, double* ms);
 
 /* MinimizerIndex::build (src/index.cpp:10-45) on the GPU: every contig sketched in segments of
  * 1,024 k-mer starts, one per thread; the occurrences radix-sorted by (hash, pos); each key placed
@@ -206,12 +211,11 @@ enum { DNALN_MODE_PLAIN = 0, DNALN_MODE_BATCHED = 1 };
 typedef struct {
   int32_t mode;             /* DNALN_MODE_PLAIN: one launch sequence per chunk of chunk_reads, with
                                host reads of the task counts between launches;
-                               DNALN_MODE_BATCHED: the whole batch as one fixed sequence of
-                               kernels (length-sorted reads, fused seed + chain buckets, SW tasks
-                               in length order); each kernel's grid is sized for the batch's
-                               worst case and its blocks read the live task counts from device
-                               memory, so no host read happens until the batch is polled, and
-                               the sequence can be captured as one HIP graph (`graph`) */
+                               DNALN_MODE_BATCHED: the whole batch as one chunk: reads bucketed
+                               by length into the fused seed + chain kernel's LDS shapes (the
+                               rest on the global-memory path), SW tasks sorted by length; the
+                               launches are separate, with a few host reads of counts (`graph`
+                               selects the forms without them) */
   uint32_t chunk_reads;     /* plain form: reads per chunk (65536 by default) */
   uint64_t max_reads;       /* capacity of the batch */
   uint64_t max_bases;
@@ -226,10 +230,13 @@ typedef struct {
                                (0: min(16 GiB, free HBM / 8)) */
   int32_t walk_kind;        /* traceback walk: 0 one thread per task, 1 one block per fill wave
                                through LDS windows of flag words */
-  int32_t graph;            /* batched form: 0 = launch the kernels one by one, with timing events
-                               around the stages (per-stage timing); 1 = replay one captured HIP
-                               graph of the same kernels per plan (timing: the whole run only).
-                               1 is refused (DNALN_E_INVALID) by builds without graph capture */
+  int32_t graph;            /* batched form: 0 = the form above (default); 2 = one fixed kernel
+                               sequence whose grids cover the batch's worst case and whose blocks
+                               read the live counts from device memory (no host read until the
+                               batch is polled); 1 = that sequence captured as one HIP graph per
+                               plan. 1 and 2 are refused (DNALN_E_INVALID) by the device build:
+                               their first run faulted the GPU and the cause is not found (the
+                               CPU emulation runs them) */
 } dnaln_batch_config;
 
 void dnaln_hip_default_batch_config(dnaln_batch_config* c);

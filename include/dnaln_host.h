@@ -53,6 +53,17 @@ int dnaln_index_load(const char* path, const uint8_t* ref_codes, const dnaln_con
 /* The digest stored in an index file: contig offsets and lengths, then their codes. */
 uint64_t dnaln_reference_digest(const uint8_t* ref_codes, const dnaln_contig* contigs, int32_t n_contigs);
 
+/* The algorithmic HBM bytes of the seed stage of a batch, for the roofline of the probe kernels (no
+ * reference counterpart; a measurement aid): each read sketched as src/minimizer.cpp:5-68 does, each
+ * minimizer looked up by linear probing as src/index.hpp:24-38 does, and the occurrences of keys
+ * with 1 <= count <= max_occ read as src/mapper.cpp:24-40 reads them. bytes = read_bases +
+ * 16 * buckets + 4 * occurrences. Reads are split over `threads` host threads. */
+typedef struct {
+  uint64_t read_bases, minimizers, buckets, occurrences, bytes;
+} dnaln_seed_traffic_counts;
+int dnaln_seed_traffic(const dnaln_index_view* idx, const uint8_t* codes, const uint64_t* offs, const int32_t* lens,
+                       uint64_t n, uint32_t max_occ, int32_t threads, dnaln_seed_traffic_counts* out);
+
 uint32_t dnaln_fnv1a(const char* s, size_t n);
 void dnaln_fnv1a_many(const char* names, const uint64_t* offs, const int32_t* lens, uint64_t n, uint32_t* out);
 void dnaln_encode(const char* s, size_t n, uint8_t* out);

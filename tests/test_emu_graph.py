@@ -6,7 +6,9 @@ readfirstlane, ds_bpermute) and barriers, device memory is host memory allocated
 size asked for, and the build carries AddressSanitizer, so any access outside a buffer, LDS array
 or kernel argument aborts the run. Both launch forms run here: the kernels one by one, and the
 captured graph replayed (this build keeps graph capture). Records must equal the reference
-binary's records of the golden sets; small read ranges keep each case to seconds."""
+binary's records of the golden sets; small read ranges keep each case to seconds. GRAPH selects
+the batched form: 0 the host-synchronized form, 2 the device-count sequence launched one by one,
+1 that sequence captured as a graph (the device build refuses 1 and 2)."""
 from __future__ import annotations
 
 import os
@@ -91,7 +93,7 @@ def test_global_path_and_wave_chain_in_the_graph(tmp_root):
 
 
 def test_direct_launches_give_reference_records(tmp_root):
-    _run("edge_uniform_bw64", tmp_root, [(0, 200)], graph=0)
+    _run("edge_uniform_bw64", tmp_root, [(0, 200)], graph=2)
 
 
 def test_anchor_overflow_runs_again(tmp_root):
@@ -119,14 +121,15 @@ def test_anchor_overflow_with_stale_device_memory(tmp_root, poison):
     it again; those stages must not read a count the overflowed chain stage left unwritten. With
     device memory holding large positive stale values, a read's unwritten candidate count once sent
     the rank-task stage far outside its buffers."""
-    out = _run("edge_repeats_bw32", tmp_root, [(0, 150)], graph=0, split=(-1, -1), apr=1, poison=poison)
+    out = _run("edge_repeats_bw32", tmp_root, [(0, 150)], graph=2, split=(-1, -1), apr=1, poison=poison)
     f = out[0].split()
     assert int(f[f.index("anchors") + 1]) > 150
 
 
-@pytest.mark.parametrize("mode", [abi.MODE_BATCHED, abi.MODE_PLAIN], ids=["batched", "plain"])
+@pytest.mark.parametrize("form", [(abi.MODE_BATCHED, 2), (abi.MODE_BATCHED, 0), (abi.MODE_PLAIN, 0)],
+                         ids=["batched-device-counts", "batched", "plain"])
 @pytest.mark.parametrize("name", ["uniform4m_150bp_10k", "edge_twins_sr_bw5", "edge_uniform_bw64"])
-def test_stale_device_memory_both_forms(tmp_root, name, mode):
-    """Every stage of both forms on device memory that starts as large positive values: no stage
+def test_stale_device_memory_every_form(tmp_root, name, form):
+    """Every stage of every form on device memory that starts as large positive values: no stage
     may consume a value it did not write first."""
-    _run(name, tmp_root, [(0, 200), (30, 230)], graph=0, poison=0x01, mode=mode)
+    _run(name, tmp_root, [(0, 200), (30, 230)], graph=form[1], poison=0x01, mode=form[0])

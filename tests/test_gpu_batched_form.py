@@ -1,8 +1,9 @@
-"""The batched form (batched_graph.hpp): one fixed sequence of kernels whose sizes live in device
-memory. Repeated runs of one batch with other device work between them, batches of many sizes
-through one batch object, the run again after the global path outgrew its anchor buffer, the
-int32 fill for a band without a packed kernel, and the captured-graph switch. Every case must give
-the reference binary's records (or the oracle's, where no golden set covers the band)."""
+"""The batched form (graph = 0, the default): the whole batch as one chunk, length buckets for the
+fused seed + chain kernel, length-sorted SW tasks. Repeated runs of one batch with other device work
+between them, batches of many sizes through one batch object, a first anchor capacity far too small,
+the int32 fill for a band without a packed kernel, and the refusal of the device-count forms
+(graph = 1, 2), which stay off on the device. Every case must give the reference binary's records
+(or the oracle's, where no golden set covers the band)."""
 from __future__ import annotations
 
 import numpy as np
@@ -68,13 +69,14 @@ def test_stage_timings_and_reference_records(name):
     print("timings", [round(x, 3) for x in ms], launches, flush=True)
 
 
-def test_captured_graph_is_refused_while_switched_off():
-    """graph = 1 (one captured HIP graph per plan) is switched off in the device build: its first
-    replay took a GPU memory fault whose cause is not found. The batch refuses it at creation,
-    before anything reaches the device."""
+@pytest.mark.parametrize("graph", [1, 2])
+def test_device_count_forms_are_refused_while_switched_off(graph):
+    """graph = 2 (the device-count sequence) and graph = 1 (it, captured) are switched off in the
+    device build: their first run faulted the GPU and the cause is not found. The batch refuses
+    them at creation, before anything reaches the device."""
     spec = sets.SETS["edge_uniform_bw16"]
     with pytest.raises(api.DnalnError) as e:
-        _context(spec).batch(mode=abi.MODE_BATCHED, graph=1)
+        _context(spec).batch(mode=abi.MODE_BATCHED, graph=graph)
     assert e.value.code == abi.E_INVALID
     assert "switched off" in str(e.value)
 
@@ -96,9 +98,9 @@ def test_batches_of_many_sizes_through_one_batch_object():
 
 
 @pytest.mark.parametrize("name", ["edge_repeats_bw32", "edge_uniform_bw64"])
-def test_anchor_buffer_overflow_runs_again(name):
-    """A first anchor capacity of one per read is too small for the global path; the run reports
-    it, the buffer grows and the batch runs again, with the reference's records."""
+def test_small_first_anchor_capacity(name):
+    """A first anchor capacity of one per read is far too small for the global path (every read
+    on it); the run grows its buffers and gives the reference's records."""
     spec = sets.SETS[name]
     g = sets.genome(spec)
     r = sets.reads(spec, g)

@@ -151,3 +151,39 @@ def test_request_trace_follows_the_serving_model():
     assert 380 < sizes.mean() < 620
     # Poisson arrivals at 1000/s: mean gap 1 ms
     assert 0.8e-3 < np.diff(arrive).mean() < 1.25e-3
+
+
+def test_seed_traffic_model_counts_match_the_oracle():
+    """dnaln_seed_traffic against a count made from the oracle: its sketch of every read, and for
+    each minimizer the buckets linear probing visits in the oracle's table (the home slot through
+    the matching or the first empty one) and the occurrences of keys with 1 <= count <= max_occ.
+    A repeat-rich genome makes probe chains and multi-occurrence keys common."""
+    g = synth.make_genome(0, 300_000, 3, 77, repeat_frac=0.3)
+    r = synth.make_reads(g, 400, synth.LEN_UNIFORM, 20, 400, 0.02, 0.003, 6, 0.5, 78, n_rate=0.003)
+    rb = api.prepare(r)
+    k, w, max_occ = 15, 10, 3
+    ix = api.Index(g, k, w)
+    got = ix.seed_traffic(rb, max_occ=max_occ, threads=3)
+    oi = oracle.Index(g.codes, g.contig_off, g.contig_len, k, w)
+    tab = oi.table()
+    mask = len(tab) - 1
+    empty = np.uint64(0xFFFFFFFFFFFFFFFF)
+    mins = buckets = occ = 0
+    for i in range(len(rb)):
+        o, L = int(rb.offs[i]), int(rb.lens[i])
+        for m in oracle.sketch(rb.codes[o:o + L], k, w):
+            mins += 1
+            s = int(m["hash"]) & mask
+            while True:
+                buckets += 1
+                if tab["key"][s] == m["hash"]:
+                    c = int(tab["count"][s])
+                    occ += c if 1 <= c <= max_occ else 0
+                    break
+                if tab["key"][s] == empty:
+                    break
+                s = (s + 1) & mask
+    bases = int(rb.lens.astype(np.int64).sum())
+    assert (got["minimizers"], got["buckets"], got["occurrences"], got["read_bases"]) == (mins, buckets, occ, bases)
+    assert got["bytes"] == bases + 16 * buckets + 4 * occ
+    assert occ > 0 and buckets > mins  # the case exercises multi-occurrence keys and probe chains
