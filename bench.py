@@ -143,8 +143,10 @@ def config_block(name: str, peak_ops: float, steps: int, warmup: int, plain_step
            "batched_over_plain": b["reads_per_s"] / plain["reads_per_s"] if plain else None,
            "stage_ms": dict(zip(api.Batch.FAMILIES, fam)),
            "traceback_fill_frac": tb_cells * OPS_PER_CELL["traceback"] / (fam[3] * 1e-3) / peak_ops if fam[3] > 0 else None,
-           "score_fill_frac": rank_cells * OPS_PER_CELL["score"] / (fam[2] * 1e-3) / peak_ops if fam[2] > 0 else None,
-           "kernels": f"k_sw_packed_g<{W}, true> / <{W}, false>",
+           "score_fill_frac": rank_cells * OPS_PER_CELL["score"] / (fam[2] * 1e-3) / peak_ops
+           if fam[2] > 0 and rank_cells >= 10**8 else None,
+           "rank_cells": rank_cells, "traceback_cells": tb_cells,
+           "kernels": f"k_sw_packed<{W}, true> / <{W}, false>",
            "seed_model": seed,
            "seed_hbm_frac": seed["bytes"] / (fam[0] * 1e-3) / HBM_PEAK if fam[0] > 0 else None,
            "index_bytes": int(ix.n_buckets) * 16 + int(ix.n_pos) * 4,
@@ -444,7 +446,7 @@ def main() -> int:
     # events around its launches on the batch stream.
     tb_ops = tb_cells * OPS_PER_CELL["traceback"]
     achieved = tb_ops / (tb_ms * 1e-3) if tb_ms > 0 else 0.0
-    roofline = {"bound": "valu-int16", "kernel": f"k_sw_packed_g<{W}, true>",
+    roofline = {"bound": "valu-int16", "kernel": f"k_sw_packed<{W}, true>",
                 "achieved": achieved / 1e12, "peak": peak_ops / 1e12, "unit": "Tops/s (packed int16)",
                 "frac": achieved / peak_ops, "traffic": None, "peak_source": peak,
                 "frac_of_nominal_peak": achieved / P16_PEAK,
