@@ -378,7 +378,7 @@ def main() -> int:
     ap.add_argument("--no-tune", action="store_true", help="batched form: skip the autotuner (it runs before timing)")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving sweep (config 5)")
     ap.add_argument("--serve-seconds", type=float, default=1.0, help="seconds of arrivals per load level")
-    ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-slots", type=int, default=6)
     ap.add_argument("--no-e2e", action="store_true", help="skip the FASTQ-to-SAM end-to-end run")
     ap.add_argument("--configs", default="uniform4m_uneven_1m_bw16,uniform4m_uneven_1m_bw64,repeats120m_250bp_4m",
                     help="further workloads timed at full size in compact sub-blocks ('' for none)")

@@ -405,7 +405,7 @@ class Server:
     """Continuous batching on the resident index (dnaln_serve_*): requests are queued, packed into
     batches as slots and the HBM budget allow, and run concurrently on the slots' streams."""
 
-    def __init__(self, ctx: Context, slots: int = 4, mode: int = abi.MODE_BATCHED, hbm_budget: int = 0,
+    def __init__(self, ctx: Context, slots: int = 6, mode: int = abi.MODE_BATCHED, hbm_budget: int = 0,
                  max_batch_reads: int = 1 << 18, max_batch_bases: int = 1 << 27, dir_budget: int = 0):
         self.ctx = ctx
         cfg = abi.ServeConfig()

@@ -682,7 +682,7 @@ __global__ void k_wave_words(const uint32_t* sorted_len, uint32_t waves, uint64_
 int scan_u64(DevBuf& tmp, const uint64_t* in, uint64_t* out, uint32_t n, hipStream_t st) {
   size_t bytes = 0;
   DN_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, n, st));
-  DN_TRY(tmp.reserve(bytes));
+  DN_TRY(tmp.reserve(bytes, st));
   DN_HIP(hipcub::DeviceScan::ExclusiveSum(tmp.p, bytes, in, out, n, st));
   return DNALN_OK;
 }
@@ -709,10 +709,10 @@ int run_sw(const ScoreP& sp, const SwTask* tasks, uint32_t n, int qmin, int qmax
     const uint32_t* order = nullptr;
     const uint64_t* woff = nullptr;
     if (qmax > qmin && n > 128) {
-      DN_TRY(wk.keys.reserve((uint64_t)n * 4));
-      DN_TRY(wk.keys2.reserve((uint64_t)n * 4));
-      DN_TRY(wk.ids.reserve((uint64_t)n * 4));
-      DN_TRY(wk.ids2.reserve((uint64_t)n * 4));
+      DN_TRY(wk.keys.reserve((uint64_t)n * 4, st));
+      DN_TRY(wk.keys2.reserve((uint64_t)n * 4, st));
+      DN_TRY(wk.ids.reserve((uint64_t)n * 4, st));
+      DN_TRY(wk.ids2.reserve((uint64_t)n * 4, st));
       hipLaunchKernelGGL(k_len_keys, dim3((n + 255) / 256), dim3(256), 0, st, tasks, n, wk.keys.as<uint32_t>(),
                          wk.ids.as<uint32_t>());
       DN_HIP(hipGetLastError());
@@ -720,12 +720,12 @@ int run_sw(const ScoreP& sp, const SwTask* tasks, uint32_t n, int qmin, int qmax
       size_t bytes = 0;
       DN_HIP(hipcub::DeviceRadixSort::SortPairsDescending(nullptr, bytes, wk.keys.as<uint32_t>(), wk.keys2.as<uint32_t>(),
                                                           wk.ids.as<uint32_t>(), wk.ids2.as<uint32_t>(), n, 0, bits, st));
-      DN_TRY(wk.tmp.reserve(bytes));
+      DN_TRY(wk.tmp.reserve(bytes, st));
       DN_HIP(hipcub::DeviceRadixSort::SortPairsDescending(wk.tmp.p, bytes, wk.keys.as<uint32_t>(), wk.keys2.as<uint32_t>(),
                                                           wk.ids.as<uint32_t>(), wk.ids2.as<uint32_t>(), n, 0, bits, st));
       order = wk.ids2.as<uint32_t>();
       if (traceback) {
-        DN_TRY(wk.woff.reserve((uint64_t)(waves + 1) * 16));
+        DN_TRY(wk.woff.reserve((uint64_t)(waves + 1) * 16, st));
         uint64_t* words = wk.woff.as<uint64_t>();
         uint64_t* offs = words + (waves + 1);
         hipLaunchKernelGGL(k_wave_words, dim3((waves + 256) / 256), dim3(256), 0, st, wk.keys2.as<uint32_t>(), waves,
@@ -757,7 +757,7 @@ int run_sw(const ScoreP& sp, const SwTask* tasks, uint32_t n, int qmin, int qmax
       for (uint32_t w0 = 0; w0 < waves; w0 += per) cuts.emplace_back(w0, std::min(waves, w0 + per));
       dir_words = (uint64_t)std::min(waves, per) * slot_words;
     }
-    if (traceback) DN_TRY(wk.dir.reserve(dir_words * 8));
+    if (traceback) DN_TRY(wk.dir.reserve(dir_words * 8, st));
     for (const auto& c : cuts) {
       const uint32_t t0 = c.first * 128, m = std::min<uint64_t>(n, (uint64_t)c.second * 128) - t0;
       if (order) {
@@ -801,8 +801,8 @@ int run_sw(const ScoreP& sp, const SwTask* tasks, uint32_t n, int qmin, int qmax
     const uint64_t stride = traceback ? (uint64_t)std::max(qmax, 1) * W : 0;
     uint32_t per_launch = n;
     if (traceback) per_launch = (uint32_t)std::min<uint64_t>(n, std::max<uint64_t>(256, budget / std::max<uint64_t>(stride, 1)));
-    DN_TRY(wk.rows.reserve((uint64_t)per_launch * 2 * (W + 1) * sizeof(int32_t)));
-    if (traceback) DN_TRY(wk.dir.reserve((uint64_t)per_launch * stride));
+    DN_TRY(wk.rows.reserve((uint64_t)per_launch * 2 * (W + 1) * sizeof(int32_t), st));
+    if (traceback) DN_TRY(wk.dir.reserve((uint64_t)per_launch * stride, st));
     for (uint32_t t0 = 0; t0 < n; t0 += per_launch) {
       const uint32_t m = std::min(per_launch, n - t0);
       g.tasks = tasks + t0;
@@ -828,7 +828,7 @@ template <class T>
 int exclusive_scan(DevBuf& tmp, const T* in, uint64_t* out, uint32_t n, hipStream_t st) {
   size_t bytes = 0;
   DN_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, n, st));
-  DN_TRY(tmp.reserve(bytes));
+  DN_TRY(tmp.reserve(bytes, st));
   DN_HIP(hipcub::DeviceScan::ExclusiveSum(tmp.p, bytes, in, out, n, st));
   return DNALN_OK;
 }
@@ -961,8 +961,8 @@ int seed_chain_global(dnaln_hip_batch* b, SeedArgs sa, ChainArgs ca, const uint3
                       size_t& sp_i, bool wave_chain = false) {
   dnaln_hip_ctx* c = b->ctx;
   hipStream_t st = b->st;
-  DN_TRY(b->acnt.reserve((uint64_t)nl * 4 + 4));
-  DN_TRY(b->aoff.reserve((uint64_t)nl * 8 + 8));
+  DN_TRY(b->acnt.reserve((uint64_t)nl * 4 + 4, st));
+  DN_TRY(b->aoff.reserve((uint64_t)nl * 8 + 8, st));
   sa.n_reads = nl;
   sa.list = list;
   sa.anchor_count = b->acnt.as<uint32_t>();
@@ -970,8 +970,8 @@ int seed_chain_global(dnaln_hip_batch* b, SeedArgs sa, ChainArgs ca, const uint3
   const size_t lds = (size_t)c->w * kSeedBlock * 12;
   const int use_lds = lds <= 48 * 1024;
   if (!use_lds) {
-    DN_TRY(b->ring_h.reserve((uint64_t)nl * c->w * 8));
-    DN_TRY(b->ring_ps.reserve((uint64_t)nl * c->w * 4));
+    DN_TRY(b->ring_h.reserve((uint64_t)nl * c->w * 8, st));
+    DN_TRY(b->ring_ps.reserve((uint64_t)nl * c->w * 4, st));
   }
   const unsigned sgrid = (nl + kSeedBlock - 1) / kSeedBlock;
   DN_TRY(b->span_begin(0, sp_i));
@@ -984,11 +984,11 @@ int seed_chain_global(dnaln_hip_batch* b, SeedArgs sa, ChainArgs ca, const uint3
   DN_TRY(b->span_end(sp_i));
   uint64_t na = 0;
   DN_TRY(scan_total(st, b->aoff.as<uint64_t>(), b->acnt.as<uint32_t>(), nl, na));
-  DN_TRY(b->keys.reserve(na * 8 + 8));
-  DN_TRY(b->f.reserve(na * 4 + 4));
-  DN_TRY(b->prev.reserve(na * 4 + 4));
-  DN_TRY(b->order.reserve(na * 8 + 8));
-  DN_TRY(b->chains.reserve(na * sizeof(ChainRec) + sizeof(ChainRec)));
+  DN_TRY(b->keys.reserve(na * 8 + 8, st));
+  DN_TRY(b->f.reserve(na * 4 + 4, st));
+  DN_TRY(b->prev.reserve(na * 4 + 4, st));
+  DN_TRY(b->order.reserve(na * 8 + 8, st));
+  DN_TRY(b->chains.reserve(na * sizeof(ChainRec) + sizeof(ChainRec), st));
   sa.keys = b->keys.as<uint64_t>();
   DN_TRY(b->span_begin(0, sp_i));
   if (nl) {
@@ -1681,8 +1681,8 @@ int run_chunk(dnaln_hip_batch* b, uint64_t r0, uint32_t n, bool first_chunk) {
   g.bw = p.sc.bw;
 
   // 1-2. seeds, chains and candidates
-  DN_TRY(b->cands.reserve((uint64_t)n * std::max(max_cand, 1) * sizeof(Cand)));
-  DN_TRY(b->ncand.reserve((uint64_t)n * 4 + 4));
+  DN_TRY(b->cands.reserve((uint64_t)n * std::max(max_cand, 1) * sizeof(Cand), st));
+  DN_TRY(b->ncand.reserve((uint64_t)n * 4 + 4, st));
   SeedArgs sa{};
   sa.codes = rv.codes;  // kReadGuard bytes into the buffer
   sa.offs = rv.offs;
@@ -1710,9 +1710,9 @@ int run_chunk(dnaln_hip_batch* b, uint64_t r0, uint32_t n, bool first_chunk) {
     const uint32_t* lorder = nullptr;
     DN_TRY(b->span_begin(0, sp_i));
     if (used > 1) {
-      DN_TRY(b->lids.reserve((uint64_t)n * 4));
-      DN_TRY(b->lkeys.reserve((uint64_t)n * 4));
-      DN_TRY(b->lorder.reserve((uint64_t)n * 4));
+      DN_TRY(b->lids.reserve((uint64_t)n * 4, st));
+      DN_TRY(b->lkeys.reserve((uint64_t)n * 4, st));
+      DN_TRY(b->lorder.reserve((uint64_t)n * 4, st));
       hipLaunchKernelGGL(k_iota, dim3((n + 255) / 256), dim3(256), 0, st, b->lids.as<uint32_t>(), n);
       DN_HIP(hipGetLastError());
       const int bits = 32 - __builtin_clz((unsigned)std::max(b->maxlen, 1));
@@ -1720,12 +1720,12 @@ int run_chunk(dnaln_hip_batch* b, uint64_t r0, uint32_t n, bool first_chunk) {
       size_t bytes = 0;
       DN_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, lens_u, b->lkeys.as<uint32_t>(), b->lids.as<uint32_t>(),
                                                 b->lorder.as<uint32_t>(), (uint32_t)n, 0, bits, st));
-      DN_TRY(b->ltmp.reserve(bytes));
+      DN_TRY(b->ltmp.reserve(bytes, st));
       DN_HIP(hipcub::DeviceRadixSort::SortPairs(b->ltmp.p, bytes, lens_u, b->lkeys.as<uint32_t>(), b->lids.as<uint32_t>(),
                                                 b->lorder.as<uint32_t>(), (uint32_t)n, 0, bits, st));
       lorder = b->lorder.as<uint32_t>();
     }
-    DN_TRY(b->ovf.reserve((uint64_t)n * 4 + 8));
+    DN_TRY(b->ovf.reserve((uint64_t)n * 4 + 8, st));
     uint32_t* ovf = b->ovf.as<uint32_t>();
     uint32_t* novf_d = ovf + n;
     DN_HIP(hipMemsetAsync(novf_d, 0, 4, st));
@@ -1733,7 +1733,7 @@ int run_chunk(dnaln_hip_batch* b, uint64_t r0, uint32_t n, bool first_chunk) {
     for (int x = 0; x < 2; ++x) {
       if (hi[x] <= lo[x]) continue;
       const FusedShape fs = fused_shape(c->k, c->w, std::min(b->split[x], b->maxlen));
-      DN_TRY(b->fmins.reserve(fused_mins_words(fs, hi[x] - lo[x]) * 8));
+      DN_TRY(b->fmins.reserve(fused_mins_words(fs, hi[x] - lo[x]) * 8, st));
     }
     for (int x = 0; x < 2; ++x) {
       if (hi[x] <= lo[x]) continue;
@@ -1752,11 +1752,11 @@ int run_chunk(dnaln_hip_batch* b, uint64_t r0, uint32_t n, bool first_chunk) {
       if (hi[x] > lo[x]) small = small || fused_shape(c->k, c->w, std::min(b->split[x], b->maxlen)).cap < 64;
     const FusedShape retry = fused_retry_shape(c->k, c->w, b->maxlen);
     if (novf && small && retry.cap) {
-      DN_TRY(b->ovf2.reserve((uint64_t)n * 4 + 8));
+      DN_TRY(b->ovf2.reserve((uint64_t)n * 4 + 8, st));
       uint32_t* ovf2 = b->ovf2.as<uint32_t>();
       uint32_t* novf2_d = ovf2 + n;
       DN_HIP(hipMemsetAsync(novf2_d, 0, 4, st));
-      DN_TRY(b->fmins.reserve(fused_mins_words(retry, novf) * 8));
+      DN_TRY(b->fmins.reserve(fused_mins_words(retry, novf) * 8, st));
       sa.list = ovf;
       sa.n_reads = novf;
       DN_TRY(launch_fused(retry, sa, ca, b->fmins.as<uint64_t>(), ovf2, novf2_d, st));
@@ -1784,14 +1784,14 @@ int run_chunk(dnaln_hip_batch* b, uint64_t r0, uint32_t n, bool first_chunk) {
   }
 
   // 3. rank pass
-  DN_TRY(b->rcnt.reserve((uint64_t)n * 4 + 4));
-  DN_TRY(b->roff.reserve((uint64_t)n * 8 + 8));
+  DN_TRY(b->rcnt.reserve((uint64_t)n * 4 + 4, st));
+  DN_TRY(b->roff.reserve((uint64_t)n * 8 + 8, st));
   if (n) hipLaunchKernelGGL(k_rank_count, dim3(sgrid), dim3(kSeedBlock), 0, st, b->ncand.as<int32_t>(), b->rcnt.as<uint32_t>(), n);
   DN_TRY(exclusive_scan(b->scan_tmp, b->rcnt.as<uint32_t>(), b->roff.as<uint64_t>(), n, st));
   uint64_t nr = 0;
   DN_TRY(scan_total(st, b->roff.as<uint64_t>(), b->rcnt.as<uint32_t>(), n, nr));
-  DN_TRY(b->rtasks.reserve(nr * sizeof(SwTask) + sizeof(SwTask)));
-  DN_TRY(b->rres.reserve(nr * sizeof(SwResult) + sizeof(SwResult)));
+  DN_TRY(b->rtasks.reserve(nr * sizeof(SwTask) + sizeof(SwTask), st));
+  DN_TRY(b->rres.reserve(nr * sizeof(SwResult) + sizeof(SwResult), st));
   if (n)
     hipLaunchKernelGGL(k_rank_tasks, dim3(sgrid), dim3(kSeedBlock), 0, st, rv, g, b->cands.as<Cand>(),
                        b->ncand.as<int32_t>(), max_cand, b->roff.as<uint64_t>(), b->rtasks.as<SwTask>(), n);
@@ -1800,10 +1800,10 @@ int run_chunk(dnaln_hip_batch* b, uint64_t r0, uint32_t n, bool first_chunk) {
   b->stats.rank_tasks += nr;
 
   // 4. select + traceback pass
-  DN_TRY(b->chosen.reserve((uint64_t)n * 4 + 4));
-  DN_TRY(b->second.reserve((uint64_t)n * 4 + 4));
-  DN_TRY(b->tbflag.reserve((uint64_t)n * 4 + 4));
-  DN_TRY(b->tboff.reserve((uint64_t)n * 8 + 8));
+  DN_TRY(b->chosen.reserve((uint64_t)n * 4 + 4, st));
+  DN_TRY(b->second.reserve((uint64_t)n * 4 + 4, st));
+  DN_TRY(b->tbflag.reserve((uint64_t)n * 4 + 4, st));
+  DN_TRY(b->tboff.reserve((uint64_t)n * 8 + 8, st));
   if (n)
     hipLaunchKernelGGL(k_select, dim3(sgrid), dim3(kSeedBlock), 0, st, rv, b->ncand.as<int32_t>(),
                        b->roff.as<uint64_t>(), b->rres.as<SwResult>(), b->chosen.as<int32_t>(),
@@ -1811,9 +1811,9 @@ int run_chunk(dnaln_hip_batch* b, uint64_t r0, uint32_t n, bool first_chunk) {
   DN_TRY(exclusive_scan(b->scan_tmp, b->tbflag.as<uint32_t>(), b->tboff.as<uint64_t>(), n, st));
   uint64_t nt = 0;
   DN_TRY(scan_total(st, b->tboff.as<uint64_t>(), b->tbflag.as<uint32_t>(), n, nt));
-  DN_TRY(b->ttasks.reserve(nt * sizeof(SwTask) + sizeof(SwTask)));
-  DN_TRY(b->tres.reserve(nt * sizeof(SwResult) + sizeof(SwResult)));
-  DN_TRY(b->tcig.reserve(nt * (uint64_t)cap * 4 + 4));
+  DN_TRY(b->ttasks.reserve(nt * sizeof(SwTask) + sizeof(SwTask), st));
+  DN_TRY(b->tres.reserve(nt * sizeof(SwResult) + sizeof(SwResult), st));
+  DN_TRY(b->tcig.reserve(nt * (uint64_t)cap * 4 + 4, st));
   if (n)
     hipLaunchKernelGGL(k_tb_tasks, dim3(sgrid), dim3(kSeedBlock), 0, st, rv, g, b->cands.as<Cand>(), max_cand,
                        b->chosen.as<int32_t>(), b->tboff.as<uint64_t>(), b->ttasks.as<SwTask>(), n);
@@ -1822,8 +1822,8 @@ int run_chunk(dnaln_hip_batch* b, uint64_t r0, uint32_t n, bool first_chunk) {
   b->stats.tb_tasks += nt;
 
   // 5. records and CIGARs
-  DN_TRY(b->ciglen.reserve((uint64_t)n * 4 + 4));
-  DN_TRY(b->cigoff.reserve((uint64_t)n * 8 + 8));
+  DN_TRY(b->ciglen.reserve((uint64_t)n * 4 + 4, st));
+  DN_TRY(b->cigoff.reserve((uint64_t)n * 8 + 8, st));
   dnaln_record* rec = b->rec.as<dnaln_record>() + r0;
   if (n)
     hipLaunchKernelGGL(k_records, dim3(sgrid), dim3(kSeedBlock), 0, st, rv, g, b->cands.as<Cand>(),

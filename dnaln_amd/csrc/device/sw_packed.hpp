@@ -412,7 +412,10 @@ DN_HD void walk_finish(const Walk& w, int i_end, int d_end, int score, int cap, 
 // Score-only: writes the scores. With flags (TB): stores the flag words into the wave's slot
 // and writes the score and the best cell (row in qe, diagonal in te) for the walk.
 // Traceback fills wider than this keep the saved best row in LDS (bh_lds) instead of VGPRs.
-constexpr int kBestRowInRegs = 33;
+#if !defined(DN_BEST_ROW_IN_REGS)
+#define DN_BEST_ROW_IN_REGS 33
+#endif
+constexpr int kBestRowInRegs = DN_BEST_ROW_IN_REGS;
 
 // Best-row words per fill wave when the row lives in LDS (one u32 per band cell and lane).
 DN_HD int best_row_lds_words(int W) { return W > kBestRowInRegs ? W * 64 : 0; }

@@ -328,7 +328,7 @@ extern "C" {
 
 void dnaln_serve_default_config(dnaln_serve_config* cfg) {
   std::memset(cfg, 0, sizeof *cfg);
-  cfg->slots = 4;
+  cfg->slots = 6;
   cfg->mode = DNALN_MODE_BATCHED;
   cfg->hbm_budget = 0;
   cfg->max_batch_reads = 1 << 17;

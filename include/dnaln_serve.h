@@ -31,7 +31,7 @@ extern "C" {
 #endif
 
 typedef struct {
-  int32_t slots;              /* batches in flight at most */
+  int32_t slots;              /* batches in flight at most (default 6) */
   int32_t mode;               /* DNALN_MODE_BATCHED or DNALN_MODE_PLAIN for every slot */
   uint64_t hbm_budget;        /* device bytes the batches in flight may use (0: free HBM / 2) */
   uint64_t max_batch_reads;   /* reads packed into one batch at most */
