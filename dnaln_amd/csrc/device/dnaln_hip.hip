@@ -255,7 +255,7 @@ __global__ __launch_bounds__(256) void k_chain_wave(ChainArgs a) {
 // interleaved by thread. The minimizer list goes to the block's part of `mins` (maxmin u64 per
 // thread, interleaved: written by the sketch, read back by the probes). The fallback sketch's
 // ring (w u64 hashes + w u32 positions) uses the anchor slots before any anchor is written.
-template <int WT>
+template <int WT, bool P8>
 __global__ __launch_bounds__(kFusedBlock) void k_seed_chain_fused(SeedArgs s, ChainArgs c, int cap, int maxmin,
                                                                   uint64_t* mins, uint32_t* ovf, uint32_t* novf) {
   DN_DYN_LDS(uint64_t, lds);
@@ -272,7 +272,7 @@ __global__ __launch_bounds__(kFusedBlock) void k_seed_chain_fused(SeedArgs s, Ch
   v.stride = kFusedBlock;
   const uint32_t r = read_id(s.list, i);
   uint64_t* my_mins = mins + (size_t)blockIdx.x * maxmin * kFusedBlock + t;
-  const int nc = seed_chain_lds<8, 10>(s, c, r, v, my_mins, kFusedBlock, maxmin, ring, cap);
+  const int nc = seed_chain_lds<8, 10, P8>(s, c, r, v, my_mins, kFusedBlock, maxmin, ring, cap);
   if (nc < 0) {
     ovf[atomicAdd(novf, 1u)] = r;
     c.n_cand[r] = 0;
@@ -468,6 +468,28 @@ __global__ void k_inline_unique(Bucket* table, uint64_t n, const uint32_t* pos) 
   if (b.key != ~0ull && b.count == 1) table[i].offset = pos[b.offset];
 }
 
+// The 8-byte probe table (probe_table.hpp) from the inlined 16-byte table: words of pos2 per
+// bucket (entry n is 0, so the exclusive scan ends with the total), then the entries, the
+// mirrored tail and the blocks of pos2.
+__global__ void k_ct_need(const Bucket* table, uint64_t n, uint32_t max_occ, uint32_t* need) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > n) return;
+  need[i] = i < n ? c_need(table[i], max_occ) : 0u;
+}
+
+__global__ void k_ct_fill(const Bucket* table, uint64_t n, uint32_t max_occ, const uint32_t* pos, const uint64_t* at,
+                          uint64_t* ctab, uint32_t* pos2) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n + kCTail) return;
+  const uint64_t slot = i < n ? i : (i - n) & (n - 1);
+  const Bucket b = table[slot];
+  ctab[i] = c_entry(b, max_occ, b.offset, (uint32_t)at[slot]);  // count 1: offset is the occurrence
+  if (i >= n || !c_need(b, max_occ)) return;
+  uint32_t* out = pos2 + at[i];
+  out[0] = b.count;
+  for (uint32_t j = 0; j < b.count; ++j) out[1 + j] = pos[b.offset + j];
+}
+
 // ---- index build on the device (src/index.cpp:10-45): segments sketched one per thread, the
 // occurrences radix-sorted by (hash, pos), each key placed by one thread with atomic linear
 // probing. Probe chains only ever grow, so every key stays reachable from its home slot.
@@ -625,6 +647,8 @@ struct dnaln_hip_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   DevBuf ref, coff, clen, table, pos, cost, lut;
+  DevBuf ctab, pos2;            // the 8-byte probe table and its occurrence blocks (probe_table.hpp)
+  bool probe8 = false;          // the seed probes use ctab / pos2
   uint8_t* ref_base = nullptr;  // reference codes, kSeqGuard bytes into `ref`
   std::vector<uint64_t> h_coff;
   std::vector<int32_t> h_cost;  // chain gap cost by drift, 0..band
@@ -1068,8 +1092,12 @@ uint64_t fused_mins_words(const FusedShape& fs, uint64_t n) {
 int launch_fused(const FusedShape& fs, const SeedArgs& sa, const ChainArgs& ca, uint64_t* mins, uint32_t* ovf,
                  uint32_t* novf, hipStream_t st) {
   const unsigned grid = (sa.n_reads + kFusedBlock - 1) / kFusedBlock;
-  hipLaunchKernelGGL((k_seed_chain_fused<10>), dim3(grid), dim3(kFusedBlock), fs.lds, st, sa, ca, fs.cap, fs.maxmin,
-                     mins, ovf, novf);
+  if (sa.ctab)
+    hipLaunchKernelGGL((k_seed_chain_fused<10, true>), dim3(grid), dim3(kFusedBlock), fs.lds, st, sa, ca, fs.cap,
+                       fs.maxmin, mins, ovf, novf);
+  else
+    hipLaunchKernelGGL((k_seed_chain_fused<10, false>), dim3(grid), dim3(kFusedBlock), fs.lds, st, sa, ca, fs.cap,
+                       fs.maxmin, mins, ovf, novf);
   DN_HIP(hipGetLastError());
   return DNALN_OK;
 }
@@ -1340,7 +1368,8 @@ int enqueue_batched(dnaln_hip_batch* b, const GraphPlan& pl, bool spans) {
   sa.lens = rv.lens;
   sa.table = c->table.as<Bucket>();
   sa.mask = c->mask;
-  sa.pos = c->pos.as<uint32_t>();
+  sa.pos = c->probe8 ? c->pos2.as<uint32_t>() : c->pos.as<uint32_t>();
+  sa.ctab = c->probe8 ? c->ctab.as<uint64_t>() : nullptr;
   sa.pos_inline = 1;
   sa.k = c->k;
   sa.w = c->w;
@@ -1689,7 +1718,8 @@ int run_chunk(dnaln_hip_batch* b, uint64_t r0, uint32_t n, bool first_chunk) {
   sa.lens = rv.lens;
   sa.table = c->table.as<Bucket>();
   sa.mask = c->mask;
-  sa.pos = c->pos.as<uint32_t>();
+  sa.pos = c->probe8 ? c->pos2.as<uint32_t>() : c->pos.as<uint32_t>();
+  sa.ctab = c->probe8 ? c->ctab.as<uint64_t>() : nullptr;
   sa.pos_inline = 1;
   sa.k = c->k;
   sa.w = c->w;
@@ -1941,6 +1971,33 @@ int build_index_dev(const uint8_t* ref, const uint64_t* coff, const uint32_t* cl
   return DNALN_OK;
 }
 
+// The 8-byte probe table and pos2 (probe_table.hpp) from the context's inlined 16-byte table,
+// where hashes fit 30 bits (k <= 15); *built = false leaves the probes on the 16-byte table.
+int build_probe_table(const DevBuf& table, uint64_t n, const DevBuf& pos, uint32_t max_occ, int k, hipStream_t st,
+                      DevBuf& ctab, DevBuf& pos2, bool* built) {
+  *built = false;
+  if (2 * k > 30 || n == 0 || n + 1 > 0xffffffffull) return DNALN_OK;
+  DevBuf need, at, tmp;
+  DN_TRY(need.reserve((n + 1) * 4));
+  DN_TRY(at.reserve((n + 1) * 8));
+  const unsigned blocks = (unsigned)((n + 1 + 255) / 256);
+  hipLaunchKernelGGL(k_ct_need, dim3(blocks), dim3(256), 0, st, table.as<Bucket>(), n, max_occ, need.as<uint32_t>());
+  DN_HIP(hipGetLastError());
+  DN_TRY(exclusive_scan(tmp, need.as<uint32_t>(), at.as<uint64_t>(), (uint32_t)(n + 1), st));
+  uint64_t total = 0;
+  DN_HIP(hipMemcpyAsync(&total, at.as<uint64_t>() + n, 8, hipMemcpyDeviceToHost, st));
+  DN_HIP(hipStreamSynchronize(st));
+  if (total >= 0xffffffffull) return DNALN_OK;
+  DN_TRY(ctab.reserve((n + kCTail) * 8));
+  DN_TRY(pos2.reserve(total * 4 + 4));
+  hipLaunchKernelGGL(k_ct_fill, dim3((unsigned)((n + kCTail + 255) / 256)), dim3(256), 0, st, table.as<Bucket>(), n,
+                     max_occ, pos.as<uint32_t>(), at.as<uint64_t>(), ctab.as<uint64_t>(), pos2.as<uint32_t>());
+  DN_HIP(hipGetLastError());
+  DN_HIP(hipStreamSynchronize(st));  // the temporaries are freed on return
+  *built = true;
+  return DNALN_OK;
+}
+
 // Reference codes in device memory between two guard regions of N (kSeqGuard bytes each): the
 // chunked readers may touch bytes just outside a contig.
 int upload_reference(DevBuf& buf, const uint8_t* codes, uint64_t len, hipStream_t st, uint8_t** base) {
@@ -2165,6 +2222,8 @@ int dnaln_hip_create(const dnaln_map_params* params, const uint8_t* ref_codes, u
     hipLaunchKernelGGL(k_inline_unique, dim3((unsigned)((c->ix.n_buckets + 255) / 256)), dim3(256), 0, c->stream,
                        c->table.as<Bucket>(), c->ix.n_buckets, c->pos.as<uint32_t>());
     DN_HIP(hipGetLastError());
+    DN_TRY(build_probe_table(c->table, c->ix.n_buckets, c->pos, (uint32_t)c->p.max_occ, c->k, c->stream, c->ctab,
+                             c->pos2, &c->probe8));
     DN_HIP(hipStreamSynchronize(c->stream));
     // Chain gap cost by drift dd (src/chain.cpp:36), evaluated in double exactly as the reference.
     std::vector<int32_t> cost(c->p.ch.band + 1, 0);

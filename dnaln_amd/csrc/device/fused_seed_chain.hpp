@@ -274,8 +274,9 @@ DN_HD int chain_extract_lds(const ChainArgs& a, const LdsView& v, int n, uint32_
 // apart: the thread's column of a global scratch array); 2. probes, eight minimizers at a time,
 // anchors into LDS; 3. sort, DP, extraction, candidates. `ring` (the fallback sketch for reads
 // with N or a window length other than WT) may alias the anchor words, which are free until
-// phase 2. Returns the candidate count or -1 (the read does not fit; global path).
-template <int MAXCH, int WT>
+// phase 2. P8: probes through the 8-byte table s.ctab (probe_table.hpp), else the 16-byte
+// buckets. Returns the candidate count or -1 (the read does not fit; global path).
+template <int MAXCH, int WT, bool P8 = true>
 DN_HD int seed_chain_lds(const SeedArgs& s, const ChainArgs& a, uint32_t r, LdsView v, uint64_t* mins, int mstride,
                          int maxmin, Ring ring, int cap) {
   const int len = s.lens[r], k = s.k;
@@ -315,26 +316,63 @@ DN_HD int seed_chain_lds(const SeedArgs& s, const ChainArgs& a, uint32_t r, LdsV
   bool overflow = false;
   for (int m0 = 0; m0 < nmin && !overflow; m0 += PG) {
     uint64_t e[PG];
-    Bucket b[PG];
-#pragma unroll
-    for (int u = 0; u < PG; ++u) {
-      e[u] = m0 + u < nmin ? mins[(m0 + u) * mstride] : 0;
-      if (m0 + u < nmin) b[u] = s.table[(e[u] >> 17) & s.mask];
-    }
     uint32_t off[PG], cnt[PG], first[PG];
+    if (P8) {
+      // The 8-byte table (s.ctab set): the four slots from each home slot in one round of loads, resolved in
+      // registers; then the count and first occurrence of every key with several, in a second
+      // round. Chains longer than four slots continue one slot at a time (rare).
+      uint64_t q[PG][4];
 #pragma unroll
-    for (int u = 0; u < PG; ++u) {
-      off[u] = cnt[u] = first[u] = 0;
-      if (m0 + u >= nmin) continue;
-      const uint64_t h = e[u] >> 17;
-      if (b[u].key == h) {
-        off[u] = b[u].offset;
-        cnt[u] = b[u].count;
-      } else if (b[u].key != ~0ULL) {
-        probe_from(s.table, s.mask, h, ((h & s.mask) + 1) & s.mask, off[u], cnt[u]);
+      for (int u = 0; u < PG; ++u) {
+        e[u] = m0 + u < nmin ? mins[(m0 + u) * mstride] : 0;
+        if (m0 + u < nmin) c_load4(s.ctab, (e[u] >> 17) & s.mask, q[u]);
       }
-      if (cnt[u] > s.max_occ) cnt[u] = 0;
-      if (cnt[u]) first[u] = first_pos(s, off[u], cnt[u]);
+      bool many[PG];
+#pragma unroll
+      for (int u = 0; u < PG; ++u) {
+        off[u] = cnt[u] = first[u] = 0;
+        many[u] = false;
+        if (m0 + u >= nmin) continue;
+        const uint32_t h = (uint32_t)(e[u] >> 17);
+        uint64_t ent = 0;
+        int f = c_find4(q[u], h, ent);
+        if (f < 0) f = c_probe_from(s.ctab, s.mask, h, ((e[u] >> 17) + 4) & s.mask, ent);
+        const uint32_t tag = c_key(ent) & 3u;
+        if (f > 0 && tag == kCOne) {
+          cnt[u] = 1;
+          first[u] = c_val(ent);
+        } else if (f > 0 && tag == kCMany) {
+          off[u] = c_val(ent) + 1;
+          many[u] = true;
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < PG; ++u)
+        if (many[u]) {
+          cnt[u] = s.pos[off[u] - 1];
+          first[u] = s.pos[off[u]];
+        }
+    } else {
+      Bucket b[PG];
+#pragma unroll
+      for (int u = 0; u < PG; ++u) {
+        e[u] = m0 + u < nmin ? mins[(m0 + u) * mstride] : 0;
+        if (m0 + u < nmin) b[u] = s.table[(e[u] >> 17) & s.mask];
+      }
+#pragma unroll
+      for (int u = 0; u < PG; ++u) {
+        off[u] = cnt[u] = first[u] = 0;
+        if (m0 + u >= nmin) continue;
+        const uint64_t h = e[u] >> 17;
+        if (b[u].key == h) {
+          off[u] = b[u].offset;
+          cnt[u] = b[u].count;
+        } else if (b[u].key != ~0ULL) {
+          probe_from(s.table, s.mask, h, ((h & s.mask) + 1) & s.mask, off[u], cnt[u]);
+        }
+        if (cnt[u] > s.max_occ) cnt[u] = 0;
+        if (cnt[u]) first[u] = first_pos(s, off[u], cnt[u]);
+      }
     }
 #pragma unroll
     for (int u = 0; u < PG; ++u) {

@@ -16,14 +16,10 @@
 #pragma once
 
 #include "common.hpp"
+#include "probe_table.hpp"
 #include "sketch_vh.hpp"
 
 namespace dn {
-
-struct Bucket {
-  uint64_t key;
-  uint32_t offset, count;
-};
 
 struct SeedArgs {
   const uint8_t* codes;     // reads, forward codes 0..4
@@ -33,7 +29,8 @@ struct SeedArgs {
   const uint32_t* list;     // read ids to process (null: 0 .. n_reads-1)
   const Bucket* table;
   uint64_t mask;
-  const uint32_t* pos;
+  const uint32_t* pos;      // occurrences; pos2 of probe_table.hpp when ctab is set
+  const uint64_t* ctab;     // the 8-byte probe table (probe_table.hpp), or null
   int pos_inline;           // a bucket with count 1 holds its occurrence itself in `offset`
   int k, w;
   uint32_t max_occ;
@@ -194,7 +191,9 @@ DN_HD void probe_from(const Bucket* table, uint64_t mask, uint64_t h, uint64_t i
 // loaded while minimizer m-1's bucket is resolved (and its first occurrence loaded) and m-2's
 // occurrences are consumed, so one read keeps several independent loads in flight. Calls
 // on(mpos, mstrand, off, cnt, first) in minimizer order for every minimizer with
-// 1 <= cnt <= max_occ (src/mapper.cpp:24-30); first = pos[off] when NeedPos.
+// 1 <= cnt <= max_occ (src/mapper.cpp:24-30); first = pos[off] when NeedPos. With the 8-byte
+// table (s.ctab) a probe loads the four slots from the home slot at once, and for a key with
+// several occurrences `off` indexes pos2 (s.pos), whose word before it holds the count.
 template <bool NeedPos, class OnHit>
 struct ProbePipe {
   const SeedArgs& s;
@@ -203,6 +202,7 @@ struct ProbePipe {
   uint64_t h1 = 0;
   uint32_t mp1 = 0, ms1 = 0;
   Bucket b1{};
+  uint64_t q1[4] = {0, 0, 0, 0};
   uint32_t mp2 = 0, ms2 = 0, off2 = 0, cnt2 = 0, first2 = 0;
   DN_HD ProbePipe(const SeedArgs& a, OnHit& f) : s(a), on(f) {}
   DN_HD void advance() {
@@ -211,31 +211,55 @@ struct ProbePipe {
       v2 = false;
     }
     if (v1) {
-      uint32_t off = 0, cnt = 0;
-      if (b1.key == h1) {
-        off = b1.offset;
-        cnt = b1.count;
-      } else if (b1.key != ~0ULL) {
-        probe_from(s.table, s.mask, h1, ((h1 & s.mask) + 1) & s.mask, off, cnt);
+      uint32_t off = 0, cnt = 0, first = 0;
+      bool hit;
+      if (s.ctab) {
+        uint64_t ent = 0;
+        int f = c_find4(q1, (uint32_t)h1, ent);
+        if (f < 0) f = c_probe_from(s.ctab, s.mask, (uint32_t)h1, ((h1 & s.mask) + 4) & s.mask, ent);
+        const uint32_t tag = c_key(ent) & 3u;
+        hit = f > 0 && tag != kCSkip;
+        if (hit && tag == kCOne) {
+          cnt = 1;
+          first = c_val(ent);
+        } else if (hit) {
+          off = c_val(ent) + 1;
+          cnt = s.pos[off - 1];
+          if (NeedPos) first = s.pos[off];
+        }
+      } else {
+        if (b1.key == h1) {
+          off = b1.offset;
+          cnt = b1.count;
+        } else if (b1.key != ~0ULL) {
+          probe_from(s.table, s.mask, h1, ((h1 & s.mask) + 1) & s.mask, off, cnt);
+        }
+        hit = cnt != 0 && cnt <= s.max_occ;
+        if (hit && NeedPos) first = first_pos(s, off, cnt);
       }
-      if (cnt != 0 && cnt <= s.max_occ) {
+      if (hit) {
         mp2 = mp1;
         ms2 = ms1;
         off2 = off;
         cnt2 = cnt;
-        first2 = NeedPos ? first_pos(s, off, cnt) : 0u;
+        first2 = first;
         v2 = true;
       }
       v1 = false;
     }
   }
   DN_HD void push(uint64_t h, uint32_t mpos, uint32_t mstrand) {
-    const Bucket bn = s.table[h & s.mask];
+    Bucket bn{};
+    uint64_t qn[4] = {0, 0, 0, 0};
+    if (s.ctab) c_load4(s.ctab, h & s.mask, qn);
+    else bn = s.table[h & s.mask];
     advance();
     h1 = h;
     mp1 = mpos;
     ms1 = mstrand;
     b1 = bn;
+#pragma unroll
+    for (int x = 0; x < 4; ++x) q1[x] = qn[x];
     v1 = true;
   }
   DN_HD void drain() {

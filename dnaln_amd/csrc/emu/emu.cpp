@@ -130,6 +130,29 @@ bool dispatch(int bw, const SwArgs& a) {
   }
 }
 
+// The 8-byte probe table and pos2 of probe_table.hpp from an inlined 16-byte table, built as
+// k_ct_need / k_ct_fill build them on the device (slot by slot, blocks in slot order).
+void build_probe_table(const std::vector<Bucket>& tab, const uint32_t* pos, uint32_t max_occ,
+                       std::vector<uint64_t>& ctab, std::vector<uint32_t>& pos2) {
+  const uint64_t n = tab.size();
+  std::vector<uint64_t> at(n);
+  uint64_t total = 0;
+  for (uint64_t i = 0; i < n; ++i) {
+    at[i] = total;
+    total += c_need(tab[i], max_occ);
+  }
+  pos2.assign(total + 1, 0);
+  ctab.assign(n + kCTail, 0);
+  for (uint64_t i = 0; i < n + kCTail; ++i) {
+    const uint64_t slot = i < n ? i : (i - n) & (n - 1);
+    const Bucket& b = tab[slot];
+    ctab[i] = c_entry(b, max_occ, b.offset, (uint32_t)at[slot]);
+    if (i >= n || !c_need(b, max_occ)) continue;
+    pos2[at[i]] = b.count;
+    for (uint32_t j = 0; j < b.count; ++j) pos2[at[i] + 1 + j] = pos[b.offset + j];
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -231,7 +254,8 @@ int emu_align_ex(const int* sp6, const uint8_t* pool, const int64_t* q_off, cons
 }
 
 // Runs both seed+chain programs on n reads (codes 0..4) and compares their candidates.
-// mp = k, w, max_occ, bw, max_cand, min_chain, lookback, max_gap, band, min_anchors.
+// mp = k, w, max_occ, bw, max_cand, min_chain, lookback, max_gap, band, min_anchors, table form
+// (0: the 16-byte buckets, 1: the 8-byte probe table).
 // Returns the number of reads whose candidates differ (-1 on bad input); stats[0] = reads the
 // LDS program sent to the global path (overflow), stats[1] = total anchors, stats[2] = max.
 // out_n / out_c (may be null): the global-memory program's candidates, n_cand per read and
@@ -260,6 +284,15 @@ long emu_seed_chain(const uint8_t* codes, const uint64_t* offs, const int32_t* l
   sa.k = mp[0];
   sa.w = mp[1];
   sa.max_occ = (uint32_t)mp[2];
+  // mp[10] = 1: probes through the 8-byte table (probe_table.hpp), as the device does for k <= 15.
+  std::vector<uint64_t> ctab;
+  std::vector<uint32_t> pos2;
+  if (mp[10]) {
+    if (2 * mp[0] > 30) return -1;
+    build_probe_table(tab, pos, sa.max_occ, ctab, pos2);
+    sa.ctab = ctab.data();
+    sa.pos = pos2.data();
+  }
   ChainArgs ca{};
   ChainSetup cs;
   cs.k = mp[0];
@@ -331,7 +364,8 @@ long emu_seed_chain(const uint8_t* codes, const uint64_t* offs, const int32_t* l
     ChainArgs l = ca;
     l.cands = c2.data();
     LdsView v{keys.data(), 1};
-    const int nc = seed_chain_lds<8, 10>(sa, l, (uint32_t)r, v, mins.data(), 1, maxmin, ring, cap);
+    const int nc = sa.ctab ? seed_chain_lds<8, 10, true>(sa, l, (uint32_t)r, v, mins.data(), 1, maxmin, ring, cap)
+                           : seed_chain_lds<8, 10, false>(sa, l, (uint32_t)r, v, mins.data(), 1, maxmin, ring, cap);
     if (nc < 0) {
       ++stats[0];
       continue;
@@ -374,6 +408,51 @@ long emu_index_occ(const uint8_t* ref, const int64_t* coff, const int64_t* clen,
     }
   }
   return n <= cap ? (long)n : -(long)n;
+}
+
+// Looks every query hash up through the 16-byte table (the reference's probe, src/index.hpp:24-38)
+// and through the 8-byte table built from it, the way the device's probes read each: the
+// four-slot load with its serial continuation (c_find4, c_probe_from) and the count and
+// occurrences from pos2. table: n buckets (n a power of two, count 1 not inlined). Returns the
+// number of queries whose occurrence lists differ (after the max_occ filter of
+// src/mapper.cpp:27-28); stats[0] = queries found, stats[1] = queries that needed more than
+// four slots.
+long emu_probe_check(const void* table, uint64_t n, const uint32_t* pos, uint32_t max_occ, const uint64_t* q,
+                     uint64_t nq, uint64_t* stats) {
+  std::vector<Bucket> tab(static_cast<const Bucket*>(table), static_cast<const Bucket*>(table) + n);
+  for (Bucket& b : tab)
+    if (b.key != ~0ull && b.count == 1) b.offset = pos[b.offset];
+  std::vector<uint64_t> ctab;
+  std::vector<uint32_t> pos2;
+  build_probe_table(tab, pos, max_occ, ctab, pos2);
+  const uint64_t mask = n - 1;
+  long bad = 0;
+  stats[0] = stats[1] = 0;
+  const Bucket* raw = static_cast<const Bucket*>(table);
+  for (uint64_t x = 0; x < nq; ++x) {
+    const uint64_t h = q[x];
+    std::vector<uint32_t> want, got;
+    uint32_t off = 0, cnt = 0;
+    probe_from(raw, mask, h, h & mask, off, cnt);
+    if (cnt && cnt <= max_occ) want.assign(pos + off, pos + off + cnt);
+    uint64_t e[4];
+    c_load4(ctab.data(), h & mask, e);
+    uint64_t ent = 0;
+    int f = c_find4(e, (uint32_t)h, ent);
+    if (f < 0) {
+      ++stats[1];
+      f = c_probe_from(ctab.data(), mask, (uint32_t)h, ((h & mask) + 4) & mask, ent);
+    }
+    const uint32_t tag = c_key(ent) & 3u;
+    if (f > 0 && tag == kCOne) got.push_back(c_val(ent));
+    if (f > 0 && tag == kCMany) {
+      const uint32_t at = c_val(ent);
+      got.assign(pos2.data() + at + 1, pos2.data() + at + 1 + pos2[at]);
+    }
+    stats[0] += cnt != 0;
+    bad += want != got;
+  }
+  return bad;
 }
 
 }  // extern "C"

@@ -113,7 +113,7 @@ def _gap_costs(k, band):
     return np.array([0] + [int(0.01 * k * dd + 0.5 * (dd.bit_length() - 1)) for dd in range(1, band + 1)], np.int32)
 
 
-def _emu_seed_chain(emu, spec, g, r, cap, maxmin, band=None, cost=None, want_cands=False):
+def _emu_seed_chain(emu, spec, g, r, cap, maxmin, band=None, cost=None, want_cands=False, table=1):
     from dnaln_amd import api
     mp = sets.map_params(spec)
     k, w, bw = mp["k"], mp["w"], mp["bw"]
@@ -124,7 +124,7 @@ def _emu_seed_chain(emu, spec, g, r, cap, maxmin, band=None, cost=None, want_can
         band = min(16, bw)  # ChainParams::band, clamped to the SW band (src/mapper.cpp:9)
     if cost is None:
         cost = _gap_costs(k, band)
-    m = np.array([k, w, mp["max_occ"], bw, mp["max_cand"], mp["min_chain"], 50, 5000, band, 2], np.int32)
+    m = np.array([k, w, mp["max_occ"], bw, mp["max_cand"], mp["min_chain"], 50, 5000, band, 2, table], np.int32)
     st = np.zeros(3, np.uint64)
     mc = max(mp["max_cand"], 1)
     out_n = np.zeros(max(len(r), 1), np.int32)
@@ -152,7 +152,8 @@ def _candidate_mismatches(n_a, c_a, n_b, c_b):
 
 @pytest.mark.parametrize("name", ["edge_uniform_bw64", "edge_uniform_bw16", "edge_repeats_bwa_bw8",
                                   "edge_repeats_params"])
-def test_emulated_candidates_and_chain_scores_equal_the_oracle(emu, name):
+@pytest.mark.parametrize("table", [0, 1])
+def test_emulated_candidates_and_chain_scores_equal_the_oracle(emu, name, table):
     """Every read's candidates — strand, contig, diagonal and the score of the chain each came
     from — as the device's seed and chain programs compute them with the parameter set-up the
     device path uses (init_chain_args), equal the oracle's (src/mapper.cpp:14-63). The reads
@@ -165,24 +166,25 @@ def test_emulated_candidates_and_chain_scores_equal_the_oracle(emu, name):
     oidx = oracle.Index(g.codes, g.contig_off, g.contig_len, mp["k"], mp["w"])
     on, oc = oracle.candidates(oracle.default_params(**mp), g, oidx, r)
     assert (on > 0).sum() > len(r) // 2
-    _, _, en, ec = _emu_seed_chain(emu, spec, g, r, 64, 56, want_cands=True)
+    _, _, en, ec = _emu_seed_chain(emu, spec, g, r, 64, 56, want_cands=True, table=table)
     assert _candidate_mismatches(en, ec, on, oc) == []
     band = min(16, mp["bw"])
-    _, _, zn, zc = _emu_seed_chain(emu, spec, g, r, 64, 56, cost=np.zeros(band + 1, np.int32), want_cands=True)
+    _, _, zn, zc = _emu_seed_chain(emu, spec, g, r, 64, 56, cost=np.zeros(band + 1, np.int32), want_cands=True,
+                                   table=table)
     assert len(_candidate_mismatches(zn, zc, on, oc)) > 0
 
 
 @pytest.mark.parametrize("name", ["uniform4m_150bp_10k", "edge_twins_bw16", "edge_repeats_params",
                                   "edge_twins_sr_bw5", "edge_uniform_linear_bw21"])
-@pytest.mark.parametrize("cap,maxmin", [(32, 32), (64, 56), (32, 24)])
-def test_emulated_fused_seed_chain_matches_global_path(emu, name, cap, maxmin):
+@pytest.mark.parametrize("cap,maxmin,table", [(32, 32, 1), (64, 56, 1), (32, 24, 1), (64, 56, 0)])
+def test_emulated_fused_seed_chain_matches_global_path(emu, name, cap, maxmin, table):
     """The fused one-read-per-thread program (register van Herk sketch, LDS minimizer list,
     batched probes, LDS chain DP and extraction) gives every read the candidates of the
     global-memory program, or declines the read (it then takes the global path)."""
     spec = sets.SETS[name]
     g = sets.genome(spec)
     r = sets.reads(spec, g)
-    differ, declined = _emu_seed_chain(emu, spec, g, r, cap, maxmin)
+    differ, declined = _emu_seed_chain(emu, spec, g, r, cap, maxmin, table=table)
     assert differ == 0
     assert declined < len(r)
 
@@ -196,3 +198,76 @@ def test_emulated_fused_seed_chain_with_a_chain_band_above_16(emu):
     differ, declined = _emu_seed_chain(emu, spec, g, r, 64, 56, band=21)
     assert differ == 0
     assert declined < len(r)
+
+
+# ------------------------------------------------------------------ the 8-byte probe table
+_BUCKET = np.dtype([("key", "<u8"), ("offset", "<u4"), ("count", "<u4")])
+
+
+def _linear_probe_table(n, keys, counts, rng):
+    """A table of n buckets (power of two) holding keys in insertion order with linear probing
+    from key & (n - 1) (src/index.cpp:32-44, src/index.hpp:24-38), and its occurrence array."""
+    tab = np.zeros(n, _BUCKET)
+    tab["key"] = np.iinfo(np.uint64).max
+    pos = []
+    for key, c in zip(keys, counts):
+        i = int(key) & (n - 1)
+        while tab["key"][i] != np.iinfo(np.uint64).max:
+            i = (i + 1) & (n - 1)
+        tab[i] = (key, len(pos), c)
+        pos.extend(sorted(int(x) for x in rng.integers(0, 2 ** 32, c, dtype=np.uint64)))
+    return tab, np.array(pos + [0], np.uint32)
+
+
+def _probe_check(emu, tab, pos, max_occ, queries):
+    st = np.zeros(2, np.uint64)
+    q = np.ascontiguousarray(queries, np.uint64)
+    emu.emu_probe_check.restype = C.c_long
+    bad = emu.emu_probe_check(C.c_void_p(tab.ctypes.data), C.c_uint64(len(tab)), C.c_void_p(pos.ctypes.data),
+                              C.c_uint32(max_occ), C.c_void_p(q.ctypes.data), C.c_uint64(len(q)),
+                              C.c_void_p(st.ctypes.data))
+    return bad, int(st[0]), int(st[1])
+
+
+@pytest.mark.parametrize("n,fill,max_occ", [(1, 0, 500), (2, 1, 500), (4, 2, 500), (4, 3, 1), (64, 40, 500),
+                                            (64, 63, 2), (1024, 512, 500), (1024, 900, 3)])
+def test_probe_table_lookups_equal_the_bucket_table(emu, n, fill, max_occ):
+    """The 8-byte probe table (probe_table.hpp) returns, for every key and for absent hashes, the
+    occurrences the 16-byte table's linear probe returns after the max_occ filter. Keys cluster
+    at the top of the table so probe chains wrap past the last slot, chains run longer than the
+    four slots a probe loads at once, and counts sit at 1, 2, max_occ and max_occ + 1."""
+    rng = np.random.default_rng(n * 1000 + fill)
+    # 30-bit hashes (k = 15), most of them homed in the top eighth of the table
+    hashes = rng.choice(2 ** 30, size=4 * fill + 8, replace=False).astype(np.uint64)
+    top = (hashes & ~np.uint64(n - 1)) | (np.uint64(n - 1) - (hashes % np.uint64(max(n // 8, 1))))
+    keys = list(dict.fromkeys(int(x) for x in top))[:fill]
+    counts = [int(rng.choice([1, 2, 3, max_occ, max_occ + 1, 7])) for _ in keys]
+    tab, pos = _linear_probe_table(n, keys, counts, rng)
+    absent = [int(x) for x in rng.integers(0, 2 ** 30, 200, dtype=np.uint64) if int(x) not in set(keys)]
+    near = [(k ^ (1 << 29)) for k in keys]  # same home slot, different key
+    queries = keys + absent + near
+    bad, found, long_chains = _probe_check(emu, tab, pos, max_occ, queries)
+    assert bad == 0
+    assert found == len(keys)
+    if fill >= 40:
+        assert long_chains > 0
+
+
+def test_probe_table_lookups_on_a_genome_index(emu):
+    """Every minimizer of a genome index and as many absent hashes: the same occurrences through
+    both tables (the index's own sorted-key insertion order and load)."""
+    from dnaln_amd import api
+    spec = sets.SETS["edge_repeats_params"]
+    g = sets.genome(spec)
+    mp = sets.map_params(spec)
+    idx = api.Index(g, mp["k"], mp["w"])
+    tab, pos = idx.table(), idx.positions()
+    tab = np.frombuffer(tab.tobytes(), _BUCKET)
+    keys = tab["key"][tab["key"] != np.iinfo(np.uint64).max]
+    rng = np.random.default_rng(3)
+    absent = rng.integers(0, 2 ** (2 * mp["k"]), len(keys), dtype=np.uint64)
+    for max_occ in (mp["max_occ"], 2, 1):
+        bad, found, _ = _probe_check(emu, tab, np.ascontiguousarray(pos, np.uint32), max_occ,
+                                     np.concatenate([keys, absent]))
+        assert bad == 0
+        assert found >= len(keys)
