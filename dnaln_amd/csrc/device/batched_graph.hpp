@@ -297,8 +297,10 @@ __global__ __launch_bounds__(kFusedBlock) void k_fused_g(SeedArgs s, ChainArgs c
   v.stride = kFusedBlock;
   uint64_t* my_mins = mins + (size_t)blockIdx.x * maxmin * kFusedBlock + t;
   const uint32_t r = bucket == 2 ? in_list[i] : rid(order, i);
-  const int nc = s.ctab ? seed_chain_lds<8, 10, true>(s, c, r, v, my_mins, kFusedBlock, maxmin, ring, cap)
-                        : seed_chain_lds<8, 10, false>(s, c, r, v, my_mins, kFusedBlock, maxmin, ring, cap);
+  const int nc = s.ctab ? (cap == 32 ? seed_chain_lds<8, 10, true, 32>(s, c, r, v, my_mins, kFusedBlock, maxmin, ring)
+                                     : seed_chain_lds<8, 10, true, 64>(s, c, r, v, my_mins, kFusedBlock, maxmin, ring))
+                        : (cap == 32 ? seed_chain_lds<8, 10, false, 32>(s, c, r, v, my_mins, kFusedBlock, maxmin, ring)
+                                     : seed_chain_lds<8, 10, false, 64>(s, c, r, v, my_mins, kFusedBlock, maxmin, ring));
   if (nc < 0) {
     c.n_cand[r] = 0;
     if (last) decl[atomicAdd(&ctl->nglobal, 1u)] = r;

@@ -255,8 +255,9 @@ __global__ __launch_bounds__(256) void k_chain_wave(ChainArgs a) {
 // interleaved by thread. The minimizer list goes to the block's part of `mins` (maxmin u64 per
 // thread, interleaved: written by the sketch, read back by the probes). The fallback sketch's
 // ring (w u64 hashes + w u32 positions) uses the anchor slots before any anchor is written.
-template <int WT, bool P8>
-__global__ __launch_bounds__(kFusedBlock) void k_seed_chain_fused(SeedArgs s, ChainArgs c, int cap, int maxmin,
+template <int WT, bool P8, int CAP>
+__global__ __launch_bounds__(kFusedBlock) DN_WAVES_PER_EU(CAP > 32 ? 2 : 3) void k_seed_chain_fused(SeedArgs s,
+                                                                                                  ChainArgs c, int maxmin,
                                                                   uint64_t* mins, uint32_t* ovf, uint32_t* novf) {
   DN_DYN_LDS(uint64_t, lds);
   const int t = threadIdx.x;
@@ -272,7 +273,7 @@ __global__ __launch_bounds__(kFusedBlock) void k_seed_chain_fused(SeedArgs s, Ch
   v.stride = kFusedBlock;
   const uint32_t r = read_id(s.list, i);
   uint64_t* my_mins = mins + (size_t)blockIdx.x * maxmin * kFusedBlock + t;
-  const int nc = seed_chain_lds<8, 10, P8>(s, c, r, v, my_mins, kFusedBlock, maxmin, ring, cap);
+  const int nc = seed_chain_lds<8, 10, P8, CAP>(s, c, r, v, my_mins, kFusedBlock, maxmin, ring);
   if (nc < 0) {
     ovf[atomicAdd(novf, 1u)] = r;
     c.n_cand[r] = 0;
@@ -1092,12 +1093,18 @@ uint64_t fused_mins_words(const FusedShape& fs, uint64_t n) {
 int launch_fused(const FusedShape& fs, const SeedArgs& sa, const ChainArgs& ca, uint64_t* mins, uint32_t* ovf,
                  uint32_t* novf, hipStream_t st) {
   const unsigned grid = (sa.n_reads + kFusedBlock - 1) / kFusedBlock;
-  if (sa.ctab)
-    hipLaunchKernelGGL((k_seed_chain_fused<10, true>), dim3(grid), dim3(kFusedBlock), fs.lds, st, sa, ca, fs.cap,
-                       fs.maxmin, mins, ovf, novf);
-  else
-    hipLaunchKernelGGL((k_seed_chain_fused<10, false>), dim3(grid), dim3(kFusedBlock), fs.lds, st, sa, ca, fs.cap,
-                       fs.maxmin, mins, ovf, novf);
+  if (fs.cap != 32 && fs.cap != 64) return fail(DNALN_E_INTERNAL, "fused seed + chain shape without a kernel");
+#define DN_FUSED(P8, CAP)                                                                                            \
+  hipLaunchKernelGGL((k_seed_chain_fused<10, P8, CAP>), dim3(grid), dim3(kFusedBlock), fs.lds, st, sa, ca, fs.maxmin, \
+                     mins, ovf, novf)
+  if (sa.ctab) {
+    if (fs.cap == 32) DN_FUSED(true, 32);
+    else DN_FUSED(true, 64);
+  } else {
+    if (fs.cap == 32) DN_FUSED(false, 32);
+    else DN_FUSED(false, 64);
+  }
+#undef DN_FUSED
   DN_HIP(hipGetLastError());
   return DNALN_OK;
 }

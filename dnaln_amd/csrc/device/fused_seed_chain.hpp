@@ -55,8 +55,9 @@ struct ChainLite {
   uint32_t rpos_strand;      // rpos | strand << 31
 };
 
-// Anchor index with the largest f among those not in `used` (ties: smallest index), or -1.
-DN_HD int next_end(const LdsView& v, int n, uint64_t used) {
+// Anchor index with the largest f among those not in `used` (ties: smallest index), or -1;
+// *f_out = its f.
+DN_HD int next_end(const LdsView& v, int n, uint64_t used, int* f_out) {
   int best = -1, bf = -1;
   for (int i = 0; i < n; ++i) {
     if ((used >> i) & 1) continue;
@@ -66,6 +67,7 @@ DN_HD int next_end(const LdsView& v, int n, uint64_t used) {
       best = i;
     }
   }
+  *f_out = bf;
   return best;
 }
 
@@ -156,13 +158,51 @@ DN_HD bool stage_read(const uint8_t* seq, int len, int tail, uint64_t* dst, int 
 // reference's either way. NIB: gap costs from the four-bit table (drift < 16) and cost_at_band
 // (drift == band), valid when cost_nib_ok; a drift above the band is never taken, so its cost
 // does not matter. Arithmetic is 32-bit: rpos < 2^31, qpos < 2^12.
+// Sorts the n anchor words ascending (n <= N): the words go to registers (+inf past n), a bitonic
+// network of N keys with compile-time indices sorts them, and they go back. Every comparator puts
+// the smaller word at the lower index (each merge starts with a flip stage, partner i ^ (k - 1)),
+// so the +inf padding stays at the end. Anchors from several loci arrive interleaved, so an
+// insertion sort's dependent LDS loads grew with the square of their number; the network costs
+// N loads, N log2(N) (log2(N) + 1) / 4 compare-exchanges in VGPRs and n stores, whatever the
+// order.
+template <int N>
+DN_HD void sort_lds(const LdsView& v, int n) {
+  uint64_t x[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) x[i] = i < n ? v.word(i) : ~0ull;
+#pragma unroll
+  for (int k = 2; k <= N; k <<= 1) {
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1) {
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        const int l = j == (k >> 1) ? (i ^ (k - 1)) : (i ^ j);
+        if (l > i) {
+          const uint64_t a = x[i], b = x[l];
+          const bool sw = b < a;
+          x[i] = sw ? b : a;
+          x[l] = sw ? a : b;
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+    if (i < n) v.keys[i * v.stride] = x[i];
+}
+
 template <bool NIB>
 DN_HD void chain_dp_lds(const ChainArgs& a, const LdsView& v, int n, int nf, int k) {
   const uint32_t max_gap = (uint32_t)a.max_gap;
   const int band = a.band;
   int fmax = 0;  // max f over this strand's anchors so far
+  // Anchors i-1 .. i-4 with their DP fields, in registers: the first group of predecessors
+  // (usually the only one) needs no LDS load, and anchor i's word is loaded one step ahead.
+  uint64_t win[4] = {0, 0, 0, 0};
+  uint64_t k_next = v.word(0);
   for (int i = 0; i < n; ++i) {
-    const uint64_t ki = v.word(i);
+    const uint64_t ki = k_next;
+    if (i + 1 < n) k_next = v.word(i + 1);
     const uint32_t hi = (uint32_t)(ki >> 32);
     const int qi = w_qpos(ki);
     int best = k, best_j = -1;
@@ -170,8 +210,13 @@ DN_HD void chain_dp_lds(const ChainArgs& a, const LdsView& v, int n, int nf, int
     const int jlo = i - a.lookback > s0 ? i - a.lookback : s0;
     for (int j0 = i - 1; j0 >= jlo; j0 -= 4) {
       uint64_t kj[4];
+      if (j0 == i - 1) {
 #pragma unroll
-      for (int u = 0; u < 4; ++u) kj[u] = v.word(j0 - u >= jlo ? j0 - u : jlo);
+        for (int u = 0; u < 4; ++u) kj[u] = win[u];  // past jlo (u > i-1-jlo): unused, the group ends the scan
+      } else {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) kj[u] = v.word(j0 - u >= jlo ? j0 - u : jlo);
+      }
       bool alive = true;
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
@@ -200,7 +245,12 @@ DN_HD void chain_dp_lds(const ChainArgs& a, const LdsView& v, int n, int nf, int
       if (dm < 3 && w_f(kj[3]) + dm + k <= best) break;
     }
     fmax = (i == 0 || i == nf || best > fmax) ? best : fmax;
-    v.set_low(i, w_low(qi, fmax - best < 3 ? fmax - best : 3, best, best_j));
+    const uint32_t low = w_low(qi, fmax - best < 3 ? fmax - best : 3, best, best_j);
+    v.set_low(i, low);
+    win[3] = win[2];
+    win[2] = win[1];
+    win[1] = win[0];
+    win[0] = (ki & ~0xffffffffull) | low;
   }
 }
 
@@ -211,7 +261,10 @@ DN_HD int chain_extract_lds(const ChainArgs& a, const LdsView& v, int n, uint32_
   uint64_t used = 0;
   ChainLite top[MAXCH];
   int count = 0, dropped_max = -1;
-  for (int end = next_end(v, n, used); end >= 0; end = next_end(v, n, used)) {
+  // A chain scores f[end] - f[stop] <= f[end] (f >= k > 0 everywhere), and ends come in
+  // descending f: once f[end] < min_score no chain left can qualify, so the pass ends there.
+  int fe = 0;
+  for (int end = next_end(v, n, used, &fe); end >= 0 && fe >= a.min_score; end = next_end(v, n, used, &fe)) {
     int cnt;
     ChainLite cur = walk_chain(v, end, used, cnt);
     if (cur.score < a.min_score || cnt < a.min_anchors) continue;
@@ -275,10 +328,12 @@ DN_HD int chain_extract_lds(const ChainArgs& a, const LdsView& v, int n, uint32_
 // anchors into LDS; 3. sort, DP, extraction, candidates. `ring` (the fallback sketch for reads
 // with N or a window length other than WT) may alias the anchor words, which are free until
 // phase 2. P8: probes through the 8-byte table s.ctab (probe_table.hpp), else the 16-byte
-// buckets. Returns the candidate count or -1 (the read does not fit; global path).
-template <int MAXCH, int WT, bool P8 = true>
+// buckets. CAP: the anchor slots per read (the LDS shape, 32 or 64), which also sizes the sort
+// network. Returns the candidate count or -1 (the read does not fit; global path).
+template <int MAXCH, int WT, bool P8, int CAP>
 DN_HD int seed_chain_lds(const SeedArgs& s, const ChainArgs& a, uint32_t r, LdsView v, uint64_t* mins, int mstride,
-                         int maxmin, Ring ring, int cap) {
+                         int maxmin, Ring ring) {
+  constexpr int cap = CAP;
   const int len = s.lens[r], k = s.k;
   if (len >= kFusedMaxLen || 2 * k > 46) return -1;
   const uint8_t* seq = s.codes + s.offs[r];
@@ -406,16 +461,8 @@ DN_HD int seed_chain_lds(const SeedArgs& s, const ChainArgs& a, uint32_t r, LdsV
   for (int x = 0; x < nrev; ++x) v.keys[(lo + x) * v.stride] = v.word(hi + 1 + x);
   if (n == 0) return 0;
 
-  // 3. sort (nearly sorted: insertion sort), DP, extraction, candidates
-  for (int i = 1; i < n; ++i) {
-    const uint64_t x = v.word(i);
-    int j = i - 1;
-    while (j >= 0 && x < v.word(j)) {
-      v.keys[(j + 1) * v.stride] = v.word(j);
-      --j;
-    }
-    v.keys[(j + 1) * v.stride] = x;
-  }
+  // 3. sort, DP, extraction, candidates
+  sort_lds<CAP>(v, n);
   if (a.cost_nib_ok) chain_dp_lds<true>(a, v, n, lo, k);
   else chain_dp_lds<false>(a, v, n, lo, k);
   return chain_extract_lds<MAXCH>(a, v, n, r);

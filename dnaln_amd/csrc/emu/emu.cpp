@@ -264,6 +264,7 @@ long emu_seed_chain(const uint8_t* codes, const uint64_t* offs, const int32_t* l
                     const void* table, uint64_t n_buckets, const uint32_t* pos, const uint64_t* contig_off,
                     int n_contigs, const int* mp, double pri_ratio, const int32_t* cost, int cap,
                     int maxmin, uint64_t* stats, int32_t* out_n, int64_t* out_c) {
+  if (cap != 32 && cap != 64) return -1;  // the fused shapes
   uint64_t total = 0;
   for (uint64_t r = 0; r < n; ++r) total = std::max<uint64_t>(total, offs[r] + (uint64_t)lens[r]);
   std::vector<uint8_t> guarded(total + 256, 0xEE);
@@ -364,8 +365,11 @@ long emu_seed_chain(const uint8_t* codes, const uint64_t* offs, const int32_t* l
     ChainArgs l = ca;
     l.cands = c2.data();
     LdsView v{keys.data(), 1};
-    const int nc = sa.ctab ? seed_chain_lds<8, 10, true>(sa, l, (uint32_t)r, v, mins.data(), 1, maxmin, ring, cap)
-                           : seed_chain_lds<8, 10, false>(sa, l, (uint32_t)r, v, mins.data(), 1, maxmin, ring, cap);
+    int nc;
+    if (sa.ctab) nc = cap == 32 ? seed_chain_lds<8, 10, true, 32>(sa, l, (uint32_t)r, v, mins.data(), 1, maxmin, ring)
+                                : seed_chain_lds<8, 10, true, 64>(sa, l, (uint32_t)r, v, mins.data(), 1, maxmin, ring);
+    else nc = cap == 32 ? seed_chain_lds<8, 10, false, 32>(sa, l, (uint32_t)r, v, mins.data(), 1, maxmin, ring)
+                        : seed_chain_lds<8, 10, false, 64>(sa, l, (uint32_t)r, v, mins.data(), 1, maxmin, ring);
     if (nc < 0) {
       ++stats[0];
       continue;
