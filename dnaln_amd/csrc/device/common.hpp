@@ -118,6 +118,12 @@ DN_HD uint32_t pk_bits(pk16 x) { return __builtin_bit_cast(uint32_t, x); }
 DN_HD pk16 pk_add(pk16 a, pk16 b) { return a + b; }
 DN_HD pk16 pk_sub(pk16 a, pk16 b) { return a - b; }
 DN_HD pk16 pk_max(pk16 a, pk16 b) { return __builtin_elementwise_max(a, b); }
+// max(a - b, 0) for halves in [0, 32767]: one v_pk_sub_u16 with clamp.
+DN_HD pk16 pk_subsat(pk16 a, pk16 b) {
+  typedef unsigned short upk16 __attribute__((ext_vector_type(2)));
+  return __builtin_bit_cast(pk16, __builtin_elementwise_sub_sat(__builtin_bit_cast(upk16, a),
+                                                                __builtin_bit_cast(upk16, b)));
+}
 #else
 struct pk16 {
   int16_t x, y;
@@ -127,6 +133,9 @@ DN_HD uint32_t pk_bits(pk16 a) { return (uint32_t)(uint16_t)a.x | ((uint32_t)(ui
 DN_HD pk16 pk_add(pk16 a, pk16 b) { return pk16{(int16_t)(a.x + b.x), (int16_t)(a.y + b.y)}; }
 DN_HD pk16 pk_sub(pk16 a, pk16 b) { return pk16{(int16_t)(a.x - b.x), (int16_t)(a.y - b.y)}; }
 DN_HD pk16 pk_max(pk16 a, pk16 b) { return pk16{a.x > b.x ? a.x : b.x, a.y > b.y ? a.y : b.y}; }
+DN_HD pk16 pk_subsat(pk16 a, pk16 b) {
+  return pk16{(int16_t)(a.x > b.x ? a.x - b.x : 0), (int16_t)(a.y > b.y ? a.y - b.y : 0)};
+}
 #endif
 DN_HD pk16 pk_splat(int v) { return pk_from((uint32_t)(uint16_t)v * 0x10001u); }
 DN_HD int pk_lo(pk16 a) { return (int16_t)(pk_bits(a) & 0xffff); }

@@ -502,11 +502,16 @@ DN_HD void sw_pair(const SwArgs& a, uint32_t ida, uint32_t idb, bool va, bool vb
         const pk16 hup = H[d + 1], eup = E[d + 1];
         const pk16 s = pk_from(perm_b32(tab0, tab1, T[d + u] ^ qsel));
         const pk16 hd = pk_add(hdiag, s);
-        const pk16 eo = pk_sub(hup, oe), ee = pk_sub(eup, ge);
+        // E and F are kept at max(E, 0) and max(F, 0) (clamped subtractions of non-negative
+        // values), which leaves H = max(hd, E, F, 0) unchanged and makes the zero floor free.
+        // Their flags then differ from the reference's only in cells with H = 0, which no walk
+        // reads: a walk stops at a cell whose score is 0, and every cell of a gap run has a
+        // positive gap score, whose larger operand is exact.
+        const pk16 eo = pk_subsat(hup, oe), ee = pk_subsat(eup, ge);
         const pk16 e = pk_max(eo, ee);
-        const pk16 fo = pk_sub(hleft, oe), fe = pk_sub(f, ge);
+        const pk16 fo = pk_subsat(hleft, oe), fe = pk_subsat(f, ge);
         f = pk_max(fo, fe);
-        const pk16 h = pk_max(pk_max(pk_max(hd, e), zero), f);
+        const pk16 h = pk_max(pk_max(hd, e), f);
         H[d] = h;
         E[d] = e;
         if (TB) {
