@@ -155,6 +155,7 @@ HOST_SYMBOLS = {
     "dnaln_seed_traffic": (i32, [vp, vp, vp, vp, u64, u32, i32, vp]),
     "dnaln_map_file": (i32, [vp, vp, C.c_char_p, i32, vp, i32, u64, i32, vp]),
     "dnaln_map_file_on": (i32, [vp, vp, C.c_char_p, i32, vp, i32, u64, i32, vp]),
+    "dnaln_map_encoded": (i32, [vp, i32, vp, vp, vp, vp, u64, u64, vp, vp, u64, vp, vp]),
     "dnaln_host_last_error": (C.c_char_p, []),
     "dnaln_evaluate": (i32, [vp, vp, u64, vp, vp, vp, vp, vp, i32, vp]),
     "dnaln_truth_check": (i32, [vp, vp, vp, u64, vp, vp, vp, vp, vp]),

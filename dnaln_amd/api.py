@@ -184,6 +184,30 @@ def prepare(reads: Reads) -> ReadBatch:
                      np.ascontiguousarray(reads.lens, np.int32), fnv)
 
 
+class PinnedArray:
+    """A numpy array in page-locked host memory (dnaln_hip_host_alloc), freed with the object."""
+
+    def __init__(self, n: int, dtype):
+        dt = np.dtype(dtype)
+        self.ptr = C.c_void_p()
+        _check(_lib.hip().dnaln_hip_host_alloc(max(n, 1) * dt.itemsize, C.byref(self.ptr)), "dnaln_hip_host_alloc")
+        buf = (C.c_uint8 * (max(n, 1) * dt.itemsize)).from_address(self.ptr.value)
+        self.a = np.frombuffer(buf, dtype=dt, count=n)
+
+    def __del__(self):
+        if getattr(self, "ptr", None) and self.ptr.value:
+            _lib.hip().dnaln_hip_host_free(self.ptr)
+            self.ptr = C.c_void_p()
+
+
+def pinned_reads(rb: ReadBatch) -> tuple[ReadBatch, list]:
+    """A copy of rb in page-locked memory, and the owners that keep it alive."""
+    own = [PinnedArray(len(x), x.dtype) for x in (rb.codes, rb.offs, rb.lens, rb.fnv)]
+    for o, x in zip(own, (rb.codes, rb.offs, rb.lens, rb.fnv)):
+        o.a[:] = x
+    return ReadBatch(*(o.a for o in own)), own
+
+
 class Context:
     """Resident reference + index + parameters on one GPU (dnaln_hip_create)."""
 
@@ -212,10 +236,11 @@ class Context:
         return {"n_buckets": nb.value, "n_pos": npos.value, "build_ms": ms.value}
 
     def close(self) -> None:
-        for pair in getattr(self, "_file_batches", {}).values():
-            for b in pair:
+        for group in list(getattr(self, "_file_batches", {}).values()) + list(getattr(self, "_enc_batches", {}).values()):
+            for b in group:
                 b.close()
         self._file_batches = {}
+        self._enc_batches = {}
         if self.ctx:
             _lib.hip().dnaln_hip_destroy(self.ctx)
             self.ctx = C.c_void_p()
@@ -278,6 +303,31 @@ class Context:
             msg = _lib.host().dnaln_host_last_error().decode(errors="replace")
             raise DnalnError(rc, f"dnaln_map_file({reads_path}): {msg}")
         return {f: getattr(st, f) for f, _ in abi.PipelineStats._fields_ if f != "reserved"}
+
+    def map_encoded(self, rb: ReadBatch, n_batches: int = 2, chunk_reads: int = 0, mode: int = abi.MODE_BATCHED,
+                    out: np.ndarray | None = None, pool: np.ndarray | None = None):
+        """Encoded reads -> (records, CIGAR pool, stats) through n_batches device batches at once
+        (dnaln_map_encoded): chunks of chunk_reads, one batch's copies overlapping another's
+        kernels. Page-locked rb / out / pool (pinned_reads, PinnedArray) make the copies direct."""
+        if not hasattr(self, "_enc_batches"):
+            self._enc_batches = {}
+        key = (mode, n_batches)
+        if key not in self._enc_batches:
+            self._enc_batches[key] = [self.batch(mode=mode) for _ in range(n_batches)]
+        bs = self._enc_batches[key]
+        arr = (C.c_void_p * n_batches)(*[b.b for b in bs])
+        if out is None:
+            out = np.zeros(len(rb), abi.RECORD_DTYPE)
+        if pool is None:
+            pool = np.zeros(8 * len(rb) + 64, np.uint32)
+        used = C.c_uint64()
+        st = abi.PipelineStats()
+        rc = _lib.host().dnaln_map_encoded(arr, n_batches, _p(rb.codes), _p(rb.offs), _p(rb.lens), _p(rb.fnv), len(rb),
+                                           chunk_reads, _p(out), _p(pool), len(pool), C.byref(used), C.byref(st))
+        if rc != abi.OK:
+            msg = _lib.host().dnaln_host_last_error().decode(errors="replace")
+            raise DnalnError(rc, f"dnaln_map_encoded: {msg}")
+        return out, pool[:used.value], {f: getattr(st, f) for f, _ in abi.PipelineStats._fields_ if f != "reserved"}
 
     def batch(self, mode: int = abi.MODE_PLAIN, chunk_reads: int = 65536, max_reads: int = 0, max_bases: int = 0,
               tune: bool = False, len_split: tuple[int, int] = (0, 0), dir_budget: int = 0,

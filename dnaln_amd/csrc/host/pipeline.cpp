@@ -4,7 +4,7 @@
 // one thread. Here three stages run at once on consecutive batches:
 //   input   parse FASTA/FASTQ (src/seqio.cpp:59-87), encode (src/common.hpp:36-42) and hash the
 //           names (src/common.hpp:136-140) into page-locked staging, on `threads` threads;
-//   device  upload, run and download on two device batches in turn, so one batch's copies
+//   device  upload, run and download, one thread per device batch (two), so one batch's copies
 //           overlap the other's kernels;
 //   output  format the SAM records (src/mapper.cpp:231-270) on `threads` threads and write them
 //           in input order.
@@ -73,6 +73,7 @@ struct Slot {
   Pinned<uint32_t> cig;
   std::vector<std::string> text;  // formatted SAM, one string per part
   uint64_t mapped = 0;
+  uint64_t seq = 0;               // position of the batch in the input
 };
 
 // Slots outlive the call that used them: their page-locked buffers (hipHostMalloc pins pages,
@@ -226,7 +227,7 @@ struct Pipe {
 
   void run() {
     const double t0 = now();
-    constexpr int kSlots = 4;
+    constexpr int kSlots = 6;  // parsing, two on the device, formatting, and one spare each side
     std::vector<std::unique_ptr<Slot>> slots = slot_pool().take(kSlots);
     struct GiveBack {
       std::vector<std::unique_ptr<Slot>>& s;
@@ -242,6 +243,7 @@ struct Pipe {
       if (err_code == DNALN_OK) err = msg, err_code = code;
     };
     double s_input = 0, s_format = 0, s_write = 0;
+    uint64_t seq = 0;  // input order of the slots the reader fills
     // input
     std::thread reader([&] {
       try {
@@ -259,6 +261,7 @@ struct Pipe {
             free_q.push(s);
             break;
           }
+          s->seq = seq++;
           ready_q.push(s);
         }
       } catch (const Failure& f) {
@@ -293,61 +296,81 @@ struct Pipe {
         free_q.push(s);
       }
     });
-    // device (this thread): batch k takes slot i while batch k ^ 1 finishes slot i - 1
-    Slot* pending[2] = {nullptr, nullptr};
-    int k = 0;
-    auto finish = [&](int j) {
-      Slot* s = pending[j];
+    // device: one thread per batch, each taking the next parsed slot through upload, run and
+    // download on its own stream, so one batch's copies overlap the other's kernels (a run
+    // blocks its thread on a few reads of counts). Slots reach the writer in input order: a
+    // slot waits for its predecessor's hand-off. Busy time is the time at least one device
+    // thread works on a slot.
+    std::mutex turn_m;
+    std::condition_variable turn_cv;
+    uint64_t next_out = 0;
+    int busy = 0;
+    double busy_since = 0, busy_s = 0;
+    auto set_busy = [&](int delta) {
+      std::lock_guard<std::mutex> g(turn_m);
       const double t = now();
-      check(dnaln_hip_batch_poll(b[j], 1), "dnaln_hip_batch_poll");
-      s->rec.reserve(s->n + 1);
-      s->cig.reserve(8 * s->n + 64);  // a few CIGAR words per read; grown to the bound if short
-      dnaln_map_stats ms{};
-      int rc = dnaln_hip_batch_download(b[j], s->rec.p, s->cig.p, s->cig.cap, &ms);
-      if (rc == DNALN_E_CAPACITY) {
-        s->cig.reserve(2 * s->bases + 4 * s->n + 4);
-        rc = dnaln_hip_batch_download(b[j], s->rec.p, s->cig.p, s->cig.cap, &ms);
-      }
-      check(rc, "dnaln_hip_batch_download");
-      st.s_device += now() - t;
-      st.reads += ms.reads;
-      st.mapped += ms.mapped;
-      st.bases += s->bases;
-      ++st.batches;
-      pending[j] = nullptr;
-      done_q.push(s);
+      if (busy == 0 && delta > 0) busy_since = t;
+      busy += delta;
+      if (busy == 0) busy_s += t - busy_since;
     };
-    try {
+    auto hand_off = [&](Slot* s, bool done, const dnaln_map_stats* ms) {
+      {
+        std::unique_lock<std::mutex> g(turn_m);
+        turn_cv.wait(g, [&] { return next_out == s->seq; });
+        if (done) {
+          st.reads += ms->reads;
+          st.mapped += ms->mapped;
+          st.bases += s->bases;
+          ++st.batches;
+          done_q.push(s);
+        } else {
+          free_q.push(s);
+        }
+        ++next_out;
+      }
+      turn_cv.notify_all();
+    };
+    auto device = [&](int j) {
       for (;;) {
         Slot* s = ready_q.pop();
-        if (!s) break;
+        if (!s) {
+          ready_q.push(nullptr);  // the end, for the other device thread too
+          return;
+        }
         if (err_code != DNALN_OK) {
-          free_q.push(s);
+          hand_off(s, false, nullptr);
           continue;
         }
-        const double t = now();
-        check(dnaln_hip_batch_upload(b[k], s->codes.p, s->offs.p, s->lens.p, s->fnv.p, s->n), "dnaln_hip_batch_upload");
-        check(dnaln_hip_batch_run(b[k]), "dnaln_hip_batch_run");
-        st.s_device += now() - t;
-        pending[k] = s;
-        if (pending[k ^ 1]) finish(k ^ 1);
-        k ^= 1;
-      }
-      for (int j : {k, k ^ 1})
-        if (pending[j]) finish(j);
-    } catch (const Failure& f) {
-      note(f.code, f.what());  // the reader stops at its next slot
-      for (int j = 0; j < 2; ++j)
-        if (pending[j]) {
-          (void)dnaln_hip_batch_poll(b[j], 1);
-          free_q.push(pending[j]);
+        dnaln_map_stats ms{};
+        bool ok = false;
+        set_busy(1);
+        try {
+          check(dnaln_hip_batch_upload(b[j], s->codes.p, s->offs.p, s->lens.p, s->fnv.p, s->n), "dnaln_hip_batch_upload");
+          check(dnaln_hip_batch_run(b[j]), "dnaln_hip_batch_run");
+          check(dnaln_hip_batch_poll(b[j], 1), "dnaln_hip_batch_poll");
+          s->rec.reserve(s->n + 1);
+          s->cig.reserve(8 * s->n + 64);  // a few CIGAR words per read; grown to the bound if short
+          int rc = dnaln_hip_batch_download(b[j], s->rec.p, s->cig.p, s->cig.cap, &ms);
+          if (rc == DNALN_E_CAPACITY) {
+            s->cig.reserve(2 * s->bases + 4 * s->n + 4);
+            rc = dnaln_hip_batch_download(b[j], s->rec.p, s->cig.p, s->cig.cap, &ms);
+          }
+          check(rc, "dnaln_hip_batch_download");
+          ok = true;
+        } catch (const Failure& f) {
+          note(f.code, f.what());  // the reader stops at its next slot
+        } catch (const std::exception& x) {
+          note(DNALN_E_INTERNAL, x.what());
         }
-      for (;;) {
-        Slot* s = ready_q.pop();
-        if (!s) break;
-        free_q.push(s);
+        if (!ok) (void)dnaln_hip_batch_poll(b[j], 1);
+        set_busy(-1);
+        hand_off(s, ok, &ms);
       }
-    }
+    };
+    std::thread dev1(device, 1);
+    device(0);
+    dev1.join();
+    st.s_device = busy_s;
     free_q.push(nullptr);
     reader.join();
     done_q.push(nullptr);
@@ -397,6 +420,105 @@ int dnaln_map_file_on(dnaln_hip_batch* b0, dnaln_hip_batch* b1, const char* read
   }
   if (stats) *stats = p.st;
   return DNALN_OK;
+}
+
+int dnaln_map_encoded(dnaln_hip_batch* const* batches, int32_t n_batches, const uint8_t* codes, const uint64_t* offs,
+                      const int32_t* lens, const uint32_t* fnv, uint64_t n, uint64_t chunk_reads, dnaln_record* out,
+                      uint32_t* cigar_pool, uint64_t cigar_cap, uint64_t* cigar_used, dnaln_pipeline_stats* stats) {
+  if (!batches || n_batches < 1 || n_batches > 64 || !cigar_used ||
+      (n && (!codes || !offs || !lens || !fnv || !out || !cigar_pool))) {
+    g_host_err = "null argument or batch count outside 1..64";
+    return DNALN_E_INVALID;
+  }
+  for (int j = 0; j < n_batches; ++j)
+    for (int x = 0; x < j; ++x)
+      if (!batches[j] || batches[j] == batches[x]) {
+        g_host_err = "null batch, or the same batch twice";
+        return DNALN_E_INVALID;
+      }
+  const uint64_t R = std::max<uint64_t>(chunk_reads ? chunk_reads : (1ull << 18), 1);
+  const uint64_t n_chunks = (n + R - 1) / R;
+  const double t0 = now();
+  std::atomic<uint64_t> next{0};
+  std::mutex turn_m;
+  std::condition_variable turn_cv;
+  uint64_t next_out = 0, used = 0, mapped = 0;
+  int err_code = DNALN_OK;
+  std::string err;
+  // Chunk c: reads [c R, min(n, (c + 1) R)), uploaded from the caller's arrays (offsets rebased
+  // to the chunk's first base), records downloaded straight into out. The chunk's CIGAR words,
+  // staged per thread, are appended to the pool in chunk order, and its records' offsets moved
+  // with them.
+  auto device = [&](int j) {
+    Pinned<uint64_t> roffs;
+    Pinned<uint32_t> cig;
+    for (;;) {
+      const uint64_t c = next.fetch_add(1);
+      if (c >= n_chunks) return;
+      const uint64_t r0 = c * R, m = std::min(n, r0 + R) - r0;
+      int rc = DNALN_OK;
+      std::string e;
+      dnaln_map_stats ms{};
+      try {
+        uint64_t lo = ~0ull;
+        for (uint64_t i = 0; i < m; ++i) lo = std::min(lo, offs[r0 + i]);
+        roffs.reserve(m + 1);
+        uint64_t bases = 0;
+        for (uint64_t i = 0; i < m; ++i) {
+          roffs.p[i] = offs[r0 + i] - lo;
+          bases += (uint64_t)std::max(lens[r0 + i], 0);
+        }
+        cig.reserve(8 * m + 64);
+        rc = dnaln_hip_batch_upload(batches[j], codes + lo, roffs.p, lens + r0, fnv + r0, m);
+        if (rc == DNALN_OK) rc = dnaln_hip_batch_run(batches[j]);
+        if (rc == DNALN_OK) rc = dnaln_hip_batch_poll(batches[j], 1);
+        if (rc == DNALN_OK) rc = dnaln_hip_batch_download(batches[j], out + r0, cig.p, cig.cap, &ms);
+        if (rc == DNALN_E_CAPACITY) {
+          cig.reserve(2 * bases + 4 * m + 4);
+          rc = dnaln_hip_batch_download(batches[j], out + r0, cig.p, cig.cap, &ms);
+        }
+        if (rc != DNALN_OK) e = dnaln_hip_last_error();
+      } catch (const std::exception& x) {
+        rc = DNALN_E_NOMEM;
+        e = x.what();
+      }
+      std::unique_lock<std::mutex> g(turn_m);
+      turn_cv.wait(g, [&] { return next_out == c; });
+      if (rc != DNALN_OK && err_code == DNALN_OK) err_code = rc, err = e;
+      if (rc == DNALN_OK && err_code == DNALN_OK) {
+        uint64_t words = 0;
+        for (uint64_t i = 0; i < m; ++i) words = std::max<uint64_t>(words, out[r0 + i].cigar_off + out[r0 + i].cigar_len);
+        if (used + words > cigar_cap) {
+          err_code = DNALN_E_CAPACITY;
+          err = "cigar_pool too small";
+        } else {
+          if (words) std::memcpy(cigar_pool + used, cig.p, words * 4);
+          for (uint64_t i = 0; i < m; ++i) out[r0 + i].cigar_off += used;
+          used += words;
+          mapped += ms.mapped;
+        }
+      }
+      ++next_out;
+      g.unlock();
+      turn_cv.notify_all();
+    }
+  };
+  {
+    std::vector<std::thread> th;
+    for (int j = 1; j < n_batches; ++j) th.emplace_back(device, j);
+    device(0);
+    for (auto& t : th) t.join();
+  }
+  *cigar_used = used;
+  if (stats) {
+    std::memset(stats, 0, sizeof *stats);
+    stats->reads = n;
+    stats->mapped = mapped;
+    stats->batches = n_chunks;
+    stats->s_device = stats->s_wall = now() - t0;
+  }
+  if (err_code != DNALN_OK) g_host_err = err;
+  return err_code;
 }
 
 int dnaln_map_file(dnaln_hip_ctx* ctx, const dnaln_batch_config* cfg, const char* reads_path, int out_fd,

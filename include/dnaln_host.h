@@ -77,8 +77,8 @@ long dnaln_sam_format(const char* name, size_t name_len, const char* seq, const 
 /* Mapper::map + append_sam over a whole FASTA/FASTQ file, plain or gzip (the driver loop of
  * src/main.cpp:47-61), as three stages that overlap on consecutive batches: input (parse,
  * encode, name hashes into page-locked staging; plain 4-line FASTQ parsed by `threads` threads,
- * anything else by the reference's sequential rules), device (two batches of configuration cfg
- * in turn: one's copies overlap the other's kernels) and output (SAM text formatted by `threads`
+ * anything else by the reference's sequential rules), device (two batches of configuration cfg,
+ * one thread each: one's copies overlap the other's kernels) and output (SAM text formatted by `threads`
  * threads, written to out_fd in input order; no header). A batch holds about batch_bytes of
  * input (0: 64 MiB). The records are those of dnaln_hip_map_batch + dnaln_sam_format. */
 typedef struct {
@@ -99,6 +99,18 @@ int dnaln_map_file(struct dnaln_hip_ctx* ctx, const dnaln_batch_config* cfg, con
  * for later calls, which then allocate nothing on the device once warm. */
 int dnaln_map_file_on(struct dnaln_hip_batch* b0, struct dnaln_hip_batch* b1, const char* reads_path, int out_fd,
                       const char* const* contig_names, int32_t n_contigs, uint64_t batch_bytes, int32_t threads,
+                      dnaln_pipeline_stats* stats);
+/* Mapper::map (src/mapper.cpp:75-223, minus SAM text) over n reads already encoded in host memory
+ * (the arguments of dnaln_hip_map_batch), through n_batches device batches of one context at
+ * once: each batch's thread takes the next chunk of chunk_reads reads (0: 262,144), uploads it
+ * from the caller's arrays (page-locked memory, dnaln_hip_host_alloc, makes these copies direct),
+ * runs it and downloads its records into out, so one batch's copies overlap another's kernels.
+ * CIGAR words are appended to cigar_pool in read order (*cigar_used words); records are those of
+ * dnaln_hip_map_batch. stats: reads, mapped, batches (chunks) and s_device = s_wall, the call's
+ * time; may be NULL. The batches stay with the caller (warm calls allocate no device memory). */
+int dnaln_map_encoded(struct dnaln_hip_batch* const* batches, int32_t n_batches, const uint8_t* codes,
+                      const uint64_t* offs, const int32_t* lens, const uint32_t* fnv, uint64_t n, uint64_t chunk_reads,
+                      dnaln_record* out, uint32_t* cigar_pool, uint64_t cigar_cap, uint64_t* cigar_used,
                       dnaln_pipeline_stats* stats);
 /* Accuracy against simulated truth (no part of the mapping path; validation of its results).
  * dnaln_evaluate is Evaluator::add (src/evaluate.cpp:40-84) over n records, record i against
@@ -125,7 +137,7 @@ int dnaln_truth_check(const dnaln_score_params* sp, const uint8_t* pool, const d
                       const dnaln_result* res, const uint32_t* cigar_pool, const uint32_t* truth_cigar,
                       const uint64_t* truth_cigar_off, dnaln_truth_stats* out);
 
-/* Message of the calling thread's last failed dnaln_map_file or dnaln_map_file_on. */
+/* Message of the calling thread's last failed dnaln_map_file, dnaln_map_file_on or dnaln_map_encoded. */
 const char* dnaln_host_last_error(void);
 
 #ifdef __cplusplus

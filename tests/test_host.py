@@ -187,3 +187,19 @@ def test_seed_traffic_model_counts_match_the_oracle():
     assert (got["minimizers"], got["buckets"], got["occurrences"], got["read_bases"]) == (mins, buckets, occ, bases)
     assert got["bytes"] == bases + 16 * buckets + 4 * occ
     assert occ > 0 and buckets > mins  # the case exercises multi-occurrence keys and probe chains
+
+
+def test_map_encoded_refuses_bad_arguments_without_a_device():
+    """dnaln_map_encoded checks its batches before touching a device: none, null ones or the same
+    batch twice give DNALN_E_INVALID with a message."""
+    import ctypes as C
+    from dnaln_amd import _lib, abi
+    lib = _lib.host()
+    used = C.c_uint64()
+    for arr, n in (((C.c_void_p * 2)(None, None), 2), ((C.c_void_p * 1)(None), 0)):
+        rc = lib.dnaln_map_encoded(arr, n, None, None, None, None, 0, 0, None, None, 0, C.byref(used), None)
+        assert rc == abi.E_INVALID
+        assert lib.dnaln_host_last_error()
+    fake = (C.c_void_p * 2)(C.c_void_p(8), C.c_void_p(8))  # never dereferenced: refused as a duplicate
+    assert lib.dnaln_map_encoded(fake, 2, None, None, None, None, 0, 0, None, None, 0, C.byref(used), None) \
+        == abi.E_INVALID
