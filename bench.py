@@ -339,7 +339,7 @@ def device_pipeline(ctx: api.Context, rb: api.ReadBatch, recs: np.ndarray, pool:
     out = api.PinnedArray(len(rb), abi.RECORD_DTYPE)
     opool = api.PinnedArray(8 * len(rb) + 64, np.uint32)
     tried = []
-    for nb, chunk in ((2, 1 << 18), (3, 1 << 18), (4, 1 << 17), (3, 1 << 19)):
+    for nb, chunk in ((2, 1 << 18), (3, 1 << 18), (4, 1 << 17), (6, 1 << 17), (8, 1 << 16)):
         ctx.map_encoded(prb, nb, chunk, out=out.a, pool=opool.a)  # warm: the batches' buffers
         best = min(ctx.map_encoded(prb, nb, chunk, out=out.a, pool=opool.a)[2]["s_wall"] for _ in range(3))
         tried.append({"batches": nb, "chunk_reads": chunk, "reads_per_s": len(rb) / best})

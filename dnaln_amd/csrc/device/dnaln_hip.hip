@@ -905,6 +905,7 @@ struct dnaln_hip_batch : SpanSink {
   bool device_form = false;          // the last run took the device-count sequence
   // workspace
   DevBuf acnt, aoff, keys, f, prev, order, chains, ring_h, ring_ps;
+  DevBuf hits, nhits;                // probe results of the global path's count pass
   DevBuf cands, ncand, rcnt, roff, rtasks, rres, chosen, second, tbflag, tboff, ttasks, tres, tcig;
   SwWork sw;
   DevBuf scan_tmp, ciglen, cigoff, err, scratch8, ovf;
@@ -988,10 +989,16 @@ int seed_chain_global(dnaln_hip_batch* b, SeedArgs sa, ChainArgs ca, const uint3
   hipStream_t st = b->st;
   DN_TRY(b->acnt.reserve((uint64_t)nl * 4 + 4, st));
   DN_TRY(b->aoff.reserve((uint64_t)nl * 8 + 8, st));
+  // The count pass keeps each read's probe results for the emit pass (12 bytes per base of the
+  // batch at most, the region of read r at 3 * offs[r] words).
+  DN_TRY(b->hits.reserve(b->bases * 12 + 16, st));
+  DN_TRY(b->nhits.reserve((uint64_t)nl * 4 + 4, st));
   sa.n_reads = nl;
   sa.list = list;
   sa.anchor_count = b->acnt.as<uint32_t>();
   sa.anchor_off = b->aoff.as<uint64_t>();
+  sa.hits = b->hits.as<uint32_t>();
+  sa.n_hits = b->nhits.as<uint32_t>();
   const size_t lds = (size_t)c->w * kSeedBlock * 12;
   const int use_lds = lds <= 48 * 1024;
   if (!use_lds) {
@@ -2855,6 +2862,8 @@ int dnaln_hip_batch_reserve(dnaln_hip_batch* b, uint64_t n, uint64_t bases, int3
     DN_TRY(b->prev.reserve(na * 4 + 4));
     DN_TRY(b->order.reserve(na * 8 + 8));
     DN_TRY(b->chains.reserve(na * sizeof(ChainRec) + sizeof(ChainRec)));
+    DN_TRY(b->hits.reserve(bases * 12 + 16));
+    DN_TRY(b->nhits.reserve(n * 4 + 4));
     // rank and traceback passes
     DN_TRY(b->rcnt.reserve(n * 4 + 4));
     DN_TRY(b->roff.reserve(n * 8 + 8));

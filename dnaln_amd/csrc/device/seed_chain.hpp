@@ -37,6 +37,13 @@ struct SeedArgs {
   uint32_t* anchor_count;   // seed_count output, by list position
   const uint64_t* anchor_off;   // exclusive scan of anchor_count
   uint64_t* keys;           // seed_emit output
+  // The probe results of seed_count_read, which seed_emit_read replays instead of sketching and
+  // probing again (null: it does both again): per minimizer with an anchor, three words
+  // (mpos << 1 | strand, off, cnt), read r's at hits[3 * offs[r] ..] (a read has at most len
+  // minimizers, so the regions of distinct reads never overlap), and their number by list
+  // position in n_hits.
+  uint32_t* hits;
+  uint32_t* n_hits;
 };
 
 DN_HD uint32_t read_id(const uint32_t* list, uint32_t i) { return list ? list[i] : i; }
@@ -221,7 +228,7 @@ struct ProbePipe {
         hit = f > 0 && tag != kCSkip;
         if (hit && tag == kCOne) {
           cnt = 1;
-          first = c_val(ent);
+          off = first = c_val(ent);  // as with inlined buckets: a single occurrence is its own offset
         } else if (hit) {
           off = c_val(ent) + 1;
           cnt = s.pos[off - 1];
@@ -284,11 +291,21 @@ DN_HD void sketch_read(const SeedArgs& a, uint32_t r, Ring ring, Emit&& emit) {
 
 DN_HD uint32_t seed_count_read(const SeedArgs& a, uint32_t i, Ring ring) {
   const uint32_t r = read_id(a.list, i);
-  uint32_t total = 0;
-  auto on = [&](uint32_t, uint32_t, uint32_t, uint32_t cnt, uint32_t) { total += cnt; };
+  uint32_t total = 0, nh = 0;
+  uint32_t* hit = a.hits ? a.hits + 3 * a.offs[r] : nullptr;
+  auto on = [&](uint32_t mpos, uint32_t mstrand, uint32_t off, uint32_t cnt, uint32_t) {
+    total += cnt;
+    if (hit) {
+      hit[3 * nh] = mpos << 1 | mstrand;
+      hit[3 * nh + 1] = off;
+      hit[3 * nh + 2] = cnt;
+      ++nh;
+    }
+  };
   ProbePipe<false, decltype(on)> pipe(a, on);
   sketch_read(a, r, ring, [&](uint64_t h, uint32_t p, uint32_t st) { pipe.push(h, p, st); });
   pipe.drain();
+  if (hit) a.n_hits[i] = nh;
   return total;
 }
 
@@ -321,6 +338,27 @@ DN_HD void seed_emit_read(const SeedArgs& a, uint32_t i, Ring ring) {
         if (j0 + x < cnt) put(w[x]);
     }
   };
+  if (a.hits) {
+    // The count pass's probe results, four at a time with their first occurrences loaded together.
+    const uint32_t* hit = a.hits + 3 * a.offs[r];
+    const uint32_t nh = a.n_hits[i];
+    for (uint32_t x0 = 0; x0 < nh; x0 += 4) {
+      uint32_t mps[4], off[4], cnt[4], first[4];
+#pragma unroll
+      for (int x = 0; x < 4; ++x) {
+        const uint32_t k = x0 + x < nh ? x0 + x : x0;
+        mps[x] = hit[3 * k];
+        off[x] = hit[3 * k + 1];
+        cnt[x] = hit[3 * k + 2];
+      }
+#pragma unroll
+      for (int x = 0; x < 4; ++x) first[x] = first_pos(a, off[x], cnt[x]);
+#pragma unroll
+      for (int x = 0; x < 4; ++x)
+        if (x0 + x < nh) on(mps[x] >> 1, mps[x] & 1, off[x], cnt[x], first[x]);
+    }
+    return;
+  }
   ProbePipe<true, decltype(on)> pipe(a, on);
   sketch_read(a, r, ring, [&](uint64_t h, uint32_t p, uint32_t st) { pipe.push(h, p, st); });
   pipe.drain();

@@ -315,17 +315,25 @@ long emu_seed_chain(const uint8_t* codes, const uint64_t* offs, const int32_t* l
   std::vector<uint64_t> keys(cap), mins(maxmin), gkeys, order;
   std::vector<int32_t> f32, prev;
   std::vector<ChainRec> chains;
+  std::vector<uint32_t> hits;
   long differ = 0;
   stats[0] = stats[1] = stats[2] = 0;
   for (uint64_t r = 0; r < n; ++r) {
     // global path for read r alone
-    uint32_t cnt = 0;
+    uint32_t cnt = 0, nh = 0;
     uint64_t off0 = 0;
     SeedArgs s1 = sa;
     const uint32_t rid = (uint32_t)r;
     s1.list = &rid;
     s1.n_reads = 1;
     s1.anchor_count = &cnt;
+    // Odd reads: the emit pass replays the count pass's probe results; even reads: it sketches
+    // and probes again. The region of read r starts at 3 * offs[r] words.
+    if (r & 1) {
+      hits.assign(3 * ((size_t)offs[r] + (size_t)std::max(lens[r], 0)) + 3, 0xA5A5A5A5u);
+      s1.hits = hits.data();
+      s1.n_hits = &nh;
+    }
     cnt = seed_count_read(s1, 0, ring);
     gkeys.assign(cnt + 1, 0);
     f32.assign(cnt + 1, 0);

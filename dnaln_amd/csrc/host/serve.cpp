@@ -229,7 +229,8 @@ struct dnaln_server {
     const uint64_t G = (uint64_t)(2 * bw + 1 + 7) / 8;
     const uint64_t per_read = 16 + sizeof(dnaln_record) + 64 + (uint64_t)(max_cand + 1) * 96 + 64 * 48 + 64 * 8;
     const uint64_t dirs = std::min<uint64_t>(bases * 4 * G + (uint64_t)maxlen * G * 512, cfg.dir_budget);
-    return bases + 256 + n * per_read + n * (2 * (uint64_t)maxlen + 4) * 4 + dirs;
+    // (12 bytes per base: the global path's probe results between its count and emit passes)
+    return bases * 13 + 256 + n * per_read + n * (2 * (uint64_t)maxlen + 4) * 4 + dirs;
   }
 
   int collect() {
