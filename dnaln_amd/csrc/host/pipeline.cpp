@@ -504,8 +504,14 @@ int dnaln_map_encoded(dnaln_hip_batch* const* batches, int32_t n_batches, const 
     }
   };
   {
+    // A thread that cannot be started leaves its chunks to the others (the calling thread takes
+    // chunks too), so the records are the same with fewer threads.
     std::vector<std::thread> th;
-    for (int j = 1; j < n_batches; ++j) th.emplace_back(device, j);
+    try {
+      th.reserve((size_t)n_batches);
+      for (int j = 1; j < n_batches; ++j) th.emplace_back(device, j);
+    } catch (const std::exception&) {
+    }
     device(0);
     for (auto& t : th) t.join();
   }
