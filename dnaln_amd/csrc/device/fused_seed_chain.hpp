@@ -196,9 +196,10 @@ DN_HD void chain_dp_lds(const ChainArgs& a, const LdsView& v, int n, int nf, int
   const uint32_t max_gap = (uint32_t)a.max_gap;
   const int band = a.band;
   int fmax = 0;  // max f over this strand's anchors so far
-  // Anchors i-1 .. i-4 with their DP fields, in registers: the first group of predecessors
-  // (usually the only one) needs no LDS load, and anchor i's word is loaded one step ahead.
-  uint64_t win[4] = {0, 0, 0, 0};
+  // Anchors i-1 .. i-8 with their DP fields, in registers: the first two groups of predecessors
+  // need no LDS load (a wave steps until its last lane's scan ends, and the lane that needs a
+  // second group is common), and anchor i's word is loaded one step ahead.
+  uint64_t win[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   uint64_t k_next = v.word(0);
   for (int i = 0; i < n; ++i) {
     const uint64_t ki = k_next;
@@ -213,6 +214,9 @@ DN_HD void chain_dp_lds(const ChainArgs& a, const LdsView& v, int n, int nf, int
       if (j0 == i - 1) {
 #pragma unroll
         for (int u = 0; u < 4; ++u) kj[u] = win[u];  // past jlo (u > i-1-jlo): unused, the group ends the scan
+      } else if (j0 == i - 5) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) kj[u] = win[4 + u];
       } else {
 #pragma unroll
         for (int u = 0; u < 4; ++u) kj[u] = v.word(j0 - u >= jlo ? j0 - u : jlo);
@@ -247,9 +251,8 @@ DN_HD void chain_dp_lds(const ChainArgs& a, const LdsView& v, int n, int nf, int
     fmax = (i == 0 || i == nf || best > fmax) ? best : fmax;
     const uint32_t low = w_low(qi, fmax - best < 3 ? fmax - best : 3, best, best_j);
     v.set_low(i, low);
-    win[3] = win[2];
-    win[2] = win[1];
-    win[1] = win[0];
+#pragma unroll
+    for (int u = 7; u > 0; --u) win[u] = win[u - 1];
     win[0] = (ki & ~0xffffffffull) | low;
   }
 }
