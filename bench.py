@@ -35,6 +35,14 @@ HBM_PEAK = 8.0e12
 OPS_PER_CELL = {"traceback": 15, "score": 11}
 
 
+def fill_kernels(W: int, lanes: int) -> str:
+    """The traceback / score-only fill kernels a batched run at band W with `lanes` lanes per pair
+    launches (the score-only fill splits only past W = 65)."""
+    tb = f"k_sw_split<{W}, {lanes}, true>" if lanes > 1 else f"k_sw_packed<{W}, true>"
+    sc = f"k_sw_split<{W}, {lanes}, false>" if lanes > 1 and W > 65 else f"k_sw_packed<{W}, false>"
+    return f"{tb} / {sc}"
+
+
 def cells_from_records(recs: np.ndarray, lens: np.ndarray, W: int) -> tuple[int, int]:
     """Band cells of the rank and traceback passes, as the reference counts them (src/mapper.cpp:139, 186)."""
     nc = recs["n_cand"].astype(np.int64)
@@ -146,12 +154,12 @@ def config_block(name: str, peak_ops: float, steps: int, warmup: int, plain_step
            "score_fill_frac": rank_cells * OPS_PER_CELL["score"] / (fam[2] * 1e-3) / peak_ops
            if fam[2] > 0 and rank_cells >= 10**8 else None,
            "rank_cells": rank_cells, "traceback_cells": tb_cells,
-           "kernels": f"k_sw_packed<{W}, true> / <{W}, false>",
+           "kernels": fill_kernels(W, b["tune"]["sw_lanes"] if b["tune"] else 1),
            "seed_model": seed,
            "seed_hbm_frac": seed["bytes"] / (fam[0] * 1e-3) / HBM_PEAK if fam[0] > 0 else None,
            "index_bytes": int(ix.n_buckets) * 16 + int(ix.n_pos) * 4,
            "autotune": {"len_split": b["tune"]["len_split"], "walk_kind": b["tune"]["walk_kind"],
-                        "tried": len(b["tune"]["tried"])} if b["tune"] else None}
+                        "sw_lanes": b["tune"]["sw_lanes"], "tried": len(b["tune"]["tried"])} if b["tune"] else None}
     if plain:
         out["plain_equals_batched"] = bool(all(np.array_equal(plain["recs"][k], recs[k])
                                                for k in ("contig", "pos", "rev", "mapq", "score", "nm")))
@@ -523,8 +531,8 @@ def main() -> int:
         "plain_stage_ms": dict(zip(api.Batch.FAMILIES, plain["family_ms"]))
         if plain else None,
         "autotune": {"len_split": batched["tune"]["len_split"], "dir_budget": batched["tune"]["dir_budget"],
-                     "walk_kind": batched["tune"]["walk_kind"],
-                     "tried": [(t["len_split"], t["dir_budget"] >> 20, t["walk_kind"], round(t["ms"], 2))
+                     "walk_kind": batched["tune"]["walk_kind"], "sw_lanes": batched["tune"]["sw_lanes"],
+                     "tried": [(t["len_split"], t["dir_budget"] >> 20, t["walk_kind"], t["sw_lanes"], round(t["ms"], 2))
                                for t in batched["tune"]["tried"]]}
         if batched["tune"] else None,
         "batched_stats": {k: int(getattr(batched["stats"], k)) for k in

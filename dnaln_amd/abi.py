@@ -68,13 +68,13 @@ class MapStats(C.Structure):
 class BatchConfig(C.Structure):
     _fields_ = [("mode", i32), ("chunk_reads", u32), ("max_reads", u64), ("max_bases", u64),
                 ("anchors_per_read", u32), ("tune", i32), ("len_split", i32 * 2), ("dir_budget", u64),
-                ("walk_kind", i32), ("graph", i32)]
+                ("walk_kind", i32), ("graph", i32), ("sw_lanes", i32), ("reserved", i32)]
 
 
 class TuneReport(C.Structure):
     _fields_ = [("len_split", i32 * 2), ("dir_budget", u64), ("walk_kind", i32), ("n_tried", i32), ("chosen", i32),
-                ("reserved", i32), ("ms", dbl * 16), ("cand_split", (i32 * 2) * 16), ("cand_budget", u64 * 16),
-                ("cand_walk", i32 * 16)]
+                ("sw_lanes", i32), ("ms", dbl * 16), ("cand_split", (i32 * 2) * 16), ("cand_budget", u64 * 16),
+                ("cand_walk", i32 * 16), ("cand_lanes", i32 * 16)]
 
 
 CAND_DTYPE = np.dtype([("rev", "<u4"), ("contig", "<i4"), ("diag", "<i8"), ("chain_score", "<i4"), ("reserved", "<i4")])
@@ -85,7 +85,7 @@ RECORD_DTYPE = np.dtype([("contig", "<i4"), ("rev", "u1"), ("mapq", "u1"), ("res
                          ("score", "<i4"), ("nm", "<i4"), ("n_cand", "<i4"), ("cigar_len", "<u4"),
                          ("cigar_off", "<u8")])
 assert CAND_DTYPE.itemsize == 24 and TASK_DTYPE.itemsize == 24 and RESULT_DTYPE.itemsize == 40 and RECORD_DTYPE.itemsize == 40
-assert C.sizeof(MapParams) == 80 and C.sizeof(BatchConfig) == 56 and C.sizeof(MapStats) == 112
+assert C.sizeof(MapParams) == 80 and C.sizeof(BatchConfig) == 64 and C.sizeof(MapStats) == 112
 
 # Every symbol include/dnaln_hip.h declares, with (restype, argtypes).
 HIP_SYMBOLS = {

@@ -331,9 +331,9 @@ class Context:
 
     def batch(self, mode: int = abi.MODE_PLAIN, chunk_reads: int = 65536, max_reads: int = 0, max_bases: int = 0,
               tune: bool = False, len_split: tuple[int, int] = (0, 0), dir_budget: int = 0,
-              walk_kind: int = 0, graph: int = 0, anchors_per_read: int = 0) -> "Batch":
+              walk_kind: int = 0, graph: int = 0, anchors_per_read: int = 0, sw_lanes: int = 0) -> "Batch":
         return Batch(self, mode, chunk_reads, max_reads, max_bases, tune, len_split, dir_budget, walk_kind, graph,
-                     anchors_per_read)
+                     anchors_per_read, sw_lanes)
 
 
 class Batch:
@@ -341,12 +341,15 @@ class Batch:
 
     def __init__(self, ctx: Context, mode: int, chunk_reads: int, max_reads: int, max_bases: int, tune: bool,
                  len_split: tuple[int, int] = (0, 0), dir_budget: int = 0, walk_kind: int = 0, graph: int = 0,
-                 anchors_per_read: int = 0):
+                 anchors_per_read: int = 0, sw_lanes: int = 0):
         """graph (batched form): 0 the host-synchronized form (default); 2 the device-count kernel
         sequence launched one by one, 1 it captured as a HIP graph per plan (both refused by the
         device build; the CPU emulation runs them).
         anchors_per_read (batched form): first anchor capacity of the global path per read (0:
-        estimated; a run that needs more grows it and runs again)."""
+        estimated; a run that needs more grows it and runs again).
+        sw_lanes: lanes per pair of alignments in the packed SW fill (1; 2 for bw 32 or 4 for
+        bw 64 splits the band over that many lanes; 0 the default: the split for the batched
+        form where the band width has one)."""
         self.ctx = ctx
         cfg = abi.BatchConfig()
         _lib.hip().dnaln_hip_default_batch_config(C.byref(cfg))
@@ -355,6 +358,7 @@ class Batch:
         cfg.dir_budget = dir_budget
         cfg.walk_kind = walk_kind
         cfg.graph = graph
+        cfg.sw_lanes = sw_lanes
         if anchors_per_read:
             cfg.anchors_per_read = anchors_per_read
         self.cfg = cfg
@@ -413,9 +417,9 @@ class Batch:
         r = abi.TuneReport()
         _check(_lib.hip().dnaln_hip_batch_tune(self.b, C.byref(r)), "dnaln_hip_batch_tune")
         tried = [{"len_split": (r.cand_split[i][0], r.cand_split[i][1]), "dir_budget": int(r.cand_budget[i]),
-                  "walk_kind": r.cand_walk[i], "ms": r.ms[i]} for i in range(r.n_tried)]
+                  "walk_kind": r.cand_walk[i], "sw_lanes": r.cand_lanes[i], "ms": r.ms[i]} for i in range(r.n_tried)]
         return {"len_split": (r.len_split[0], r.len_split[1]), "dir_budget": int(r.dir_budget),
-                "walk_kind": r.walk_kind, "chosen": r.chosen, "tried": tried}
+                "walk_kind": r.walk_kind, "sw_lanes": r.sw_lanes, "chosen": r.chosen, "tried": tried}
 
     def close(self) -> None:
         if self.b:

@@ -26,6 +26,7 @@
 #include "seed_chain.hpp"
 #include "sw_generic.hpp"
 #include "sw_packed.hpp"
+#include "sw_split.hpp"
 #include "fused_seed_chain.hpp"
 #include "chain_wave.hpp"
 #include "index_build.hpp"
@@ -142,11 +143,12 @@ struct SwWork {
   std::vector<uint64_t> woff_h;
   uint64_t dir_budget = 1ull << 30;  // bytes of direction words per launch
   int walk_kind = 0;                 // launch_walk_bw
+  int lanes = 1;                     // lanes per pair of the packed fill (split_available)
 };
 
 // Layouts shared with other languages' bindings.
 static_assert(sizeof(dnaln_map_params) == 80, "dnaln_map_params layout");
-static_assert(sizeof(dnaln_batch_config) == 56, "dnaln_batch_config layout");
+static_assert(sizeof(dnaln_batch_config) == 64, "dnaln_batch_config layout");
 static_assert(sizeof(dnaln_map_stats) == 112, "dnaln_map_stats layout");
 static_assert(sizeof(dnaln_record) == 40 && sizeof(dnaln_result) == 40, "record/result layout");
 
@@ -384,14 +386,42 @@ __global__ __launch_bounds__(kSwBlock) DN_WAVES_PER_EU(sw_min_waves(W, TB)) void
   }
 }
 
-// Traceback with one thread per task (walk_task).
-template <int W>
+// The packed fill split over P lanes per pair (sw_split.hpp): fill group g (tasks 128g ..
+// 128g + 127, one direction slot) runs as waves gP .. gP + P-1; wave j holds pairs
+// j*64/P .. (j+1)*64/P - 1, lane l segment l % P of pair j*64/P + l / P. Each wave keeps its
+// lanes' best rows in LDS (S columns x 64 lanes x {A, B}). Two waves per SIMD: at S = 33 the
+// traceback fill takes 224 registers; the score-only fill held to three waves spills 17.
+template <int W, int P, bool TB>
+__global__ __launch_bounds__(kSwBlock) DN_WAVES_PER_EU(2) void k_sw_split(SwArgs a) {
+  constexpr int S = seg_width(W, P);
+  const int lane = threadIdx.x & 63;
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint32_t grp = wave / P, j = wave % P;
+  const uint64_t base = (uint64_t)grp * 128;
+  if (base >= a.n) return;
+  const uint32_t ka = (uint32_t)base + j * (64 / P) + lane / P, kb = ka + 64;
+  const bool va = ka < a.n, vb = kb < a.n;
+  const uint32_t ida = va ? (a.order ? a.order[ka] : ka) : 0u;
+  const uint32_t idb = vb ? (a.order ? a.order[kb] : kb) : 0u;
+  int ql = max(va ? a.tasks[ida].qlen : 0, vb ? a.tasks[idb].qlen : 0);
+  ql = wave_max(ql);
+  uint64_t* slot = TB ? dir_slot(a, grp) : nullptr;
+  uint16_t* bh = nullptr;
+  if (TB) {
+    DN_DYN_LDS(uint16_t, bh_rows);
+    bh = bh_rows + (size_t)(threadIdx.x >> 6) * S * 128 + 2 * lane;
+  }
+  sw_split<W, P, TB>(a, ida, idb, va, vb, ql, slot, (int)j, lane, bh, 64);
+}
+
+// Traceback with one thread per task (walk_task), fills with P lanes per pair.
+template <int W, int P = 1>
 __global__ __launch_bounds__(256) void k_walk(SwArgs a) {
   __shared__ uint32_t seqbuf[kWalkSeqDwords * 256];
   const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= a.n) return;
   WalkSeq seq(seqbuf + threadIdx.x, 256);
-  walk_task<W>(a, k, seq);
+  walk_task<W, P>(a, k, seq);
 }
 
 // Traceback of the 128 tasks of one fill wave by one block, through an LDS window of `rows`
@@ -595,6 +625,49 @@ bool launch_packed_bw(int bw, const SwArgs& a, unsigned grid, hipStream_t st) {
   }
 }
 
+// True when band half-width bw has a fill split over `lanes` lanes per pair.
+constexpr bool split_available(int bw, int lanes) {
+#define DN_CASE(B, P) (bw == B && lanes == P) ||
+  return DN_SPLIT_LIST(DN_CASE) false;
+#undef DN_CASE
+}
+
+// The split fill: groups = 128-task fill groups of the launch.
+template <bool TB>
+bool launch_split_bw(int bw, int lanes, const SwArgs& a, unsigned groups, hipStream_t st) {
+#define DN_CASE(B, P)                                                                                       \
+  if (bw == B && lanes == P) {                                                                              \
+    constexpr int kW = 2 * B + 1;                                                                           \
+    hipLaunchKernelGGL((k_sw_split<kW, P, TB>), dim3((groups * P + kSwBlock / 64 - 1) / (kSwBlock / 64)),   \
+                       dim3(kSwBlock), TB ? (size_t)seg_width(kW, P) * 256 * (kSwBlock / 64) : 0, st, a);   \
+    return true;                                                                                            \
+  }
+  DN_SPLIT_LIST(DN_CASE)
+#undef DN_CASE
+  return false;
+}
+
+// Lanes per pair of the packed fill when the batch config leaves it open: the split where the
+// band width has one, for the batched form.
+int default_sw_lanes(int bw, int mode) {
+  if (mode != DNALN_MODE_BATCHED) return 1;
+  for (const int lanes : {2, 4})
+    if (split_available(bw, lanes)) return lanes;
+  return 1;
+}
+
+// The walk after a split fill (one thread per task).
+bool launch_walk_split(int bw, int lanes, const SwArgs& a, hipStream_t st) {
+#define DN_CASE(B, P)                                                                                  \
+  if (bw == B && lanes == P) {                                                                         \
+    hipLaunchKernelGGL((k_walk<2 * B + 1, P>), dim3((a.n + 255) / 256), dim3(256), 0, st, a);          \
+    return true;                                                                                       \
+  }
+  DN_SPLIT_LIST(DN_CASE)
+#undef DN_CASE
+  return false;
+}
+
 // Rows per LDS window of the walk: 16 KiB of flag words, at least 4 rows.
 int walk_rows(int W) { return std::max(4, 16384 / (((W + 7) / 8) * 512)); }
 
@@ -696,12 +769,14 @@ __global__ void k_len_keys(const SwTask* tasks, uint32_t n, uint32_t* keys, uint
   ids[i] = i;
 }
 
-// Slot words of every fill wave, from its longest query (the first of its 128 tasks, which are
-// sorted longest first); entry `waves` is 0 so that the exclusive scan ends with the total.
-__global__ void k_wave_words(const uint32_t* sorted_len, uint32_t waves, uint64_t row_words, uint64_t* words) {
+// Slot words of every fill group, from its longest query (the first of its 128 tasks, which are
+// sorted longest first) and the fill's lanes per pair; entry `waves` is 0 so that the exclusive
+// scan ends with the total.
+__global__ void k_wave_words(const uint32_t* sorted_len, uint32_t waves, uint64_t row_words, int lanes,
+                             uint64_t* words) {
   const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
   if (w > waves) return;
-  words[w] = w < waves ? fill_rows((int)sorted_len[(uint64_t)w * 128]) * row_words : 0;
+  words[w] = w < waves ? slot_rows((int)sorted_len[(uint64_t)w * 128], lanes) * row_words : 0;
 }
 
 int scan_u64(DevBuf& tmp, const uint64_t* in, uint64_t* out, uint32_t n, hipStream_t st) {
@@ -715,7 +790,9 @@ int scan_u64(DevBuf& tmp, const uint64_t* in, uint64_t* out, uint32_t n, hipStre
 // Fills (and with traceback walks) n tasks. Query lengths lie in [qmin, qmax]. When they differ,
 // the tasks are sorted longest first on the device, so each wave of 128 runs to nearly its own
 // tasks' length, and each wave's direction slot is sized by its longest query. Launches are cut
-// at wave boundaries so that one launch's direction words fit wk.dir_budget.
+// at wave boundaries so that one launch's direction words fit wk.dir_budget. With wk.lanes > 1
+// (where the band width has that split) each group of 128 tasks runs as wk.lanes waves of the
+// split fill, walked one thread per task.
 int run_sw(const ScoreP& sp, const SwTask* tasks, uint32_t n, int qmin, int qmax, bool traceback, SwResult* res,
            uint32_t* cig, int cap, SwWork& wk, hipStream_t st, uint64_t* launches, SpanSink* spans = nullptr,
            int fill_family = 0) {
@@ -728,8 +805,12 @@ int run_sw(const ScoreP& sp, const SwTask* tasks, uint32_t n, int qmin, int qmax
     a.sp = sp;
     score_table(sp, a.tab0, a.tab1);
     a.cigar_cap = cap;
-    const uint64_t row_words = (uint64_t)((W + 7) / 8) * 64;
-    const uint64_t slot_words = fill_rows(qmax) * row_words;
+    // The score-only fill stays unsplit up to W = 65, where it keeps two waves per SIMD without
+    // spills and runs faster than split (4.66 against 4.83 ms per step on the 4M x 250 bp repeat
+    // workload); wider, unsplit it holds one wave.
+    const int lanes = split_available(sp.bw, wk.lanes) && (traceback || W > 65) ? wk.lanes : 1;
+    const uint64_t row_words = lanes > 1 ? slot_row_words(W, lanes) : (uint64_t)((W + 7) / 8) * 64;
+    const uint64_t slot_words = slot_rows(qmax, lanes) * row_words;
     const uint32_t waves = (n + 127) / 128;
     const uint32_t* order = nullptr;
     const uint64_t* woff = nullptr;
@@ -754,7 +835,7 @@ int run_sw(const ScoreP& sp, const SwTask* tasks, uint32_t n, int qmin, int qmax
         uint64_t* words = wk.woff.as<uint64_t>();
         uint64_t* offs = words + (waves + 1);
         hipLaunchKernelGGL(k_wave_words, dim3((waves + 256) / 256), dim3(256), 0, st, wk.keys2.as<uint32_t>(), waves,
-                           row_words, words);
+                           row_words, lanes, words);
         DN_HIP(hipGetLastError());
         DN_TRY(scan_u64(wk.tmp, words, offs, waves + 1, st));
         wk.woff_h.resize(waves + 1);
@@ -805,14 +886,19 @@ int run_sw(const ScoreP& sp, const SwTask* tasks, uint32_t n, int qmin, int qmax
       const unsigned grid = (lw + kSwBlock / 64 - 1) / (kSwBlock / 64);
       size_t si = 0;
       if (spans) DN_TRY(spans->begin(fill_family, si));
-      const bool ok = traceback ? launch_packed_bw<true>(sp.bw, a, grid, st) : launch_packed_bw<false>(sp.bw, a, grid, st);
+      bool ok;
+      if (lanes > 1)
+        ok = traceback ? launch_split_bw<true>(sp.bw, lanes, a, lw, st) : launch_split_bw<false>(sp.bw, lanes, a, lw, st);
+      else
+        ok = traceback ? launch_packed_bw<true>(sp.bw, a, grid, st) : launch_packed_bw<false>(sp.bw, a, grid, st);
       if (!ok) return fail(DNALN_E_INTERNAL, "no packed kernel for this band width");
       DN_HIP(hipGetLastError());
       if (spans) DN_TRY(spans->end(si));
       if (launches) ++launches[fill_family];
       if (traceback) {
         if (spans) DN_TRY(spans->begin(4, si));
-        if (!launch_walk_bw(sp.bw, a, wk.walk_kind, st)) return fail(DNALN_E_INTERNAL, "no walk kernel for this band width");
+        const bool wok = lanes > 1 ? launch_walk_split(sp.bw, lanes, a, st) : launch_walk_bw(sp.bw, a, wk.walk_kind, st);
+        if (!wok) return fail(DNALN_E_INTERNAL, "no walk kernel for this band width");
         DN_HIP(hipGetLastError());
         if (spans) DN_TRY(spans->end(si));
         if (launches) ++launches[4];
@@ -2461,6 +2547,9 @@ int dnaln_hip_batch_create(dnaln_hip_ctx* c, const dnaln_batch_config* cfg, dnal
     DN_HIP(hipSetDevice(c->device));
     b->sw.dir_budget = b->cfg.dir_budget ? std::max<uint64_t>(b->cfg.dir_budget, 1 << 20) : default_dir_budget();
     if (b->cfg.walk_kind < 0 || b->cfg.walk_kind > 1) return fail(DNALN_E_INVALID, "walk_kind must be 0 or 1");
+    if (b->cfg.sw_lanes < 0 || (b->cfg.sw_lanes > 1 && !split_available(c->p.sc.bw, b->cfg.sw_lanes)))
+      return fail(DNALN_E_INVALID, "sw_lanes = " + std::to_string(b->cfg.sw_lanes) + ": no such split for bw " +
+                                       std::to_string(c->p.sc.bw));
     if (b->cfg.graph < 0 || b->cfg.graph > 2) return fail(DNALN_E_INVALID, "graph must be 0, 1 or 2");
 #if !defined(DN_GRAPH_CAPTURE)
     // The device-count sequence (graph = 2) and its captured form (graph = 1) faulted the GPU on
@@ -2473,6 +2562,7 @@ int dnaln_hip_batch_create(dnaln_hip_ctx* c, const dnaln_batch_config* cfg, dnal
                                        ": the device-count forms of the batched pipeline are switched off");
 #endif
     b->sw.walk_kind = b->cfg.walk_kind;
+    b->sw.lanes = b->cfg.sw_lanes ? b->cfg.sw_lanes : default_sw_lanes(c->p.sc.bw, b->cfg.mode);
     DN_HIP(hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking));
     DN_HIP(hipEventCreate(&b->done));
     DN_TRY(b->err.reserve(64));
@@ -2614,14 +2704,16 @@ int dnaln_hip_batch_tune(dnaln_hip_batch* b, dnaln_tune_report* rep) {
     b->tuned = true;
     dnaln_tune_report r{};
     // One candidate: apply, run twice (the first run grows the workspace), keep the second time.
-    auto trial = [&](int s0, int s1, uint64_t budget, int walk, bool& skipped) -> int {
+    auto trial = [&](int s0, int s1, uint64_t budget, int walk, int lanes, bool& skipped) -> int {
       set_splits(b, s0, s1);
       b->sw.dir_budget = budget;
       b->sw.walk_kind = walk;
+      b->sw.lanes = lanes;
       const int e0 = b->n_short > 0 ? b->split[0] : 0, e1 = b->n_mid > b->n_short ? b->split[1] : 0;
       skipped = false;
       for (int x = 0; x < r.n_tried; ++x)
-        if (r.cand_split[x][0] == e0 && r.cand_split[x][1] == e1 && r.cand_budget[x] == budget && r.cand_walk[x] == walk)
+        if (r.cand_split[x][0] == e0 && r.cand_split[x][1] == e1 && r.cand_budget[x] == budget &&
+            r.cand_walk[x] == walk && r.cand_lanes[x] == lanes)
           skipped = true;
       if (skipped || r.n_tried >= 16) {
         skipped = true;
@@ -2639,38 +2731,56 @@ int dnaln_hip_batch_tune(dnaln_hip_batch* b, dnaln_tune_report* rep) {
       r.cand_split[t][1] = e1;
       r.cand_budget[t] = budget;
       r.cand_walk[t] = walk;
+      r.cand_lanes[t] = lanes;
       return DNALN_OK;
     };
+    const int bw = b->ctx->p.sc.bw;
+    const int lanes0 = b->sw.lanes;
+    int split = 1;  // the band width's split fill, if any
+    for (const int l : {2, 4})
+      if (split_available(bw, l)) split = l;
     const uint64_t big = default_dir_budget();
     std::vector<uint64_t> budgets{big};
     if (big != (1ull << 30)) budgets.push_back(1ull << 30);
     bool skipped = false;
-    // Stage 1: read-length bucketing x direction budget, with the per-thread walk.
+    // Stage 1: read-length bucketing x direction budget, with the per-thread walk and the
+    // configured lanes per pair.
     for (const auto& sp : {std::make_pair(0, 0), std::make_pair(0, -1), std::make_pair(-1, 0), std::make_pair(-1, -1)})
-      for (const uint64_t bud : budgets) DN_TRY(trial(sp.first, sp.second, bud, 0, skipped));
+      for (const uint64_t bud : budgets) DN_TRY(trial(sp.first, sp.second, bud, 0, lanes0, skipped));
     auto best_of = [&]() {
       int bi = 0;
       for (int x = 1; x < r.n_tried; ++x)
         if (r.ms[x] < r.ms[bi]) bi = x;
       return bi;
     };
-    // Stage 2: the other walk on the best setting.
+    // Stage 2: the SW fill's other lanes per pair (where the band width has a split), then the
+    // other walk (which applies to the unsplit fill) on the best setting.
     int bi = best_of();
-    DN_TRY(trial(r.cand_split[bi][0] > 0 ? r.cand_split[bi][0] : -1, r.cand_split[bi][1] > 0 ? r.cand_split[bi][1] : -1,
-                 r.cand_budget[bi], 1, skipped));
-    bi = best_of();
+    auto s0_of = [&](int x) { return r.cand_split[x][0] > 0 ? r.cand_split[x][0] : -1; };
+    auto s1_of = [&](int x) { return r.cand_split[x][1] > 0 ? r.cand_split[x][1] : -1; };
+    if (split > 1) {
+      DN_TRY(trial(s0_of(bi), s1_of(bi), r.cand_budget[bi], 0, lanes0 > 1 ? 1 : split, skipped));
+      bi = best_of();
+    }
+    if (r.cand_lanes[bi] == 1) {
+      DN_TRY(trial(s0_of(bi), s1_of(bi), r.cand_budget[bi], 1, 1, skipped));
+      bi = best_of();
+    }
     r.chosen = bi;
     r.len_split[0] = r.cand_split[bi][0];
     r.len_split[1] = r.cand_split[bi][1];
     r.dir_budget = r.cand_budget[bi];
     r.walk_kind = r.cand_walk[bi];
+    r.sw_lanes = r.cand_lanes[bi];
     b->cfg.len_split[0] = r.len_split[0] > 0 ? r.len_split[0] : -1;
     b->cfg.len_split[1] = r.len_split[1] > 0 ? r.len_split[1] : -1;
     b->cfg.dir_budget = r.dir_budget;
     b->cfg.walk_kind = r.walk_kind;
+    b->cfg.sw_lanes = r.sw_lanes;
     set_splits(b, b->cfg.len_split[0], b->cfg.len_split[1]);
     b->sw.dir_budget = r.dir_budget;
     b->sw.walk_kind = r.walk_kind;
+    b->sw.lanes = r.sw_lanes;
     if (rep) *rep = r;
     return DNALN_OK;
   });
@@ -2880,8 +2990,11 @@ int dnaln_hip_batch_reserve(dnaln_hip_batch* b, uint64_t n, uint64_t bases, int3
     DN_TRY(b->cigoff.reserve(n * 8 + 8));
     // directions of one launch (the plain form runs its tasks in read order, fixed slots)
     const int W = 2 * c->p.sc.bw + 1;
-    const uint64_t dir = std::min<uint64_t>(b->sw.dir_budget, ((n + 127) / 128 + 1) * (uint64_t)std::max(maxlen, 1) *
-                                                                  ((W + 7) / 8) * 512);
+    uint64_t slot = fill_rows(std::max(maxlen, 1)) * ((W + 7) / 8) * 512;
+    for (const int lanes : {2, 4})  // the split fill's slots, which the autotuner may choose
+      if (split_available(c->p.sc.bw, lanes))
+        slot = std::max<uint64_t>(slot, slot_rows(std::max(maxlen, 1), lanes) * slot_row_words(W, lanes) * 8);
+    const uint64_t dir = std::min<uint64_t>(b->sw.dir_budget, ((n + 127) / 128 + 1) * slot);
     DN_TRY(b->sw.dir.reserve(dir));
     // scan temporaries for the largest counts
     size_t t1 = 0;

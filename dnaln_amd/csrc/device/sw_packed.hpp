@@ -135,6 +135,19 @@ DN_HD uint32_t align4(uint32_t lo, uint32_t hi, uint32_t sh) {
 // the wave's longest query rounded up to a multiple of 4 (rows past a query are never read).
 DN_HD uint64_t fill_rows(int qmax) { return (uint64_t)(((qmax > 1 ? qmax : 1) + 3) & ~3); }
 
+// A band of W columns split over P lanes (sw_split.hpp; P = 1 is sw_pair): lane `seg` of a pair
+// holds columns seg*S .. seg*S + S-1 and runs row r at step r + seg. A slot holds, per step, the
+// flag words of its 128 tasks' segments: word (step s, group g of a segment, half h, segment seg,
+// pair p) at u32 index ((s * GS + g) * 2 + h) * 64P + seg * 64 + p, which for P = 1 is the
+// sw_pair layout. The 64 pairs of one segment lie side by side, so the walks of a wave's tasks,
+// which move through nearby cells, read nearby words. A task's pair p (its index mod 64) runs in
+// fill wave p / (64/P) of its group, lanes (p mod 64/P) * P + seg.
+DN_HD constexpr int seg_width(int W, int P) { return (W + P - 1) / P; }
+DN_HD constexpr int seg_groups(int W, int P) { return (seg_width(W, P) + 7) / 8; }
+// u64 direction words per step of a slot, and steps per slot.
+DN_HD constexpr uint64_t slot_row_words(int W, int P) { return (uint64_t)seg_groups(W, P) * (uint64_t)P * 64; }
+DN_HD uint64_t slot_rows(int qmax, int P) { return fill_rows(qmax + P - 1); }
+
 // The sequence streams of one lane's two alignments (h = 0: A, 1: B), read one block of four
 // rows ahead. For block r0 (a multiple of 4) a query stream needs the codes of rows r0..r0+3
 // (q[r0..r0+3], or q[qlen-4-r0 .. qlen-1-r0] reversed) and a target stream the codes of
@@ -143,18 +156,20 @@ DN_HD uint64_t fill_rows(int qmax) { return (uint64_t)(((qmax > 1 ? qmax : 1) + 
 // waits on it; the byte offset within the 8 bytes is the same for every block of a stream.
 // Offsets are clamped to the range where a block with any valid row or position reads its
 // bytes: loads stay within a few bytes of the sequence, and a clamped block is fully masked.
-template <int W>
+// LAG (sw_split.hpp): rows below row0 also read as N, and the target limits may be negative.
+template <int W, bool LAG = false>
 struct BlockStreams {
   uint64_t q[2], t[2];
   int32_t qlen[2];
-  uint32_t tlim[2];                             // tlo | thi << 16
+  int32_t row0;
+  uint32_t tlim[2];                             // tlo | thi << 16, as int16
   uint32_t qlo[2], qhi[2], tlo_w[2], thi_w[2];  // raw bytes of the next block
   uint32_t sh;  // per byte (query A, B, target A, B): bits 1:0 the byte offset; bit 7 of the
                 // query bytes: reverse-complement
 
   DN_HD bool rev(int h) const { return (sh >> (8 * h + 7)) & 1; }
-  DN_HD int tlo(int h) const { return (int)(tlim[h] & 0xffff); }
-  DN_HD int thi(int h) const { return (int)(tlim[h] >> 16); }
+  DN_HD int tlo(int h) const { return LAG ? (int)(int16_t)(tlim[h] & 0xffff) : (int)(tlim[h] & 0xffff); }
+  DN_HD int thi(int h) const { return LAG ? (int)(int16_t)(tlim[h] >> 16) : (int)(tlim[h] >> 16); }
   DN_HD int qoff(int h, int r0) const {
     const int o = rev(h) ? qlen[h] - 4 - r0 : r0;
     const int lo = -3, hi = qlen[h] - 1;
@@ -172,16 +187,17 @@ struct BlockStreams {
       load8((t[h] + (uint64_t)(int64_t)toff(h, r0)) & ~(uint64_t)3, tlo_w[h], thi_w[h]);
     }
   }
-  // tlo and thi lie in [0, 32768) (thi <= qlen + 2*bw).
-  DN_HD void init(const SwTask& a, const SwTask& b) {
+  // tlo and thi lie in [0, 32768) (thi <= qlen + 2*bw); with LAG in (-32768, 32768).
+  DN_HD void init(const SwTask& a, const SwTask& b, int first_row = 0) {
     const SwTask* tk[2] = {&a, &b};
     sh = 0;
+    row0 = first_row;
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       q[h] = tk[h]->q;
       t[h] = tk[h]->t;
       qlen[h] = tk[h]->qlen;
-      tlim[h] = (uint32_t)tk[h]->tlo | (uint32_t)tk[h]->thi << 16;
+      tlim[h] = (uint32_t)(uint16_t)tk[h]->tlo | (uint32_t)(uint16_t)tk[h]->thi << 16;
       const bool r = tk[h]->flags & 1;
       const uint32_t sq = (uint32_t)(q[h] + (r ? (uint64_t)(int64_t)(qlen[h] - 4) : 0)) & 3;
       const uint32_t st = (uint32_t)(t[h] + W) & 3;
@@ -201,7 +217,8 @@ struct BlockStreams {
       const uint32_t tw = align4(tlo_w[h], thi_w[h], sh >> (16 + 8 * h));
       // Reversed queries read backwards and complement (c ^ 3 swaps A/T and C/G; N = 4 becomes 7).
       const uint32_t c = rev(h) ? perm_b32(0, qw, 0x00010203u) ^ 0x03030303u : qw;
-      qc[h] = bfi(bytes_from(qlen[h] - r0), 0xF0F0F0F0u, perm_b32(0xF0F0F0F0u, 0x03020100u, c));
+      const uint32_t qbad = bytes_from(qlen[h] - r0) | (LAG ? bytes_below(row0 - r0) : 0u);
+      qc[h] = bfi(qbad, 0xF0F0F0F0u, perm_b32(0xF0F0F0F0u, 0x03020100u, c));
       const int j0 = r0 + W;
       const uint32_t tbad = bytes_below(tlo(h) - j0) | bytes_from(thi(h) - j0);
       tt[h] = bfi(tbad, 0x30303030u, perm_b32(0x30303030u, 0x03020100u, tw));
@@ -310,14 +327,15 @@ DN_HD void walk_flush(Walk& w, uint32_t* out, int cap) {
   w.pend_len = 0;
 }
 
-// One traceback step. plane holds the flag words of the walk's own half (its lane and task half
-// of the fill wave) from row row0 on: the word of row i, group g at ((i - row0) * G + g) * 128,
-// bytes extF, extE, E and not-diagonal, bit d & 7 in each. A step reads only row w.i, and none
-// when w.i < 0.
-template <int W>
+// One traceback step. plane holds the flag words of the walk's own task (its pair and half of
+// the fill group) from step row0 on: column d lies in segment seg = d / S at offset c = d % S
+// (S = seg_width(W, P)); the word of row i, group c / 8 at ((i + seg - row0) * GS + c / 8) * 128P
+// + seg * 64, bytes extF, extE, E and not-diagonal, bit c & 7 in each (for P = 1: row i, group
+// d / 8 at ((i - row0) * G + d / 8) * 128). A step reads only row w.i, and none when w.i < 0.
+template <int W, int P = 1>
 DN_HD void walk_step(Walk& w, const SwTask& tk, const uint32_t* plane, const ScoreP& sp, uint32_t* out, int cap,
                      WalkSeq& seq, int row0 = 0) {
-  constexpr int G = (W + 7) / 8;
+  constexpr int S = seg_width(W, P), GS = seg_groups(W, P);
   const int oe = sp.gap_open + sp.gap_ext, ge = sp.gap_ext;
   if (w.state == 0 && (w.i < 0 || w.v <= 0)) {
     if (w.run) walk_push(w, 0, w.run, out, cap);
@@ -332,9 +350,10 @@ DN_HD void walk_step(Walk& w, const SwTask& tk, const uint32_t* plane, const Sco
     return;
   }
   // The flag word and, speculatively, the two bases of the cell (mostly from registers).
-  const uint32_t word = plane[(uint64_t)((w.i - row0) * G + (w.d >> 3)) * 128];
+  const int seg = P == 1 ? 0 : w.d / S, col = w.d - seg * S;
+  const uint32_t word = plane[(uint64_t)((w.i + seg - row0) * GS + (col >> 3)) * (128 * P) + seg * 64];
   const uint8_t a = seq.qb(tk, w.i), c = seq.tb(tk, w.i + w.d);
-  const int b = w.d & 7;
+  const int b = col & 7;
   if (w.state == 0) {
     if (!((word >> (24 + b)) & 1)) {  // diagonal
       w.v -= sub_score(sp, a, c);
@@ -608,10 +627,10 @@ DN_HD void sw_pair(const SwArgs& a, uint32_t ida, uint32_t idb, bool va, bool vb
   }
 }
 
-// Traceback of one alignment from the end cell sw_pair<W, true> left in its result. k is the
-// launch-relative position of the task (the fill's wave k / 128 holds it, in lane k % 64, half
-// (k / 64) % 2): one thread per task reads its flag words straight from the slot.
-template <int W>
+// Traceback of one alignment from the end cell sw_pair<W, true> (or sw_split with P lanes per
+// pair) left in its result. k is the launch-relative position of the task (fill group k / 128,
+// pair k % 64, half (k / 64) % 2): one thread per task reads its flag words straight from the slot.
+template <int W, int P = 1>
 DN_HD void walk_task(const SwArgs& a, uint32_t k, WalkSeq& seq) {
   const uint32_t id = a.order ? a.order[k] : k;
   SwResult& r = a.res[id];
@@ -621,12 +640,12 @@ DN_HD void walk_task(const SwArgs& a, uint32_t k, WalkSeq& seq) {
     return;
   }
   const SwTask tk = a.tasks[id];
-  const uint32_t wave = k >> 7, lane = k & 63, half = (k >> 6) & 1;
-  const uint32_t* plane = reinterpret_cast<const uint32_t*>(dir_slot(a, wave)) + half * 64 + lane;
+  const uint32_t wave = k >> 7, pair = k & 63, half = (k >> 6) & 1;
+  const uint32_t* plane = reinterpret_cast<const uint32_t*>(dir_slot(a, wave)) + half * 64 * P + pair;
   uint32_t* out = a.cigar + (uint64_t)id * (uint64_t)a.cigar_cap;
   Walk w = walk_init(i_end, d_end, score);
   seq.q.at = seq.t.at = ~0ull;
-  while (!w.done) walk_step<W>(w, tk, plane, a.sp, out, a.cigar_cap, seq);
+  while (!w.done) walk_step<W, P>(w, tk, plane, a.sp, out, a.cigar_cap, seq);
   walk_finish(w, i_end, d_end, score, a.cigar_cap, r);
 }
 

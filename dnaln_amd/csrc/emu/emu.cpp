@@ -3,7 +3,9 @@
 // Runs sw_pair / sw_generic_one exactly as a wave64 would, lane by lane, with the same
 // task pairing (lane l holds tasks l and l + 64 of each 128-task group), the same
 // direction-slot layout and the same dispatch rules, so the kernels' integer tricks can be
-// checked against the oracle and the reference vectors without a GPU.
+// checked against the oracle and the reference vectors without a GPU. The split fill
+// (sw_split.hpp) runs its P lanes of a pair phase by phase, in the order the wave's lock step
+// gives them.
 #include <stdint.h>
 #include <string.h>
 
@@ -17,6 +19,7 @@
 #include "device/index_build.hpp"
 #include "device/sw_generic.hpp"
 #include "device/sw_packed.hpp"
+#include "device/sw_split.hpp"
 
 using namespace dn;
 
@@ -116,6 +119,83 @@ void run_packed(const SwArgs& a0) {
   }
 }
 
+// The split fill over P lanes per pair and its walk, one fill group (128 tasks, P waves) at a
+// time. Each wave runs to its own longest query, as on the device.
+template <int W, int P, bool TB>
+void run_split(const SwArgs& a0) {
+  using F = SegFill<W, P, TB>;
+  constexpr int U = F::U, S = F::S;
+  int qall = 1;
+  for (uint32_t k = 0; k < a0.n; ++k) qall = std::max(qall, a0.tasks[k].qlen);
+  const uint64_t slot_words = slot_rows(qall, P) * slot_row_words(W, P);
+  const uint32_t groups = (a0.n + 127) / 128;
+  std::vector<uint64_t> dir(TB ? (size_t)groups * slot_words : 1, 0xdeadbeefdeadbeefULL);
+  SwArgs a = a0;
+  a.dir = dir.data();
+  a.dir_slot_words = slot_words;
+  std::vector<F> L(P);
+  std::vector<uint32_t> bh((size_t)P * S);  // each lane's best row: S words of two half-words
+  const pk16 zero = pk_splat(0);
+  for (uint32_t g = 0; g < groups; ++g) {
+    for (int j = 0; j < P; ++j) {
+      int qmax = 0;
+      for (int pl = 0; pl < 64 / P; ++pl) {
+        const uint32_t ka = g * 128 + j * (64 / P) + pl, kb = ka + 64;
+        if (ka < a.n) qmax = std::max(qmax, a.tasks[ka].qlen);
+        if (kb < a.n) qmax = std::max(qmax, a.tasks[kb].qlen);
+      }
+      for (int pl = 0; pl < 64 / P; ++pl) {
+        const uint32_t ka = g * 128 + j * (64 / P) + pl, kb = ka + 64;
+        const bool va = ka < a.n, vb = kb < a.n;
+        for (int seg = 0; seg < P; ++seg)
+          L[seg].init(a, a.tasks[va ? ka : 0], a.tasks[vb ? kb : 0], va, vb, seg,
+                      TB ? a.dir + (uint64_t)g * slot_words : nullptr, j, pl * P + seg,
+                      reinterpret_cast<uint16_t*>(bh.data() + (size_t)seg * S), 1);
+        const int steps = qmax + P - 1;
+        for (int s0 = 0; s0 < steps; s0 += U) {
+          for (F& x : L) x.step_begin();
+          for (int u = 0; u < U; ++u) {
+            // Cell 0 reads the left neighbour's last cell of its previous row (rest has not run
+            // yet this row); the rest reads the right neighbour's cell 0 of this row.
+            for (int seg = 0; seg < P; ++seg)
+              L[seg].cell0(u, (s0 & 3) + u, s0 + u, seg ? L[seg - 1].x_hl : zero, seg ? L[seg - 1].x_fl : zero);
+            for (int seg = 0; seg < P; ++seg)
+              L[seg].rest(u, s0 + u, seg + 1 < P ? L[seg + 1].x_h0 : zero, seg + 1 < P ? L[seg + 1].x_e0 : zero);
+          }
+          for (F& x : L) x.step_end(s0);
+        }
+        uint64_t kmax[2] = {0, 0};
+        for (int seg = 0; seg < P; ++seg) {
+          uint64_t k2[2];
+          L[seg].keys(k2[0], k2[1]);
+          kmax[0] = std::max(kmax[0], k2[0]);
+          kmax[1] = std::max(kmax[1], k2[1]);
+        }
+        if (va) split_result<TB>(a.res[ka], kmax[0]);
+        if (vb) split_result<TB>(a.res[kb], kmax[1]);
+      }
+    }
+  }
+  if (TB)
+    for (uint32_t k = 0; k < a.n; ++k) {
+      uint32_t buf[kWalkSeqDwords];
+      WalkSeq seq(buf, 1);
+      walk_task<W, P>(a, k, seq);
+    }
+}
+
+template <bool TB>
+bool dispatch_split(int bw, int lanes, const SwArgs& a) {
+#define DN_CASE(B, P)                     \
+  if (bw == B && lanes == P) {            \
+    run_split<2 * B + 1, P, TB>(a);       \
+    return true;                          \
+  }
+  DN_SPLIT_LIST(DN_CASE)
+#undef DN_CASE
+  return false;
+}
+
 template <bool TB>
 bool dispatch(int bw, const SwArgs& a) {
   switch (bw) {
@@ -163,6 +243,10 @@ int emu_align_ex(const int* sp6, const uint8_t* pool, const int64_t* q_off, cons
                  const int64_t* t_off, const int32_t* t_len, const uint8_t* rev, const int32_t* tlo, size_t n,
                  int traceback, int32_t* res6, uint32_t* cig, int64_t* cig_off, int32_t* cig_n, size_t cap,
                  int force);
+int emu_align_lanes(const int* sp6, const uint8_t* pool, const int64_t* q_off, const int32_t* q_len,
+                    const int64_t* t_off, const int32_t* t_len, const uint8_t* rev, const int32_t* tlo, size_t n,
+                    int traceback, int32_t* res6, uint32_t* cig, int64_t* cig_off, int32_t* cig_n, size_t cap,
+                    int force, int lanes);
 
 int emu_align(const int* sp6, const uint8_t* pool, const int64_t* q_off, const int32_t* q_len, const int64_t* t_off,
               const int32_t* t_len, size_t n, int traceback, int32_t* res6, uint32_t* cig, int64_t* cig_off,
@@ -177,6 +261,16 @@ int emu_align_ex(const int* sp6, const uint8_t* pool, const int64_t* q_off, cons
                  const int64_t* t_off, const int32_t* t_len, const uint8_t* rev, const int32_t* tlo, size_t n,
                  int traceback, int32_t* res6, uint32_t* cig, int64_t* cig_off, int32_t* cig_n, size_t cap,
                  int force) {
+  return emu_align_lanes(sp6, pool, q_off, q_len, t_off, t_len, rev, tlo, n, traceback, res6, cig, cig_off, cig_n,
+                         cap, force, 1);
+}
+
+// lanes > 1 with force = 2: the packed fill split over that many lanes per pair (fails with -2
+// where the band width has no such form).
+int emu_align_lanes(const int* sp6, const uint8_t* pool, const int64_t* q_off, const int32_t* q_len,
+                    const int64_t* t_off, const int32_t* t_len, const uint8_t* rev, const int32_t* tlo, size_t n,
+                    int traceback, int32_t* res6, uint32_t* cig, int64_t* cig_off, int32_t* cig_n, size_t cap,
+                    int force, int lanes) {
   ScoreP sp{sp6[0], sp6[1], sp6[2], sp6[3], sp6[4], sp6[5]};
   // The packed kernel streams aligned dwords around each sequence: give the pool guard bytes.
   uint64_t pool_len = 0;
@@ -213,7 +307,10 @@ int emu_align_ex(const int* sp6, const uint8_t* pool, const int64_t* q_off, cons
     a.cigar_cap = ccap;
     a.sp = sp;
     score_table(sp, a.tab0, a.tab1);
-    done = traceback ? dispatch<true>(sp.bw, a) : dispatch<false>(sp.bw, a);
+    if (lanes > 1)
+      done = traceback ? dispatch_split<true>(sp.bw, lanes, a) : dispatch_split<false>(sp.bw, lanes, a);
+    else
+      done = traceback ? dispatch<true>(sp.bw, a) : dispatch<false>(sp.bw, a);
   }
   if (!done) {
     if (force == 2) return -2;

@@ -1,5 +1,6 @@
 // serve.cpp — continuous batching of mapping requests on the resident index (dnaln_serve.h).
 #include "dnaln_serve.h"
+#include "device/bw_list.hpp"
 #include "host/c_guard.hpp"
 
 #include <algorithm>
@@ -226,9 +227,20 @@ struct dnaln_server {
     // global path, minimizer lists of the fused kernel, and the traceback directions of one SW
     // launch (4 bytes per group of 8 band cells per row per alignment), bounded by the
     // per-launch direction budget.
-    const uint64_t G = (uint64_t)(2 * bw + 1 + 7) / 8;
+    // Where the fill may split a pair over P lanes, a row holds P segments of ceil(ceil(W/P)/8)
+    // groups, and each group of 128 tasks holds P - 1 more rows.
+    const int W = 2 * bw + 1;
+    uint64_t G = (uint64_t)(W + 7) / 8, lag = 0;
+#define DN_CASE(B, P)                                                                           \
+  if (bw == B) {                                                                                \
+    G = std::max<uint64_t>(G, (uint64_t)(((W + P - 1) / P + 7) / 8) * P);                      \
+    lag = std::max<uint64_t>(lag, P - 1);                                                      \
+  }
+    DN_SPLIT_LIST(DN_CASE)
+#undef DN_CASE
     const uint64_t per_read = 16 + sizeof(dnaln_record) + 64 + (uint64_t)(max_cand + 1) * 96 + 64 * 48 + 64 * 8;
-    const uint64_t dirs = std::min<uint64_t>(bases * 4 * G + (uint64_t)maxlen * G * 512, cfg.dir_budget);
+    const uint64_t dirs = std::min<uint64_t>(
+        bases * 4 * G + ((uint64_t)maxlen + 4 + lag) * G * 512 + (n + 127) / 128 * (lag + 3) * G * 512, cfg.dir_budget);
     // (12 bytes per base: the global path's probe results between its count and emit passes)
     return bases * 13 + 256 + n * per_read + n * (2 * (uint64_t)maxlen + 4) * 4 + dirs;
   }

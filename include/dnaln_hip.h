@@ -237,6 +237,13 @@ typedef struct {
                                plan. 1 and 2 are refused (DNALN_E_INVALID) by the device build:
                                their first run faulted the GPU and the cause is not found (the
                                CPU emulation runs them) */
+  int32_t sw_lanes;         /* lanes per pair of alignments in the packed SW fill: 1 = the band in
+                               one lane's registers; 2 (bw 32) or 4 (bw 64) = the band split over
+                               that many lanes, walked one thread per task (walk_kind is then
+                               ignored); 0 = the default (the split for the batched form where the
+                               band width has one, else 1); a split the band width lacks is
+                               refused (DNALN_E_INVALID) */
+  int32_t reserved;         /* 0 */
 } dnaln_batch_config;
 
 void dnaln_hip_default_batch_config(dnaln_batch_config* c);
@@ -297,11 +304,12 @@ typedef struct {
   int32_t walk_kind;        /* chosen walk */
   int32_t n_tried;
   int32_t chosen;           /* index into ms[] */
-  int32_t reserved;
+  int32_t sw_lanes;         /* chosen lanes per pair of the packed SW fill */
   double ms[16];            /* whole-run time of each candidate, in the order tried */
   int32_t cand_split[16][2];
   uint64_t cand_budget[16];
   int32_t cand_walk[16];
+  int32_t cand_lanes[16];
 } dnaln_tune_report;
 
 int dnaln_hip_batch_tune(struct dnaln_hip_batch* b, dnaln_tune_report* report);

@@ -55,6 +55,65 @@ def test_emulated_kernels_match_reference_vectors(emu, si, force):
         assert np.array_equal(cig, kv[f"s{si}_tb{tb}_cig"])
 
 
+def _run_lanes(emu, sp, pool, qo, ql, to, tl, rev, tlo, tb, lanes):
+    n = len(ql)
+    res = np.zeros((n, 6), np.int32)
+    cap = int(ql.astype(np.int64).sum()) * 2 + 8 * n + 8
+    cig = np.zeros(cap, np.uint32)
+    co = np.zeros(n, np.int64)
+    cn = np.zeros(n, np.int32)
+    v = C.c_void_p
+    rc = emu.emu_align_lanes(v(sp.ctypes.data), v(pool.ctypes.data), v(qo.ctypes.data), v(ql.ctypes.data),
+                             v(to.ctypes.data), v(tl.ctypes.data), v(rev.ctypes.data), v(tlo.ctypes.data),
+                             C.c_size_t(n), tb, v(res.ctypes.data), v(cig.ctypes.data), v(co.ctypes.data),
+                             v(cn.ctypes.data), C.c_size_t(cap), 2, lanes)
+    assert rc == 1, rc  # the split fill ran
+    return res, cig[:int(cn.sum())], cn
+
+
+@pytest.mark.parametrize("si,lanes", [(4, 2), (5, 4)])
+def test_emulated_split_fill_matches_reference_vectors(emu, si, lanes):
+    """The fill split over 2 (bw 32) or 4 (bw 64) lanes per pair of alignments, lanes in lock
+    step with their neighbour exchanges, against the reference binary's kernel vectors."""
+    kv = golden_io.kernel_vectors()
+    sp = kv[f"s{si}_sp"].astype(np.int32)
+    args = [kv[f"s{si}_{k}"] for k in ("pool", "qoff", "qlen", "toff", "tlen")]
+    n = len(args[2])
+    for tb in (0, 1):
+        res, cig, cn = _run_lanes(emu, sp, *args, np.zeros(n, np.uint8), np.zeros(n, np.int32), tb, lanes)
+        assert np.array_equal(res, kv[f"s{si}_tb{tb}_res"])
+        assert np.array_equal(cn, kv[f"s{si}_tb{tb}_cn"])
+        assert np.array_equal(cig, kv[f"s{si}_tb{tb}_cig"])
+
+
+@pytest.mark.parametrize("bw,lanes", [(32, 2), (64, 4)])
+def test_emulated_split_fill_on_reverse_clipped_and_n_rich_tasks(emu, bw, lanes):
+    """Reverse-complemented queries, windows starting inside the target and N-rich queries
+    through the split fill: the oracle on the equivalent plain task decides."""
+    rng = np.random.default_rng(300 + bw)
+    sc = dict(match=2, mismatch=4, gap_open=4, gap_ext=2, ambig=1, bw=bw)
+    tasks = sets.random_tasks(rng, 150, bw, max_len=900)
+    for q, t in tasks[:15]:
+        q[rng.random(len(q)) < 0.2] = 4
+    pool, qo, ql, to, tl = sets.pack_tasks(tasks)
+    n = len(tasks)
+    rev = (rng.random(n) < 0.5).astype(np.uint8)
+    tlo = np.where(rng.random(n) < 0.3, rng.integers(0, 40, n), 0).astype(np.int32)
+    sp = np.array(list(sc.values()), np.int32)
+    res, cig, cn = _run_lanes(emu, sp, pool, qo, ql, to, tl, rev, tlo, 1, lanes)
+    sres, _, _ = _run_lanes(emu, sp, pool, qo, ql, to, tl, rev, tlo, 0, lanes)
+    ospt = oracle.score_t(**sc)
+    offs = np.concatenate([[0], np.cumsum(cn)])
+    for i, (q, t) in enumerate(tasks):
+        qq = np.where(q[::-1] < 4, 3 - q[::-1], 4).astype(np.uint8) if rev[i] else q
+        tt = t.copy()
+        tt[:min(int(tlo[i]), len(tt))] = 4
+        exp = oracle.align(ospt, qq, tt, True)
+        assert tuple(res[i]) == tuple(exp[:6]), (bw, i)
+        assert cig[offs[i]:offs[i + 1]].tolist() == exp[6], (bw, i)
+        assert sres[i][0] == exp[0], (bw, i)
+
+
 def test_emulated_packed_kernel_on_long_and_n_rich_tasks(emu):
     rng = np.random.default_rng(77)
     for bw in (16, 32):

@@ -146,6 +146,30 @@ def test_both_traceback_walks_give_reference_records(name, walk_kind, mode):
     _compare(name, recs, pool, g)
 
 
+@pytest.mark.parametrize("name,lanes", [("edge_uniform_bw64", 4), ("edge_repeats_bw32", 2)])
+@pytest.mark.parametrize("sw_lanes", [1, 0])
+def test_split_fill_gives_reference_records(name, lanes, sw_lanes):
+    """The packed fill with one pair of alignments per lane, and split over 2 (bw 32) or 4
+    (bw 64) lanes per pair, the neighbours exchanging their edge cells by DPP; a small direction
+    budget cuts the traceback fill into several launches. The batched form's default is the split."""
+    spec = sets.SETS[name]
+    g = sets.genome(spec)
+    r = sets.reads(spec, g)
+    print(f"set {name} sw_lanes {sw_lanes}", flush=True)
+    b = _context(spec).batch(mode=abi.MODE_BATCHED, sw_lanes=sw_lanes or lanes, dir_budget=1 << 22)
+    b.upload(api.prepare(r))
+    b.run()
+    recs, pool, _ = b.download()
+    b.close()
+    _compare(name, recs, pool, g)
+
+
+def test_split_lanes_the_band_lacks_are_refused():
+    spec = sets.SETS["edge_uniform_bw16"]
+    with pytest.raises(api.DnalnError):
+        _context(spec).batch(mode=abi.MODE_BATCHED, sw_lanes=2)
+
+
 def _oracle_candidates(spec, g, r):
     import oracle
     mp = sets.map_params(spec)
