@@ -33,6 +33,10 @@ METRIC = "reads/s + GCUPS, batched vs plain GPU form, bit-exact SAM; p50/p99 lat
 P16_PEAK = 256 * 4 * 16 * 2 * 2.4e9     # packed int16 VALU ops/s (256 CU x 4 SIMD x 16 lanes x 2 x 2.4 GHz)
 HBM_PEAK = 8.0e12
 OPS_PER_CELL = {"traceback": 15, "score": 11}
+DEFAULT_WORKLOAD = "uniform4m_150bp_2m"
+# Lanes per pair of the packed SW fill the batched form uses by default (the band widths with a
+# split fill); the autotuner's report names what it chose.
+DEFAULT_SW_LANES = {32: 2, 64: 4}
 
 
 def fill_kernels(W: int, lanes: int) -> str:
@@ -197,9 +201,11 @@ def _pmc_rows(path: str) -> list[dict]:
         return list(csv.DictReader(fp))
 
 
-def summarize_pmc(fetch_csv: str, write_csv: str, calib: int) -> int:
-    """HBM bytes per launch of the traceback fill kernel from FETCH_SIZE / WRITE_SIZE (KiB per
-    dispatch), each scaled by what the same counter reads for the calibration copy."""
+def summarize_pmc(fetch_csv: str, write_csv: str, calib: int, workload: str) -> int:
+    """HBM bytes per launch of the traceback fill kernel (k_sw_packed or k_sw_split) from
+    FETCH_SIZE / WRITE_SIZE (KiB per dispatch), each scaled by what the same counter reads for the
+    calibration copy. For the default workload the summary also goes to profiles/traffic.json,
+    which the benchmark's roofline block reads."""
     out = {}
     for counter, path in (("FETCH_SIZE", fetch_csv), ("WRITE_SIZE", write_csv)):
         rows = [r for r in _pmc_rows(path) if r.get("Counter_Name") == counter]
@@ -213,8 +219,10 @@ def summarize_pmc(fetch_csv: str, write_csv: str, calib: int) -> int:
                 top[k], per[k] = g, []
             if g == top[k]:
                 per[k].append(float(r["Counter_Value"]))
-        copy = [v for k, vs in per.items() if "copyBuffer" in k for v in vs]
-        fill = [v for k, vs in per.items() if re.search(r"::k_sw_packed(_g)?<", k) and ", true>" in k for v in vs]
+        # the calibration copy: the copy launch that counted the most bytes (other copies may have
+        # larger grids)
+        copy = [float(r["Counter_Value"]) for r in rows if "copyBuffer" in r["Kernel_Name"]]
+        fill = [v for k, vs in per.items() if re.search(r"::k_sw_(packed|split)(_g)?<", k) and ", true>" in k for v in vs]
         walk = [v for k, vs in per.items() if re.search(r"k_walk(_g)?<", k) for v in vs]
         fused = [v for k, vs in per.items() if re.search(r"k_seed_chain_fused<|k_fused_g<", k) for v in vs]
         out[counter] = {"copy_kib": max(copy) if copy else None, "fill_kib_per_launch": sum(fill) / len(fill) if fill else None,
@@ -236,8 +244,10 @@ def summarize_pmc(fetch_csv: str, write_csv: str, calib: int) -> int:
             if f[key] is not None and w[key] is not None:
                 res[name] = f[key] * 1024 * f["scale"] + w[key] * 1024 * w["scale"]
     res["copy_bytes"] = calib
-    (ROOT / "profiles").mkdir(exist_ok=True)
-    (ROOT / "profiles" / "traffic.json").write_text(json.dumps(res, indent=1) + "\n")
+    res["workload"] = workload
+    if workload == DEFAULT_WORKLOAD:
+        (ROOT / "profiles").mkdir(exist_ok=True)
+        (ROOT / "profiles" / "traffic.json").write_text(json.dumps(res, indent=1) + "\n")
     print(json.dumps(res, indent=1))
     return 0
 
@@ -408,7 +418,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--plain-steps", type=int, default=2)
-    ap.add_argument("--workload", default="uniform4m_150bp_2m")
+    ap.add_argument("--workload", default=DEFAULT_WORKLOAD)
     ap.add_argument("--reads", type=int, default=0, help="override the read count (smaller runs)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-parity", action="store_true")
@@ -437,7 +447,7 @@ def main() -> int:
     if a.summarize_pmc:
         if not a.calibrate_copy:
             ap.error("--summarize-pmc needs --calibrate-copy BYTES (the size the profiled run copied)")
-        return summarize_pmc(*a.summarize_pmc, a.calibrate_copy)
+        return summarize_pmc(*a.summarize_pmc, a.calibrate_copy, a.workload)
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported (reads shard "
                                    "trivially across GPUs, but no multi-GPU path is built)"}))
@@ -486,7 +496,8 @@ def main() -> int:
     # events around its launches on the batch stream.
     tb_ops = tb_cells * OPS_PER_CELL["traceback"]
     achieved = tb_ops / (tb_ms * 1e-3) if tb_ms > 0 else 0.0
-    roofline = {"bound": "valu-int16", "kernel": f"k_sw_packed<{W}, true>",
+    lanes = batched["tune"]["sw_lanes"] if batched["tune"] else DEFAULT_SW_LANES.get(w.bw, 1)
+    roofline = {"bound": "valu-int16", "kernel": fill_kernels(W, lanes).split(" / ")[0],
                 "achieved": achieved / 1e12, "peak": peak_ops / 1e12, "unit": "Tops/s (packed int16)",
                 "frac": achieved / peak_ops, "traffic": None, "peak_source": peak,
                 "frac_of_nominal_peak": achieved / P16_PEAK,
@@ -502,8 +513,8 @@ def main() -> int:
                                 "note": "read bases + 16 B per bucket visited + 4 B per occurrence read, over the "
                                         "seed stage's event time (fused seed + chain kernels)"}}
     tpath = ROOT / "profiles" / "traffic.json"
-    if tpath.exists():
-        tr = json.loads(tpath.read_text())
+    tr = json.loads(tpath.read_text()) if tpath.exists() else None
+    if tr and tr.get("workload", DEFAULT_WORKLOAD) == w.name:
         roofline["traffic"] = tr.get("k_sw_packed_tb_bytes_per_launch")
         # HBM-side view of the two memory-bound kernels: counted bytes per launch (from the
         # committed counter runs) over this run's measured time per launch.
