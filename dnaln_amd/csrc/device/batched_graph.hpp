@@ -486,6 +486,10 @@ __global__ __launch_bounds__(kSwBlock) DN_WAVES_PER_EU(sw_min_waves(W, TB)) void
     DN_DYN_LDS(uint16_t, bh_rows);
     uint16_t* bh = bh_rows + (size_t)(threadIdx.x >> 6) * best_row_lds_words(W) * 2 + 2 * lane;
     sw_pair<W, TB>(a, ka, kb, va, vb, ql, slot, lane, bh, 64);
+  } else if (TB && flag_granules<W, 1>()) {
+    DN_DYN_LDS(uint32_t, stage);
+    sw_pair<W, TB>(a, ka, kb, va, vb, ql, slot, lane, nullptr, 0,
+                   stage + (size_t)(threadIdx.x >> 6) * fill_lds_words(W) + lane, 64);
   } else {
     sw_pair<W, TB>(a, ka, kb, va, vb, ql, slot, lane);
   }
@@ -508,6 +512,7 @@ __global__ __launch_bounds__(256) void k_walk_g(SwArgs a, const uint32_t* n_ptr,
 template <int W>
 __global__ __launch_bounds__(128) void k_walk_win_g(SwArgs a, int rows, const uint32_t* n_ptr, uint32_t first) {
   constexpr int G = (W + 7) / 8;
+  constexpr bool kGran = flag_granules<W, 1>();
   DN_DYN_LDS(uint64_t, win);
   __shared__ int s_top;
   __shared__ uint32_t seqbuf[kWalkSeqDwords * 128];
@@ -546,8 +551,11 @@ __global__ __launch_bounds__(128) void k_walk_win_g(SwArgs a, int rows, const ui
     __syncthreads();
     const int top = s_top;
     if (top < 0) break;
-    const int lo = max(0, top - rows + 1);
-    const uint64_t n_words = (uint64_t)(top - lo + 1) * G * 64;
+    // Flag granules: the window covers whole granules, [lo, top | 3] with lo a multiple of 4
+    // (rows is a multiple of 4, and slots hold whole granules).
+    const int lo = kGran ? max(0, (top - rows + 4) & ~3) : max(0, top - rows + 1);
+    const int hi = kGran ? (top | 3) : top;
+    const uint64_t n_words = (uint64_t)(hi - lo + 1) * G * 64;
     const uint64_t* src = slot + (uint64_t)lo * G * 64;
     for (uint64_t x = 2 * (uint64_t)t; x < n_words; x += 256) {
       const uint4 vv = *reinterpret_cast<const uint4*>(src + x);
@@ -555,7 +563,8 @@ __global__ __launch_bounds__(128) void k_walk_win_g(SwArgs a, int rows, const ui
     }
     __syncthreads();
     while (active && !w.done && (w.i >= lo || w.i < 0))
-      walk_step<W>(w, tk, reinterpret_cast<const uint32_t*>(win) + half * 64 + lane, a.sp, out, a.cigar_cap, seq, lo);
+      walk_step<W>(w, tk, reinterpret_cast<const uint32_t*>(win) + (kGran ? (half * 64 + lane) * 4 : half * 64 + lane),
+                   a.sp, out, a.cigar_cap, seq, lo);
     __syncthreads();
   }
   if (active) walk_finish(w, i_end, d_end, score, a.cigar_cap, a.res[k]);
@@ -611,7 +620,7 @@ bool launch_packed_g(int bw, const SwArgs& a, uint64_t waves, const uint32_t* n_
     const unsigned threads = (unsigned)sw_block(kW, TB);                                                          \
     const unsigned blocks = (unsigned)std::max<uint64_t>(1, (waves * 64 + threads - 1) / threads);                \
     hipLaunchKernelGGL((k_sw_packed_g<kW, TB>), dim3(blocks), dim3(threads),                                      \
-                       TB ? (size_t)best_row_lds_words(kW) * 4 * (threads / 64) : 0, st, a, n_ptr, first);         \
+                       TB ? (size_t)fill_lds_words(kW) * 4 * (threads / 64) : 0, st, a, n_ptr, first);             \
     return true;                                                                                                  \
   }
     DN_PACKED_BW_LIST(DN_CASE)

@@ -102,6 +102,19 @@ DN_HD void store_stream(uint64_t* p, uint64_t v) {
   *p = v;
 #endif
 }
+// 16 bytes at the 16-aligned p, one non-temporal 16-byte store on the device.
+DN_HD void store_stream16(uint32_t* p, const Quad& q) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  typedef uint32_t v4u __attribute__((ext_vector_type(4)));
+  const v4u v = {q.x, q.y, q.z, q.w};
+  __builtin_nontemporal_store(v, reinterpret_cast<v4u*>(p));
+#else
+  p[0] = q.x;
+  p[1] = q.y;
+  p[2] = q.z;
+  p[3] = q.w;
+#endif
+}
 DN_HD void store_stream(uint32_t* p, uint32_t v) {
 #if defined(__HIP_DEVICE_COMPILE__)
   __builtin_nontemporal_store(v, p);

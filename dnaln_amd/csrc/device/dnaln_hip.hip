@@ -381,6 +381,10 @@ __global__ __launch_bounds__(kSwBlock) DN_WAVES_PER_EU(sw_min_waves(W, TB)) void
     DN_DYN_LDS(uint16_t, bh_rows);  // per wave of the block: W x 64 lanes x {A, B}
     uint16_t* bh = bh_rows + (size_t)(threadIdx.x >> 6) * best_row_lds_words(W) * 2 + 2 * lane;
     sw_pair<W, TB>(a, ida, idb, va, vb, ql, slot, lane, bh, 64);
+  } else if (TB && flag_granules<W, 1>()) {
+    DN_DYN_LDS(uint32_t, stage);  // per wave of the block: one step's flag words, lane-interleaved
+    sw_pair<W, TB>(a, ida, idb, va, vb, ql, slot, lane, nullptr, 0,
+                   stage + (size_t)(threadIdx.x >> 6) * fill_lds_words(W) + lane, 64);
   } else {
     sw_pair<W, TB>(a, ida, idb, va, vb, ql, slot, lane);
   }
@@ -433,6 +437,7 @@ __global__ __launch_bounds__(256) void k_walk(SwArgs a) {
 template <int W>
 __global__ __launch_bounds__(128) void k_walk_win(SwArgs a, int rows) {
   constexpr int G = (W + 7) / 8;
+  constexpr bool kGran = flag_granules<W, 1>();
   DN_DYN_LDS(uint64_t, win);
   __shared__ int s_top;
   __shared__ uint32_t seqbuf[kWalkSeqDwords * 128];
@@ -469,8 +474,11 @@ __global__ __launch_bounds__(128) void k_walk_win(SwArgs a, int rows) {
     __syncthreads();
     const int top = s_top;
     if (top < 0) break;
-    const int lo = max(0, top - rows + 1);
-    const uint64_t n_words = (uint64_t)(top - lo + 1) * G * 64;
+    // Flag granules: the window covers whole granules, [lo, top | 3] with lo a multiple of 4
+    // (rows is a multiple of 4, and slots hold whole granules).
+    const int lo = kGran ? max(0, (top - rows + 4) & ~3) : max(0, top - rows + 1);
+    const int hi = kGran ? (top | 3) : top;
+    const uint64_t n_words = (uint64_t)(hi - lo + 1) * G * 64;
     const uint64_t* src = slot + (uint64_t)lo * G * 64;
     for (uint64_t x = 2 * (uint64_t)t; x < n_words; x += 256) {
       const uint4 v = *reinterpret_cast<const uint4*>(src + x);
@@ -478,7 +486,8 @@ __global__ __launch_bounds__(128) void k_walk_win(SwArgs a, int rows) {
     }
     __syncthreads();
     while (active && !w.done && (w.i >= lo || w.i < 0))
-      walk_step<W>(w, tk, reinterpret_cast<const uint32_t*>(win) + half * 64 + lane, a.sp, out, a.cigar_cap, seq, lo);
+      walk_step<W>(w, tk, reinterpret_cast<const uint32_t*>(win) + (kGran ? (half * 64 + lane) * 4 : half * 64 + lane),
+                   a.sp, out, a.cigar_cap, seq, lo);
     __syncthreads();
   }
   if (active) walk_finish(w, i_end, d_end, score, a.cigar_cap, a.res[id]);
@@ -615,7 +624,7 @@ bool launch_packed_bw(int bw, const SwArgs& a, unsigned grid, hipStream_t st) {
   case B:                                                                             \
     hipLaunchKernelGGL((k_sw_packed<2 * B + 1, TB>), dim3(grid * (kSwBlock / sw_block(2 * B + 1, TB))),  \
                        dim3(sw_block(2 * B + 1, TB)),                                                     \
-                       TB ? (size_t)best_row_lds_words(2 * B + 1) * 4 * (sw_block(2 * B + 1, TB) / 64) : 0, \
+                       TB ? (size_t)fill_lds_words(2 * B + 1) * 4 * (sw_block(2 * B + 1, TB) / 64) : 0,     \
                        st, a);                                                                            \
     return true;
     DN_PACKED_BW_LIST(DN_CASE)
@@ -668,8 +677,9 @@ bool launch_walk_split(int bw, int lanes, const SwArgs& a, hipStream_t st) {
   return false;
 }
 
-// Rows per LDS window of the walk: 16 KiB of flag words, at least 4 rows.
-int walk_rows(int W) { return std::max(4, 16384 / (((W + 7) / 8) * 512)); }
+// Rows per LDS window of the walk: 16 KiB of flag words, a multiple of 4 rows (whole flag
+// granules), at least 4.
+int walk_rows(int W) { return std::max(4, (16384 / (((W + 7) / 8) * 512)) & ~3); }
 
 // walk_kind 0: one thread per task (k_walk); 1: one block per fill wave through LDS windows
 // (k_walk_win).

@@ -21,8 +21,11 @@
 // Traceback storage: 4 flag bits per cell and alignment. Flags are the sign bits of four packed
 // differences, gathered by v_perm_b32 into byte masks and accumulated 8 cells to a byte:
 //   byte 0/1 extF (A/B), 2/3 extE, 4/5 notE, 6/7 notDiag — bit (d & 7) of the 8-byte word of
-// group d >> 3. The words of one wave are interleaved by lane: word(i, g, lane) at
-// slot[(i * G + g) * 64 + lane], so every store is one coalesced 512-byte write.
+// group d >> 3, split by two perms into one 32-bit plane per task half. With four-row steps
+// (W <= kGranuleW) the words of rows 4k .. 4k+3 of one group, half and lane form a 16-byte granule
+// (flag_granules below), staged in LDS during the step and stored as one coalesced 1-KiB write per
+// group and half; wider bands store each row's words interleaved by lane, one coalesced 256-byte
+// write per group and half.
 #pragma once
 
 #include "common.hpp"
@@ -134,6 +137,14 @@ DN_HD uint32_t align4(uint32_t lo, uint32_t hi, uint32_t sh) {
 // Rows per fill slot: the traceback fill runs its rows in steps of up to 4, so a slot holds
 // the wave's longest query rounded up to a multiple of 4 (rows past a query are never read).
 DN_HD uint64_t fill_rows(int qmax) { return (uint64_t)(((qmax > 1 ? qmax : 1) + 3) & ~3); }
+
+// Traceback fills with four-row steps (W <= kGranuleW, one lane per pair) keep each lane's flag
+// words of rows 4k .. 4k+3 together: granule (k, group g, half h, lane l) at u32 index
+// (((k * G + g) * 2 + h) * 64 + l) * 4, word r of it row 4k + r. A walk then reads a diagonal run
+// of four rows from one 16-byte load, where the row layout below needs a line per row.
+constexpr int kGranuleW = 33;
+template <int W, int P>
+DN_HD constexpr bool flag_granules() { return P == 1 && W <= kGranuleW; }
 
 // A band of W columns split over P lanes (sw_split.hpp; P = 1 is sw_pair): lane `seg` of a pair
 // holds columns seg*S .. seg*S + S-1 and runs row r at step r + seg. A slot holds, per step, the
@@ -331,11 +342,15 @@ DN_HD void walk_flush(Walk& w, uint32_t* out, int cap) {
 // the fill group) from step row0 on: column d lies in segment seg = d / S at offset c = d % S
 // (S = seg_width(W, P)); the word of row i, group c / 8 at ((i + seg - row0) * GS + c / 8) * 128P
 // + seg * 64, bytes extF, extE, E and not-diagonal, bit c & 7 in each (for P = 1: row i, group
-// d / 8 at ((i - row0) * G + d / 8) * 128). A step reads only row w.i, and none when w.i < 0.
+// d / 8 at ((i - row0) * G + d / 8) * 128). With flag_granules, plane is the task's granule
+// column and the word of row i, group g is word (i - row0) & 3 of granule ((i - row0) >> 2) * G
+// + g, 512 words apart (row0 a multiple of 4): a diagonal run reads four rows from one line. A
+// step reads only row w.i, and none when w.i < 0.
 template <int W, int P = 1>
 DN_HD void walk_step(Walk& w, const SwTask& tk, const uint32_t* plane, const ScoreP& sp, uint32_t* out, int cap,
                      WalkSeq& seq, int row0 = 0) {
   constexpr int S = seg_width(W, P), GS = seg_groups(W, P);
+  constexpr bool kGran = flag_granules<W, P>();
   const int oe = sp.gap_open + sp.gap_ext, ge = sp.gap_ext;
   if (w.state == 0 && (w.i < 0 || w.v <= 0)) {
     if (w.run) walk_push(w, 0, w.run, out, cap);
@@ -351,7 +366,13 @@ DN_HD void walk_step(Walk& w, const SwTask& tk, const uint32_t* plane, const Sco
   }
   // The flag word and, speculatively, the two bases of the cell (mostly from registers).
   const int seg = P == 1 ? 0 : w.d / S, col = w.d - seg * S;
-  const uint32_t word = plane[(uint64_t)((w.i + seg - row0) * GS + (col >> 3)) * (128 * P) + seg * 64];
+  uint32_t word;
+  if (kGran) {
+    const int r = w.i - row0;
+    word = plane[(uint64_t)((r >> 2) * GS + (col >> 3)) * 512 + (r & 3)];
+  } else {
+    word = plane[(uint64_t)((w.i + seg - row0) * GS + (col >> 3)) * (128 * P) + seg * 64];
+  }
   const uint8_t a = seq.qb(tk, w.i), c = seq.tb(tk, w.i + w.d);
   const int b = col & 7;
   if (w.state == 0) {
@@ -438,10 +459,17 @@ constexpr int kBestRowInRegs = DN_BEST_ROW_IN_REGS;
 
 // Best-row words per fill wave when the row lives in LDS (one u32 per band cell and lane).
 DN_HD int best_row_lds_words(int W) { return W > kBestRowInRegs ? W * 64 : 0; }
+// LDS words per fill wave of the traceback fill (the saved best row, or the flag granules'
+// staging: four rows of every group and half per lane).
+static_assert(kBestRowInRegs >= kGranuleW, "a fill stages flag granules or keeps its best row in LDS, not both");
+DN_HD int fill_lds_words(int W) { return W > kBestRowInRegs ? W * 64 : W <= kGranuleW ? ((W + 7) / 8) * 8 * 64 : 0; }
 
+// stage (flag_granules): this lane's LDS words for one step's flags, word ((g * 2 + h) * 4 + r)
+// at stage[((g * 2 + h) * 4 + r) * stage_stride].
 template <int W, bool TB>
 DN_HD void sw_pair(const SwArgs& a, uint32_t ida, uint32_t idb, bool va, bool vb, int qmax, uint64_t* slot,
-                   int lane, uint16_t* bh_lds = nullptr, int bh_stride = 0) {
+                   int lane, uint16_t* bh_lds = nullptr, int bh_stride = 0, uint32_t* stage = nullptr,
+                   int stage_stride = 0) {
   constexpr int G = (W + 7) / 8;
   // An empty half reads task 0's sequences (every launch has a task 0) with no valid row or
   // target position, so its loads stay in bounds and its cells score as N.
@@ -459,7 +487,8 @@ DN_HD void sw_pair(const SwArgs& a, uint32_t ida, uint32_t idb, bool va, bool vb
   // way); the score-only fill would drop from 3 to 2 waves, wider bands would spill more.
   // A step always runs all U rows; rows past qmax score below the best and store their flags
   // into the slot's padding rows (fill_rows).
-  constexpr int U = TB && W <= 33 ? 4 : 1;
+  constexpr int U = TB && W <= kGranuleW ? 4 : 1;
+  constexpr bool kGran = TB && flag_granules<W, 1>();
   pk16 H[W + 1], E[W + 1];
   uint32_t T[W + U - 1];
 #pragma unroll
@@ -542,9 +571,20 @@ DN_HD void sw_pair(const SwArgs& a, uint32_t ida, uint32_t idb, bool va, bool vb
           if ((d & 7) == 7 || d == W - 1) {
             // One 32-bit plane per half (bytes: extF, extE, E, not-diagonal), so that a walk,
             // which follows one half, reads only its own plane.
-            uint32_t* row = slot_lane + (uint64_t)(i * G + (d >> 3)) * 128;
-            store_stream(row, perm_b32(acc34, acc12, 0x06040200u));
-            store_stream(row + 64, perm_b32(acc34, acc12, 0x07050301u));
+            const uint32_t fa = perm_b32(acc34, acc12, 0x06040200u), fb = perm_b32(acc34, acc12, 0x07050301u);
+            if (kGran) {
+              stage[(((d >> 3) * 2) * 4 + u) * stage_stride] = fa;
+              stage[(((d >> 3) * 2 + 1) * 4 + u) * stage_stride] = fb;
+#if defined(__HIP_DEVICE_COMPILE__)
+              // Left free, the scheduler sinks these LDS stores into later rows and the fill
+              // spills 800 registers; with the barrier it keeps 249 and no scratch.
+              __builtin_amdgcn_sched_barrier(0);
+#endif
+            } else {
+              uint32_t* row = slot_lane + (uint64_t)(i * G + (d >> 3)) * 128;
+              store_stream(row, fa);
+              store_stream(row + 64, fb);
+            }
             acc12 = acc34 = 0;
           }
           rowmax = pk_max(rowmax, h);
@@ -574,6 +614,19 @@ DN_HD void sw_pair(const SwArgs& a, uint32_t ida, uint32_t idb, bool va, bool vb
           best = pk_max(best, rowmax);
           best_i = bfi(m, (uint32_t)i * 0x10001u, best_i);
         }
+      }
+    }
+    if (kGran) {
+      // The step's four rows of each group and half, from LDS, as one 16-byte granule store.
+      uint32_t* gran = reinterpret_cast<uint32_t*>(slot) + (uint64_t)(i0 >> 2) * G * 512 + (uint64_t)lane * 4;
+#pragma unroll
+      for (int x = 0; x < 2 * G; ++x) {
+        Quad q;
+        q.x = stage[(x * 4 + 0) * stage_stride];
+        q.y = stage[(x * 4 + 1) * stage_stride];
+        q.z = stage[(x * 4 + 2) * stage_stride];
+        q.w = stage[(x * 4 + 3) * stage_stride];
+        store_stream16(gran + (uint64_t)x * 256, q);
       }
     }
     // Shift the band U positions and bring in target position i0 + W + U - 1; move the block's
@@ -641,7 +694,8 @@ DN_HD void walk_task(const SwArgs& a, uint32_t k, WalkSeq& seq) {
   }
   const SwTask tk = a.tasks[id];
   const uint32_t wave = k >> 7, pair = k & 63, half = (k >> 6) & 1;
-  const uint32_t* plane = reinterpret_cast<const uint32_t*>(dir_slot(a, wave)) + half * 64 * P + pair;
+  const uint32_t* plane = reinterpret_cast<const uint32_t*>(dir_slot(a, wave)) +
+                          (flag_granules<W, P>() ? (half * 64 + pair) * 4 : half * 64 * P + pair);
   uint32_t* out = a.cigar + (uint64_t)id * (uint64_t)a.cigar_cap;
   Walk w = walk_init(i_end, d_end, score);
   seq.q.at = seq.t.at = ~0ull;

@@ -67,15 +67,18 @@ void walk_wave_window(const SwArgs& a, uint32_t wave, int rows) {
     for (const Th& x : th)
       if (x.active && !x.w.done) top = std::max(top, x.w.i);
     if (top < 0) break;
-    const int lo = std::max(0, top - rows + 1);
-    const uint64_t n_words = (uint64_t)(top - lo + 1) * G * 64;
+    const bool gran = flag_granules<W, 1>();  // whole granules: [lo, top | 3], lo a multiple of 4
+    const int lo = gran ? std::max(0, (top - rows + 4) & ~3) : std::max(0, top - rows + 1);
+    const int hi = gran ? (top | 3) : top;
+    const uint64_t n_words = (uint64_t)(hi - lo + 1) * G * 64;
     std::fill(win.begin(), win.end(), 0x5555aaaa5555aaaaULL);
     std::copy(slot + (uint64_t)lo * G * 64, slot + (uint64_t)lo * G * 64 + n_words, win.begin());
     for (int t = 0; t < 128; ++t) {
       Th& x = th[t];
       while (x.active && !x.w.done && (x.w.i >= lo || x.w.i < 0))
-        walk_step<W>(x.w, x.tk, reinterpret_cast<const uint32_t*>(win.data()) + (t >> 6) * 64 + (t & 63), a.sp, x.out,
-                     a.cigar_cap, x.seq, lo);
+        walk_step<W>(x.w, x.tk,
+                     reinterpret_cast<const uint32_t*>(win.data()) + (gran ? ((t >> 6) * 64 + (t & 63)) * 4 : (t >> 6) * 64 + (t & 63)),
+                     a.sp, x.out, a.cigar_cap, x.seq, lo);
     }
   }
   for (Th& x : th)
@@ -100,15 +103,16 @@ void run_packed(const SwArgs& a0) {
     for (int lane = 0; lane < 64; ++lane) {
       const uint32_t ia = base + lane, ib = base + 64 + lane;
       uint16_t bh[2 * W];
+      uint32_t stage[8 * G];  // one step's flag words (flag granules)
       sw_pair<W, TB>(a, ia < a.n ? ia : 0, ib < a.n ? ib : 0, ia < a.n, ib < a.n, qmax,
-                     TB ? a.dir + (uint64_t)wv * slot_words : nullptr, lane, bh, 1);
+                     TB ? a.dir + (uint64_t)wv * slot_words : nullptr, lane, bh, 1, stage, 1);
     }
   }
   if (TB) {
-    // Both walks: windows of 3 rows (fewer than the device's, so many window moves) on even
-    // fill waves, one thread per task on odd ones.
+    // Both walks: windows of 3 rows (4, one granule, with flag granules; fewer than the device's,
+    // so many window moves) on even fill waves, one thread per task on odd ones.
     for (uint32_t wv = 0; wv < waves; ++wv) {
-      if (wv % 2 == 0) walk_wave_window<W>(a, wv, 3);
+      if (wv % 2 == 0) walk_wave_window<W>(a, wv, flag_granules<W, 1>() ? 4 : 3);
       else
         for (uint32_t k = wv * 128; k < a.n && k < wv * 128 + 128; ++k) {
           uint32_t buf[kWalkSeqDwords];
