@@ -393,8 +393,9 @@ __global__ __launch_bounds__(kSwBlock) DN_WAVES_PER_EU(sw_min_waves(W, TB)) void
 // The packed fill split over P lanes per pair (sw_split.hpp): fill group g (tasks 128g ..
 // 128g + 127, one direction slot) runs as waves gP .. gP + P-1; wave j holds pairs
 // j*64/P .. (j+1)*64/P - 1, lane l segment l % P of pair j*64/P + l / P. Each wave keeps its
-// lanes' best rows in LDS (S columns x 64 lanes x {A, B}). Two waves per SIMD: at S = 33 the
-// traceback fill takes 224 registers; the score-only fill held to three waves spills 17.
+// lanes' best rows (S columns x 64 lanes x {A, B}) and one step's flag words (split_stage_words)
+// in LDS. Two waves per SIMD: at S = 33 the traceback fill takes about 224 registers; the
+// score-only fill held to three waves spills 17.
 template <int W, int P, bool TB>
 __global__ __launch_bounds__(kSwBlock) DN_WAVES_PER_EU(2) void k_sw_split(SwArgs a) {
   constexpr int S = seg_width(W, P);
@@ -411,11 +412,13 @@ __global__ __launch_bounds__(kSwBlock) DN_WAVES_PER_EU(2) void k_sw_split(SwArgs
   ql = wave_max(ql);
   uint64_t* slot = TB ? dir_slot(a, grp) : nullptr;
   uint16_t* bh = nullptr;
+  uint32_t* stage = nullptr;
   if (TB) {
-    DN_DYN_LDS(uint16_t, bh_rows);
-    bh = bh_rows + (size_t)(threadIdx.x >> 6) * S * 128 + 2 * lane;
+    DN_DYN_LDS(uint32_t, split_lds);  // best rows of the block's waves, then their flag staging
+    bh = reinterpret_cast<uint16_t*>(split_lds + (size_t)(threadIdx.x >> 6) * S * 64) + 2 * lane;
+    stage = split_lds + (size_t)(kSwBlock / 64) * S * 64 + (size_t)(threadIdx.x >> 6) * split_stage_words(W, P) + lane;
   }
-  sw_split<W, P, TB>(a, ida, idb, va, vb, ql, slot, (int)j, lane, bh, 64);
+  sw_split<W, P, TB>(a, ida, idb, va, vb, ql, slot, (int)j, lane, bh, 64, stage, 64);
 }
 
 // Traceback with one thread per task (walk_task), fills with P lanes per pair.
@@ -648,7 +651,9 @@ bool launch_split_bw(int bw, int lanes, const SwArgs& a, unsigned groups, hipStr
   if (bw == B && lanes == P) {                                                                              \
     constexpr int kW = 2 * B + 1;                                                                           \
     hipLaunchKernelGGL((k_sw_split<kW, P, TB>), dim3((groups * P + kSwBlock / 64 - 1) / (kSwBlock / 64)),   \
-                       dim3(kSwBlock), TB ? (size_t)seg_width(kW, P) * 256 * (kSwBlock / 64) : 0, st, a);   \
+                       dim3(kSwBlock),                                                                      \
+                       TB ? (size_t)(seg_width(kW, P) * 64 + split_stage_words(kW, P)) * 4 * (kSwBlock / 64) : 0, \
+                       st, a);                                                                              \
     return true;                                                                                            \
   }
   DN_SPLIT_LIST(DN_CASE)

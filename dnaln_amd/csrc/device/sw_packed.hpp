@@ -138,13 +138,14 @@ DN_HD uint32_t align4(uint32_t lo, uint32_t hi, uint32_t sh) {
 // the wave's longest query rounded up to a multiple of 4 (rows past a query are never read).
 DN_HD uint64_t fill_rows(int qmax) { return (uint64_t)(((qmax > 1 ? qmax : 1) + 3) & ~3); }
 
-// Traceback fills with four-row steps (W <= kGranuleW, one lane per pair) keep each lane's flag
-// words of rows 4k .. 4k+3 together: granule (k, group g, half h, lane l) at u32 index
-// (((k * G + g) * 2 + h) * 64 + l) * 4, word r of it row 4k + r. A walk then reads a diagonal run
-// of four rows from one 16-byte load, where the row layout below needs a line per row.
+// Traceback fills with four-row steps (W <= kGranuleW with one lane per pair, and every split
+// fill) keep a lane's flag words of four consecutive steps together in a 16-byte granule: granule
+// (k, group g, half h, segment seg, pair p) at u32 index ((((k * GS + g) * 2 + h) * 64P + seg * 64
+// + p) * 4, word r of it step 4k + r (one lane per pair: P = 1, seg = 0, steps = rows). A walk then
+// reads a diagonal run of four rows from one line, where the row layout below needs a line per row.
 constexpr int kGranuleW = 33;
 template <int W, int P>
-DN_HD constexpr bool flag_granules() { return P == 1 && W <= kGranuleW; }
+DN_HD constexpr bool flag_granules() { return P > 1 || W <= kGranuleW; }
 
 // A band of W columns split over P lanes (sw_split.hpp; P = 1 is sw_pair): lane `seg` of a pair
 // holds columns seg*S .. seg*S + S-1 and runs row r at step r + seg. A slot holds, per step, the
@@ -343,9 +344,10 @@ DN_HD void walk_flush(Walk& w, uint32_t* out, int cap) {
 // (S = seg_width(W, P)); the word of row i, group c / 8 at ((i + seg - row0) * GS + c / 8) * 128P
 // + seg * 64, bytes extF, extE, E and not-diagonal, bit c & 7 in each (for P = 1: row i, group
 // d / 8 at ((i - row0) * G + d / 8) * 128). With flag_granules, plane is the task's granule
-// column and the word of row i, group g is word (i - row0) & 3 of granule ((i - row0) >> 2) * G
-// + g, 512 words apart (row0 a multiple of 4): a diagonal run reads four rows from one line. A
-// step reads only row w.i, and none when w.i < 0.
+// column (half h, pair p: (h * 64P + p) * 4) and the word of row i, column c of segment seg is
+// word s & 3 of granule (s >> 2) * GS + c / 8 of that segment, s = i + seg - row0 (row0 a multiple
+// of 4): a diagonal run reads four rows from one line. A step reads only row w.i, and none when
+// w.i < 0.
 template <int W, int P = 1>
 DN_HD void walk_step(Walk& w, const SwTask& tk, const uint32_t* plane, const ScoreP& sp, uint32_t* out, int cap,
                      WalkSeq& seq, int row0 = 0) {
@@ -368,8 +370,8 @@ DN_HD void walk_step(Walk& w, const SwTask& tk, const uint32_t* plane, const Sco
   const int seg = P == 1 ? 0 : w.d / S, col = w.d - seg * S;
   uint32_t word;
   if (kGran) {
-    const int r = w.i - row0;
-    word = plane[(uint64_t)((r >> 2) * GS + (col >> 3)) * 512 + (r & 3)];
+    const int r = w.i + seg - row0;
+    word = plane[(uint64_t)((r >> 2) * GS + (col >> 3)) * (512 * P) + seg * 256 + (r & 3)];
   } else {
     word = plane[(uint64_t)((w.i + seg - row0) * GS + (col >> 3)) * (128 * P) + seg * 64];
   }
@@ -695,7 +697,7 @@ DN_HD void walk_task(const SwArgs& a, uint32_t k, WalkSeq& seq) {
   const SwTask tk = a.tasks[id];
   const uint32_t wave = k >> 7, pair = k & 63, half = (k >> 6) & 1;
   const uint32_t* plane = reinterpret_cast<const uint32_t*>(dir_slot(a, wave)) +
-                          (flag_granules<W, P>() ? (half * 64 + pair) * 4 : half * 64 * P + pair);
+                          (flag_granules<W, P>() ? (half * 64 * P + pair) * 4 : half * 64 * P + pair);
   uint32_t* out = a.cigar + (uint64_t)id * (uint64_t)a.cigar_cap;
   Walk w = walk_init(i_end, d_end, score);
   seq.q.at = seq.t.at = ~0ull;

@@ -16,9 +16,10 @@
 // other and are left out of the row maximum and the best cell. Rows before 0 (the first steps of
 // lanes seg > 0) read as N and stay 0, so nothing reaches the neighbours early.
 //
-// Flags go to the slot in step order (slot_row_words / slot_rows in sw_packed.hpp), each segment's
-// 64 pairs side by side: a flag store of a wave writes P runs of 64/P consecutive words, and the
-// walks of neighbouring tasks read neighbouring words. Each lane keeps the first cell of its own
+// Flags go to the slot in step order as four-step granules (flag_granules in sw_packed.hpp),
+// each segment's 64 pairs side by side: a step's flag words wait in LDS, and every four steps a
+// wave writes each group and half as P runs of 64/P granules (256 bytes or more each). Each lane
+// keeps the first cell of its own
 // segment attaining its maximum (saved best row in LDS, as sw_pair's wide bands); the pair's
 // best cell is the lane maximum ordered by score, then lowest row, then lowest column.
 #pragma once
@@ -36,6 +37,10 @@ DN_HD void bh_store2(uint16_t* p, uint32_t v) {
   p[1] = (uint16_t)(v >> 16);
 #endif
 }
+
+// LDS words per wave of a split traceback fill's flag staging: four steps of every group and
+// half of a segment, per lane.
+DN_HD constexpr int split_stage_words(int W, int P) { return seg_groups(W, P) * 8 * 64; }
 
 template <int W, int P, bool TB>
 struct SegFill {
@@ -65,14 +70,16 @@ struct SegFill {
   uint32_t best_i;
   uint16_t* bh;
   int bh_stride;
-  uint32_t* slot_lane;
+  uint32_t* gran;   // this lane's granule column of the slot
+  uint32_t* stage;  // this lane's LDS words for one step's flags: ((g * 2 + h) * 4 + r) * stride
+  int stage_stride;
 
   // ta / tb: the pair's tasks (an invalid half is emptied). slot: the fill group's slot; wave:
   // this lane's wave within the group (0 .. P-1); lane: its lane in the wave (pair
   // wave * 64/P + lane / P, segment lane % P). bh: this lane's
   // best-row words, column c at bh[2 * c * stride] (A) and the half-word after it (B).
   DN_HD void init(const SwArgs& a, SwTask ta, SwTask tb, bool va, bool vb, int seg_, uint64_t* slot, int wave,
-                  int lane, uint16_t* bh_, int stride) {
+                  int lane, uint16_t* bh_, int stride, uint32_t* stage_ = nullptr, int stage_stride_ = 0) {
     if (!va) ta.qlen = ta.tlo = ta.thi = 0, ta.flags = 0;
     if (!vb) tb.qlen = tb.tlo = tb.thi = 0, tb.flags = 0;
     seg = seg_;
@@ -116,7 +123,9 @@ struct SegFill {
     if (TB) {
 #pragma unroll
       for (int c = 0; c < S; ++c) bh_store2(bh + 2 * c * bh_stride, 0);
-      slot_lane = reinterpret_cast<uint32_t*>(slot) + seg * 64 + wave * (64 / P) + lane / P;
+      gran = reinterpret_cast<uint32_t*>(slot) + (uint64_t)(seg * 64 + wave * (64 / P) + lane / P) * 4;
+      stage = stage_;
+      stage_stride = stage_stride_;
     }
     const uint32_t zs = (uint32_t)a.sp.bw >> 16;  // 0 (bw <= 1000): keeps the masks in SGPRs
 #pragma unroll
@@ -149,9 +158,11 @@ struct SegFill {
       acc12 = and_or(p12, bit, acc12);
       acc34 = and_or(p34, bit, acc34);
       if ((c & 7) == 7 || c == S - 1) {
-        uint32_t* row = slot_lane + (uint64_t)((s * GS + (c >> 3)) * 2) * (64 * P);
-        store_stream(row, perm_b32(acc34, acc12, 0x06040200u));
-        store_stream(row + 64 * P, perm_b32(acc34, acc12, 0x07050301u));
+        stage[(((c >> 3) * 2) * 4 + u) * stage_stride] = perm_b32(acc34, acc12, 0x06040200u);
+        stage[(((c >> 3) * 2 + 1) * 4 + u) * stage_stride] = perm_b32(acc34, acc12, 0x07050301u);
+#if defined(__HIP_DEVICE_COMPILE__)
+        __builtin_amdgcn_sched_barrier(0);  // as in sw_pair: keeps the stores from sinking into later rows
+#endif
         acc12 = acc34 = 0;
       }
     }
@@ -205,6 +216,19 @@ struct SegFill {
   }
 
   DN_HD void step_end(int s0) {
+    if (TB) {
+      // The four steps' flag words of every group and half, as one 16-byte granule store each.
+      uint32_t* g0 = gran + (uint64_t)(s0 >> 2) * GS * 2 * 256 * P;
+#pragma unroll
+      for (int x = 0; x < 2 * GS; ++x) {
+        Quad q;
+        q.x = stage[(x * 4 + 0) * stage_stride];
+        q.y = stage[(x * 4 + 1) * stage_stride];
+        q.z = stage[(x * 4 + 2) * stage_stride];
+        q.w = stage[(x * 4 + 3) * stage_stride];
+        store_stream16(g0 + (uint64_t)x * 256 * P, q);
+      }
+    }
 #pragma unroll
     for (int c = 0; c + 1 < S; ++c) T[c] = T[c + U];
     T[S - 1] = TS[U - 1];
@@ -271,13 +295,14 @@ inline uint32_t from_lane_above(uint32_t v) {
 // pass of hipcc compiles it without the exchanges; only the device pass runs it.)
 template <int W, int P, bool TB>
 __device__ void sw_split(const SwArgs& a, uint32_t ida, uint32_t idb, bool va, bool vb, int qmax, uint64_t* slot,
-                         int wave, int lane, uint16_t* bh, int stride) {
+                         int wave, int lane, uint16_t* bh, int stride, uint32_t* stage, int stage_stride) {
   using F = SegFill<W, P, TB>;
   constexpr int U = F::U;
   const int seg = lane % P;
   const bool first = seg == 0, last = seg == P - 1;
   F L;
-  L.init(a, a.tasks[va ? ida : 0], a.tasks[vb ? idb : 0], va, vb, seg, slot, wave, lane, bh, stride);
+  L.init(a, a.tasks[va ? ida : 0], a.tasks[vb ? idb : 0], va, vb, seg, slot, wave, lane, bh, stride, stage,
+         stage_stride);
   const int steps = qmax + P - 1;
   for (int s0 = 0; s0 < steps; s0 += U) {
     L.step_begin();

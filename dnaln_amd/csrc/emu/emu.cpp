@@ -139,6 +139,7 @@ void run_split(const SwArgs& a0) {
   a.dir_slot_words = slot_words;
   std::vector<F> L(P);
   std::vector<uint32_t> bh((size_t)P * S);  // each lane's best row: S words of two half-words
+  std::vector<uint32_t> stage((size_t)P * seg_groups(W, P) * 8);  // each lane's flag staging
   const pk16 zero = pk_splat(0);
   for (uint32_t g = 0; g < groups; ++g) {
     for (int j = 0; j < P; ++j) {
@@ -154,7 +155,8 @@ void run_split(const SwArgs& a0) {
         for (int seg = 0; seg < P; ++seg)
           L[seg].init(a, a.tasks[va ? ka : 0], a.tasks[vb ? kb : 0], va, vb, seg,
                       TB ? a.dir + (uint64_t)g * slot_words : nullptr, j, pl * P + seg,
-                      reinterpret_cast<uint16_t*>(bh.data() + (size_t)seg * S), 1);
+                      reinterpret_cast<uint16_t*>(bh.data() + (size_t)seg * S), 1,
+                      stage.data() + (size_t)seg * seg_groups(W, P) * 8, 1);
         const int steps = qmax + P - 1;
         for (int s0 = 0; s0 < steps; s0 += U) {
           for (F& x : L) x.step_begin();
