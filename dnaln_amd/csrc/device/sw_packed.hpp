@@ -354,66 +354,65 @@ DN_HD void walk_step(Walk& w, const SwTask& tk, const uint32_t* plane, const Sco
   constexpr int S = seg_width(W, P), GS = seg_groups(W, P);
   constexpr bool kGran = flag_granules<W, P>();
   const int oe = sp.gap_open + sp.gap_ext, ge = sp.gap_ext;
-  if (w.state == 0 && (w.i < 0 || w.v <= 0)) {
-    if (w.run) walk_push(w, 0, w.run, out, cap);
-    w.run = 0;
-    walk_flush(w, out, cap);
-    w.done = true;
-    return;
+  // Branch-free: every lane of a wave computes the same step, whatever state it is in, and the
+  // CIGAR pushes are two predicated stores (a branch per state made each wave step run the union
+  // of its lanes' paths). The outcome equals walk_task's reference order of cases:
+  //   fin   (state H, row < 0 or score <= 0): push the match run, flush, done;
+  //   bad   (cell outside the band; never reached by a consistent walk): status 1, done;
+  //   diag  (state H, diagonal flag): score and NM of the pair, run + 1, row - 1;
+  //   open  (state H, not diagonal): push the match run, enter E or F by the notE flag;
+  //   ext / close (state E or F): one gap cell (E: row - 1, column + 1; F: column - 1); a cell
+  //         without its extension flag closes the gap: push it, NM += its length, back to H.
+  const bool fin = w.state == 0 && (w.i < 0 || w.v <= 0);
+  const bool bad = !fin && (w.i < 0 || w.d < 0 || w.d >= W);
+  const bool live = !fin && !bad;
+  const int seg = P == 1 ? 0 : (live ? w.d : 0) / S, col = (live ? w.d : 0) - seg * S;
+  uint32_t word = 0;
+  if (live) {
+    if (kGran) {
+      const int r = w.i + seg - row0;
+      word = plane[(uint64_t)((r >> 2) * GS + (col >> 3)) * (512 * P) + seg * 256 + (r & 3)];
+    } else {
+      word = plane[(uint64_t)((w.i + seg - row0) * GS + (col >> 3)) * (128 * P) + seg * 64];
+    }
   }
-  if (w.i < 0 || w.d < 0 || w.d >= W) {  // never reached by a consistent walk
-    w.status = 1;
-    w.done = true;
-    return;
-  }
-  // The flag word and, speculatively, the two bases of the cell (mostly from registers).
-  const int seg = P == 1 ? 0 : w.d / S, col = w.d - seg * S;
-  uint32_t word;
-  if (kGran) {
-    const int r = w.i + seg - row0;
-    word = plane[(uint64_t)((r >> 2) * GS + (col >> 3)) * (512 * P) + seg * 256 + (r & 3)];
-  } else {
-    word = plane[(uint64_t)((w.i + seg - row0) * GS + (col >> 3)) * (128 * P) + seg * 64];
-  }
+  // The two bases of the cell (4 outside the sequences; mostly from the windows).
   const uint8_t a = seq.qb(tk, w.i), c = seq.tb(tk, w.i + w.d);
   const int b = col & 7;
-  if (w.state == 0) {
-    if (!((word >> (24 + b)) & 1)) {  // diagonal
-      w.v -= sub_score(sp, a, c);
-      w.nm += (a != c || a > 3);
-      ++w.run;
-      --w.i;
-      return;
-    }
-    if (w.run) walk_push(w, 0, w.run, out, cap);
-    w.run = 0;
-    w.state = ((word >> (16 + b)) & 1) ? 2 : 1;  // notE set: deletion
-    return;
-  }
-  if (w.state == 1) {
-    const bool ext = (word >> (8 + b)) & 1;
-    w.v += ext ? ge : oe;
-    ++w.run;
-    --w.i;
-    ++w.d;
-    if (!ext) {
-      walk_push(w, 1, w.run, out, cap);
-      w.nm += w.run;
-      w.run = 0;
-      w.state = 0;
-    }
-    return;
-  }
-  const bool ext = (word >> b) & 1;
-  w.v += ext ? ge : oe;
-  ++w.run;
-  --w.d;
-  if (!ext) {
-    walk_push(w, 2, w.run, out, cap);
-    w.nm += w.run;
-    w.run = 0;
-    w.state = 0;
-  }
+  const bool s0 = w.state == 0, s1 = w.state == 1;
+  const bool diag = live && s0 && !((word >> (24 + b)) & 1);
+  const bool open = live && s0 && !diag;
+  const bool gap = live && !s0;
+  const bool ext = gap && ((s1 ? (word >> (8 + b)) : (word >> b)) & 1);
+  const bool close = gap && !ext;
+  // Run push (walk_push): the match run on fin / open, the gap on close.
+  const int run1 = w.run + 1;
+  const bool push = ((fin || open) && w.run > 0) || close;
+  const uint32_t pop = close ? (s1 ? 1u : 2u) : 0u;
+  const int plen = close ? run1 : w.run;
+  const bool merge = push && w.pend_len > 0 && w.pend_op == pop;
+  const bool emit = push && !merge && w.pend_len > 0;
+  if (emit && w.n < cap) out[w.n] = (uint32_t)w.pend_len << 4 | w.pend_op;
+  int n = w.n + (emit ? 1 : 0);
+  int pend_len = merge ? w.pend_len + plen : push ? plen : w.pend_len;
+  const uint32_t pend_op = push && !merge ? pop : w.pend_op;
+  // Flush on fin (walk_flush).
+  const bool flush = fin && pend_len > 0;
+  if (flush && n < cap) out[n] = (uint32_t)pend_len << 4 | pend_op;
+  n += flush ? 1 : 0;
+  pend_len = fin ? 0 : pend_len;
+  w.n = n;
+  w.pend_len = pend_len;
+  w.pend_op = pend_op;
+  // Score, NM, run, coordinates, state.
+  w.v += diag ? -sub_score(sp, a, c) : gap ? (ext ? ge : oe) : 0;
+  w.nm += diag ? (int)(a != c || a > 3) : close ? run1 : 0;
+  w.run = diag || ext ? run1 : (fin || open || close) ? 0 : w.run;
+  w.i -= diag || (gap && s1) ? 1 : 0;
+  w.d += gap ? (s1 ? 1 : -1) : 0;
+  w.state = open ? (((word >> (16 + b)) & 1) ? 2 : 1) : close ? 0 : w.state;
+  w.status = bad ? 1 : w.status;
+  w.done = fin || bad;
 }
 
 DN_HD Walk walk_init(int i_end, int d_end, int score) {
