@@ -538,6 +538,7 @@ __global__ __launch_bounds__(128) void k_walk_win_g(SwArgs a, int rows, const ui
       r.qb = r.qe = r.tb = r.te = r.nm = r.n_cigar = r.status = 0;
     } else {
       tk = a.tasks[k];
+      seq.begin(tk);
       out = a.cigar + (uint64_t)k * (uint64_t)a.cigar_cap;
       w = walk_init(i_end, d_end, score);
       active = true;

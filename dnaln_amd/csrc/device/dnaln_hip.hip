@@ -464,6 +464,7 @@ __global__ __launch_bounds__(128) void k_walk_win(SwArgs a, int rows) {
       r.qb = r.qe = r.tb = r.te = r.nm = r.n_cigar = r.status = 0;
     } else {
       tk = a.tasks[id];
+      seq.begin(tk);
       out = a.cigar + (uint64_t)id * (uint64_t)a.cigar_cap;
       w = walk_init(i_end, d_end, score);
       active = true;

@@ -272,49 +272,64 @@ struct Walk {
 // bytes.
 constexpr int kSeqWindow = 32;  // bytes
 
-struct SeqWindow {
-  uint32_t* buf = nullptr;
-  int stride = 1;
-  uint64_t at = ~0ull;
-  DN_HD uint8_t get(uint64_t addr) {
-    const uint64_t a = addr & ~(uint64_t)(kSeqWindow - 1);
-    if (a != at) {
-      Quad c[kSeqWindow / 16];
-#pragma unroll
-      for (int x = 0; x < kSeqWindow / 16; ++x) c[x] = load16(a + 16 * x);
-#pragma unroll
-      for (int x = 0; x < kSeqWindow / 16; ++x) {
-        buf[(4 * x) * stride] = c[x].x;
-        buf[(4 * x + 1) * stride] = c[x].y;
-        buf[(4 * x + 2) * stride] = c[x].z;
-        buf[(4 * x + 3) * stride] = c[x].w;
-      }
-      at = a;
-    }
-    const uint32_t x = (uint32_t)(addr & (kSeqWindow - 1));
-    return (uint8_t)(buf[(x >> 2) * stride] >> (8 * (x & 3)));
-  }
-};
-
-// The query and target windows of one walk: kWalkSeqDwords dwords at buf, stride apart.
+// The query and target windows of one walk: kWalkSeqDwords dwords at buf, stride apart (query
+// window first). begin() takes the task's constants once (where row i's query byte lies, and
+// whether to complement it); bases() refreshes a window only when the walk leaves it and then
+// issues both LDS reads before using either, with no branch on the strand.
 constexpr int kWalkSeqDwords = 2 * kSeqWindow / 4;
 struct WalkSeq {
-  SeqWindow q, t;
+  uint32_t* buf = nullptr;
+  int stride = 1;
+  uint64_t qbase = 0, tbase = 0;  // the query byte of row i at qbase + qstep * i, target j at tbase + j
+  int qstep = 1;
+  uint32_t qrc = 0;  // 3 for a reversed query: its codes are complemented (c ^ 3 = 3 - c)
+  int qlen = 0, tlo = 0, thi = 0;
+  uint64_t qat = ~0ull, tat = ~0ull;  // the windows' addresses
   WalkSeq() = default;
-  DN_HD WalkSeq(uint32_t* buf, int stride) {
-    q.buf = buf;
-    q.stride = stride;
-    t.buf = buf + (kSeqWindow / 4) * stride;
-    t.stride = stride;
+  DN_HD WalkSeq(uint32_t* b, int s) : buf(b), stride(s) {}
+  DN_HD void begin(const SwTask& tk) {
+    const bool rev = tk.flags & 1;
+    qbase = rev ? tk.q + (uint64_t)(int64_t)(tk.qlen - 1) : tk.q;
+    qstep = rev ? -1 : 1;
+    qrc = rev ? 3u : 0u;
+    tbase = tk.t;
+    qlen = tk.qlen;
+    tlo = tk.tlo;
+    thi = tk.thi;
+    qat = tat = ~0ull;
   }
-  DN_HD uint8_t qb(const SwTask& tk, int i) {
-    if (i < 0 || i >= tk.qlen) return 4;
-    if (tk.flags & 1) return comp_code(q.get(tk.q + (uint64_t)(tk.qlen - 1 - i)));
-    return q.get(tk.q + (uint64_t)i);
+  // The kSeqWindow aligned bytes at a into the window at dword offset off.
+  DN_HD void fill(int off, uint64_t a) {
+    Quad c[kSeqWindow / 16];
+#pragma unroll
+    for (int x = 0; x < kSeqWindow / 16; ++x) c[x] = load16(a + 16 * x);
+#pragma unroll
+    for (int x = 0; x < kSeqWindow / 16; ++x) {
+      buf[(off + 4 * x) * stride] = c[x].x;
+      buf[(off + 4 * x + 1) * stride] = c[x].y;
+      buf[(off + 4 * x + 2) * stride] = c[x].z;
+      buf[(off + 4 * x + 3) * stride] = c[x].w;
+    }
   }
-  DN_HD uint8_t tb(const SwTask& tk, int j) {
-    if (j < tk.tlo || j >= tk.thi) return 4;
-    return t.get(tk.t + (uint64_t)(int64_t)j);
+  // The query code of row i and the target code of position j, 4 outside the sequences (a
+  // reversed query's codes complemented, as comp_code).
+  DN_HD void bases(int i, int j, uint8_t& qc, uint8_t& tc) {
+    const bool qin = i >= 0 && i < qlen, tin = j >= tlo && j < thi;
+    const uint64_t aq = qbase + (uint64_t)(int64_t)(qstep * i), at = tbase + (uint64_t)(int64_t)j;
+    const uint64_t wq = aq & ~(uint64_t)(kSeqWindow - 1), wt = at & ~(uint64_t)(kSeqWindow - 1);
+    if (qin && wq != qat) {
+      fill(0, wq);
+      qat = wq;
+    }
+    if (tin && wt != tat) {
+      fill(kSeqWindow / 4, wt);
+      tat = wt;
+    }
+    const uint32_t xq = (uint32_t)aq & (kSeqWindow - 1), xt = (uint32_t)at & (kSeqWindow - 1);
+    const uint32_t dq = buf[(xq >> 2) * stride], dt = buf[(kSeqWindow / 4 + (xt >> 2)) * stride];
+    const uint32_t cq = (dq >> (8 * (xq & 3))) & 0xffu, ct = (dt >> (8 * (xt & 3))) & 0xffu;
+    qc = qin ? (uint8_t)(cq < 4 ? cq ^ qrc : 4u) : (uint8_t)4;
+    tc = tin ? (uint8_t)ct : (uint8_t)4;
   }
 };
 
@@ -377,7 +392,8 @@ DN_HD void walk_step(Walk& w, const SwTask& tk, const uint32_t* plane, const Sco
     }
   }
   // The two bases of the cell (4 outside the sequences; mostly from the windows).
-  const uint8_t a = seq.qb(tk, w.i), c = seq.tb(tk, w.i + w.d);
+  uint8_t a, c;
+  seq.bases(w.i, w.i + w.d, a, c);
   const int b = col & 7;
   const bool s0 = w.state == 0, s1 = w.state == 1;
   const bool diag = live && s0 && !((word >> (24 + b)) & 1);
@@ -699,7 +715,7 @@ DN_HD void walk_task(const SwArgs& a, uint32_t k, WalkSeq& seq) {
                           (flag_granules<W, P>() ? (half * 64 * P + pair) * 4 : half * 64 * P + pair);
   uint32_t* out = a.cigar + (uint64_t)id * (uint64_t)a.cigar_cap;
   Walk w = walk_init(i_end, d_end, score);
-  seq.q.at = seq.t.at = ~0ull;
+  seq.begin(tk);
   while (!w.done) walk_step<W, P>(w, tk, plane, a.sp, out, a.cigar_cap, seq);
   walk_finish(w, i_end, d_end, score, a.cigar_cap, r);
 }

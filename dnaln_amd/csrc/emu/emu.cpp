@@ -56,6 +56,7 @@ void walk_wave_window(const SwArgs& a, uint32_t wave, int rows) {
       continue;
     }
     x.tk = a.tasks[x.id];
+    x.seq.begin(x.tk);
     x.out = a.cigar + (uint64_t)x.id * (uint64_t)a.cigar_cap;
     x.w = walk_init(x.i_end, x.d_end, x.score);
     x.active = true;
