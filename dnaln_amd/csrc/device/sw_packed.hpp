@@ -271,6 +271,7 @@ struct Walk {
 // kSeqWindow - 1 bytes before and after the bytes read: every sequence buffer carries guard
 // bytes.
 constexpr int kSeqWindow = 32;  // bytes
+static_assert(kSeqWindow == 32, "WalkSeq::bases divides offsets by 32 with a shift");
 
 // The query and target windows of one walk: kWalkSeqDwords dwords at buf, stride apart (query
 // window first). begin() takes the task's constants once (where row i's query byte lies, and
@@ -280,23 +281,28 @@ constexpr int kWalkSeqDwords = 2 * kSeqWindow / 4;
 struct WalkSeq {
   uint32_t* buf = nullptr;
   int stride = 1;
-  uint64_t qbase = 0, tbase = 0;  // the query byte of row i at qbase + qstep * i, target j at tbase + j
-  int qstep = 1;
+  // Positions as 32-bit byte offsets from kSeqWindow-aligned origins: the query byte of row i at
+  // qorg + qofs + qstep * i, target position j at torg + tofs + j (64-bit only on a refill).
+  uint64_t qorg = 0, torg = 0;
+  int qofs = 0, tofs = 0, qstep = 1;
   uint32_t qrc = 0;  // 3 for a reversed query: its codes are complemented (c ^ 3 = 3 - c)
   int qlen = 0, tlo = 0, thi = 0;
-  uint64_t qat = ~0ull, tat = ~0ull;  // the windows' addresses
+  int qwin = 0, twin = 0;  // the windows held (offset / kSeqWindow); a begin() sets impossible ones
   WalkSeq() = default;
   DN_HD WalkSeq(uint32_t* b, int s) : buf(b), stride(s) {}
   DN_HD void begin(const SwTask& tk) {
     const bool rev = tk.flags & 1;
-    qbase = rev ? tk.q + (uint64_t)(int64_t)(tk.qlen - 1) : tk.q;
+    const uint64_t q0 = rev ? tk.q + (uint64_t)(int64_t)(tk.qlen - 1) : tk.q;
+    qorg = q0 & ~(uint64_t)(kSeqWindow - 1);
+    qofs = (int)(q0 & (kSeqWindow - 1));
+    torg = tk.t & ~(uint64_t)(kSeqWindow - 1);
+    tofs = (int)(tk.t & (kSeqWindow - 1));
     qstep = rev ? -1 : 1;
     qrc = rev ? 3u : 0u;
-    tbase = tk.t;
     qlen = tk.qlen;
     tlo = tk.tlo;
     thi = tk.thi;
-    qat = tat = ~0ull;
+    qwin = twin = -(1 << 30);
   }
   // The kSeqWindow aligned bytes at a into the window at dword offset off.
   DN_HD void fill(int off, uint64_t a) {
@@ -315,17 +321,17 @@ struct WalkSeq {
   // reversed query's codes complemented, as comp_code).
   DN_HD void bases(int i, int j, uint8_t& qc, uint8_t& tc) {
     const bool qin = i >= 0 && i < qlen, tin = j >= tlo && j < thi;
-    const uint64_t aq = qbase + (uint64_t)(int64_t)(qstep * i), at = tbase + (uint64_t)(int64_t)j;
-    const uint64_t wq = aq & ~(uint64_t)(kSeqWindow - 1), wt = at & ~(uint64_t)(kSeqWindow - 1);
-    if (qin && wq != qat) {
-      fill(0, wq);
-      qat = wq;
+    const int oq = qofs + qstep * i, ot = tofs + j;
+    const int wq = oq >> 5, wt = ot >> 5;  // kSeqWindow = 32
+    if (qin && wq != qwin) {
+      fill(0, qorg + (uint64_t)(int64_t)(wq * kSeqWindow));
+      qwin = wq;
     }
-    if (tin && wt != tat) {
-      fill(kSeqWindow / 4, wt);
-      tat = wt;
+    if (tin && wt != twin) {
+      fill(kSeqWindow / 4, torg + (uint64_t)(int64_t)(wt * kSeqWindow));
+      twin = wt;
     }
-    const uint32_t xq = (uint32_t)aq & (kSeqWindow - 1), xt = (uint32_t)at & (kSeqWindow - 1);
+    const uint32_t xq = (uint32_t)oq & (kSeqWindow - 1), xt = (uint32_t)ot & (kSeqWindow - 1);
     const uint32_t dq = buf[(xq >> 2) * stride], dt = buf[(kSeqWindow / 4 + (xt >> 2)) * stride];
     const uint32_t cq = (dq >> (8 * (xq & 3))) & 0xffu, ct = (dt >> (8 * (xt & 3))) & 0xffu;
     qc = qin ? (uint8_t)(cq < 4 ? cq ^ qrc : 4u) : (uint8_t)4;
@@ -384,11 +390,12 @@ DN_HD void walk_step(Walk& w, const SwTask& tk, const uint32_t* plane, const Sco
   const int seg = P == 1 ? 0 : (live ? w.d : 0) / S, col = (live ? w.d : 0) - seg * S;
   uint32_t word = 0;
   if (live) {
+    // 32-bit word indexes: a slot holds far fewer than 2^32 words.
     if (kGran) {
       const int r = w.i + seg - row0;
-      word = plane[(uint64_t)((r >> 2) * GS + (col >> 3)) * (512 * P) + seg * 256 + (r & 3)];
+      word = plane[(uint32_t)(((r >> 2) * GS + (col >> 3)) * (512 * P) + seg * 256 + (r & 3))];
     } else {
-      word = plane[(uint64_t)((w.i + seg - row0) * GS + (col >> 3)) * (128 * P) + seg * 64];
+      word = plane[(uint32_t)(((w.i + seg - row0) * GS + (col >> 3)) * (128 * P) + seg * 64)];
     }
   }
   // The two bases of the cell (4 outside the sequences; mostly from the windows).
