@@ -265,9 +265,11 @@ struct Walk {
 // position per step, so a window is refilled (16-byte loads) every kSeqWindow steps. Measured on
 // MI355X (2M x 150 bp, W = 33): a dword cached in a register and refilled every 4 steps walks in
 // 2.7 ms, since with a wave's worth of streams per SIMD its refills miss L2; 64-byte windows in
-// 1.8 ms, 32 bytes in 1.2 ms, 16 bytes in 1.24 ms (the walk is latency-bound, so the LDS each
-// thread holds, through occupancy, matters more than how often it refills); prefetching the
-// next 64 bytes into registers cost more occupancy than it saved (2.6 ms). Windows reach up to
+// 1.8 ms, 32 bytes in 1.2 ms, 16 bytes in 1.24 ms (the LDS each thread holds sets the occupancy
+// that hides the flag loads, which matters more than how often a window refills); prefetching
+// the next 64 bytes into registers cost more occupancy than it saved (2.6 ms). The walk is then
+// bound by its instruction issue more than by its loads (about 108 VALU instructions per wave
+// step kept the SIMDs ~74% busy), which is why a step is written branch-free. Windows reach up to
 // kSeqWindow - 1 bytes before and after the bytes read: every sequence buffer carries guard
 // bytes.
 constexpr int kSeqWindow = 32;  // bytes
